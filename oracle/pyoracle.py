@@ -1,0 +1,216 @@
+"""ctypes view of the CPU restatement (liboracle.so) and of reference dumps.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+benchmark's CPU-baseline leg; never by the product.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import struct
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_BIN = os.path.join(HERE, "_ref", "pfref")
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("phases", ctypes.c_int32), ("mu_pos", ctypes.c_uint32), ("mu_neg", ctypes.c_uint32),
+                ("lcve_min", ctypes.c_uint32), ("lits_min", ctypes.c_int64), ("shrink_rate", ctypes.c_int32),
+                ("ve_clause_limit", ctypes.c_int32), ("ve_lbound_en", ctypes.c_int32),
+                ("xor_max_arity", ctypes.c_int32), ("sub_limit", ctypes.c_int32), ("ere_limit", ctypes.c_int32),
+                ("ere_max_resolvent", ctypes.c_int32), ("sub_en", ctypes.c_int32), ("ve_en", ctypes.c_int32),
+                ("ere_en", ctypes.c_int32), ("lbd_tier1", ctypes.c_int32), ("stable_sort", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            subprocess.run(["make", "-C", HERE, "all"], check=True, capture_output=True)
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+        u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
+        L.pfo_default_opts.argtypes = [ctypes.POINTER(Opts)]
+        L.pfo_create.restype = vp
+        L.pfo_create.argtypes = [ctypes.c_uint32, ctypes.POINTER(Opts), u32p, u64p, ctypes.c_uint64, vp]
+        L.pfo_destroy.argtypes = [vp]
+        L.pfo_enqueue.argtypes = [vp, ctypes.c_uint32]
+        for f in ("pfo_sigmify", "pfo_prop", "pfo_lcve", "pfo_unsat", "pfo_phase"):
+            getattr(L, f).restype = ctypes.c_int
+            getattr(L, f).argtypes = [vp]
+        for f in ("pfo_resize_cnf", "pfo_create_ot", "pfo_sort_ot", "pfo_ere", "pfo_sub", "pfo_ve",
+                  "pfo_filter_pvs", "pfo_shrink", "pfo_next_phase"):
+            getattr(L, f).restype = None
+            getattr(L, f).argtypes = [vp]
+        L.pfo_stop.restype = ctypes.c_int
+        L.pfo_stop.argtypes = [vp, ctypes.c_int64]
+        L.pfo_count_all.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.pfo_hist.argtypes = [vp, u32p]
+        L.pfo_mu_inc.restype = ctypes.c_uint32
+        L.pfo_mu_inc.argtypes = [vp]
+        L.pfo_num_refs.restype = ctypes.c_uint64
+        L.pfo_num_refs.argtypes = [vp]
+        L.pfo_dump_cnf.restype = ctypes.c_uint64
+        L.pfo_dump_cnf.argtypes = [vp, vp]
+        L.pfo_trail.restype = ctypes.c_uint32
+        L.pfo_trail.argtypes = [vp, vp]
+        L.pfo_resolved.restype = ctypes.c_uint64
+        L.pfo_resolved.argtypes = [vp, vp]
+        L.pfo_vstate.argtypes = [vp, vp]
+        L.pfo_pvs.restype = ctypes.c_uint32
+        L.pfo_pvs.argtypes = [vp, vp]
+        L.pfo_ot_list.restype = ctypes.c_int32
+        L.pfo_ot_list.argtypes = [vp, ctypes.c_uint32, vp]
+        L.pfo_ot_size.restype = ctypes.c_uint32
+        L.pfo_ot_size.argtypes = [vp, ctypes.c_uint32]
+        L.pfo_introsort_u32.argtypes = [u32p, ctypes.c_int64, u32p]
+        L.pfo_extend_model.argtypes = [vp, vp]
+        _lib = L
+    return _lib
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().pfo_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+@dataclass
+class State:
+    """Simplifier state in a comparable form."""
+    nvars: int
+    cnf: np.ndarray          # uint32 words, per clause [hdr0, sig, size, lits...] in scnf order
+    trail: np.ndarray        # uint32
+    resolved: np.ndarray     # uint32
+    vstate: np.ndarray       # uint8, nvars + 1
+    pvs: np.ndarray          # uint32
+    lists: list = field(default_factory=list)   # per PV literal: np.int64 scnf indices
+    unsat: int = 0
+
+    def clauses(self):
+        w, i, out = self.cnf, 0, []
+        while i < len(w):
+            n = int(w[i + 2])
+            out.append((int(w[i]), int(w[i + 1]), tuple(int(x) for x in w[i + 3:i + 3 + n])))
+            i += 3 + n
+        return out
+
+
+class Oracle:
+    def __init__(self, cnf, opts: Opts | None = None, hdr0=None):
+        self.L = lib()
+        self.nvars = cnf.nvars
+        self.opts = opts or default_opts()
+        h = None if hdr0 is None else np.ascontiguousarray(hdr0, np.uint32).ctypes.data
+        self.h = self.L.pfo_create(cnf.nvars, ctypes.byref(self.opts), np.ascontiguousarray(cnf.lits, np.uint32),
+                                   np.ascontiguousarray(cnf.offs, np.uint64), cnf.ncls, h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.pfo_destroy(self.h)
+            self.h = None
+
+    def sigmify(self) -> int:
+        return self.L.pfo_sigmify(self.h)
+
+    def state(self, lists=False) -> State:
+        L, h = self.L, self.h
+        nw = L.pfo_dump_cnf(h, None)
+        cnf = np.empty(nw, np.uint32)
+        L.pfo_dump_cnf(h, cnf.ctypes.data)
+        nt = L.pfo_trail(h, None)
+        trail = np.empty(nt, np.uint32)
+        L.pfo_trail(h, trail.ctypes.data)
+        nr = L.pfo_resolved(h, None)
+        res = np.empty(nr, np.uint32)
+        L.pfo_resolved(h, res.ctypes.data)
+        vs = np.empty(self.nvars + 1, np.uint8)
+        L.pfo_vstate(h, vs.ctypes.data)
+        npv = L.pfo_pvs(h, None)
+        pvs = np.empty(npv, np.uint32)
+        L.pfo_pvs(h, pvs.ctypes.data)
+        st = State(self.nvars, cnf, trail, res, vs, pvs, unsat=L.pfo_unsat(h))
+        if lists:
+            for v in pvs:
+                for sg in (0, 1):
+                    lit = 2 * int(v) + sg
+                    n = L.pfo_ot_size(h, lit)
+                    idx = np.empty(n, np.int64)
+                    L.pfo_ot_list(h, lit, idx.ctypes.data)
+                    st.lists.append(idx)
+        return st
+
+    def extend_model(self, value: np.ndarray) -> np.ndarray:
+        v = np.ascontiguousarray(value, np.uint8).copy()
+        self.L.pfo_extend_model(self.h, v.ctypes.data)
+        return v
+
+
+def read_dump(path: str) -> State:
+    with open(path, "rb") as f:
+        b = f.read()
+    off = 0
+
+    def take(fmt):
+        nonlocal off
+        v = struct.unpack_from(fmt, b, off)
+        off += struct.calcsize(fmt)
+        return v
+
+    magic, nvars = take("<II")
+    assert magic == 0x44524650
+    _, words = take("<QQ")
+    cnf = np.frombuffer(b, np.uint32, words, off).copy(); off += 4 * words
+    (nt,) = take("<I")
+    trail = np.frombuffer(b, np.uint32, nt, off).copy(); off += 4 * nt
+    (nr,) = take("<Q")
+    res = np.frombuffer(b, np.uint32, nr, off).copy(); off += 4 * nr
+    vs = np.frombuffer(b, np.uint8, nvars + 1, off).copy(); off += nvars + 1
+    (npv,) = take("<I")
+    pvs = np.frombuffer(b, np.uint32, npv, off).copy(); off += 4 * npv
+    (nl,) = take("<I")
+    lists = []
+    for _ in range(nl):
+        (n,) = take("<I")
+        lists.append(np.frombuffer(b, np.int64, n, off).copy()); off += 8 * n
+    (status,) = take("<I")
+    return State(nvars, cnf, trail, res, vs, pvs, lists, unsat=status)
+
+
+def ref_opts_args(o: Opts) -> list[str]:
+    """Reference command-line options equal to an Opts value (src/cpu/options.cpp names)."""
+    a = [f"--phases={o.phases}", f"--mupos={o.mu_pos}", f"--muneg={o.mu_neg}", f"--lcvemin={o.lcve_min}",
+         f"--vephasemin={o.lits_min}", f"--gcfreq={o.shrink_rate}", f"--veclausemax={o.ve_clause_limit}",
+         f"--xormaxarity={o.xor_max_arity}", f"--submax={o.sub_limit}", f"--eremax={o.ere_limit}",
+         f"--ereclausemax={o.ere_max_resolvent}", "-velbound" if o.ve_lbound_en else "-no-velbound",
+         "-ere" if o.ere_en else "-no-ere", "--mapperc=1"]
+    return a
+
+
+def run_reference(cnf_path: str, out_path: str, o: Opts, trace_prefix: str | None = None,
+                  timeout: float = 600) -> State:
+    env = dict(os.environ)
+    if trace_prefix:
+        env["PFREF_TRACE"] = trace_prefix
+    if os.path.exists(out_path):
+        os.remove(out_path)
+    r = subprocess.run([REF_BIN, cnf_path, out_path] + ref_opts_args(o), env=env,
+                       capture_output=True, timeout=timeout)
+    if not os.path.exists(out_path):
+        # the reference ends the process on a proven contradiction (killSolver)
+        if b"UNSATISFIABLE" in r.stdout:
+            return State(0, np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint32),
+                         np.zeros(0, np.uint8), np.zeros(0, np.uint32), unsat=1)
+        raise RuntimeError(f"reference run failed ({r.returncode}): {r.stdout[-500:]!r} {r.stderr[-500:]!r}")
+    return read_dump(out_path)
