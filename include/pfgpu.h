@@ -1,0 +1,121 @@
+/*
+ * pfgpu.h — C ABI of the MI355X SIGmA simplifier (libpfgpu.so).
+ *
+ * One opaque context per device-resident CNF instance. A context owns its HBM
+ * pools (clause pool, clause references, occurrence table, variable state,
+ * unit trail, model-reconstruction stack) and one HIP stream. Every call returns
+ * a status code and never exits the process. Calls on one context are not
+ * re-entrant; distinct contexts may be driven from distinct host threads.
+ *
+ * Data contract (kept from ParaFROST):
+ *   literal      = 2*var + sign (sign 1 = negative), variables 1..nvars
+ *                  (src/gpu/const.cuh, src/cpu/const.h:72-78)
+ *   clause image = [word0 {status:2, molten:1, added:1, usage:2, lbd:26}, sig, size, lits...]
+ *                  (SCLAUSE, src/gpu/sclause.h:37-186); references are uint64 word offsets
+ *                  into the clause pool (S_REF, src/gpu/sclause.h)
+ *   occurrence list per literal = {uint64 offset, uint32 size, uint32 capacity} over a
+ *                  uint64 reference array (OT/OL, src/gpu/simptypes.cuh:279-318)
+ *   resolved stack = model-reconstruction records (src/gpu/model.cuh:28-53)
+ * Semantics of every pass are those of the reference simplifier ParaFROST src/cpu
+ * (bit-exact, including clause order); see the per-function notes.
+ */
+#ifndef PFGPU_H
+#define PFGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PFG_OK = 0,
+    PFG_EINVAL = 1,     /* bad argument or call order                         */
+    PFG_ENOMEM = 2,     /* HBM budget exceeded (reference CNFALLOC/OTALLOC_FAIL) */
+    PFG_EDEVICE = 3,    /* HIP runtime error                                  */
+    PFG_EUNSAT = 4,     /* formula proven unsatisfiable (reference learnEmpty)   */
+    PFG_EINTERNAL = 5   /* an internal invariant failed                      */
+};
+
+typedef struct pfg_ctx pfg_ctx;
+
+/* Simplifier options; defaults equal the reference (src/cpu/options.cpp:24-46). */
+typedef struct pfg_opts {
+    int32_t phases;            /* 5    */
+    uint32_t mu_pos, mu_neg;   /* 32   */
+    uint32_t lcve_min;         /* 1    */
+    int64_t lits_min;          /* 500  (vephasemin) */
+    int32_t shrink_rate;       /* 2    (gcfreq) */
+    int32_t ve_clause_limit;   /* 100  */
+    int32_t ve_lbound_en;      /* 0    */
+    int32_t xor_max_arity;     /* 10   */
+    int32_t sub_limit;         /* 3000 */
+    int32_t ere_limit;         /* 3000 */
+    int32_t ere_max_resolvent; /* 1000 */
+    int32_t sub_en, ve_en, ere_en;  /* 1, 1, 1 */
+    int32_t lbd_tier1;         /* 2    */
+    int32_t stable_sort;       /* 0: lists > 20 ordered as libstdc++ std::sort (reference) */
+} pfg_opts;
+
+/* per-pass device time accumulated over the context's life, in milliseconds */
+typedef struct pfg_times {
+    double upload, hist, ot, sort_ot, prop, lcve, sub, ve, ere, gc, count, total;
+} pfg_times;
+
+void pfg_default_opts(pfg_opts *o);
+
+/* ParaFROST: cuMM::init + allocations of awaken() (src/gpu/simplify.cu:91-153, memory.cu:82-206).
+ * hbm_budget = 0 means no limit beyond the device. */
+int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, const pfg_opts *opts);
+int pfg_destroy(pfg_ctx *ctx);
+
+/* ParaFROST: extract + reflectCNF + prepareCNFAsync (transfer.cu:24-42, kernels.cu:61-70):
+ * clauses in CSR form (host memory), literal order free; hdr0 (nullable) = packed status word. */
+int pfg_upload_cnf(pfg_ctx *ctx, const uint32_t *lits, const uint64_t *offs, uint64_t ncls,
+                   const uint32_t *hdr0);
+/* the same with lits/offs already in device memory (HBM-resident input) */
+int pfg_upload_cnf_device(pfg_ctx *ctx, const uint32_t *d_lits, const uint64_t *d_offs, uint64_t ncls,
+                          uint64_t nlits);
+/* root-level units to propagate (ParaFROST trail entries before sigmify) */
+int pfg_enqueue_units(pfg_ctx *ctx, const uint32_t *lits, uint32_t n);
+
+/* ParaFROST::sigmify (src/gpu/simplify.cu:75-89 / src/cpu/simplify.cpp:115-214): the whole
+ * round loop on the device, ending with GC of deleted clauses. */
+int pfg_sigmify(pfg_ctx *ctx);
+
+/* Individual passes, in the reference round order (src/cpu/simplify.cpp:182-200). */
+int pfg_resize_cnf(pfg_ctx *ctx);                 /* resizeCNF/reallocCNF  (simplify.cu:30-46)        */
+int pfg_create_ot(pfg_ctx *ctx);                  /* histSimp+resizeOTAsync+createOTAsync              */
+int pfg_reduce_ot(pfg_ctx *ctx);                  /* reduceOTAsync         (kernels.cu:98-105)          */
+int pfg_prop(pfg_ctx *ctx);                       /* prop                  (elimbcp.cu:103-193)         */
+int pfg_lcve(pfg_ctx *ctx, uint32_t *npvs);       /* varReorder + LCVE     (lcve.cu:23-130)             */
+int pfg_sort_ot(pfg_ctx *ctx);                    /* sortOT                (segsort.cu:32-41)           */
+int pfg_sub(pfg_ctx *ctx);                        /* subAsync              (kernels.cu:1133-1154)       */
+int pfg_ve(pfg_ctx *ctx);                         /* veAsync + postVE      (kernels.cu:1085-1120)       */
+int pfg_ere(pfg_ctx *ctx);                        /* ereAsync              (kernels.cu:1171-1195)       */
+int pfg_count(pfg_ctx *ctx, int64_t *cls, int64_t *lits); /* parcountAll (kernels.cu:855-925)           */
+int pfg_filter_pvs(pfg_ctx *ctx);                 /* filterPVs / DeviceSelect::If (elimination.cu:37-50) */
+int pfg_gc(pfg_ctx *ctx);                         /* compactCNF            (recycle.cu:71-97)           */
+int pfg_stop(pfg_ctx *ctx, int64_t lr, int *stop);/* stop                  (solve.h:773-778)            */
+int pfg_next_phase(pfg_ctx *ctx);                 /* phase++, mu_inc       (simplify.cu:200-201)        */
+
+/* histogram of live literals (histSimp, histogram.cu:34-51) into host hist[2*(nvars+1)] */
+int pfg_hist(pfg_ctx *ctx, uint32_t *hist);
+
+/* state download (cacheCNF / cacheUnits / cacheResolved / cacheEliminated, transfer.cu:104-179).
+ * Sizes are returned through the size pointers; a NULL buffer only queries the size. */
+int pfg_download_cnf(pfg_ctx *ctx, uint32_t *words, uint64_t *nwords, uint64_t *ncls);
+int pfg_download_trail(pfg_ctx *ctx, uint32_t *lits, uint32_t *n);
+int pfg_download_resolved(pfg_ctx *ctx, uint32_t *words, uint64_t *n);
+int pfg_download_vstate(pfg_ctx *ctx, uint8_t *vstate);          /* nvars + 1 bytes */
+int pfg_download_pvs(pfg_ctx *ctx, uint32_t *pvs, uint32_t *n);
+/* occurrence list of one literal as clause positions (index in clause order) */
+int pfg_download_ot_list(pfg_ctx *ctx, uint32_t lit, int64_t *idx, uint32_t *n);
+
+int pfg_times_get(pfg_ctx *ctx, pfg_times *t);
+int pfg_is_unsat(pfg_ctx *ctx);
+const char *pfg_last_error(pfg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,241 @@
+"""MI355X-native SIGmA simplifier (ParaFROST's GPU inprocessing engine, rebuilt).
+
+Python view of libpfgpu.so. ``Simplifier`` mirrors the reference entry points
+(ParaFROST::sigmify and the pass wrappers of src/cpu/simplify.cpp / src/gpu/simplify.cu);
+every method calls the C ABI declared in include/pfgpu.h. There is no fallback:
+if the native library is missing or the device call fails, an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import gen  # noqa: F401  (workload generators)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+PFG_OK, PFG_EINVAL, PFG_ENOMEM, PFG_EDEVICE, PFG_EUNSAT, PFG_EINTERNAL = range(6)
+_ERRNAMES = {1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EUNSAT", 5: "EINTERNAL"}
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("phases", ctypes.c_int32), ("mu_pos", ctypes.c_uint32), ("mu_neg", ctypes.c_uint32),
+                ("lcve_min", ctypes.c_uint32), ("lits_min", ctypes.c_int64), ("shrink_rate", ctypes.c_int32),
+                ("ve_clause_limit", ctypes.c_int32), ("ve_lbound_en", ctypes.c_int32),
+                ("xor_max_arity", ctypes.c_int32), ("sub_limit", ctypes.c_int32), ("ere_limit", ctypes.c_int32),
+                ("ere_max_resolvent", ctypes.c_int32), ("sub_en", ctypes.c_int32), ("ve_en", ctypes.c_int32),
+                ("ere_en", ctypes.c_int32), ("lbd_tier1", ctypes.c_int32), ("stable_sort", ctypes.c_int32)]
+
+
+class Times(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("upload", "hist", "ot", "sort_ot", "prop", "lcve", "sub", "ve", "ere", "gc", "count", "total")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_upload_cnf_device",
+       "pfg_enqueue_units", "pfg_sigmify", "pfg_resize_cnf", "pfg_create_ot", "pfg_reduce_ot", "pfg_prop",
+       "pfg_lcve", "pfg_sort_ot", "pfg_sub", "pfg_ve", "pfg_ere", "pfg_count", "pfg_filter_pvs", "pfg_gc",
+       "pfg_stop", "pfg_next_phase", "pfg_hist", "pfg_download_cnf", "pfg_download_trail",
+       "pfg_download_resolved", "pfg_download_vstate", "pfg_download_pvs", "pfg_download_ot_list",
+       "pfg_times_get", "pfg_is_unsat", "pfg_last_error"]
+
+
+def lib_path() -> str:
+    return os.path.join(_HERE, "libpfgpu.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        vp, i32, u32, u64, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64
+        L.pfg_default_opts.argtypes = [ctypes.POINTER(Opts)]
+        L.pfg_create.argtypes = [ctypes.POINTER(vp), i32, u64, u32, ctypes.POINTER(Opts)]
+        L.pfg_upload_cnf.argtypes = [vp, vp, vp, u64, vp]
+        L.pfg_upload_cnf_device.argtypes = [vp, vp, vp, u64, u64]
+        L.pfg_enqueue_units.argtypes = [vp, vp, u32]
+        for f in ("pfg_destroy", "pfg_sigmify", "pfg_resize_cnf", "pfg_create_ot", "pfg_reduce_ot", "pfg_prop",
+                  "pfg_sort_ot", "pfg_sub", "pfg_ve", "pfg_ere", "pfg_filter_pvs", "pfg_gc", "pfg_next_phase",
+                  "pfg_is_unsat"):
+            getattr(L, f).argtypes = [vp]
+        L.pfg_lcve.argtypes = [vp, ctypes.POINTER(u32)]
+        L.pfg_count.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.pfg_stop.argtypes = [vp, i64, ctypes.POINTER(i32)]
+        L.pfg_hist.argtypes = [vp, vp]
+        L.pfg_download_cnf.argtypes = [vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.pfg_download_trail.argtypes = [vp, vp, ctypes.POINTER(u32)]
+        L.pfg_download_resolved.argtypes = [vp, vp, ctypes.POINTER(u64)]
+        L.pfg_download_vstate.argtypes = [vp, vp]
+        L.pfg_download_pvs.argtypes = [vp, vp, ctypes.POINTER(u32)]
+        L.pfg_download_ot_list.argtypes = [vp, u32, vp, ctypes.POINTER(u32)]
+        L.pfg_times_get.argtypes = [vp, ctypes.POINTER(Times)]
+        L.pfg_last_error.argtypes = [vp]
+        L.pfg_last_error.restype = ctypes.c_char_p
+        for f in API:
+            if getattr(L, f).restype is None and f not in ("pfg_default_opts",):
+                getattr(L, f).restype = ctypes.c_int
+        L.pfg_default_opts.restype = None
+        L.pfg_last_error.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().pfg_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class SimplifierError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Unsat(SimplifierError):
+    pass
+
+
+class Simplifier:
+    """One device-resident CNF instance (a pfg_ctx)."""
+
+    def __init__(self, nvars: int, opts: Opts | None = None, device: int = 0, hbm_budget: int = 0):
+        self.L = lib()
+        self.nvars = nvars
+        self.opts = opts or default_opts()
+        h = ctypes.c_void_p()
+        self._check(self.L.pfg_create(ctypes.byref(h), device, hbm_budget, nvars, ctypes.byref(self.opts)), None)
+        self.h = h
+
+    def _check(self, rc, h="self"):
+        if rc == PFG_OK:
+            return
+        msg = self.L.pfg_last_error(self.h if h == "self" else None)
+        msg = msg.decode() if msg else ""
+        if rc == PFG_EUNSAT:
+            raise Unsat(rc, "formula is unsatisfiable")
+        raise SimplifierError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pfg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- input
+    def upload(self, cnf, hdr0=None):
+        lits = np.ascontiguousarray(cnf.lits, np.uint32)
+        offs = np.ascontiguousarray(cnf.offs, np.uint64)
+        h = None if hdr0 is None else np.ascontiguousarray(hdr0, np.uint32)
+        self._keep = (lits, offs, h)
+        self._check(self.L.pfg_upload_cnf(self.h, lits.ctypes.data, offs.ctypes.data, cnf.ncls,
+                                          None if h is None else h.ctypes.data))
+
+    def upload_device(self, d_lits: int, d_offs: int, ncls: int, nlits: int):
+        """Clauses already in HBM (device pointers, e.g. torch tensor data_ptr())."""
+        self._check(self.L.pfg_upload_cnf_device(self.h, d_lits, d_offs, ncls, nlits))
+
+    def enqueue_units(self, lits):
+        a = np.ascontiguousarray(lits, np.uint32)
+        self._check(self.L.pfg_enqueue_units(self.h, a.ctypes.data, len(a)))
+
+    # ---- reference entry points
+    def sigmify(self):
+        self._check(self.L.pfg_sigmify(self.h))
+
+    def resizeCNF(self): self._check(self.L.pfg_resize_cnf(self.h))
+    def createOT(self): self._check(self.L.pfg_create_ot(self.h))
+    def reduceOT(self): self._check(self.L.pfg_reduce_ot(self.h))
+    def prop(self): self._check(self.L.pfg_prop(self.h))
+    def sortOT(self): self._check(self.L.pfg_sort_ot(self.h))
+    def SUB(self): self._check(self.L.pfg_sub(self.h))
+    def VE(self): self._check(self.L.pfg_ve(self.h))
+    def ERE(self): self._check(self.L.pfg_ere(self.h))
+    def filterPVs(self): self._check(self.L.pfg_filter_pvs(self.h))
+    def shrinkSimp(self): self._check(self.L.pfg_gc(self.h))
+    def next_phase(self): self._check(self.L.pfg_next_phase(self.h))
+
+    def LCVE(self) -> int:
+        n = ctypes.c_uint32()
+        self._check(self.L.pfg_lcve(self.h, ctypes.byref(n)))
+        return n.value
+
+    def countAll(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.L.pfg_count(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def stop(self, lr: int) -> bool:
+        s = ctypes.c_int()
+        self._check(self.L.pfg_stop(self.h, lr, ctypes.byref(s)))
+        return bool(s.value)
+
+    def hist(self) -> np.ndarray:
+        h = np.empty(2 * (self.nvars + 1), np.uint32)
+        self._check(self.L.pfg_hist(self.h, h.ctypes.data))
+        return h
+
+    # ---- output
+    def cnf_words(self) -> np.ndarray:
+        nw, nc = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.L.pfg_download_cnf(self.h, None, ctypes.byref(nw), ctypes.byref(nc)))
+        w = np.empty(nw.value, np.uint32)
+        self._check(self.L.pfg_download_cnf(self.h, w.ctypes.data, ctypes.byref(nw), ctypes.byref(nc)))
+        return w
+
+    def trail(self) -> np.ndarray:
+        n = ctypes.c_uint32()
+        self._check(self.L.pfg_download_trail(self.h, None, ctypes.byref(n)))
+        a = np.empty(n.value, np.uint32)
+        self._check(self.L.pfg_download_trail(self.h, a.ctypes.data, ctypes.byref(n)))
+        return a
+
+    def resolved(self) -> np.ndarray:
+        n = ctypes.c_uint64()
+        self._check(self.L.pfg_download_resolved(self.h, None, ctypes.byref(n)))
+        a = np.empty(n.value, np.uint32)
+        self._check(self.L.pfg_download_resolved(self.h, a.ctypes.data, ctypes.byref(n)))
+        return a
+
+    def vstate(self) -> np.ndarray:
+        a = np.empty(self.nvars + 1, np.uint8)
+        self._check(self.L.pfg_download_vstate(self.h, a.ctypes.data))
+        return a
+
+    def pvs(self) -> np.ndarray:
+        n = ctypes.c_uint32()
+        self._check(self.L.pfg_download_pvs(self.h, None, ctypes.byref(n)))
+        a = np.empty(n.value, np.uint32)
+        self._check(self.L.pfg_download_pvs(self.h, a.ctypes.data, ctypes.byref(n)))
+        return a
+
+    def ot_list(self, lit: int) -> np.ndarray:
+        n = ctypes.c_uint32()
+        self._check(self.L.pfg_download_ot_list(self.h, lit, None, ctypes.byref(n)))
+        a = np.empty(n.value, np.int64)
+        self._check(self.L.pfg_download_ot_list(self.h, lit, a.ctypes.data, ctypes.byref(n)))
+        return a
+
+    def times(self) -> dict:
+        t = Times()
+        self._check(self.L.pfg_times_get(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def is_unsat(self) -> bool:
+        return bool(self.L.pfg_is_unsat(self.h))

@@ -1,0 +1,740 @@
+// Clause database kernels: upload, histogram, occurrence table (build, reduce,
+// per-list ordering), unit propagation, counting, garbage collection, PV
+// filtering and the PV-list sort by the reference clause key.
+//
+// Order contract: clause references are kept in clause ("scnf") order and the
+// pool is laid out in that order, so a reference's value orders clauses the way
+// ParaFROST's scnf vector does (src/cpu/simplify.cpp:95-112, bve.cpp:214-218).
+// Every occurrence list is left sorted by reference, which is the order in
+// which createOT pushes (src/cpu/simplify.cpp:34-53).
+#include "device.hpp"
+#include "passes.hpp"
+#include "prims.hpp"
+#include <vector>
+#include <algorithm>
+
+namespace pfg {
+
+// ------------------------------------------------------------------ upload
+// one thread per clause: header, sorted literals, signature (prep_cnf_k, kernels.cu:61-70)
+__global__ void k_upload(const uint32_t *__restrict__ lits, const uint64_t *__restrict__ offs, uint64_t ncls,
+                         const uint32_t *__restrict__ hdr0, uint32_t *__restrict__ mem, uint64_t *__restrict__ refs)
+{
+    for (uint64_t i = gtid(); i < ncls; i += gstride()) {
+        const uint64_t b = offs[i], e = offs[i + 1];
+        const int n = (int)(e - b);
+        const uint64_t r = b + 3 * i;
+        uint32_t *c = mem + r;
+        uint32_t *out = c + 3;
+        uint32_t sig = 0;
+        if (n <= 16) {
+            uint32_t t[16];
+            for (int k = 0; k < n; k++) t[k] = lits[b + k];
+            sort_u32_small(t, n);
+            for (int k = 0; k < n; k++) { out[k] = t[k]; sig |= maphash(t[k]); }
+        }
+        else {
+            for (int k = 0; k < n; k++) out[k] = lits[b + k];
+            sort_u32_small(out, n);
+            for (int k = 0; k < n; k++) sig |= maphash(out[k]);
+        }
+        c[0] = hdr0 ? hdr0[i] : ST_ORIGINAL;
+        c[1] = sig;
+        c[2] = (uint32_t)n;
+        refs[i] = r;
+    }
+}
+
+void upload(Ctx &c, const uint32_t *d_lits, const uint64_t *d_offs, uint64_t ncls, uint64_t nlits, const uint32_t *d_hdr0)
+{
+    const uint64_t words = nlits + 3 * ncls;
+    ensure(c, c.mem, words + words / 2 + 4096);
+    ensure(c, c.refs, ncls + ncls / 2 + 1024);
+    k_upload<<<grid_for(ncls, 256, 16384), 256, 0, c.stream>>>(d_lits, d_offs, ncls, d_hdr0, c.mem.p, c.refs.p);
+    c.msz = words;
+    c.nrefs = ncls;
+    c.nclauses = (int64_t)ncls;
+    c.nliterals = (int64_t)nlits;
+    c.ot_valid = false;
+}
+
+// ------------------------------------------------------------------ histogram
+// thread per clause over live (non-deleted) clauses (histSimp, histogram.cu:34-51)
+__global__ void k_hist(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                       uint32_t *__restrict__ hist)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t *c = mem + refs[i];
+        if (c_del(c)) continue;
+        const int sz = c_size(c);
+        for (int k = 0; k < sz; k++) atomicAdd(&hist[c[3 + k]], 1u);
+    }
+}
+
+void hist(Ctx &c)
+{
+    ensure(c, c.hist, c.nlits2);
+    PFG_HIP(hipMemsetAsync(c.hist.p, 0, c.nlits2 * 4, c.stream));
+    if (c.nrefs)
+        k_hist<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.hist.p);
+}
+
+// ------------------------------------------------------------------ occurrence table
+__global__ void k_ot_init(OL *__restrict__ ot, const uint64_t *__restrict__ segs, const uint32_t *__restrict__ hist, uint64_t n)
+{
+    for (uint64_t l = gtid(); l < n; l += gstride()) {
+        OL o;
+        o.off = segs[l];
+        o.sz = 0;
+        o.cap = hist[l];
+        ot[l] = o;
+    }
+}
+
+// createOT: every live clause pushes its reference to the lists of its literals
+__global__ void k_ot_fill(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                          OL *__restrict__ ot, uint64_t *__restrict__ occ)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint64_t r = refs[i];
+        const uint32_t *c = mem + r;
+        if (c_del(c)) continue;
+        const int sz = c_size(c);
+        for (int k = 0; k < sz; k++) {
+            OL *o = ot + c[3 + k];
+            const uint32_t p = atomicAdd(&o->sz, 1u);
+            occ[o->off + p] = r;
+        }
+    }
+}
+
+// classify lists by length for the ordering step
+__global__ void k_ot_classify(const OL *__restrict__ ot, uint64_t n, uint32_t *__restrict__ small_ids,
+                              uint32_t *__restrict__ mid_ids, uint32_t *__restrict__ big_ids, unsigned long long *cnt)
+{
+    for (uint64_t l = gtid(); l < n; l += gstride()) {
+        const uint32_t sz = ot[l].sz;
+        if (sz <= 1) continue;
+        if (sz <= 32) small_ids[atomicAdd(&cnt[CNT_A], 1ull)] = (uint32_t)l;
+        else if (sz <= 2048) mid_ids[atomicAdd(&cnt[CNT_B], 1ull)] = (uint32_t)l;
+        else big_ids[atomicAdd(&cnt[CNT_C], 1ull)] = (uint32_t)l;
+    }
+}
+
+__global__ void k_sort_small(const uint32_t *__restrict__ ids, uint64_t n, const OL *__restrict__ ot, uint64_t *__restrict__ occ)
+{
+    for (uint64_t q = gtid(); q < n; q += gstride()) {
+        const OL o = ot[ids[q]];
+        uint64_t *d = occ + o.off;
+        uint64_t t[32];
+        const int sz = (int)o.sz;
+        for (int k = 0; k < sz; k++) t[k] = d[k];
+        for (int i = 1; i < sz; i++) {
+            const uint64_t v = t[i];
+            int j = i;
+            while (j > 0 && t[j - 1] > v) { t[j] = t[j - 1]; j--; }
+            t[j] = v;
+        }
+        for (int k = 0; k < sz; k++) d[k] = t[k];
+    }
+}
+
+// block per list, bitonic sort of up to 2048 references in LDS
+__global__ void __launch_bounds__(256) k_sort_mid(const uint32_t *__restrict__ ids, const OL *__restrict__ ot, uint64_t *__restrict__ occ)
+{
+    __shared__ uint64_t s[2048];
+    const OL o = ot[ids[blockIdx.x]];
+    uint64_t *d = occ + o.off;
+    const int sz = (int)o.sz;
+    int N = 64;
+    while (N < sz) N <<= 1;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s[i] = i < sz ? d[i] : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= N; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < N; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint64_t a = s[i], b = s[l];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) { s[i] = b; s[l] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < sz; i += blockDim.x) d[i] = s[i];
+}
+
+__global__ void k_big_gather(const uint32_t *__restrict__ ids, const uint64_t *__restrict__ base, uint32_t nbig,
+                             const OL *__restrict__ ot, const uint64_t *__restrict__ occ, uint64_t *__restrict__ keys)
+{
+    for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const OL o = ot[ids[b]];
+        for (uint32_t k = threadIdx.x; k < o.sz; k += blockDim.x)
+            keys[base[b] + k] = ((uint64_t)b << 32) | occ[o.off + k];
+    }
+}
+
+__global__ void k_big_scatter(const uint32_t *__restrict__ ids, const uint64_t *__restrict__ base, uint32_t nbig,
+                              const OL *__restrict__ ot, uint64_t *__restrict__ occ, const uint64_t *__restrict__ keys)
+{
+    for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const OL o = ot[ids[b]];
+        for (uint32_t k = threadIdx.x; k < o.sz; k += blockDim.x)
+            occ[o.off + k] = keys[base[b] + k] & 0xFFFFFFFFull;
+    }
+}
+
+static int bits_for(uint64_t v)
+{
+    int b = 0;
+    while (b < 64 && (v >> b)) b++;
+    return b;
+}
+
+// order every list by reference (= clause order), the createOT push order
+static void order_lists_by_ref(Ctx &c)
+{
+    const uint64_t n = c.nlits2;
+    ensure(c, c.tmpu32a, n);
+    ensure(c, c.tmpu32b, n);
+    ensure(c, c.eligible, n);
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 3 * 8, c.stream));
+    k_ot_classify<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.ot.p, n, c.tmpu32a.p, c.tmpu32b.p, c.eligible.p,
+                                                                 (unsigned long long *)c.cnt.p);
+    read_counters(c, 4);
+    const uint64_t ns = c.hcnt[CNT_A], nm = c.hcnt[CNT_B], nb = c.hcnt[CNT_C];
+    if (ns) k_sort_small<<<grid_for(ns, 128, 16384), 128, 0, c.stream>>>(c.tmpu32a.p, ns, c.ot.p, c.occ.p);
+    if (nm) k_sort_mid<<<(unsigned)nm, 256, 0, c.stream>>>(c.tmpu32b.p, c.ot.p, c.occ.p);
+    if (nb) {
+        if (c.msz >= (1ull << 32)) throw Error(PFG_EINTERNAL, "clause pool beyond 2^32 words");
+        std::vector<uint32_t> ids(nb);
+        PFG_HIP(hipMemcpyAsync(ids.data(), c.eligible.p, nb * 4, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+        std::vector<OL> heads(nb);
+        for (uint64_t b = 0; b < nb; b++)
+            PFG_HIP(hipMemcpyAsync(&heads[b], c.ot.p + ids[b], sizeof(OL), hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+        std::vector<uint64_t> base(nb);
+        uint64_t tot = 0;
+        for (uint64_t b = 0; b < nb; b++) { base[b] = tot; tot += heads[b].sz; }
+        ensure(c, c.tmpu64b, tot);
+        ensure(c, c.tmpu64c, nb + 1);
+        PFG_HIP(hipMemcpyAsync(c.tmpu64c.p, base.data(), nb * 8, hipMemcpyHostToDevice, c.stream));
+        k_big_gather<<<(unsigned)std::min<uint64_t>(nb, 4096), 256, 0, c.stream>>>(c.eligible.p, c.tmpu64c.p, (uint32_t)nb,
+                                                                                 c.ot.p, c.occ.p, c.tmpu64b.p);
+        radix_sort_u64_keys(c, c.tmpu64b.p, tot, 32 + bits_for(nb));
+        k_big_scatter<<<(unsigned)std::min<uint64_t>(nb, 4096), 256, 0, c.stream>>>(c.eligible.p, c.tmpu64c.p, (uint32_t)nb,
+                                                                                  c.ot.p, c.occ.p, c.tmpu64b.p);
+        PFG_HIP(hipStreamSynchronize(c.stream));  // host vectors go out of scope
+    }
+}
+
+void create_ot(Ctx &c)
+{
+    hist(c);
+    ensure(c, c.ot, c.nlits2);
+    ensure(c, c.tmpu64a, c.nlits2);
+    scan_u32_to_u64(c, c.hist.p, c.tmpu64a.p, c.nlits2, c.cnt.p + CNT_TOTAL);
+    read_counters(c, 1);
+    const uint64_t total = c.hcnt[CNT_TOTAL];
+    ensure(c, c.occ, total + 1024);
+    k_ot_init<<<grid_for(c.nlits2, 256, 8192), 256, 0, c.stream>>>(c.ot.p, c.tmpu64a.p, c.hist.p, c.nlits2);
+    if (c.nrefs) k_ot_fill<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.ot.p, c.occ.p);
+    order_lists_by_ref(c);
+    c.ot_valid = true;
+}
+
+// reduceOL (src/cpu/simplify.cpp:55-65): in-order filter of deleted references
+__global__ void k_reduce_ot(const uint32_t *__restrict__ mem, OL *__restrict__ ot, uint64_t *__restrict__ occ, uint64_t n)
+{
+    for (uint64_t l = gtid(); l < n; l += gstride()) {
+        OL o = ot[l];
+        uint64_t *d = occ + o.off;
+        uint32_t j = 0;
+        for (uint32_t i = 0; i < o.sz; i++) {
+            const uint64_t r = d[i];
+            if (!c_del(mem + r)) d[j++] = r;
+        }
+        ot[l].sz = j;
+    }
+}
+
+void reduce_ot(Ctx &c)
+{
+    k_reduce_ot<<<grid_for(c.nlits2, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.ot.p, c.occ.p, c.nlits2);
+}
+
+// ------------------------------------------------------------------ unit propagation
+// prop (src/cpu/elimination.cpp:64-97) in batches: all clauses on the lists of the
+// batch's units (and of their negations) are claimed once and brought to the state
+// the sequential loop leaves them in: satisfied -> deleted, false literals removed
+// (order kept, signature recomputed), new units assigned and appended to the trail.
+__global__ void k_prop_collect(const uint32_t *__restrict__ trail, uint32_t b0, uint32_t b1, const OL *__restrict__ ot,
+                               const uint64_t *__restrict__ occ, uint32_t *__restrict__ claim,
+                               uint64_t *__restrict__ cand, unsigned long long *cnt)
+{
+    const uint32_t nu = b1 - b0;
+    for (uint64_t q = blockIdx.x; q < 2ull * nu; q += gridDim.x) {
+        const uint32_t u = trail[b0 + (uint32_t)(q >> 1)];
+        const uint32_t l = (q & 1) ? u : (u ^ 1u);
+        const OL o = ot[l];
+        for (uint32_t k = threadIdx.x; k < o.sz; k += blockDim.x) {
+            const uint64_t r = occ[o.off + k];
+            const uint64_t key = r >> 1;
+            const uint32_t bit = 1u << (key & 31u);
+            const uint32_t old = atomicOr(&claim[key >> 5], bit);
+            if (!(old & bit)) cand[atomicAdd(&cnt[CNT_A], 1ull)] = r;
+        }
+    }
+}
+
+__global__ void k_prop_apply(uint32_t *__restrict__ mem, const uint64_t *__restrict__ cand, uint64_t n, int8_t *value,
+                             uint8_t *vstate, uint32_t *trail, unsigned long long *cnt, uint32_t *__restrict__ claim)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint64_t r = cand[i];
+        const uint64_t key = r >> 1;
+        atomicAnd(&claim[key >> 5], ~(1u << (key & 31u)));
+        uint32_t *c = mem + r;
+        if (c_del(c)) continue;
+        const int sz = c_size(c);
+        uint32_t *lits = c + 3;
+        bool sat = false;
+        for (int k = 0; k < sz; k++)
+            if (__atomic_load_n(&value[lits[k]], __ATOMIC_RELAXED) > 0) { sat = true; break; }
+        if (sat) { c_markdel(c); continue; }
+        int j = 0;
+        uint32_t sig = 0;
+        for (int k = 0; k < sz; k++) {
+            const uint32_t l = lits[k];
+            if (__atomic_load_n(&value[l], __ATOMIC_RELAXED) == 0) continue;
+            lits[j++] = l;
+            sig |= maphash(l);
+        }
+        c[1] = sig;
+        c[2] = (uint32_t)j;
+        if (j == 0) atomicOr(&cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT);
+        else if (j == 1) enqueue_dev(value, vstate, trail, cnt, lits[0]);
+    }
+}
+
+__global__ void k_prop_clear(const uint32_t *__restrict__ trail, uint32_t b0, uint32_t b1, OL *__restrict__ ot)
+{
+    for (uint64_t q = gtid() + b0; q < b1; q += gstride()) {
+        const uint32_t u = trail[q];
+        ot[u].sz = 0;
+        ot[u ^ 1u].sz = 0;
+    }
+}
+
+bool prop(Ctx &c)
+{
+    if (c.propagated >= c.ntrail) return !c.unsat;
+    const uint32_t before = c.propagated;
+    ensure(c, c.claim, (c.mem.cap >> 6) + 2);
+    PFG_HIP(hipMemsetAsync(c.claim.p, 0, c.claim.cap * 4, c.stream));
+    ensure(c, c.tmpu64b, c.occ.cap);
+    while (c.propagated < c.ntrail) {
+        const uint32_t b0 = c.propagated, b1 = c.ntrail;
+        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 8, c.stream));
+        k_prop_collect<<<grid_for(2ull * (b1 - b0) * 64, 64, 8192), 64, 0, c.stream>>>(
+            c.trail.p, b0, b1, c.ot.p, c.occ.p, c.claim.p, c.tmpu64b.p, (unsigned long long *)c.cnt.p);
+        read_counters(c, 2);
+        const uint64_t nc = c.hcnt[CNT_A];
+        if (nc) k_prop_apply<<<grid_for(nc, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.tmpu64b.p, nc, c.value.p, c.vstate.p,
+                                                                             c.trail.p, (unsigned long long *)c.cnt.p, c.claim.p);
+        k_prop_clear<<<grid_for(b1 - b0, 256, 1024), 256, 0, c.stream>>>(c.trail.p, b0, b1, c.ot.p);
+        read_counters(c, 10);
+        c.propagated = b1;
+        c.ntrail = (uint32_t)c.hcnt[CNT_NTRAIL];
+        if (c.hcnt[CNT_FLAGS] & FLAG_UNSAT) { c.unsat = true; return false; }
+    }
+    if (c.propagated != before && !c.opts.sub_en) reduce_ot(c);
+    return true;
+}
+
+// ------------------------------------------------------------------ counting
+__global__ void k_count(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n, unsigned long long *cnt)
+{
+    unsigned long long nc = 0, nl = 0;
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t *c = mem + refs[i];
+        if (!c_del(c)) { nc++; nl += (unsigned)c_size(c); }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { nc += __shfl_down(nc, o, 64); nl += __shfl_down(nl, o, 64); }
+    if (__lane_id() == 0) { atomicAdd(&cnt[CNT_A], nc); atomicAdd(&cnt[CNT_B], nl); }
+}
+
+void count_all(Ctx &c, int64_t &cls, int64_t &lits)
+{
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));
+    if (c.nrefs) k_count<<<grid_for(c.nrefs, 256, 4096), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, (unsigned long long *)c.cnt.p);
+    read_counters(c, 3);
+    cls = (int64_t)c.hcnt[CNT_A];
+    lits = (int64_t)c.hcnt[CNT_B];
+}
+
+// ------------------------------------------------------------------ garbage collection
+// stable compaction of live clauses into the second pool (compactCNF, recycle.cu:71-97)
+__global__ void k_gc_sizes(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                           uint64_t *__restrict__ words, uint32_t *__restrict__ live)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t *c = mem + refs[i];
+        const bool l = !c_del(c);
+        words[i] = l ? 3ull + (uint64_t)c_size(c) : 0ull;
+        live[i] = l ? 1u : 0u;
+    }
+}
+
+__global__ void k_gc_move(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                          const uint64_t *__restrict__ woff, const uint64_t *__restrict__ idx,
+                          const uint32_t *__restrict__ live, uint32_t *__restrict__ mem2, uint64_t *__restrict__ refs2)
+{
+    // one wave per clause slot group: lanes copy words
+    const int lane = __lane_id();
+    const uint64_t wid = gtid() >> 6, nw = gstride() >> 6;
+    for (uint64_t i = wid; i < n; i += nw) {
+        if (!live[i]) continue;
+        const uint32_t *c = mem + refs[i];
+        const uint64_t words = 3ull + (uint64_t)c_size(c);
+        uint32_t *d = mem2 + woff[i];
+        for (uint64_t k = lane; k < words; k += 64) d[k] = c[k];
+        if (lane == 0) refs2[idx[i]] = woff[i];
+    }
+}
+
+void shrink(Ctx &c)
+{
+    const uint64_t n = c.nrefs;
+    if (!n) return;
+    ensure(c, c.tmpu64a, n);
+    ensure(c, c.tmpu64b, n);
+    ensure(c, c.tmpu64c, n);
+    ensure(c, c.tmpu32a, n);
+    k_gc_sizes<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu64a.p, c.tmpu32a.p);
+    scan_u64(c, c.tmpu64a.p, c.tmpu64b.p, n, c.cnt.p + CNT_A);
+    scan_u32_to_u64(c, c.tmpu32a.p, c.tmpu64c.p, n, c.cnt.p + CNT_B);
+    read_counters(c, 3);
+    const uint64_t words = c.hcnt[CNT_A], live = c.hcnt[CNT_B];
+    ensure(c, c.mem2, std::max<uint64_t>(c.mem.cap, words + 1));
+    ensure(c, c.refs2, std::max<uint64_t>(c.refs.cap, live + 1));
+    k_gc_move<<<grid_for(n * 64, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu64b.p, c.tmpu64c.p,
+                                                                   c.tmpu32a.p, c.mem2.p, c.refs2.p);
+    std::swap(c.mem, c.mem2);
+    std::swap(c.refs, c.refs2);
+    c.msz = words;
+    c.nrefs = live;
+    c.ot_valid = false;
+}
+
+// ------------------------------------------------------------------ ordered compaction
+__global__ void k_compact_scatter(const uint32_t *__restrict__ in, const uint32_t *__restrict__ flags,
+                                  const uint64_t *__restrict__ pos, uint64_t n, uint32_t *__restrict__ out)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride())
+        if (flags[i]) out[pos[i]] = in[i];
+}
+
+uint64_t compact_u32(Ctx &c, const uint32_t *in, const uint32_t *flags, uint64_t n, uint32_t *out)
+{
+    if (!n) return 0;
+    ensure(c, c.tmpu64c, n);
+    scan_u32_to_u64(c, flags, c.tmpu64c.p, n, c.cnt.p + CNT_A);
+    k_compact_scatter<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(in, flags, c.tmpu64c.p, n, out);
+    read_counters(c, 2);
+    return c.hcnt[CNT_A];
+}
+
+// filterPVs (src/cpu/solve.h:697-707)
+__global__ void k_pv_flags(const uint32_t *__restrict__ pvs, uint64_t n, const uint8_t *__restrict__ vstate, uint32_t *__restrict__ f)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t x = pvs[i];
+        f[i] = (x && !vstate[x]) ? 1u : 0u;
+    }
+}
+
+void filter_pvs(Ctx &c)
+{
+    if (!c.npvs) return;
+    ensure(c, c.tmpu32a, c.npvs);
+    ensure(c, c.tmpu32b, c.npvs);
+    k_pv_flags<<<grid_for(c.npvs, 256, 4096), 256, 0, c.stream>>>(c.pvs.p, c.npvs, c.vstate.p, c.tmpu32a.p);
+    const uint64_t n = compact_u32(c, c.pvs.p, c.tmpu32a.p, c.npvs, c.tmpu32b.p);
+    if (n) PFG_HIP(hipMemcpyAsync(c.pvs.p, c.tmpu32b.p, n * 4, hipMemcpyDeviceToDevice, c.stream));
+    c.npvs = (uint32_t)n;
+}
+
+// ------------------------------------------------------------------ units from passes
+__global__ void k_assign_host_units(const uint32_t *__restrict__ lits, uint32_t n, int8_t *value, uint8_t *vstate,
+                                    uint32_t *trail, unsigned long long *cnt)
+{
+    if (gtid() != 0) return;
+    for (uint32_t i = 0; i < n; i++) enqueue_dev(value, vstate, trail, cnt, lits[i]);
+}
+
+void enqueue_units_host(Ctx &c, const uint32_t *lits, uint32_t n)
+{
+    if (!n) return;
+    ensure(c, c.tmpu32a, n);
+    PFG_HIP(hipMemcpyAsync(c.tmpu32a.p, lits, n * 4, hipMemcpyHostToDevice, c.stream));
+    k_assign_host_units<<<1, 64, 0, c.stream>>>(c.tmpu32a.p, n, c.value.p, c.vstate.p, c.trail.p, (unsigned long long *)c.cnt.p);
+    read_counters(c, 10);
+    c.ntrail = (uint32_t)c.hcnt[CNT_NTRAIL];
+    if (c.hcnt[CNT_FLAGS] & FLAG_UNSAT) c.unsat = true;
+}
+
+// Units found by BVE (newResolvent / substitute_single, bve.cpp:199-211, equivalence.h:55-84)
+// in PV order: the first occurrence of a variable is enqueued if unassigned; a later
+// occurrence of its negation, or a unit that is already false, proves a contradiction.
+__global__ void k_units_first(const uint32_t *__restrict__ u, uint64_t n, uint32_t *__restrict__ first)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) atomicMin(&first[u[i] >> 1], (uint32_t)i);
+}
+
+__global__ void k_units_keep(const uint32_t *__restrict__ u, uint64_t n, const uint32_t *__restrict__ first,
+                             const int8_t *__restrict__ value, uint32_t *__restrict__ keep, unsigned long long *cnt)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t l = u[i];
+        const int8_t v = value[l];
+        uint32_t k = 0;
+        if (v >= 0) { if (v == 0) atomicOr(&cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT); }
+        else {
+            const uint32_t f = first[l >> 1];
+            if (f == (uint32_t)i) k = 1;
+            else if (u[f] != l) atomicOr(&cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT);
+        }
+        keep[i] = k;
+    }
+}
+
+__global__ void k_units_append(const uint32_t *__restrict__ kept, uint64_t n, uint32_t base, int8_t *value, uint8_t *vstate,
+                               uint32_t *trail, uint32_t *__restrict__ first)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t l = kept[i];
+        value[l] = 1;
+        value[l ^ 1u] = 0;
+        vstate[l >> 1] = FROZEN_M;
+        trail[base + i] = l;
+    }
+}
+
+__global__ void k_units_reset(const uint32_t *__restrict__ u, uint64_t n, uint32_t *__restrict__ first)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) first[u[i] >> 1] = 0xFFFFFFFFu;
+}
+
+void merge_units(Ctx &c, const uint32_t *d_units, uint64_t n)
+{
+    if (!n) return;
+    // ufirst: per-variable first occurrence, all-ones outside this function
+    ensure(c, c.tmpu32a, n);
+    ensure(c, c.tmpu32b, n);
+    k_units_first<<<grid_for(n, 256, 4096), 256, 0, c.stream>>>(d_units, n, c.ufirst.p);
+    k_units_keep<<<grid_for(n, 256, 4096), 256, 0, c.stream>>>(d_units, n, c.ufirst.p, c.value.p, c.tmpu32a.p,
+                                                                (unsigned long long *)c.cnt.p);
+    k_units_reset<<<grid_for(n, 256, 4096), 256, 0, c.stream>>>(d_units, n, c.ufirst.p);
+    const uint64_t nk = compact_u32(c, d_units, c.tmpu32a.p, n, c.tmpu32b.p);
+    if (nk) k_units_append<<<grid_for(nk, 256, 4096), 256, 0, c.stream>>>(c.tmpu32b.p, nk, c.ntrail, c.value.p, c.vstate.p,
+                                                                          c.trail.p, c.ufirst.p);
+    c.ntrail += (uint32_t)nk;
+    {
+        const unsigned long long nt = c.ntrail;
+        // keep the device trail counter in step with the host
+        PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_NTRAIL, &nt, 8, hipMemcpyHostToDevice, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+    }
+    read_counters(c, 10);
+    if (c.hcnt[CNT_FLAGS] & FLAG_UNSAT) c.unsat = true;
+}
+
+// ------------------------------------------------------------------ PV list sort (sortOT)
+// CNF_CMP_KEY (src/cpu/simplify.h:29-43): size, first literal, last literal, signature.
+struct KeyLess {
+    const uint32_t *mem;
+    __device__ bool operator()(uint64_t x, uint64_t y) const
+    {
+        const uint32_t *a = mem + x, *b = mem + y;
+        const int xs = (int)a[2], ys = (int)b[2];
+        if (xs != ys) return xs < ys;
+        if (a[3] != b[3]) return a[3] < b[3];
+        const uint32_t xb = a[3 + xs - 1], yb = b[3 + ys - 1];
+        if (xb != yb) return xb < yb;
+        return a[1] < b[1];
+    }
+};
+
+template <typename LT>
+__device__ void ins_sort_sorth(uint64_t *d, int n, LT lt)
+{   // insertionSort (src/cpu/sort.h:41-54)
+    if (n == 2 && lt(d[1], d[0])) { uint64_t t = d[1]; d[1] = d[0]; d[0] = t; }
+    else if (n > 2) {
+        for (int i = 1; i < n; i++) {
+            const uint64_t t = d[i];
+            int j;
+            for (j = i; j > 0 && lt(t, d[j - 1]); j--) d[j] = d[j - 1];
+            d[j] = t;
+        }
+    }
+}
+
+// libstdc++ std::sort (introsort) order, restated for the device
+template <typename LT>
+__device__ void is_adjust_heap(uint64_t *f, int64_t hole, int64_t len, uint64_t value, LT lt)
+{
+    const int64_t top = hole;
+    int64_t child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (lt(f[child], f[child - 1])) child--;
+        f[hole] = f[child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        f[hole] = f[child - 1];
+        hole = child - 1;
+    }
+    int64_t parent = (hole - 1) / 2;
+    while (hole > top && lt(f[parent], value)) {
+        f[hole] = f[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    f[hole] = value;
+}
+
+template <typename LT>
+__device__ void is_heapsort(uint64_t *f, int64_t len, LT lt)
+{
+    if (len >= 2) {
+        for (int64_t parent = (len - 2) / 2;; parent--) {
+            is_adjust_heap(f, parent, len, f[parent], lt);
+            if (parent == 0) break;
+        }
+    }
+    while (len > 1) {
+        len--;
+        const uint64_t v = f[len];
+        f[len] = f[0];
+        is_adjust_heap(f, 0, len, v, lt);
+    }
+}
+
+template <typename LT>
+__device__ void is_linear_insert(uint64_t *last, LT lt)
+{
+    const uint64_t v = *last;
+    uint64_t *next = last - 1;
+    while (lt(v, *next)) { *last = *next; last = next; --next; }
+    *last = v;
+}
+
+template <typename LT>
+__device__ void is_insertion(uint64_t *first, uint64_t *last, LT lt)
+{
+    if (first == last) return;
+    for (uint64_t *i = first + 1; i != last; ++i) {
+        if (lt(*i, *first)) {
+            const uint64_t v = *i;
+            for (uint64_t *p = i; p != first; --p) *p = *(p - 1);
+            *first = v;
+        }
+        else is_linear_insert(i, lt);
+    }
+}
+
+template <typename LT>
+__device__ void introsort_dev(uint64_t *d, int64_t n, LT lt)
+{
+    if (n < 2) return;
+    struct Frame { int64_t f, l, depth; };
+    Frame st[72];
+    int sp = 0;
+    int64_t lg = 63 - __clzll((long long)n);
+    st[sp++] = {0, n, 2 * lg};
+    while (sp) {
+        Frame fr = st[--sp];
+        int64_t f = fr.f, l = fr.l, depth = fr.depth;
+        while (l - f > 16) {
+            if (depth == 0) { is_heapsort(d + f, l - f, lt); break; }
+            --depth;
+            uint64_t *first = d + f, *last = d + l;
+            uint64_t *a = first + 1, *b = first + (l - f) / 2, *cc = last - 1, *pick;
+            if (lt(*a, *b)) {
+                if (lt(*b, *cc)) pick = b;
+                else if (lt(*a, *cc)) pick = cc;
+                else pick = a;
+            }
+            else if (lt(*a, *cc)) pick = a;
+            else if (lt(*b, *cc)) pick = cc;
+            else pick = b;
+            { const uint64_t t = *first; *first = *pick; *pick = t; }
+            uint64_t *lo = first + 1, *hi = last;
+            for (;;) {
+                while (lt(*lo, *first)) ++lo;
+                --hi;
+                while (lt(*first, *hi)) --hi;
+                if (!(lo < hi)) break;
+                const uint64_t t = *lo; *lo = *hi; *hi = t;
+                ++lo;
+            }
+            const int64_t cut = lo - d;
+            st[sp++] = {cut, l, depth};   // the right part is sorted independently
+            l = cut;
+        }
+    }
+    if (n > 16) {
+        is_insertion(d, d + 16, lt);
+        for (uint64_t *i = d + 16; i != d + n; ++i) is_linear_insert(i, lt);
+    }
+    else is_insertion(d, d + n, lt);
+}
+
+template <typename LT>
+__device__ void stable_sort_dev(uint64_t *d, uint64_t *tmp, int64_t n, LT lt)
+{
+    for (int64_t w = 1; w < n; w *= 2) {
+        for (int64_t lo = 0; lo < n; lo += 2 * w) {
+            const int64_t mid = min(lo + w, n), hi = min(lo + 2 * w, n);
+            int64_t i = lo, j = mid, k = lo;
+            while (i < mid && j < hi) tmp[k++] = lt(d[j], d[i]) ? d[j++] : d[i++];
+            while (i < mid) tmp[k++] = d[i++];
+            while (j < hi) tmp[k++] = d[j++];
+        }
+        for (int64_t k = 0; k < n; k++) d[k] = tmp[k];
+    }
+}
+
+__global__ void k_sort_pv_lists(const uint32_t *__restrict__ pvs, uint32_t npvs, const OL *__restrict__ ot, uint64_t *occ,
+                                const uint32_t *__restrict__ mem, int stable, uint64_t *tmp)
+{
+    const KeyLess lt{mem};
+    for (uint64_t q = gtid(); q < 2ull * npvs; q += gstride()) {
+        const uint32_t lit = 2 * pvs[q >> 1] + (uint32_t)(q & 1);
+        const OL o = ot[lit];
+        uint64_t *d = occ + o.off;
+        const int64_t n = o.sz;
+        if (n <= 1) continue;
+        if (n <= 20) ins_sort_sorth(d, (int)n, lt);
+        else if (stable) stable_sort_dev(d, tmp + o.off, n, lt);
+        else introsort_dev(d, n, lt);
+    }
+}
+
+void sort_ot(Ctx &c)
+{
+    if (!c.npvs) return;
+    uint64_t *tmp = nullptr;
+    if (c.opts.stable_sort) { ensure(c, c.tmpu64b, c.occ.cap); tmp = c.tmpu64b.p; }
+    k_sort_pv_lists<<<grid_for(2ull * c.npvs, 64, 16384), 64, 0, c.stream>>>(c.pvs.p, c.npvs, c.ot.p, c.occ.p, c.mem.p,
+                                                                              c.opts.stable_sort, tmp);
+}
+
+}  // namespace pfg

@@ -1,0 +1,134 @@
+// Internal context of one device-resident simplifier instance.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <stdexcept>
+#include "../../include/pfgpu.h"
+
+namespace pfg {
+
+// occurrence list header (OL, src/gpu/simptypes.cuh:66-67): offset into the
+// occurrence array, current size, capacity (= histogram count at build time)
+struct OL {
+    uint64_t off;
+    uint32_t sz, cap;
+};
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define PFG_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::pfg::Error(PFG_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Growable device buffer of T.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    uint64_t cap = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0, allocated = 0;
+    pfg_opts opts{};
+    uint32_t nvars = 0;
+    uint64_t nlits2 = 0;  // 2 * (nvars + 1)
+
+    // clause pool (SCLAUSE images) and references in clause order
+    DBuf<uint32_t> mem, mem2;   // mem2: destination of compaction
+    uint64_t msz = 0;           // words in use
+    DBuf<uint64_t> refs, refs2;
+    uint64_t nrefs = 0;
+
+    // occurrence table
+    DBuf<OL> ot;
+    DBuf<uint64_t> occ;
+    DBuf<uint32_t> hist;        // per literal
+    bool ot_valid = false;
+
+    // variables
+    DBuf<int8_t> value;         // per literal: -1 unassigned, 0 false, 1 true
+    DBuf<uint8_t> vstate;       // per variable: MELTED 1, FROZEN 2
+    DBuf<uint32_t> trail;       // units (nvars + 1)
+    uint32_t ntrail = 0, propagated = 0;
+    DBuf<uint32_t> resolved;    // model stack
+    uint64_t nres = 0;
+    DBuf<uint32_t> pvs;         // elected variables
+    DBuf<uint32_t> units;       // units produced by one BVE pass, in elected order
+    uint32_t npvs = 0;
+
+    // scratch
+    DBuf<uint32_t> scores, eligible, rank, tmpu32a, tmpu32b;
+    DBuf<uint8_t> tmpu8;
+    DBuf<uint64_t> tmpu64a, tmpu64b, tmpu64c;
+    DBuf<uint32_t> owner;       // per (ref >> 1): ERE target claims (all-ones when idle)
+    DBuf<uint32_t> claim;       // bitmap over (ref >> 1): prop claims
+    DBuf<uint32_t> ufirst;      // per variable: first unit occurrence (all-ones when idle)
+    DBuf<uint64_t> cnt;         // small device counters
+    uint64_t *hcnt = nullptr;   // pinned mirror
+    DBuf<uint8_t> scan_tmp;     // scan partial sums
+    DBuf<uint8_t> radix_tmp;    // radix sort temporaries
+
+    // round state
+    int32_t phase = 0;
+    uint32_t mu_inc = 0;
+    int64_t nclauses = 0, nliterals = 0;
+    bool unsat = false;
+    bool sorted_lists = false;
+
+    pfg_times times{};
+    hipEvent_t ev[8][2] = {};   // per nesting depth of pass timers
+    int ev_depth = 0;
+    std::string last_error;
+};
+
+// allocation helpers (budget-checked)
+void dalloc_raw(Ctx &c, void **p, uint64_t bytes);
+void dfree_raw(Ctx &c, void *p, uint64_t bytes);
+
+template <typename T>
+void ensure(Ctx &c, DBuf<T> &b, uint64_t n, bool keep = false)
+{
+    if (n <= b.cap) return;
+    uint64_t ncap = b.cap ? b.cap : 1024;
+    while (ncap < n) ncap += ncap / 2 + 1024;
+    void *np = nullptr;
+    dalloc_raw(c, &np, ncap * sizeof(T));
+    if (keep && b.p && b.cap)
+        PFG_HIP(hipMemcpyAsync(np, b.p, b.cap * sizeof(T), hipMemcpyDeviceToDevice, c.stream));
+    if (b.p) {
+        if (keep) PFG_HIP(hipStreamSynchronize(c.stream));
+        dfree_raw(c, b.p, b.cap * sizeof(T));
+    }
+    b.p = (T *)np;
+    b.cap = ncap;
+}
+
+template <typename T>
+void release(Ctx &c, DBuf<T> &b)
+{
+    if (b.p) dfree_raw(c, b.p, b.cap * sizeof(T));
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// read a few device counters into the pinned mirror (synchronizes the stream)
+void read_counters(Ctx &c, int n);
+
+struct PassTimer {
+    Ctx &c;
+    double *acc;
+    int d;
+    PassTimer(Ctx &cc, double *a);
+    ~PassTimer();
+};
+
+}  // namespace pfg

@@ -1,0 +1,132 @@
+// Device-side clause model and small helpers shared by the simplifier kernels.
+// Clause image (SCLAUSE, ParaFROST src/gpu/sclause.h:37-186):
+//   word0 = status:2 | molten:1 | added:1 | usage:2 | lbd:26, word1 = signature,
+//   word2 = size, words 3.. = literals (ascending).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "ctx.hpp"
+
+namespace pfg {
+
+constexpr uint32_t ST_ORIGINAL = 0u, ST_LEARNT = 1u, ST_DELETED = 2u;
+constexpr uint8_t MELTED_M = 1u, FROZEN_M = 2u;
+constexpr int HSE_MAX_CL_SIZE = 1000;  // subsume.h:25
+
+__device__ __forceinline__ uint32_t absv(uint32_t l) { return l >> 1; }
+__device__ __forceinline__ uint32_t flip(uint32_t l) { return l ^ 1u; }
+__device__ __forceinline__ uint32_t maphash(uint32_t l) { return 1u << (l & 31u); }
+
+__device__ __forceinline__ uint32_t c_st(const uint32_t *c) { return c[0] & 3u; }
+__device__ __forceinline__ bool c_orig(const uint32_t *c) { return (c[0] & 3u) == 0; }
+__device__ __forceinline__ bool c_del(const uint32_t *c) { return (c[0] & ST_DELETED) != 0; }
+__device__ __forceinline__ bool c_learnt(const uint32_t *c) { return (c[0] & ST_LEARNT) != 0; }
+__device__ __forceinline__ bool c_molten(const uint32_t *c) { return (c[0] >> 2) & 1u; }
+__device__ __forceinline__ void c_melt(uint32_t *c) { c[0] |= 4u; }
+__device__ __forceinline__ void c_freeze(uint32_t *c) { c[0] &= ~4u; }
+__device__ __forceinline__ void c_markdel(uint32_t *c) { c[0] = (c[0] & ~3u) | ST_DELETED; }
+__device__ __forceinline__ void c_setst(uint32_t *c, uint32_t st) { c[0] = (c[0] & ~3u) | st; }
+__device__ __forceinline__ int c_size(const uint32_t *c) { return (int)c[2]; }
+
+// signature filters (src/cpu/simplify.h:81-87)
+__device__ __forceinline__ bool sig_sub(uint32_t A, uint32_t B) { return !(A & ~B); }
+__device__ __forceinline__ bool sig_selfsub(uint32_t A, uint32_t B)
+{
+    const uint32_t t = B | ((B & 0xAAAAAAAAu) >> 1) | ((B & 0x55555555u) << 1);
+    return !(A & ~t);
+}
+
+// grid-stride helpers
+__device__ __forceinline__ uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
+
+inline unsigned grid_for(uint64_t n, unsigned block, unsigned maxblocks = 8192)
+{
+    uint64_t g = (n + block - 1) / block;
+    if (g == 0) g = 1;
+    if (g > maxblocks) g = maxblocks;
+    return (unsigned)g;
+}
+
+// resolvent length without writing it; 0 on tautology (src/cpu/simplify.h:168-193)
+__device__ __forceinline__ int merge_len(uint32_t x, const uint32_t *c1, const uint32_t *c2)
+{
+    const int n1 = c_size(c1), n2 = c_size(c2);
+    const uint32_t *a = c1 + 3, *b = c2 + 3;
+    int it1 = 0, it2 = 0, len = n1 + n2 - 2;
+    while (it1 < n1 && it2 < n2) {
+        const uint32_t l1 = a[it1], l2 = b[it2], v1 = l1 >> 1, v2 = l2 >> 1;
+        if (v1 == x) it1++;
+        else if (v2 == x) it2++;
+        else if ((l1 ^ l2) == 1u) return 0;
+        else if (v1 < v2) it1++;
+        else if (v2 < v1) it2++;
+        else { it1++; it2++; len--; }
+    }
+    return len;
+}
+
+// resolvent written to out; false on tautology (src/cpu/simplify.h:131-166)
+__device__ __forceinline__ bool merge_out(uint32_t x, const uint32_t *c1, const uint32_t *c2, uint32_t *out, int &n)
+{
+    n = 0;
+    const int n1 = c_size(c1), n2 = c_size(c2);
+    const uint32_t *a = c1 + 3, *b = c2 + 3;
+    int it1 = 0, it2 = 0;
+    while (it1 < n1 && it2 < n2) {
+        const uint32_t l1 = a[it1], l2 = b[it2], v1 = l1 >> 1, v2 = l2 >> 1;
+        if (v1 == x) it1++;
+        else if (v2 == x) it2++;
+        else if ((l1 ^ l2) == 1u) return false;
+        else if (v1 < v2) { it1++; out[n++] = l1; }
+        else if (v2 < v1) { it2++; out[n++] = l2; }
+        else { it1++; it2++; out[n++] = l1; }
+    }
+    while (it1 < n1) { const uint32_t l1 = a[it1++]; if ((l1 >> 1) != x) out[n++] = l1; }
+    while (it2 < n2) { const uint32_t l2 = b[it2++]; if ((l2 >> 1) != x) out[n++] = l2; }
+    return true;
+}
+
+__device__ __forceinline__ void sort_u32_small(uint32_t *d, int n)
+{
+    for (int i = 1; i < n; i++) {
+        uint32_t t = d[i];
+        int j = i;
+        while (j > 0 && d[j - 1] > t) { d[j] = d[j - 1]; j--; }
+        d[j] = t;
+    }
+}
+
+// device counter slots (Ctx::cnt) and flags
+enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_NTRAIL = 8, CNT_FLAGS = 9 };
+enum { FLAG_UNSAT = 1, FLAG_INTERNAL = 2 };
+
+// returns 1 if lit became true, 0 if it was true already, -1 if it was false
+__device__ __forceinline__ int try_assign(int8_t *value, uint32_t lit)
+{
+    uint32_t *w = (uint32_t *)(value + (lit & ~3u));
+    const int sl = (int)(lit & 3u) * 8, sn = (int)((lit ^ 1u) & 3u) * 8;
+    uint32_t old = __atomic_load_n(w, __ATOMIC_RELAXED), assumed;
+    do {
+        const int8_t cur = (int8_t)((old >> sl) & 0xFFu);
+        if (cur >= 0) return cur ? 0 : -1;
+        uint32_t nw = old & ~(0xFFu << sl) & ~(0xFFu << sn);
+        nw |= (1u << sl);
+        assumed = old;
+        old = atomicCAS(w, assumed, nw);
+    } while (old != assumed);
+    return 1;
+}
+
+__device__ __forceinline__ void enqueue_dev(int8_t *value, uint8_t *vstate, uint32_t *trail, unsigned long long *cnt, uint32_t lit)
+{
+    const int r = try_assign(value, lit);
+    if (r == 1) {
+        vstate[lit >> 1] = FROZEN_M;   // learnUnit -> markFrozen (src/cpu/solve.h:475-479)
+        const unsigned long long pos = atomicAdd(&cnt[CNT_NTRAIL], 1ull);
+        trail[pos] = lit;
+    }
+    else if (r < 0) atomicOr(&cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT);
+}
+
+}  // namespace pfg

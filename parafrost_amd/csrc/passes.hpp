@@ -1,0 +1,31 @@
+// Host launchers of the simplifier passes (one translation unit per pass family).
+#pragma once
+#include "ctx.hpp"
+
+namespace pfg {
+
+// cnf.hip
+void upload(Ctx &c, const uint32_t *d_lits, const uint64_t *d_offs, uint64_t ncls, uint64_t nlits,
+            const uint32_t *d_hdr0);
+void hist(Ctx &c);                       // c.hist = live literal counts
+void create_ot(Ctx &c);                  // histogram, list layout, fill in clause order
+void reduce_ot(Ctx &c);
+bool prop(Ctx &c);                       // false on conflict
+void count_all(Ctx &c, int64_t &cls, int64_t &lits);
+void shrink(Ctx &c);                     // stable compaction of live clauses
+void filter_pvs(Ctx &c);
+void sort_ot(Ctx &c);                    // PV lists by the reference clause key
+uint64_t compact_u32(Ctx &c, const uint32_t *in, const uint32_t *flags, uint64_t n, uint32_t *out);
+void enqueue_units_host(Ctx &c, const uint32_t *lits, uint32_t n);
+void merge_units(Ctx &c, const uint32_t *d_units, uint64_t n);  // ordered dedupe into the trail
+
+// lcve.hip
+uint32_t lcve(Ctx &c);                   // returns #elected
+// sub.hip
+void sub(Ctx &c);
+// ve.hip
+void ve(Ctx &c);
+// ere.hip
+void ere(Ctx &c);
+
+}  // namespace pfg

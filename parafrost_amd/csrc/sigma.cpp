@@ -1,0 +1,433 @@
+// Host side of the MI355X simplifier: the reference's simplification entry
+// points (ParaFROST::sigmify / sigmifying and the pass wrappers) driving the
+// device passes, and the C ABI declared in include/pfgpu.h.
+#include "ctx.hpp"
+#include "passes.hpp"
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <new>
+
+namespace pfg {
+
+void dalloc_raw(Ctx &c, void **p, uint64_t bytes)
+{
+    if (c.budget && c.allocated + bytes > c.budget)
+        throw Error(PFG_ENOMEM, "HBM budget exceeded (" + std::to_string(c.allocated + bytes) + " > " +
+                                    std::to_string(c.budget) + " bytes)");
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(e == hipErrorOutOfMemory ? PFG_ENOMEM : PFG_EDEVICE,
+                    std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    c.allocated += bytes;
+}
+
+void dfree_raw(Ctx &c, void *p, uint64_t bytes)
+{
+    (void)hipFree(p);
+    c.allocated -= std::min(bytes, c.allocated);
+}
+
+void read_counters(Ctx &c, int n)
+{
+    PFG_HIP(hipMemcpyAsync(c.hcnt, c.cnt.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+}
+
+PassTimer::PassTimer(Ctx &cc, double *a) : c(cc), acc(a), d(cc.ev_depth < 8 ? cc.ev_depth++ : -1)
+{
+    if (d >= 0) (void)hipEventRecord(c.ev[d][0], c.stream);
+}
+
+PassTimer::~PassTimer()
+{
+    if (d < 0) return;
+    (void)hipEventRecord(c.ev[d][1], c.stream);
+    if (hipEventSynchronize(c.ev[d][1]) == hipSuccess) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c.ev[d][0], c.ev[d][1]) == hipSuccess) *acc += ms;
+    }
+    c.ev_depth--;
+}
+
+// The reference simplifier's round structure (src/cpu/simplify.cpp:150-214,
+// src/gpu/simplify.cu:155-234), one method per reference entry point.
+struct Sigma {
+    Ctx &c;
+    explicit Sigma(Ctx &cc) : c(cc) {}
+
+    void resizeCNF()
+    {   // solve.h:684-688 / simplify.cu:30-46: GC every shrink_rate rounds
+        const int times = c.phase + 1;
+        if (times > 1 && times != c.opts.phases && c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0) shrinkSimp();
+    }
+    void shrinkSimp() { PassTimer t(c, &c.times.gc); shrink(c); }
+    void createOT() { PassTimer t(c, &c.times.ot); create_ot(c); }
+    bool propagate() { PassTimer t(c, &c.times.prop); return prop(c); }
+    bool LCVE()
+    {
+        PassTimer t(c, &c.times.lcve);
+        return lcve(c) >= c.opts.lcve_min;
+    }
+    void sortOT() { PassTimer t(c, &c.times.sort_ot); sort_ot(c); }
+    bool stop(int64_t lr) const { return (c.phase == c.opts.phases) || (lr <= c.opts.lits_min && c.phase > 2); }
+    void ERE() { PassTimer t(c, &c.times.ere); ere(c); }
+    void SUB() { if (c.opts.sub_en || c.opts.ve_en) { PassTimer t(c, &c.times.sub); sub(c); } }
+    void VE() { if (c.opts.ve_en) { PassTimer t(c, &c.times.ve); ve(c); } }
+    void countAll(int64_t &cls, int64_t &lits) { PassTimer t(c, &c.times.count); count_all(c, cls, lits); }
+    void filterPVs() { filter_pvs(c); }
+    void nextPhase()
+    {
+        c.phase++;
+        c.mu_inc++;
+        c.mu_inc += (c.phase == c.opts.phases);
+    }
+
+    int sigmifying()
+    {
+        c.phase = 0;
+        c.mu_inc = 0;
+        int64_t litsbefore = c.nliterals, diff = INT64_MAX;
+        while (litsbefore) {
+            resizeCNF();
+            createOT();
+            if (!propagate()) return PFG_EUNSAT;
+            if (!LCVE()) break;
+            sortOT();
+            if (stop(diff)) { ERE(); break; }
+            SUB();
+            if (c.unsat) return PFG_EUNSAT;
+            VE();
+            if (c.unsat) return PFG_EUNSAT;
+            int64_t nc, nl;
+            countAll(nc, nl);
+            filterPVs();
+            c.nclauses = nc;
+            c.nliterals = nl;
+            diff = litsbefore - c.nliterals;
+            litsbefore = c.nliterals;
+            nextPhase();
+        }
+        if (c.ntrail > c.propagated) {
+            if (!c.ot_valid) createOT();
+            if (!propagate()) return PFG_EUNSAT;
+        }
+        int64_t nc, nl;
+        countAll(nc, nl);
+        c.nclauses = nc;
+        c.nliterals = nl;
+        shrinkSimp();
+        return PFG_OK;
+    }
+};
+
+}  // namespace pfg
+
+using namespace pfg;
+
+struct pfg_ctx {
+    Ctx c;
+};
+
+#define API_BEGIN                  \
+    if (!ctx) return PFG_EINVAL;   \
+    Ctx &c = ctx->c;               \
+    try {                          \
+        PFG_HIP(hipSetDevice(c.device));
+#define API_END                                  \
+    }                                            \
+    catch (const Error &e) {                     \
+        c.last_error = e.what();                 \
+        return e.code;                           \
+    }                                            \
+    catch (const std::bad_alloc &) {             \
+        c.last_error = "host allocation failed"; \
+        return PFG_ENOMEM;                       \
+    }                                            \
+    return c.unsat ? PFG_EUNSAT : PFG_OK;
+
+extern "C" {
+
+void pfg_default_opts(pfg_opts *o)
+{
+    o->phases = 5; o->mu_pos = 32; o->mu_neg = 32; o->lcve_min = 1; o->lits_min = 500;
+    o->shrink_rate = 2; o->ve_clause_limit = 100; o->ve_lbound_en = 0; o->xor_max_arity = 10;
+    o->sub_limit = 3000; o->ere_limit = 3000; o->ere_max_resolvent = 1000;
+    o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
+}
+
+int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, const pfg_opts *opts)
+{
+    if (!out || !nvars) return PFG_EINVAL;
+    *out = nullptr;
+    pfg_ctx *ctx = new (std::nothrow) pfg_ctx();
+    if (!ctx) return PFG_ENOMEM;
+    Ctx &c = ctx->c;
+    c.device = device;
+    c.budget = hbm_budget;
+    c.nvars = nvars;
+    c.nlits2 = 2ull * ((uint64_t)nvars + 1);
+    if (opts) c.opts = *opts; else pfg_default_opts(&c.opts);
+    try {
+        PFG_HIP(hipSetDevice(device));
+        PFG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        for (auto &e : c.ev) { PFG_HIP(hipEventCreate(&e[0])); PFG_HIP(hipEventCreate(&e[1])); }
+        PFG_HIP(hipHostMalloc((void **)&c.hcnt, 16 * sizeof(uint64_t), hipHostMallocDefault));
+        ensure(c, c.cnt, 16);
+        PFG_HIP(hipMemsetAsync(c.cnt.p, 0, 16 * 8, c.stream));
+        ensure(c, c.value, c.nlits2 + 8);
+        PFG_HIP(hipMemsetAsync(c.value.p, 0xFF, c.value.cap, c.stream));
+        ensure(c, c.vstate, (uint64_t)nvars + 1);
+        PFG_HIP(hipMemsetAsync(c.vstate.p, 0, c.vstate.cap, c.stream));
+        ensure(c, c.trail, (uint64_t)nvars + 1);
+        ensure(c, c.ufirst, (uint64_t)nvars + 1);
+        PFG_HIP(hipMemsetAsync(c.ufirst.p, 0xFF, c.ufirst.cap * 4, c.stream));
+        ensure(c, c.pvs, (uint64_t)nvars + 1);
+        ensure(c, c.resolved, 1024);
+        PFG_HIP(hipStreamSynchronize(c.stream));
+    }
+    catch (const Error &e) {
+        c.last_error = e.what();
+        int code = e.code;
+        pfg_destroy(ctx);
+        return code;
+    }
+    *out = ctx;
+    return PFG_OK;
+}
+
+int pfg_destroy(pfg_ctx *ctx)
+{
+    if (!ctx) return PFG_EINVAL;
+    Ctx &c = ctx->c;
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    release(c, c.mem); release(c, c.mem2); release(c, c.refs); release(c, c.refs2);
+    release(c, c.ot); release(c, c.occ); release(c, c.hist); release(c, c.value); release(c, c.vstate);
+    release(c, c.trail); release(c, c.resolved); release(c, c.pvs); release(c, c.units);
+    release(c, c.scores); release(c, c.eligible); release(c, c.rank); release(c, c.tmpu32a); release(c, c.tmpu32b);
+    release(c, c.tmpu8); release(c, c.tmpu64a); release(c, c.tmpu64b); release(c, c.tmpu64c);
+    release(c, c.owner); release(c, c.claim); release(c, c.ufirst); release(c, c.cnt);
+    release(c, c.scan_tmp); release(c, c.radix_tmp);
+    if (c.hcnt) (void)hipHostFree(c.hcnt);
+    for (auto &e : c.ev) {
+        if (e[0]) (void)hipEventDestroy(e[0]);
+        if (e[1]) (void)hipEventDestroy(e[1]);
+    }
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    delete ctx;
+    return PFG_OK;
+}
+
+int pfg_upload_cnf(pfg_ctx *ctx, const uint32_t *lits, const uint64_t *offs, uint64_t ncls, const uint32_t *hdr0)
+{
+    if (!ctx || (!lits && ncls) || !offs) return PFG_EINVAL;
+    API_BEGIN
+    const uint64_t nlits = offs[ncls];
+    for (uint64_t i = 0; i < nlits; i++)
+        if ((lits[i] >> 1) == 0 || (lits[i] >> 1) > c.nvars) throw Error(PFG_EINVAL, "literal out of range");
+    PassTimer t(c, &c.times.upload);
+    uint32_t *dl = nullptr, *dh = nullptr;
+    uint64_t *doff = nullptr;
+    dalloc_raw(c, (void **)&dl, nlits * 4 + 4);
+    dalloc_raw(c, (void **)&doff, (ncls + 1) * 8);
+    PFG_HIP(hipMemcpyAsync(dl, lits, nlits * 4, hipMemcpyHostToDevice, c.stream));
+    PFG_HIP(hipMemcpyAsync(doff, offs, (ncls + 1) * 8, hipMemcpyHostToDevice, c.stream));
+    if (hdr0) {
+        dalloc_raw(c, (void **)&dh, ncls * 4 + 4);
+        PFG_HIP(hipMemcpyAsync(dh, hdr0, ncls * 4, hipMemcpyHostToDevice, c.stream));
+    }
+    upload(c, dl, doff, ncls, nlits, dh);
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    dfree_raw(c, dl, nlits * 4 + 4);
+    dfree_raw(c, doff, (ncls + 1) * 8);
+    if (dh) dfree_raw(c, dh, ncls * 4 + 4);
+    API_END
+}
+
+int pfg_upload_cnf_device(pfg_ctx *ctx, const uint32_t *d_lits, const uint64_t *d_offs, uint64_t ncls, uint64_t nlits)
+{
+    if (!ctx || !d_offs) return PFG_EINVAL;
+    API_BEGIN
+    PassTimer t(c, &c.times.upload);
+    upload(c, d_lits, d_offs, ncls, nlits, nullptr);
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    API_END
+}
+
+int pfg_enqueue_units(pfg_ctx *ctx, const uint32_t *lits, uint32_t n)
+{
+    if (!ctx || (n && !lits)) return PFG_EINVAL;
+    API_BEGIN
+    for (uint32_t i = 0; i < n; i++)
+        if ((lits[i] >> 1) == 0 || (lits[i] >> 1) > c.nvars) throw Error(PFG_EINVAL, "literal out of range");
+    enqueue_units_host(c, lits, n);
+    API_END
+}
+
+int pfg_sigmify(pfg_ctx *ctx)
+{
+    API_BEGIN
+    double t0 = 0;
+    int rc;
+    {
+        PassTimer t(c, &t0);
+        Sigma s(c);
+        rc = s.sigmifying();
+    }
+    c.times.total += t0;
+    if (rc != PFG_OK) return rc;
+    API_END
+}
+
+int pfg_resize_cnf(pfg_ctx *ctx) { API_BEGIN Sigma(c).resizeCNF(); API_END }
+int pfg_create_ot(pfg_ctx *ctx) { API_BEGIN Sigma(c).createOT(); API_END }
+int pfg_reduce_ot(pfg_ctx *ctx) { API_BEGIN if (!c.ot_valid) throw Error(PFG_EINVAL, "no occurrence table"); reduce_ot(c); API_END }
+int pfg_prop(pfg_ctx *ctx) { API_BEGIN if (!c.ot_valid && c.ntrail > c.propagated) throw Error(PFG_EINVAL, "no occurrence table"); Sigma(c).propagate(); API_END }
+
+int pfg_lcve(pfg_ctx *ctx, uint32_t *npvs)
+{
+    API_BEGIN
+    if (!c.ot_valid) throw Error(PFG_EINVAL, "no occurrence table");
+    Sigma(c).LCVE();
+    if (npvs) *npvs = c.npvs;
+    API_END
+}
+
+int pfg_sort_ot(pfg_ctx *ctx) { API_BEGIN Sigma(c).sortOT(); API_END }
+int pfg_sub(pfg_ctx *ctx) { API_BEGIN Sigma(c).SUB(); API_END }
+int pfg_ve(pfg_ctx *ctx) { API_BEGIN Sigma(c).VE(); API_END }
+int pfg_ere(pfg_ctx *ctx) { API_BEGIN Sigma(c).ERE(); API_END }
+
+int pfg_count(pfg_ctx *ctx, int64_t *cls, int64_t *lits)
+{
+    API_BEGIN
+    int64_t a, b;
+    Sigma(c).countAll(a, b);
+    c.nclauses = a;
+    c.nliterals = b;
+    if (cls) *cls = a;
+    if (lits) *lits = b;
+    API_END
+}
+
+int pfg_filter_pvs(pfg_ctx *ctx) { API_BEGIN Sigma(c).filterPVs(); API_END }
+int pfg_gc(pfg_ctx *ctx) { API_BEGIN Sigma(c).shrinkSimp(); API_END }
+int pfg_next_phase(pfg_ctx *ctx) { API_BEGIN Sigma(c).nextPhase(); API_END }
+
+int pfg_stop(pfg_ctx *ctx, int64_t lr, int *stop)
+{
+    API_BEGIN
+    if (stop) *stop = Sigma(c).stop(lr) ? 1 : 0;
+    API_END
+}
+
+int pfg_hist(pfg_ctx *ctx, uint32_t *h)
+{
+    API_BEGIN
+    PassTimer t(c, &c.times.hist);
+    hist(c);
+    if (h) {
+        PFG_HIP(hipMemcpyAsync(h, c.hist.p, c.nlits2 * 4, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+    }
+    API_END
+}
+
+int pfg_download_cnf(pfg_ctx *ctx, uint32_t *words, uint64_t *nwords, uint64_t *ncls)
+{
+    API_BEGIN
+    std::vector<uint64_t> refs(c.nrefs);
+    std::vector<uint32_t> mem(c.msz);
+    if (c.nrefs) PFG_HIP(hipMemcpyAsync(refs.data(), c.refs.p, c.nrefs * 8, hipMemcpyDeviceToHost, c.stream));
+    if (c.msz) PFG_HIP(hipMemcpyAsync(mem.data(), c.mem.p, c.msz * 4, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    uint64_t w = 0;
+    for (uint64_t i = 0; i < c.nrefs; i++) {
+        const uint32_t *cl = mem.data() + refs[i];
+        const uint64_t n = 3 + (uint64_t)cl[2];
+        if (words) std::memcpy(words + w, cl, n * 4);
+        w += n;
+    }
+    if (nwords) *nwords = w;
+    if (ncls) *ncls = c.nrefs;
+    API_END
+}
+
+int pfg_download_trail(pfg_ctx *ctx, uint32_t *lits, uint32_t *n)
+{
+    API_BEGIN
+    if (lits && c.ntrail) {
+        PFG_HIP(hipMemcpyAsync(lits, c.trail.p, c.ntrail * 4ull, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (n) *n = c.ntrail;
+    API_END
+}
+
+int pfg_download_resolved(pfg_ctx *ctx, uint32_t *words, uint64_t *n)
+{
+    API_BEGIN
+    if (words && c.nres) {
+        PFG_HIP(hipMemcpyAsync(words, c.resolved.p, c.nres * 4, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (n) *n = c.nres;
+    API_END
+}
+
+int pfg_download_vstate(pfg_ctx *ctx, uint8_t *vstate)
+{
+    API_BEGIN
+    if (!vstate) throw Error(PFG_EINVAL, "null buffer");
+    PFG_HIP(hipMemcpyAsync(vstate, c.vstate.p, (uint64_t)c.nvars + 1, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    API_END
+}
+
+int pfg_download_pvs(pfg_ctx *ctx, uint32_t *pvs, uint32_t *n)
+{
+    API_BEGIN
+    if (pvs && c.npvs) {
+        PFG_HIP(hipMemcpyAsync(pvs, c.pvs.p, c.npvs * 4ull, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (n) *n = c.npvs;
+    API_END
+}
+
+int pfg_download_ot_list(pfg_ctx *ctx, uint32_t lit, int64_t *idx, uint32_t *n)
+{
+    API_BEGIN
+    if (!c.ot_valid || lit >= c.nlits2) throw Error(PFG_EINVAL, "no occurrence table / bad literal");
+    OL o;
+    PFG_HIP(hipMemcpyAsync(&o, c.ot.p + lit, sizeof(OL), hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    if (n) *n = o.sz;
+    if (idx && o.sz) {
+        std::vector<uint64_t> refs(c.nrefs), ent(o.sz);
+        PFG_HIP(hipMemcpyAsync(refs.data(), c.refs.p, c.nrefs * 8, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipMemcpyAsync(ent.data(), c.occ.p + o.off, o.sz * 8ull, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+        for (uint32_t k = 0; k < o.sz; k++) {
+            auto it = std::lower_bound(refs.begin(), refs.end(), ent[k]);
+            idx[k] = (it != refs.end() && *it == ent[k]) ? (int64_t)(it - refs.begin()) : -1;
+        }
+    }
+    API_END
+}
+
+int pfg_times_get(pfg_ctx *ctx, pfg_times *t)
+{
+    if (!ctx || !t) return PFG_EINVAL;
+    *t = ctx->c.times;
+    return PFG_OK;
+}
+
+int pfg_is_unsat(pfg_ctx *ctx) { return ctx && ctx->c.unsat ? 1 : 0; }
+
+const char *pfg_last_error(pfg_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
+
+}  // extern "C"

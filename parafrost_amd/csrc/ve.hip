@@ -1,0 +1,559 @@
+// Bounded variable elimination with gate substitution on the elected variables
+// (ParaFROST src/cpu/bve.cpp:28-220, simplify.h, and.h, equivalence.h,
+// ifthenelse.h, xor.h).
+//
+// Three phases, as the reference GPU engine does it (kernels.cu:1085-1120), with
+// the reference CPU decisions:
+//   1. detect (thread per variable): pure literal, equivalence, AND/OR, ITE, XOR
+//      gates, else plain resolution under the clause bound; record the kind and
+//      the exact number of resolvents, resolvent words, unit resolvents and
+//      model-stack words. Gate search marks clauses molten as in the reference.
+//   2. exclusive scans over the elected order place every variable's resolvents,
+//      clause references, units and model records (deterministic, equal to the
+//      sequential append order of the reference).
+//   3. apply (thread per variable): write model records and resolvents, rewrite
+//      substituted clauses, delete antecedents, mark the variable eliminated.
+#include "device.hpp"
+#include "passes.hpp"
+#include "prims.hpp"
+
+namespace pfg {
+
+enum { T_NONE = 0, T_PURE = 1, T_BN = 2, T_SUBST = 3, T_RES = 4 };
+
+struct VeArgs {
+    uint32_t *mem;
+    OL *ot;
+    uint64_t *occ;
+    int rlimit;
+    int lbound;
+    int xor_max_arity;
+};
+
+__device__ __forceinline__ uint32_t *cl_at(const VeArgs &A, uint32_t lit, uint32_t i)
+{
+    return A.mem + A.occ[A.ot[lit].off + i];
+}
+
+__device__ int count_orgs(const VeArgs &A, uint32_t lit)
+{
+    const uint32_t sz = A.ot[lit].sz;
+    int n = 0;
+    for (uint32_t i = 0; i < sz; i++) n += c_orig(cl_at(A, lit, i));
+    return n;
+}
+
+// countSubstituted / countResolvents (src/cpu/simplify.h:227-287); on success the
+// totals needed for placement are returned through words/units.
+__device__ bool count_subst(const VeArgs &A, uint32_t x, int clsbefore, uint32_t me, uint32_t other, bool molten_rule,
+                            int &nAdded, uint64_t &words, uint32_t &units)
+{
+    int nAddedLits = 0;
+    uint32_t nunit = 0;
+    const uint32_t ms = A.ot[me].sz, os = A.ot[other].sz;
+    for (uint32_t i = 0; i < ms; i++) {
+        const uint32_t *ci = cl_at(A, me, i);
+        if (!c_orig(ci)) continue;
+        const bool a = c_molten(ci);
+        for (uint32_t j = 0; j < os; j++) {
+            const uint32_t *cj = cl_at(A, other, j);
+            if (!c_orig(cj)) continue;
+            if (molten_rule && a == c_molten(cj)) continue;
+            const int rsize = merge_len(x, ci, cj);
+            if (rsize > 1) {
+                if (++nAdded > clsbefore || (A.rlimit && rsize > A.rlimit)) return true;
+                nAddedLits += rsize;
+            }
+            else if (rsize == 1) nunit++;
+        }
+    }
+    if (A.lbound) {
+        int before = 0;
+        for (uint32_t i = 0; i < ms; i++) { const uint32_t *c = cl_at(A, me, i); if (c_orig(c)) before += c_size(c); }
+        for (uint32_t j = 0; j < os; j++) { const uint32_t *c = cl_at(A, other, j); if (c_orig(c)) before += c_size(c); }
+        if (nAddedLits > before) return true;
+    }
+    words = 3ull * (uint64_t)nAdded + (uint64_t)nAddedLits;
+    units = nAdded ? nunit : 0;
+    return false;
+}
+
+__device__ void freeze_binaries(const VeArgs &A, uint32_t lit)
+{
+    const uint32_t sz = A.ot[lit].sz;
+    for (uint32_t i = 0; i < sz; i++) {
+        uint32_t *c = cl_at(A, lit, i);
+        if (c_orig(c) && c_size(c) == 2) c_freeze(c);
+    }
+}
+
+// find_sfanin + find_BN_gate (equivalence.h:88-130)
+__device__ uint32_t find_bn_gate(const VeArgs &A, uint32_t p)
+{
+    const uint32_t n = p ^ 1u;
+    const uint32_t ps = A.ot[p].sz;
+    uint32_t imp = 0;
+    int nImps = 0;
+    uint32_t first = 0;
+    bool many = false;
+    for (uint32_t i = 0; i < ps; i++) {
+        uint32_t *c = cl_at(A, p, i);
+        if (c_orig(c) && c_size(c) == 2) {
+            imp = (c[3] ^ c[4] ^ p) ^ 1u;
+            c_melt(c);
+            nImps++;
+        }
+        if (nImps > 1) { many = true; break; }
+    }
+    first = many ? 0 : imp;
+    if (first) {
+        uint32_t second = n, def = first;
+        if (second < first) { first = second; second = def; }
+        const uint32_t ns = A.ot[n].sz;
+        for (uint32_t i = 0; i < ns; i++) {
+            uint32_t *c = cl_at(A, n, i);
+            if (c_orig(c) && c_size(c) == 2 && c[3] == first && c[4] == second) {
+                c_melt(c);
+                return def;
+            }
+        }
+    }
+    freeze_binaries(A, p);
+    return 0;
+}
+
+__device__ bool has_lit(const uint32_t *c, uint32_t lit)
+{
+    const int sz = c_size(c);
+    for (int i = 0; i < sz; i++) if (c[3 + i] == lit) return true;
+    return false;
+}
+
+// find_AO_gate (and.h:25-85) without materialising the fan-in vector: the fan-out
+// clause must have |fanins|+1 distinct literals, all of them fan-ins or ~dx
+__device__ bool find_ao_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words, uint32_t &units)
+{
+    const uint32_t x = dx >> 1, f_dx = dx ^ 1u;
+    const uint32_t is = A.ot[dx].sz;
+    uint32_t sig = 0;
+    int nbin = 0;
+    for (uint32_t i = 0; i < is; i++) {
+        uint32_t *c = cl_at(A, dx, i);
+        if (c_orig(c) && c_size(c) == 2) {
+            const uint32_t imp = (c[3] ^ c[4] ^ dx) ^ 1u;
+            sig |= maphash(imp);
+            nbin++;
+            c_melt(c);
+        }
+    }
+    if (nbin > 1) {
+        sig |= maphash(f_dx);
+        const int want = nbin + 1;
+        const uint32_t os = A.ot[f_dx].sz;
+        for (uint32_t i = 0; i < os; i++) {
+            uint32_t *c = cl_at(A, f_dx, i);
+            if (!(c_orig(c) && c_size(c) == want && sig_sub(c[1], sig))) continue;
+            bool eq = true;
+            for (int k = 0; k < want && eq; k++) {
+                const uint32_t l = c[3 + k];
+                if (l == f_dx) continue;
+                bool found = false;
+                for (uint32_t t = 0; t < is && !found; t++) {
+                    const uint32_t *b = cl_at(A, dx, t);
+                    if (c_orig(b) && c_size(b) == 2 && ((b[3] ^ b[4] ^ dx) ^ 1u) == l) found = true;
+                }
+                eq = found;
+            }
+            if (!eq) continue;
+            c_melt(c);
+            nAdded = 0;
+            if (count_subst(A, x, nOrgCls, dx, f_dx, true, nAdded, words, units)) { c_freeze(c); break; }
+            return true;
+        }
+    }
+    freeze_binaries(A, dx);
+    return false;
+}
+
+// fast_equality_check (ifthenelse.h:24-40)
+__device__ uint32_t *fast_equality_check(const VeArgs &A, uint32_t x, uint32_t y, uint32_t z)
+{
+    uint32_t t;
+    if (A.ot[y].sz > A.ot[z].sz) { t = y; y = z; z = t; }
+    if (A.ot[x].sz > A.ot[y].sz) { t = x; x = y; y = t; }
+    const uint32_t lx = x;
+    if (y > z) { t = y; y = z; z = t; }
+    if (x > z) { t = x; x = z; z = t; }
+    if (x > y) { t = x; x = y; y = t; }
+    const uint32_t sz = A.ot[lx].sz;
+    for (uint32_t i = 0; i < sz; i++) {
+        uint32_t *c = cl_at(A, lx, i);
+        if (c_molten(c)) continue;
+        if (c_orig(c) && c_size(c) == 3 && c[3] == x && c[4] == y && c[5] == z) return c;
+    }
+    return nullptr;
+}
+
+// find_ITE_gate (ifthenelse.h:42-95)
+__device__ bool find_ite_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words, uint32_t &units)
+{
+    const uint32_t fx = dx ^ 1u;
+    const uint32_t is = A.ot[dx].sz;
+    for (uint32_t i = 0; i < is; i++) {
+        uint32_t *ci = cl_at(A, dx, i);
+        if (!(c_orig(ci) && c_size(ci) == 3)) continue;
+        uint32_t xi = ci[3], yi = ci[4], zi = ci[5], t;
+        if (yi == dx) { t = xi; xi = yi; yi = t; }
+        if (zi == dx) { t = xi; xi = zi; zi = t; }
+        for (uint32_t j = i + 1; j < is; j++) {
+            uint32_t *cj = cl_at(A, dx, j);
+            if (!(c_orig(cj) && c_size(cj) == 3)) continue;
+            uint32_t xj = cj[3], yj = cj[4], zj = cj[5];
+            if (yj == dx) { t = xj; xj = yj; yj = t; }
+            if (zj == dx) { t = xj; xj = zj; zj = t; }
+            if ((yi >> 1) == (zj >> 1)) { t = yj; yj = zj; zj = t; }
+            if ((zi >> 1) == (zj >> 1)) continue;
+            if (yi != (yj ^ 1u)) continue;
+            uint32_t *d1 = fast_equality_check(A, fx, yi, zi ^ 1u);
+            if (!d1) continue;
+            uint32_t *d2 = fast_equality_check(A, fx, yj, zj ^ 1u);
+            if (!d2) continue;
+            c_melt(ci); c_melt(cj); c_melt(d1); c_melt(d2);
+            nAdded = 0;
+            if (count_subst(A, dx >> 1, nOrgCls, dx, fx, true, nAdded, words, units)) {
+                c_freeze(ci); c_freeze(cj); c_freeze(d1); c_freeze(d2);
+                return false;
+            }
+            return true;
+        }
+    }
+    return false;
+}
+
+// XOR gates (xor.h:30-147)
+__device__ void freeze_arities(const VeArgs &A, uint32_t me, uint32_t other)
+{
+    for (int s = 0; s < 2; s++) {
+        const uint32_t l = s ? other : me, sz = A.ot[l].sz;
+        for (uint32_t i = 0; i < sz; i++) {
+            uint32_t *c = cl_at(A, l, i);
+            if (c_size(c) > 2 && c_molten(c)) c_freeze(c);
+        }
+    }
+}
+
+__device__ bool make_arity(const VeArgs &A, uint32_t &parity, uint32_t *lits, int size)
+{
+    const uint32_t old = parity;
+    while (__builtin_parity(++parity));
+    for (int k = 0; k < size; k++) {
+        const uint32_t bit = 1u << k;
+        if ((parity & bit) != (old & bit)) lits[k] ^= 1u;
+    }
+    uint32_t best = lits[0];
+    uint32_t minsize = A.ot[best].sz;
+    for (int k = 1; k < size; k++) {
+        const uint32_t ls = A.ot[lits[k]].sz;
+        if (ls < minsize) { minsize = ls; best = lits[k]; }
+    }
+    const uint32_t sz = A.ot[best].sz;
+    for (uint32_t i = 0; i < sz; i++) {
+        uint32_t *c = cl_at(A, best, i);
+        if (!(c_orig(c) && c_size(c) == size)) continue;
+        bool ok = true;
+        for (int k = 0; k < size && ok; k++) {
+            const uint32_t l = c[3 + k];
+            int j;
+            for (j = 0; j < size; j++) if (l == lits[j]) break;
+            ok = j < size;
+        }
+        if (ok) { c_melt(c); return true; }
+    }
+    return false;
+}
+
+__device__ bool find_xor_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words, uint32_t &units)
+{
+    const uint32_t fx = dx ^ 1u, v = dx >> 1;
+    const uint32_t is = A.ot[dx].sz;
+    uint32_t lits[21];
+    for (uint32_t i = 0; i < is; i++) {
+        uint32_t *ci = cl_at(A, dx, i);
+        if (!c_orig(ci)) continue;
+        const int size = c_size(ci), arity = size - 1;
+        if (size < 3 || arity > A.xor_max_arity) continue;
+        for (int k = 0; k < size; k++) lits[k] = ci[3 + k];
+        uint32_t parity = 0;
+        int itargets = 1 << arity;
+        while (--itargets && make_arity(A, parity, lits, size));
+        if (itargets) freeze_arities(A, dx, fx);
+        else {
+            c_melt(ci);
+            nAdded = 0;
+            if (count_subst(A, v, nOrgCls, dx, fx, true, nAdded, words, units)) { freeze_arities(A, dx, fx); break; }
+            return true;
+        }
+    }
+    return false;
+}
+
+// model words written by toblivion / save_BN_gate for the smaller original side
+__device__ uint64_t model_words(const VeArgs &A, uint32_t p, int pOrgs, int nOrgs)
+{
+    const uint32_t side = pOrgs > nOrgs ? (p ^ 1u) : p;
+    const uint32_t sz = A.ot[side].sz;
+    uint64_t w = 2;
+    for (uint32_t i = 0; i < sz; i++) {
+        const uint32_t *c = cl_at(A, side, i);
+        if (c_orig(c)) w += (uint64_t)c_size(c) + 1;
+    }
+    return w;
+}
+
+__device__ uint32_t bn_units(const VeArgs &A, uint32_t p, uint32_t def)
+{   // substitute_single (equivalence.h:42-86): clauses that collapse to a unit
+    const uint32_t n = p ^ 1u, def_f = def ^ 1u;
+    uint32_t u = 0;
+    for (int s = 0; s < 2; s++) {
+        const uint32_t lit = s ? p : n;          // negatives first
+        const uint32_t kill = s ? def_f : def;   // literal whose presence deletes the clause
+        const uint32_t into = s ? def : def_f;   // literal substituted for lit
+        const uint32_t sz = A.ot[lit].sz;
+        for (uint32_t i = 0; i < sz; i++) {
+            const uint32_t *c = cl_at(A, lit, i);
+            if (c_learnt(c) || c_molten(c) || has_lit(c, kill)) continue;
+            if (c_orig(c) && c_size(c) - (has_lit(c, into) ? 1 : 0) == 1) u++;
+        }
+    }
+    return u;
+}
+
+__global__ void k_ve_detect(VeArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs, uint32_t *__restrict__ type,
+                            uint32_t *__restrict__ aux, uint32_t *__restrict__ nres, uint64_t *__restrict__ nwords,
+                            uint32_t *__restrict__ nunits, uint64_t *__restrict__ nmodel)
+{
+    for (uint64_t q = gtid(); q < npvs; q += gstride()) {
+        const uint32_t v = pvs[q], p = 2 * v, n = p + 1;
+        const int pOrgs = count_orgs(A, p), nOrgs = count_orgs(A, n);
+        uint32_t t = T_NONE, a = 0, nr = 0, nu = 0;
+        uint64_t nw = 0, nm = 0;
+        if (!pOrgs || !nOrgs) {
+            t = T_PURE;
+            nm = model_words(A, p, pOrgs, nOrgs);
+            a = (uint32_t)(pOrgs > nOrgs);   // model side
+        }
+        else {
+            const uint32_t def = find_bn_gate(A, p);
+            if (def) {
+                t = T_BN;
+                a = def;
+                nm = model_words(A, p, pOrgs, nOrgs);
+                nu = bn_units(A, p, def);
+            }
+            else {
+                const int nOrgCls = pOrgs + nOrgs;
+                int nAdded = 0;
+                uint64_t words = 0;
+                uint32_t units = 0;
+                bool gate = false;
+                if (nOrgCls > 2) {
+                    if (find_ao_gate(A, n, nOrgCls, nAdded, words, units)) gate = true;
+                    else if (!nAdded && find_ao_gate(A, p, nOrgCls, nAdded, words, units)) gate = true;
+                }
+                if (!gate && nOrgCls > 3) {
+                    if (find_ite_gate(A, p, nOrgCls, nAdded, words, units)) gate = true;
+                    else if (!nAdded && find_ite_gate(A, n, nOrgCls, nAdded, words, units)) gate = true;
+                    else if (find_xor_gate(A, p, nOrgCls, nAdded, words, units)) gate = true;
+                    else if (!nAdded && find_xor_gate(A, n, nOrgCls, nAdded, words, units)) gate = true;
+                }
+                if (gate) t = T_SUBST;
+                else if (!nAdded && !count_subst(A, v, nOrgCls, p, n, false, nAdded, words, units)) t = T_RES;
+                if (t != T_NONE) {
+                    nr = (uint32_t)nAdded;
+                    nw = words;
+                    nu = units;
+                    nm = model_words(A, p, pOrgs, nOrgs);
+                    a = (uint32_t)(pOrgs > nOrgs);   // model side
+                }
+            }
+        }
+        type[q] = t | ((t == T_BN ? (uint32_t)(pOrgs > nOrgs) : 0u) << 8);
+        aux[q] = a;
+        nres[q] = nr;
+        nwords[q] = nw;
+        nunits[q] = nu;
+        nmodel[q] = nm;
+    }
+}
+
+struct VeOut {
+    uint32_t *resolved;
+    uint32_t *units;
+    uint64_t *refs;
+    uint8_t *vstate;
+    uint32_t *pvs;
+    uint64_t ref_base, data_base, model_base;
+};
+
+__device__ void save_side(const VeArgs &A, uint32_t *&rs, uint32_t p, bool pbig)
+{   // saveClause/saveWitness for the smaller side (simplify.h:299-331, model.cpp:263-307)
+    const uint32_t n = p ^ 1u;
+    const uint32_t side = pbig ? n : p;
+    const uint32_t sz = A.ot[side].sz;
+    for (uint32_t i = 0; i < sz; i++) {
+        const uint32_t *c = cl_at(A, side, i);
+        if (!c_orig(c)) continue;
+        const int cs = c_size(c);
+        int pos = -1;
+        for (int k = 0; k < cs; k++) { rs[k] = c[3 + k]; if (c[3 + k] == side) pos = k; }
+        if (pos > 0) { const uint32_t t = rs[pos]; rs[pos] = rs[0]; rs[0] = t; }
+        rs[cs] = (uint32_t)cs;
+        rs += cs + 1;
+    }
+    rs[0] = pbig ? p : n;   // witness
+    rs[1] = 1;
+    rs += 2;
+}
+
+__device__ void delete_lists(const VeArgs &A, uint32_t p)
+{   // toblivion: every clause on both lists is deleted, lists cleared
+    for (int s = 0; s < 2; s++) {
+        OL *o = A.ot + (p ^ (uint32_t)s);
+        const uint64_t *d = A.occ + o->off;
+        for (uint32_t i = 0; i < o->sz; i++) c_markdel(A.mem + d[i]);
+        o->sz = 0;
+    }
+}
+
+__device__ void substitute_one(uint32_t dx, uint32_t *org, uint32_t def, uint32_t *&us)
+{   // substitute_single(dx, org, def) (equivalence.h:24-40)
+    uint32_t *lits = org + 3;
+    const int sz = c_size(org);
+    int n = 0;
+    for (int i = 0; i < sz; i++) {
+        const uint32_t lit = lits[i];
+        if (lit == dx) lits[n++] = def;
+        else if (lit != def) lits[n++] = lit;
+    }
+    org[2] = (uint32_t)n;
+    if (n == 1) { *us++ = lits[0]; return; }
+    sort_u32_small(lits, n);
+    uint32_t sig = 0;
+    for (int i = 0; i < n; i++) sig |= maphash(lits[i]);
+    org[1] = sig;
+}
+
+__global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type, const uint32_t *__restrict__ aux,
+                           uint32_t npvs, const uint64_t *__restrict__ roff, const uint64_t *__restrict__ woff,
+                           const uint64_t *__restrict__ uoff, const uint64_t *__restrict__ moff)
+{
+    for (uint64_t q = gtid(); q < npvs; q += gstride()) {
+        const uint32_t tw = type[q], t = tw & 0xFFu;
+        if (t == T_NONE) continue;
+        const uint32_t v = O.pvs[q], p = 2 * v, n = p + 1;
+        uint32_t *rs = O.resolved + O.model_base + moff[q];
+        uint32_t *us = O.units + uoff[q];
+        if (t == T_BN) {
+            const uint32_t def = aux[q], def_f = def ^ 1u;
+            save_side(A, rs, p, (tw >> 8) & 1u);
+            // negatives, then positives
+            for (int s = 0; s < 2; s++) {
+                const uint32_t lit = s ? p : n, kill = s ? def_f : def, into = s ? def : def_f;
+                const OL o = A.ot[lit];
+                for (uint32_t i = 0; i < o.sz; i++) {
+                    uint32_t *c = A.mem + A.occ[o.off + i];
+                    if (c_learnt(c) || c_molten(c) || has_lit(c, kill)) c_markdel(c);
+                    else if (c_orig(c)) substitute_one(lit, c, into, us);
+                }
+            }
+        }
+        else {
+            const bool pbig = aux[q] != 0;
+            if ((t == T_SUBST || t == T_RES) && roff[q + 1] > roff[q]) {   // xsubstitute/xresolve only if nAddedCls
+                uint32_t dx = p, fx = n;
+                if (A.ot[dx].sz > A.ot[fx].sz) { dx = n; fx = p; }
+                const OL om = A.ot[dx], oo = A.ot[fx];
+                uint64_t w = O.data_base + woff[q];
+                uint64_t r = O.ref_base + roff[q];
+                const bool subst = t == T_SUBST;
+                for (uint32_t i = 0; i < om.sz; i++) {
+                    const uint32_t *ci = A.mem + A.occ[om.off + i];
+                    if (!c_orig(ci)) continue;
+                    const bool a = c_molten(ci);
+                    for (uint32_t j = 0; j < oo.sz; j++) {
+                        const uint32_t *cj = A.mem + A.occ[oo.off + j];
+                        if (!c_orig(cj)) continue;
+                        if (subst && a == c_molten(cj)) continue;
+                        // the length is known first so that nothing is written past this
+                        // variable's region (merge_out writes before it sees a tautology)
+                        const int rl = merge_len(v, ci, cj);
+                        if (rl == 0) continue;
+                        int len;
+                        if (rl == 1) {
+                            uint32_t one[2];
+                            merge_out(v, ci, cj, one, len);
+                            *us++ = one[0];
+                            continue;
+                        }
+                        uint32_t *dst = A.mem + w;
+                        merge_out(v, ci, cj, dst + 3, len);
+                        uint32_t sig = 0;
+                        for (int k = 0; k < len; k++) sig |= maphash(dst[3 + k]);
+                        dst[0] = ST_ORIGINAL | 8u;   // markAdded
+                        dst[1] = sig;
+                        dst[2] = (uint32_t)len;
+                        O.refs[r++] = w;
+                        w += 3 + (uint64_t)len;
+                    }
+                }
+            }
+            save_side(A, rs, p, pbig);
+            delete_lists(A, p);
+        }
+        O.vstate[v] = MELTED_M;
+        O.pvs[q] = 0;
+    }
+}
+
+void ve(Ctx &c)
+{
+    if (!c.opts.ve_en || !c.npvs) return;
+    const uint32_t np = c.npvs;
+    VeArgs A{c.mem.p, c.ot.p, c.occ.p, c.opts.ve_clause_limit, c.opts.ve_lbound_en, c.opts.xor_max_arity};
+    // per-variable outputs: type, aux (u32) | nres, nunits (u32 -> scanned) | nwords, nmodel (u64)
+    ensure(c, c.tmpu32a, 2ull * np + 2);
+    ensure(c, c.tmpu32b, 2ull * np + 2);
+    ensure(c, c.tmpu64a, 2ull * np + 2);
+    ensure(c, c.tmpu64b, 4ull * np + 8);
+    uint32_t *type = c.tmpu32a.p, *aux = c.tmpu32a.p + np + 1;
+    uint32_t *nres = c.tmpu32b.p, *nunits = c.tmpu32b.p + np + 1;
+    uint64_t *nwords = c.tmpu64a.p, *nmodel = c.tmpu64a.p + np + 1;
+    uint64_t *roff = c.tmpu64b.p, *uoff = roff + np + 2, *woff = uoff + np + 2, *moff = woff + np + 2;
+    k_ve_detect<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, np, type, aux, nres, nwords, nunits, nmodel);
+    // exclusive scans with the totals stored one past the end
+    scan_u32_to_u64(c, nres, roff, np, roff + np);
+    scan_u32_to_u64(c, nunits, uoff, np, uoff + np);
+    scan_u64(c, nwords, woff, np, woff + np);
+    scan_u64(c, nmodel, moff, np, moff + np);
+    uint64_t tot[4 * 2];
+    PFG_HIP(hipMemcpyAsync(&tot[0], roff + np, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipMemcpyAsync(&tot[1], uoff + np, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipMemcpyAsync(&tot[2], woff + np, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipMemcpyAsync(&tot[3], moff + np, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    const uint64_t add_refs = tot[0], add_units = tot[1], add_words = tot[2], add_model = tot[3];
+    ensure(c, c.mem, c.msz + add_words + 1, true);
+    ensure(c, c.refs, c.nrefs + add_refs + 1, true);
+    ensure(c, c.resolved, c.nres + add_model + 1, true);
+    ensure(c, c.units, add_units + 1);
+    A.mem = c.mem.p;
+    uint32_t *units = c.units.p;
+    VeOut O{c.resolved.p, units, c.refs.p, c.vstate.p, c.pvs.p, c.nrefs, c.msz, c.nres};
+    k_ve_apply<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, O, type, aux, np, roff, woff, uoff, moff);
+    c.nrefs += add_refs;
+    c.msz += add_words;
+    c.nres += add_model;
+    if (add_units) merge_units(c, units, add_units);
+}
+
+}  // namespace pfg
