@@ -112,6 +112,19 @@ int pfg_download_pvs(pfg_ctx *ctx, uint32_t *pvs, uint32_t *n);
 int pfg_download_ot_list(pfg_ctx *ctx, uint32_t lit, int64_t *idx, uint32_t *n);
 
 int pfg_times_get(pfg_ctx *ctx, pfg_times *t);
+
+/* per-kernel device time (HIP events around each launch) and algorithmic HBM bytes */
+typedef struct pfg_kstat {
+    const char *name;
+    double ms;
+    uint64_t launches;
+    double bytes;
+} pfg_kstat;
+int pfg_kernel_stats(pfg_ctx *ctx, pfg_kstat *out, int max, int *n);
+int pfg_reset_stats(pfg_ctx *ctx);
+/* forget the clause database and all variable state (allocations are kept); the
+   context can then take a new upload of the same variable range */
+int pfg_reset(pfg_ctx *ctx);
 int pfg_is_unsat(pfg_ctx *ctx);
 const char *pfg_last_error(pfg_ctx *ctx);
 

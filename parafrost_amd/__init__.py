@@ -38,12 +38,17 @@ class Times(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class KStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("ms", ctypes.c_double), ("launches", ctypes.c_uint64),
+                ("bytes", ctypes.c_double)]
+
+
 API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_upload_cnf_device",
        "pfg_enqueue_units", "pfg_sigmify", "pfg_resize_cnf", "pfg_create_ot", "pfg_reduce_ot", "pfg_prop",
        "pfg_lcve", "pfg_sort_ot", "pfg_sub", "pfg_ve", "pfg_ere", "pfg_count", "pfg_filter_pvs", "pfg_gc",
        "pfg_stop", "pfg_next_phase", "pfg_hist", "pfg_download_cnf", "pfg_download_trail",
        "pfg_download_resolved", "pfg_download_vstate", "pfg_download_pvs", "pfg_download_ot_list",
-       "pfg_times_get", "pfg_is_unsat", "pfg_last_error"]
+       "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset"]
 
 
 def lib_path() -> str:
@@ -78,6 +83,9 @@ def lib():
         L.pfg_download_pvs.argtypes = [vp, vp, ctypes.POINTER(u32)]
         L.pfg_download_ot_list.argtypes = [vp, u32, vp, ctypes.POINTER(u32)]
         L.pfg_times_get.argtypes = [vp, ctypes.POINTER(Times)]
+        L.pfg_kernel_stats.argtypes = [vp, ctypes.POINTER(KStat), i32, ctypes.POINTER(i32)]
+        L.pfg_reset_stats.argtypes = [vp]
+        L.pfg_reset.argtypes = [vp]
         L.pfg_last_error.argtypes = [vp]
         L.pfg_last_error.restype = ctypes.c_char_p
         for f in API:
@@ -236,6 +244,23 @@ class Simplifier:
         t = Times()
         self._check(self.L.pfg_times_get(self.h, ctypes.byref(t)))
         return t.as_dict()
+
+    def kernel_times(self) -> dict:
+        arr = (KStat * 32)()
+        n = ctypes.c_int()
+        self._check(self.L.pfg_kernel_stats(self.h, arr, 32, ctypes.byref(n)))
+        return {arr[i].name.decode(): {"ms": arr[i].ms, "launches": int(arr[i].launches), "bytes": arr[i].bytes}
+                for i in range(n.value)}
+
+    def reset_times(self):
+        self._check(self.L.pfg_reset_stats(self.h))
+
+    def reset_for_upload(self):
+        self._check(self.L.pfg_reset(self.h))
+
+    def stats(self) -> dict:
+        ncls, nl = self.countAll()
+        return {"clauses": ncls, "literals": nl, "eliminated": int((self.vstate() == 1).sum())}
 
     def is_unsat(self) -> bool:
         return bool(self.L.pfg_is_unsat(self.h))

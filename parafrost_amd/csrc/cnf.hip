@@ -50,7 +50,10 @@ void upload(Ctx &c, const uint32_t *d_lits, const uint64_t *d_offs, uint64_t ncl
     const uint64_t words = nlits + 3 * ncls;
     ensure(c, c.mem, words + words / 2 + 4096);
     ensure(c, c.refs, ncls + ncls / 2 + 1024);
-    k_upload<<<grid_for(ncls, 256, 16384), 256, 0, c.stream>>>(d_lits, d_offs, ncls, d_hdr0, c.mem.p, c.refs.p);
+    {
+        KTimer kt(c, K_UPLOAD, 8.0 * nlits + 28.0 * ncls);
+        k_upload<<<grid_for(ncls, 256, 16384), 256, 0, c.stream>>>(d_lits, d_offs, ncls, d_hdr0, c.mem.p, c.refs.p);
+    }
     c.msz = words;
     c.nrefs = ncls;
     c.nclauses = (int64_t)ncls;
@@ -75,8 +78,10 @@ void hist(Ctx &c)
 {
     ensure(c, c.hist, c.nlits2);
     PFG_HIP(hipMemsetAsync(c.hist.p, 0, c.nlits2 * 4, c.stream));
-    if (c.nrefs)
+    if (c.nrefs) {
+        KTimer kt(c, K_HIST, 20.0 * c.nrefs + 4.0 * c.nliterals + 4.0 * c.nlits2);
         k_hist<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.hist.p);
+    }
 }
 
 // ------------------------------------------------------------------ occurrence table
@@ -241,8 +246,14 @@ void create_ot(Ctx &c)
     const uint64_t total = c.hcnt[CNT_TOTAL];
     ensure(c, c.occ, total + 1024);
     k_ot_init<<<grid_for(c.nlits2, 256, 8192), 256, 0, c.stream>>>(c.ot.p, c.tmpu64a.p, c.hist.p, c.nlits2);
-    if (c.nrefs) k_ot_fill<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.ot.p, c.occ.p);
-    order_lists_by_ref(c);
+    if (c.nrefs) {
+        KTimer kt(c, K_OT_FILL, 20.0 * c.nrefs + 12.0 * total + 16.0 * c.nlits2);
+        k_ot_fill<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.ot.p, c.occ.p);
+    }
+    {
+        KTimer kt(c, K_OT_SORT, 16.0 * total);
+        order_lists_by_ref(c);
+    }
     c.ot_valid = true;
 }
 
@@ -371,7 +382,10 @@ __global__ void k_count(const uint32_t *__restrict__ mem, const uint64_t *__rest
 void count_all(Ctx &c, int64_t &cls, int64_t &lits)
 {
     PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));
-    if (c.nrefs) k_count<<<grid_for(c.nrefs, 256, 4096), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, (unsigned long long *)c.cnt.p);
+    if (c.nrefs) {
+        KTimer kt(c, K_COUNT, 20.0 * c.nrefs + 4.0 * c.nliterals);
+        k_count<<<grid_for(c.nrefs, 256, 4096), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, (unsigned long long *)c.cnt.p);
+    }
     read_counters(c, 3);
     cls = (int64_t)c.hcnt[CNT_A];
     lits = (int64_t)c.hcnt[CNT_B];
@@ -422,8 +436,11 @@ void shrink(Ctx &c)
     const uint64_t words = c.hcnt[CNT_A], live = c.hcnt[CNT_B];
     ensure(c, c.mem2, std::max<uint64_t>(c.mem.cap, words + 1));
     ensure(c, c.refs2, std::max<uint64_t>(c.refs.cap, live + 1));
-    k_gc_move<<<grid_for(n * 64, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu64b.p, c.tmpu64c.p,
-                                                                   c.tmpu32a.p, c.mem2.p, c.refs2.p);
+    {
+        KTimer kt(c, K_GC, 8.0 * words + 36.0 * n);
+        k_gc_move<<<grid_for(n * 64, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu64b.p, c.tmpu64c.p,
+                                                                       c.tmpu32a.p, c.mem2.p, c.refs2.p);
+    }
     std::swap(c.mem, c.mem2);
     std::swap(c.refs, c.refs2);
     c.msz = words;
@@ -733,6 +750,7 @@ void sort_ot(Ctx &c)
     if (!c.npvs) return;
     uint64_t *tmp = nullptr;
     if (c.opts.stable_sort) { ensure(c, c.tmpu64b, c.occ.cap); tmp = c.tmpu64b.p; }
+    KTimer kt(c, K_SORT_PV, 0.0);
     k_sort_pv_lists<<<grid_for(2ull * c.npvs, 64, 16384), 64, 0, c.stream>>>(c.pvs.p, c.npvs, c.ot.p, c.occ.p, c.mem.p,
                                                                               c.opts.stable_sort, tmp);
 }

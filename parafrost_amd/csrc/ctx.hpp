@@ -4,6 +4,8 @@
 #include <cstdint>
 #include <string>
 #include <stdexcept>
+#include <vector>
+#include <array>
 #include "../../include/pfgpu.h"
 
 namespace pfg {
@@ -32,6 +34,18 @@ template <typename T>
 struct DBuf {
     T *p = nullptr;
     uint64_t cap = 0;
+};
+
+// kernels with their own event timers (the roofline reports the dominant one)
+enum KId {
+    K_UPLOAD, K_HIST, K_OT_FILL, K_OT_SORT, K_PROP, K_LCVE_RADIX, K_LCVE_MIS, K_SORT_PV, K_SUB,
+    K_VE_DETECT, K_VE_APPLY, K_ERE, K_GC, K_COUNT, K_NUM
+};
+extern const char *const kernel_names[K_NUM];
+
+struct KStat {
+    double ms = 0, bytes = 0;
+    uint64_t launches = 0;
 };
 
 struct Ctx {
@@ -85,6 +99,9 @@ struct Ctx {
     bool sorted_lists = false;
 
     pfg_times times{};
+    KStat kstat[K_NUM];
+    std::vector<std::array<hipEvent_t, 2>> kev;     // event pairs of pending kernel timers
+    std::vector<std::pair<int, double>> kpend;      // (kernel, algorithmic bytes) per pending pair
     hipEvent_t ev[8][2] = {};   // per nesting depth of pass timers
     int ev_depth = 0;
     std::string last_error;
@@ -122,6 +139,15 @@ void release(Ctx &c, DBuf<T> &b)
 
 // read a few device counters into the pinned mirror (synchronizes the stream)
 void read_counters(Ctx &c, int n);
+
+// event pair around one kernel launch; resolved lazily (no stream sync per launch)
+struct KTimer {
+    Ctx &c;
+    size_t slot;
+    KTimer(Ctx &cc, int kid, double bytes);
+    ~KTimer();
+};
+void resolve_ktimers(Ctx &c);
 
 struct PassTimer {
     Ctx &c;

@@ -144,7 +144,10 @@ void ere(Ctx &c)
     EreArgs A{c.mem.p, c.ot.p, c.occ.p, c.opts.ere_limit, c.opts.ere_max_resolvent, c.owner.p, c.tmpu64b.p,
               (unsigned long long *)c.cnt.p};
     PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));
-    k_ere_claim<<<grid_for(c.npvs, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, c.npvs);
+    {
+        KTimer kt(c, K_ERE, 0.0);
+        k_ere_claim<<<grid_for(c.npvs, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, c.npvs);
+    }
     read_counters(c, 3);
     const uint64_t nt = c.hcnt[CNT_A];
     const bool collision = c.hcnt[CNT_B] != 0;

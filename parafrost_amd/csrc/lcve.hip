@@ -136,7 +136,10 @@ uint32_t lcve(Ctx &c)
     const uint32_t maxs = reduce_max_u32(c, c.scores.p, nv);
     int bits = 0;
     while (bits < 32 && (maxs >> bits)) bits++;
-    radix_sort_u32_pairs(c, c.scores.p, c.eligible.p, nv, ((bits + 7) / 8) * 8);
+    {
+        KTimer kt(c, K_LCVE_RADIX, 16.0 * nv * ((bits + 7) / 8));
+        radix_sort_u32_pairs(c, c.scores.p, c.eligible.p, nv, ((bits + 7) / 8) * 8);
+    }
     k_rank<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, c.rank.p);
     uint8_t *kind = c.tmpu8.p, *status = c.tmpu8.p + nv + 1;
     const uint32_t pt = c.opts.mu_pos << c.mu_inc, nt = c.opts.mu_neg << c.mu_inc;
@@ -153,6 +156,7 @@ uint32_t lcve(Ctx &c)
     int rounds = 0;
     while (nwork) {
         PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 8, c.stream));
+        KTimer kt(c, K_LCVE_MIS, 0.0);
         k_mis_round<<<grid_for(nwork * 64, 256, 8192), 256, 0, c.stream>>>(work, nwork, c.eligible.p, c.rank.p, kind, status,
                                                                             c.ot.p, c.occ.p, c.mem.p, next,
                                                                             (unsigned long long *)c.cnt.p);

@@ -52,6 +52,42 @@ PassTimer::~PassTimer()
     c.ev_depth--;
 }
 
+const char *const kernel_names[K_NUM] = {
+    "upload_prep", "hist", "ot_fill", "ot_sort", "prop", "lcve_radix", "lcve_mis", "sort_pv_lists", "sub",
+    "ve_detect", "ve_apply", "ere", "gc_move", "count"};
+
+KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
+{
+    if (c.kpend.size() >= 2048) resolve_ktimers(c);
+    slot = c.kpend.size();
+    if (c.kev.size() <= slot) {
+        std::array<hipEvent_t, 2> e{};
+        PFG_HIP(hipEventCreate(&e[0]));
+        PFG_HIP(hipEventCreate(&e[1]));
+        c.kev.push_back(e);
+    }
+    c.kpend.emplace_back(kid, bytes);
+    (void)hipEventRecord(c.kev[slot][0], c.stream);
+}
+
+KTimer::~KTimer() { (void)hipEventRecord(c.kev[slot][1], c.stream); }
+
+void resolve_ktimers(Ctx &c)
+{
+    if (c.kpend.empty()) return;
+    (void)hipStreamSynchronize(c.stream);
+    for (size_t i = 0; i < c.kpend.size(); i++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c.kev[i][0], c.kev[i][1]) == hipSuccess) {
+            KStat &k = c.kstat[c.kpend[i].first];
+            k.ms += ms;
+            k.bytes += c.kpend[i].second;
+            k.launches++;
+        }
+    }
+    c.kpend.clear();
+}
+
 // The reference simplifier's round structure (src/cpu/simplify.cpp:150-214,
 // src/gpu/simplify.cu:155-234), one method per reference entry point.
 struct Sigma {
@@ -212,6 +248,7 @@ int pfg_destroy(pfg_ctx *ctx)
     release(c, c.owner); release(c, c.claim); release(c, c.ufirst); release(c, c.cnt);
     release(c, c.scan_tmp); release(c, c.radix_tmp);
     if (c.hcnt) (void)hipHostFree(c.hcnt);
+    for (auto &e : c.kev) { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); }
     for (auto &e : c.ev) {
         if (e[0]) (void)hipEventDestroy(e[0]);
         if (e[1]) (void)hipEventDestroy(e[1]);
@@ -277,6 +314,7 @@ int pfg_sigmify(pfg_ctx *ctx)
         Sigma s(c);
         rc = s.sigmifying();
     }
+    resolve_ktimers(c);
     c.times.total += t0;
     if (rc != PFG_OK) return rc;
     API_END
@@ -416,6 +454,54 @@ int pfg_download_ot_list(pfg_ctx *ctx, uint32_t lit, int64_t *idx, uint32_t *n)
             idx[k] = (it != refs.end() && *it == ent[k]) ? (int64_t)(it - refs.begin()) : -1;
         }
     }
+    API_END
+}
+
+int pfg_kernel_stats(pfg_ctx *ctx, pfg_kstat *out, int max, int *n)
+{
+    API_BEGIN
+    resolve_ktimers(c);
+    int k = 0;
+    for (int i = 0; i < K_NUM && k < max; i++) {
+        if (!c.kstat[i].launches) continue;
+        if (out) {
+            out[k].name = kernel_names[i];
+            out[k].ms = c.kstat[i].ms;
+            out[k].launches = c.kstat[i].launches;
+            out[k].bytes = c.kstat[i].bytes;
+        }
+        k++;
+    }
+    if (n) *n = k;
+    API_END
+}
+
+int pfg_reset_stats(pfg_ctx *ctx)
+{
+    API_BEGIN
+    resolve_ktimers(c);
+    for (auto &k : c.kstat) k = KStat{};
+    c.times = pfg_times{};
+    API_END
+}
+
+int pfg_reset(pfg_ctx *ctx)
+{
+    API_BEGIN
+    PFG_HIP(hipMemsetAsync(c.value.p, 0xFF, c.value.cap, c.stream));
+    PFG_HIP(hipMemsetAsync(c.vstate.p, 0, c.vstate.cap, c.stream));
+    PFG_HIP(hipMemsetAsync(c.cnt.p, 0, 16 * 8, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    c.ntrail = c.propagated = 0;
+    c.nres = 0;
+    c.npvs = 0;
+    c.msz = 0;
+    c.nrefs = 0;
+    c.phase = 0;
+    c.mu_inc = 0;
+    c.nclauses = c.nliterals = 0;
+    c.unsat = false;
+    c.ot_valid = false;
     API_END
 }
 

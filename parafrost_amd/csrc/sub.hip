@@ -229,7 +229,10 @@ void sub(Ctx &c)
     }
     uint32_t b = 0;
     auto run_range = [&](uint32_t e) {
-        if (e > b) k_sub_range<<<grid_for(e - b, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, b, e);
+        if (e > b) {
+            KTimer kt(c, K_SUB, 0.0);
+            k_sub_range<<<grid_for(e - b, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, b, e);
+        }
     };
     for (uint64_t k = 0; k < nu; k++) {
         const uint32_t u = ucap[k];

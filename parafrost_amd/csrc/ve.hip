@@ -529,7 +529,10 @@ void ve(Ctx &c)
     uint32_t *nres = c.tmpu32b.p, *nunits = c.tmpu32b.p + np + 1;
     uint64_t *nwords = c.tmpu64a.p, *nmodel = c.tmpu64a.p + np + 1;
     uint64_t *roff = c.tmpu64b.p, *uoff = roff + np + 2, *woff = uoff + np + 2, *moff = woff + np + 2;
-    k_ve_detect<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, np, type, aux, nres, nwords, nunits, nmodel);
+    {
+        KTimer kt(c, K_VE_DETECT, 0.0);
+        k_ve_detect<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, np, type, aux, nres, nwords, nunits, nmodel);
+    }
     // exclusive scans with the totals stored one past the end
     scan_u32_to_u64(c, nres, roff, np, roff + np);
     scan_u32_to_u64(c, nunits, uoff, np, uoff + np);
@@ -549,7 +552,10 @@ void ve(Ctx &c)
     A.mem = c.mem.p;
     uint32_t *units = c.units.p;
     VeOut O{c.resolved.p, units, c.refs.p, c.vstate.p, c.pvs.p, c.nrefs, c.msz, c.nres};
-    k_ve_apply<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, O, type, aux, np, roff, woff, uoff, moff);
+    {
+        KTimer kt(c, K_VE_APPLY, 4.0 * add_words + 8.0 * add_refs + 4.0 * add_model);
+        k_ve_apply<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, O, type, aux, np, roff, woff, uoff, moff);
+    }
     c.nrefs += add_refs;
     c.msz += add_words;
     c.nres += add_model;

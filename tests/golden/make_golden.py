@@ -1,0 +1,84 @@
+"""Write golden vectors of the reference CPU simplifier (ParaFROST src/cpu).
+
+Runs the reference simplifier, compiled from the reference's own sources by
+oracle/Makefile (target `ref`), on small seeded instances and stores, per case,
+the input clauses and the reference's final state (clause images in clause
+order, unit trail, model stack, variable states) plus per-pass dumps of the
+first rounds. Output: tests/golden/reference_vectors.npz (data only).
+
+usage: python tests/golden/make_golden.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import pyoracle as po  # noqa: E402
+from parafrost_amd import gen  # noqa: E402
+
+PASSES = ("lcve", "sort", "sub", "ve", "ere")
+
+
+def cases():
+    out = []
+    for seed in range(48):
+        rng = np.random.default_rng(1000 + seed)
+        nv = int(rng.integers(30, 250))
+        kind = seed % 3
+        if kind == 0:
+            cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(2.0, 4.3)), 3, 1000 + seed)
+        elif kind == 1:
+            cnf = gen.geometric(nv, int(nv * rng.uniform(1.5, 4.0)), 2, 8, 1000 + seed)
+        else:
+            nv = max(nv, 64)
+            cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 4.0)), 2.5, 0.3, 1000 + seed)
+        opts = dict(phases=int(rng.integers(1, 6)), mu_pos=int(rng.integers(10, 60)),
+                    mu_neg=int(rng.integers(10, 60)), shrink_rate=int(rng.integers(1, 4)))
+        out.append((f"g{seed}", cnf, opts))
+    return out
+
+
+def main():
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True, capture_output=True)
+    arrays = {}
+    names = []
+    with tempfile.TemporaryDirectory() as td:
+        for name, cnf, opts in cases():
+            path = os.path.join(td, name + ".cnf")
+            cnf.write_dimacs(path)
+            o = po.default_opts(**opts)
+            st = po.run_reference(path, os.path.join(td, name + ".bin"), o, trace_prefix=os.path.join(td, name))
+            arrays[f"{name}/lits"] = cnf.lits
+            arrays[f"{name}/offs"] = cnf.offs
+            arrays[f"{name}/meta"] = np.array([cnf.nvars, opts["phases"], opts["mu_pos"], opts["mu_neg"],
+                                               opts["shrink_rate"], st.unsat], np.int64)
+            for f in ("cnf", "trail", "resolved", "vstate"):
+                arrays[f"{name}/final_{f}"] = getattr(st, f)
+            # per-pass states of round 0 and 1
+            for r in (0, 1):
+                for p in PASSES:
+                    tp = os.path.join(td, f"{name}.{r}.{p}.bin")
+                    if not os.path.exists(tp):
+                        continue
+                    ts = po.read_dump(tp)
+                    arrays[f"{name}/r{r}_{p}_cnf"] = ts.cnf
+                    arrays[f"{name}/r{r}_{p}_pvs"] = ts.pvs
+                    arrays[f"{name}/r{r}_{p}_trail"] = ts.trail
+                    if p == "sort":
+                        arrays[f"{name}/r{r}_{p}_lists"] = (np.concatenate([np.array([len(x)], np.int64) for x in ts.lists] +
+                                                                           [x for x in ts.lists]) if ts.lists else np.zeros(0, np.int64))
+            names.append(name)
+    arrays["names"] = np.array(names)
+    np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **arrays)
+    print("wrote", len(names), "cases")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,177 @@
+"""GPU simplifier parity with the oracle (bit-exact), through the C ABI.
+
+Lock-step cases compare the complete state after every pass of every round
+(tests/lockstep.py); end-to-end cases compare the final state of a whole
+sigmify. Larger workloads add size-independent properties.
+"""
+import numpy as np
+import pytest
+
+import lockstep as ls
+import pyoracle as po
+from parafrost_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+
+def _small(seed):
+    rng = np.random.default_rng(7000 + seed)
+    nv = int(rng.integers(20, 400))
+    kind = seed % 3
+    if kind == 0:
+        cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(2.0, 4.5)), 3, 7000 + seed)
+    elif kind == 1:
+        cnf = gen.geometric(nv, int(nv * rng.uniform(1.5, 4.0)), 2, 8, 7000 + seed)
+    else:
+        nv = max(nv, 64)
+        cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 4.0)), 2.5, 0.3, 7000 + seed)
+    kw = dict(phases=int(rng.integers(1, 7)), mu_pos=int(rng.integers(10, 60)), mu_neg=int(rng.integers(10, 60)),
+              shrink_rate=int(rng.integers(1, 4)))
+    return cnf, kw
+
+
+@pytest.mark.parametrize("seed", range(48))
+def test_lockstep_small(seed):
+    print(f"lockstep small seed={seed}", flush=True)
+    cnf, kw = _small(seed)
+    ls.run_lockstep(cnf, **kw)
+
+
+OPTION_VARIANTS = [
+    dict(stable_sort=1), dict(ere_en=0), dict(ve_lbound_en=1), dict(xor_max_arity=2), dict(xor_max_arity=20),
+    dict(sub_limit=20, ere_limit=20), dict(ve_clause_limit=0), dict(ve_clause_limit=3), dict(ere_max_resolvent=0),
+    dict(lits_min=1 << 40), dict(phases=8, mu_pos=10, mu_neg=10), dict(sub_en=0), dict(lcve_min=50),
+]
+
+
+@pytest.mark.parametrize("variant", range(len(OPTION_VARIANTS)))
+def test_lockstep_options(variant):
+    kw = OPTION_VARIANTS[variant]
+    print(f"lockstep options {kw}", flush=True)
+    for seed in (1, 2, 5):
+        cnf, base = _small(seed)
+        base.update(kw)
+        ls.run_lockstep(cnf, **base)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_lockstep_long_lists(seed):
+    """Mid-size scale-free instances with high temperatures: elected variables with
+    occurrence lists longer than 20 (std::sort order) and gate structure."""
+    print(f"lockstep long lists seed={seed}", flush=True)
+    rng = np.random.default_rng(9000 + seed)
+    nv = int(rng.integers(3000, 12000))
+    cnf = gen.scalefree(nv, int(nv * rng.uniform(3.0, 5.0)), 2.5, 0.2, 9000 + seed)
+    ls.run_lockstep(cnf, lists=True, phases=5, mu_pos=int(rng.integers(64, 400)), mu_neg=int(rng.integers(64, 400)))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_lockstep_learnt_clauses(seed):
+    """Learnt clauses (status LEARNT with an LBD): ERE types, subsumption promotion,
+    strengthening LBD bump (subsume.h:101-117, elimination.cpp:27-39)."""
+    print(f"lockstep learnt seed={seed}", flush=True)
+    cnf, kw = _small(seed + 100)
+    rng = np.random.default_rng(seed)
+    hdr = np.where(rng.random(cnf.ncls) < 0.3, 1 | (rng.integers(1, 30, cnf.ncls) << 6) | (1 << 4), 0).astype(np.uint32)
+    oo, go = ls._opts_pair(**kw)
+    o = po.Oracle(cnf, oo, hdr0=hdr)
+    o.sigmify()
+    import parafrost_amd as pf
+    g = pf.Simplifier(cnf.nvars, go)
+    g.upload(cnf, hdr0=hdr)
+    try:
+        g.sigmify()
+        ls.compare(o.state(), ls.state_gpu(g), "learnt sigmify")
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_lockstep_root_units(seed):
+    """Units on the trail before simplification: batched device propagation."""
+    print(f"lockstep units seed={seed}", flush=True)
+    cnf, kw = _small(seed + 200)
+    rng = np.random.default_rng(seed)
+    vs = rng.choice(np.arange(1, cnf.nvars + 1), size=max(1, cnf.nvars // 25), replace=False)
+    units = (2 * vs + rng.integers(0, 2, len(vs))).astype(np.uint32)
+    ls.run_lockstep(cnf, units=units, **kw)
+
+
+def test_unsat_by_propagation():
+    import parafrost_amd as pf
+    lits = np.array([2 * 1 + 1, 2 * 2, 2 * 2 + 1, 2 * 3, 2 * 3 + 1, 2 * 1], np.uint32)  # (~1|2) (~2|3) (~3|1)... closed by units
+    cnf = gen.Cnf(3, lits, np.array([0, 2, 4, 6], np.uint64))
+    g = pf.Simplifier(3)
+    g.upload(cnf)
+    g.enqueue_units(np.array([2 * 1, 2 * 3 + 1], np.uint32))   # 1, ~3  -> 2 forced, then ~3 conflicts
+    with pytest.raises(pf.Unsat):
+        g.sigmify()
+    g.close()
+
+
+def test_empty_and_tiny():
+    import parafrost_amd as pf
+    g = pf.Simplifier(5)
+    g.upload(gen.Cnf(5, np.zeros(0, np.uint32), np.zeros(1, np.uint64)))
+    g.sigmify()
+    assert len(g.cnf_words()) == 0
+    g.close()
+    cnf = gen.Cnf(2, np.array([2, 4], np.uint32), np.array([0, 2], np.uint64))
+    ls.run_full(cnf)
+
+
+def test_c1_lockstep():
+    print("c1 lockstep", flush=True)
+    ls.run_lockstep(gen.config("c1"))
+
+
+def test_c1_reupload_deterministic():
+    import parafrost_amd as pf
+    cnf = gen.config("c1")
+    g = pf.Simplifier(cnf.nvars)
+    outs = []
+    for _ in range(3):
+        g.reset_for_upload()
+        g.upload(cnf)
+        g.sigmify()
+        outs.append((g.cnf_words(), g.resolved(), g.vstate()))
+    g.close()
+    for a in outs[1:]:
+        for x, y in zip(outs[0], a):
+            assert np.array_equal(x, y)
+
+
+@pytest.mark.slow
+def test_c2_full_matches_oracle():
+    """The benchmark workload end to end (2M vars, 8.4M clauses)."""
+    print("c2 full", flush=True)
+    ls.run_full(gen.config("c2"))
+
+
+@pytest.mark.slow
+def test_c3_full_matches_oracle_and_invariants():
+    """Scale-free 5M vars / 20M clauses: oracle parity plus size-independent checks:
+    eliminated variables occur in no surviving clause, literals sorted and unique
+    per clause, signatures consistent, model stack well formed."""
+    print("c3 full", flush=True)
+    cnf = gen.config("c3")
+    st, _ = ls.run_full(cnf)
+    w = st.cnf
+    elim = st.vstate == 1
+    i, n, k = 0, len(w), 0
+    while i < n and k < 200_000:
+        sz = int(w[i + 2])
+        cl = w[i + 3:i + 3 + sz]
+        assert not elim[cl >> 1].any()
+        assert np.all(np.diff(cl.astype(np.int64)) > 0)
+        assert int(w[i + 1]) == int(np.bitwise_or.reduce(np.left_shift(1, cl & 31).astype(np.uint32))) if sz else True
+        i += 3 + sz
+        k += 1
+    # model stack: records end with their size, walk backwards
+    r = st.resolved
+    j = len(r) - 1
+    while j > 0:
+        k = int(r[j])
+        assert k >= 1
+        j -= k + 1
+    assert j == -1

@@ -1,0 +1,48 @@
+"""Oracle against the reference CPU simplifier run live (oracle/_ref/pfref, built
+from the reference sources by `make -C oracle ref`). Skipped where that build is
+absent; the committed golden vectors cover the same ground offline."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from parafrost_amd import gen
+
+pytestmark = pytest.mark.skipif(not os.path.exists(po.REF_BIN), reason="reference build absent")
+
+
+def _run(cnf, opts):
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "f.cnf")
+        cnf.write_dimacs(p)
+        ref = po.run_reference(p, os.path.join(td, "f.bin"), opts)
+    o = po.Oracle(cnf, opts)
+    rc = o.sigmify()
+    st = o.state()
+    if ref.unsat:
+        assert rc != 0
+        return
+    for f in ("cnf", "trail", "resolved", "vstate"):
+        assert np.array_equal(getattr(st, f), getattr(ref, f)), f
+
+
+def test_c1_full():
+    _run(gen.config("c1"), po.default_opts())
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_mid_size(seed):
+    rng = np.random.default_rng(500 + seed)
+    nv = int(rng.integers(2000, 20000))
+    kind = seed % 3
+    if kind == 0:
+        cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(3.0, 4.3)), 3, 500 + seed)
+    elif kind == 1:
+        cnf = gen.geometric(nv, int(nv * rng.uniform(2.0, 4.0)), 2, 8, 500 + seed)
+    else:
+        cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 5.0)), 2.5, 0.2, 500 + seed)
+    opts = po.default_opts(phases=int(rng.integers(2, 7)), mu_pos=int(rng.integers(10, 200)),
+                           mu_neg=int(rng.integers(10, 200)), shrink_rate=int(rng.integers(1, 4)))
+    _run(cnf, opts)
