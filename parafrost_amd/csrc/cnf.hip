@@ -59,6 +59,7 @@ void upload(Ctx &c, const uint32_t *d_lits, const uint64_t *d_offs, uint64_t ncl
     c.nclauses = (int64_t)ncls;
     c.nliterals = (int64_t)nlits;
     c.ot_valid = false;
+    c.hist_live = false;
 }
 
 // ------------------------------------------------------------------ histogram
@@ -113,34 +114,47 @@ __global__ void k_ot_fill(const uint32_t *__restrict__ mem, const uint64_t *__re
     }
 }
 
-// classify lists by length for the ordering step
-__global__ void k_ot_classify(const OL *__restrict__ ot, uint64_t n, uint32_t *__restrict__ small_ids,
-                              uint32_t *__restrict__ mid_ids, uint32_t *__restrict__ big_ids, unsigned long long *cnt)
+__device__ __forceinline__ void cswap64(uint64_t &a, uint64_t &b)
 {
-    for (uint64_t l = gtid(); l < n; l += gstride()) {
-        const uint32_t sz = ot[l].sz;
-        if (sz <= 1) continue;
-        if (sz <= 32) small_ids[atomicAdd(&cnt[CNT_A], 1ull)] = (uint32_t)l;
-        else if (sz <= 2048) mid_ids[atomicAdd(&cnt[CNT_B], 1ull)] = (uint32_t)l;
-        else big_ids[atomicAdd(&cnt[CNT_C], 1ull)] = (uint32_t)l;
-    }
+    const uint64_t x = a < b ? a : b, y = a < b ? b : a;
+    a = x;
+    b = y;
 }
 
-__global__ void k_sort_small(const uint32_t *__restrict__ ids, uint64_t n, const OL *__restrict__ ot, uint64_t *__restrict__ occ)
+// one thread per list: lists of up to 8 references are sorted in registers by a
+// fixed network, up to 32 by insertion in place; longer lists are handed on
+__global__ void __launch_bounds__(256) k_sort_small(const OL *__restrict__ ot, uint64_t n, uint64_t *__restrict__ occ,
+                                                    uint32_t *__restrict__ mid_ids, uint32_t *__restrict__ big_ids,
+                                                    unsigned long long *cnt)
 {
-    for (uint64_t q = gtid(); q < n; q += gstride()) {
-        const OL o = ot[ids[q]];
-        uint64_t *d = occ + o.off;
-        uint64_t t[32];
-        const int sz = (int)o.sz;
-        for (int k = 0; k < sz; k++) t[k] = d[k];
-        for (int i = 1; i < sz; i++) {
-            const uint64_t v = t[i];
-            int j = i;
-            while (j > 0 && t[j - 1] > v) { t[j] = t[j - 1]; j--; }
-            t[j] = v;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
+        const uint64_t l = b + threadIdx.x;
+        const uint32_t sz = l < n ? ot[l].sz : 0u;
+        if (sz >= 2 && sz <= 8) {
+            uint64_t *d = occ + ot[l].off;
+            uint64_t t[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) t[k] = k < (int)sz ? d[k] : ~0ull;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+#pragma unroll
+                for (int k = (i & 1); k + 1 < 8; k += 2) cswap64(t[k], t[k + 1]);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (k < (int)sz) d[k] = t[k];
         }
-        for (int k = 0; k < sz; k++) d[k] = t[k];
+        else if (sz > 8 && sz <= 32) {
+            uint64_t *d = occ + ot[l].off;
+            for (uint32_t i = 1; i < sz; i++) {
+                const uint64_t v = d[i];
+                uint32_t j = i;
+                while (j > 0 && d[j - 1] > v) { d[j] = d[j - 1]; j--; }
+                d[j] = v;
+            }
+        }
+        wave_append(sz > 32 && sz <= 2048, (uint32_t)l, mid_ids, &cnt[CNT_B]);
+        wave_append(sz > 2048, (uint32_t)l, big_ids, &cnt[CNT_C]);
     }
 }
 
@@ -206,11 +220,10 @@ static void order_lists_by_ref(Ctx &c)
     ensure(c, c.tmpu32b, n);
     ensure(c, c.eligible, n);
     PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 3 * 8, c.stream));
-    k_ot_classify<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.ot.p, n, c.tmpu32a.p, c.tmpu32b.p, c.eligible.p,
-                                                                 (unsigned long long *)c.cnt.p);
+    k_sort_small<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.ot.p, n, c.occ.p, c.tmpu32b.p, c.eligible.p,
+                                                                (unsigned long long *)c.cnt.p);
     read_counters(c, 4);
-    const uint64_t ns = c.hcnt[CNT_A], nm = c.hcnt[CNT_B], nb = c.hcnt[CNT_C];
-    if (ns) k_sort_small<<<grid_for(ns, 128, 16384), 128, 0, c.stream>>>(c.tmpu32a.p, ns, c.ot.p, c.occ.p);
+    const uint64_t nm = c.hcnt[CNT_B], nb = c.hcnt[CNT_C];
     if (nm) k_sort_mid<<<(unsigned)nm, 256, 0, c.stream>>>(c.tmpu32b.p, c.ot.p, c.occ.p);
     if (nb) {
         if (c.msz >= (1ull << 32)) throw Error(PFG_EINTERNAL, "clause pool beyond 2^32 words");
@@ -255,6 +268,7 @@ void create_ot(Ctx &c)
         order_lists_by_ref(c);
     }
     c.ot_valid = true;
+    c.hist_live = true;
 }
 
 // reduceOL (src/cpu/simplify.cpp:55-65): in-order filter of deleted references
@@ -362,6 +376,7 @@ bool prop(Ctx &c)
         c.ntrail = (uint32_t)c.hcnt[CNT_NTRAIL];
         if (c.hcnt[CNT_FLAGS] & FLAG_UNSAT) { c.unsat = true; return false; }
     }
+    if (c.propagated != before) c.hist_live = false;
     if (c.propagated != before && !c.opts.sub_en) reduce_ot(c);
     return true;
 }
@@ -499,6 +514,7 @@ void enqueue_units_host(Ctx &c, const uint32_t *lits, uint32_t n)
     if (!n) return;
     ensure(c, c.tmpu32a, n);
     PFG_HIP(hipMemcpyAsync(c.tmpu32a.p, lits, n * 4, hipMemcpyHostToDevice, c.stream));
+    c.hist_live = false;
     k_assign_host_units<<<1, 64, 0, c.stream>>>(c.tmpu32a.p, n, c.value.p, c.vstate.p, c.trail.p, (unsigned long long *)c.cnt.p);
     read_counters(c, 10);
     c.ntrail = (uint32_t)c.hcnt[CNT_NTRAIL];

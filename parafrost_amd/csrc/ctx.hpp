@@ -67,6 +67,8 @@ struct Ctx {
     DBuf<uint64_t> occ;
     DBuf<uint32_t> hist;        // per literal
     bool ot_valid = false;
+    bool hist_live = false;     // hist holds the live-literal counts of the current database
+    bool trace = false;         // PFG_TRACE set: per-round progress on stderr
 
     // variables
     DBuf<int8_t> value;         // per literal: -1 unassigned, 0 false, 1 true
@@ -80,7 +82,7 @@ struct Ctx {
     uint32_t npvs = 0;
 
     // scratch
-    DBuf<uint32_t> scores, eligible, rank, tmpu32a, tmpu32b;
+    DBuf<uint32_t> scores, eligible, rank, tmpu32a, tmpu32b, tmpu32c, tmpu32d;
     DBuf<uint8_t> tmpu8;
     DBuf<uint64_t> tmpu64a, tmpu64b, tmpu64c;
     DBuf<uint32_t> owner;       // per (ref >> 1): ERE target claims (all-ones when idle)

@@ -129,4 +129,17 @@ __device__ __forceinline__ void enqueue_dev(int8_t *value, uint8_t *vstate, uint
     else if (r < 0) atomicOr(&cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT);
 }
 
+// wave-aggregated append of flagged lanes (one atomic per wave and list kind)
+__device__ __forceinline__ void wave_append(bool flag, uint32_t value, uint32_t *out, unsigned long long *counter)
+{
+    const uint64_t m = __ballot(flag);
+    if (!m) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (flag) out[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
 }  // namespace pfg

@@ -5,18 +5,26 @@
 // the first candidate whose both occurrence lists reach the temperatures, and
 // freezing every cold neighbour of an elected variable (src/cpu/lcve.cpp:46-99).
 // That sequential greedy is reproduced exactly by a deterministic parallel
-// greedy-MIS over candidate ranks:
+// greedy maximal independent set over candidate ranks:
 //   * candidates before the static break position B0 (first non-skipped candidate
 //     that is hot by list size and cannot be frozen) are all freezable, and the
 //     "shares a live clause" relation between them is symmetric;
 //   * a candidate is elected iff no earlier-ranked elected candidate shares a live
-//     clause with it; rounds decide a candidate once every earlier neighbour is
-//     decided (or an earlier elected neighbour is found);
+//     clause with it;
 //   * a candidate that is hot by list size but freezable ends the election where
 //     the sequential loop would reach it unfrozen (dynamic break); it never freezes.
+// Rounds are clause-centric streaming passes over the live clauses that hold two
+// or more candidates:
+//   A. a clause with an elected (freezing) candidate marks its undecided candidates
+//      out; otherwise every undecided candidate ranked after the clause's earliest
+//      undecided freezing candidate is blocked for this round;
+//   B. every undecided candidate that is not blocked is elected.
+// The earliest undecided candidate is never blocked, so every round makes progress;
+// settled clauses and decided candidates are dropped between rounds.
 #include "device.hpp"
 #include "passes.hpp"
 #include "prims.hpp"
+#include <cstdio>
 
 namespace pfg {
 
@@ -39,7 +47,8 @@ __global__ void k_rank(const uint32_t *__restrict__ elig, uint32_t nvars, uint32
 
 __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, const uint32_t *__restrict__ hist,
                            const uint8_t *__restrict__ vstate, const OL *__restrict__ ot, uint32_t pt, uint32_t nt,
-                           uint8_t *__restrict__ kind, uint8_t *__restrict__ status, unsigned long long *cnt)
+                           uint8_t *__restrict__ kind, uint8_t *__restrict__ status, uint8_t *__restrict__ blocked,
+                           unsigned long long *cnt)
 {
     for (uint64_t i = gtid(); i < nvars; i += gstride()) {
         const uint32_t v = elig[i];
@@ -53,58 +62,126 @@ __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, co
         }
         kind[i] = k;
         status[i] = S_UNDEC;
+        blocked[i] = 0;
     }
 }
 
-__global__ void k_work_init(const uint8_t *__restrict__ kind, uint64_t b0, uint32_t *__restrict__ work, unsigned long long *cnt)
+__global__ void __launch_bounds__(256) k_work_init(const uint8_t *__restrict__ kind, uint64_t b0, uint32_t *__restrict__ work,
+                                                   unsigned long long *cnt)
 {
-    for (uint64_t i = gtid(); i < b0; i += gstride()) {
-        const uint8_t k = kind[i];
-        if (k == K_CAND || k == K_DYNBREAK) work[atomicAdd(&cnt[CNT_A], 1ull)] = (uint32_t)i;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < b0; b += gstride()) {
+        const uint64_t i = b + threadIdx.x;
+        const uint8_t k = i < b0 ? kind[i] : (uint8_t)K_SKIP;
+        wave_append(k == K_CAND || k == K_DYNBREAK, (uint32_t)i, work, &cnt[CNT_A]);
     }
 }
 
-// one wave per undecided candidate; lanes stride over the candidate's live clauses
-__global__ void __launch_bounds__(256) k_mis_round(const uint32_t *__restrict__ work, uint64_t nwork,
-                                                   const uint32_t *__restrict__ elig, const uint32_t *__restrict__ rank,
-                                                   const uint8_t *__restrict__ kind, uint8_t *status, const OL *__restrict__ ot,
-                                                   const uint64_t *__restrict__ occ, const uint32_t *__restrict__ mem,
-                                                   uint32_t *__restrict__ next, unsigned long long *cnt)
+struct ElectArgs {
+    const uint32_t *mem;
+    const uint32_t *rank;
+    const uint8_t *kind;
+    uint8_t *status, *blocked;
+    uint64_t b0;
+};
+
+__device__ __forceinline__ bool is_cand(const ElectArgs &A, uint32_t j)
 {
-    const int lane = __lane_id();
-    const uint64_t wid = gtid() >> 6, nw = gstride() >> 6;
-    for (uint64_t q = wid; q < nwork; q += nw) {
-        const uint32_t i = work[q];
-        const uint32_t v = elig[i];
-        bool in_found = false, pending = false;
-        for (int sg = 0; sg < 2 && !in_found; sg++) {
-            const OL o = ot[2 * v + sg];
-            for (uint32_t base = 0; base < o.sz; base += 64) {
-                const uint32_t k = base + lane;
-                if (k < o.sz) {
-                    const uint32_t *c = mem + occ[o.off + k];
-                    if (!c_del(c)) {
-                        const int sz = c_size(c);
-                        for (int t = 0; t < sz; t++) {
-                            const uint32_t u = c[3 + t] >> 1;
-                            if (u == v) continue;
-                            const uint32_t j = rank[u];
-                            if (j >= i || kind[j] != K_CAND) continue;
-                            const uint8_t st = __atomic_load_n(&status[j], __ATOMIC_RELAXED);
-                            if (st == S_IN) { in_found = true; break; }
-                            if (st == S_UNDEC) pending = true;
-                        }
-                    }
-                }
-                if (__ballot(in_found)) { in_found = true; break; }
+    return j < A.b0 && (A.kind[j] == K_CAND || A.kind[j] == K_DYNBREAK);
+}
+
+// live clauses with at least two candidates take part in the election
+__global__ void __launch_bounds__(256) k_elect_clauses(ElectArgs A, const uint64_t *__restrict__ refs, uint64_t n,
+                                                       uint64_t *__restrict__ active, unsigned long long *cnt)
+{
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
+        const uint64_t i = b + threadIdx.x;
+        bool keep = false;
+        uint64_t r = 0;
+        if (i < n) {
+            r = refs[i];
+            const uint32_t *c = A.mem + r;
+            if (!c_del(c)) {
+                const int sz = c_size(c);
+                uint32_t k = 0;
+                for (int t = 0; t < sz; t++) k += is_cand(A, A.rank[c[3 + t] >> 1]) ? 1u : 0u;
+                keep = k >= 2u;
             }
         }
-        pending = __ballot(pending) != 0;
-        if (lane == 0) {
-            if (in_found) __atomic_store_n(&status[i], (uint8_t)S_OUT, __ATOMIC_RELAXED);
-            else if (!pending) __atomic_store_n(&status[i], (uint8_t)S_IN, __ATOMIC_RELAXED);
-            else next[atomicAdd(&cnt[CNT_A], 1ull)] = i;
+        const uint64_t m = __ballot(keep);
+        if (m) {
+            const int lane = __lane_id(), leader = __ffsll((long long)m) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&cnt[CNT_C], (unsigned long long)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (keep) active[base + __popcll(m & ((1ull << lane) - 1ull))] = r;
         }
+    }
+}
+
+// pass A over the active clauses
+__global__ void __launch_bounds__(256) k_elect_pass_a(ElectArgs A, const uint64_t *__restrict__ active, uint64_t n,
+                                                      uint64_t *__restrict__ next, unsigned long long *cnt)
+{
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
+        const uint64_t i = b + threadIdx.x;
+        bool keep = false;
+        uint64_t r = 0;
+        if (i < n) {
+            r = active[i];
+            const uint32_t *c = A.mem + r;
+            const int sz = c_size(c);
+            bool in_freezer = false;
+            uint32_t m = 0xFFFFFFFFu;
+            int undec = 0, alive = 0;
+            for (int t = 0; t < sz; t++) {
+                const uint32_t j = A.rank[c[3 + t] >> 1];
+                if (!is_cand(A, j)) continue;
+                const uint8_t st = A.status[j];
+                const bool freezer = A.kind[j] == K_CAND;
+                if (st == S_IN && freezer) in_freezer = true;
+                if (st == S_UNDEC) {
+                    undec++;
+                    if (freezer && j < m) m = j;
+                }
+                if (st == S_UNDEC || (st == S_IN && freezer)) alive++;
+            }
+            if (undec) {
+                for (int t = 0; t < sz; t++) {
+                    const uint32_t j = A.rank[c[3 + t] >> 1];
+                    if (!is_cand(A, j) || A.status[j] != S_UNDEC) continue;
+                    if (in_freezer) A.status[j] = S_OUT;
+                    else if (j > m) A.blocked[j] = 1;
+                }
+            }
+            keep = !in_freezer && undec >= 1 && alive >= 2;
+        }
+        const uint64_t mk = __ballot(keep);
+        if (mk) {
+            const int lane = __lane_id(), leader = __ffsll((long long)mk) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&cnt[CNT_C], (unsigned long long)__popcll(mk));
+            base = __shfl(base, leader, 64);
+            if (keep) next[base + __popcll(mk & ((1ull << lane) - 1ull))] = r;
+        }
+    }
+}
+
+// pass B over the undecided candidates
+__global__ void __launch_bounds__(256) k_elect_pass_b(ElectArgs A, const uint32_t *__restrict__ work, uint64_t n,
+                                                      uint32_t *__restrict__ next, unsigned long long *cnt)
+{
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
+        const uint64_t q = b + threadIdx.x;
+        bool keep = false;
+        uint32_t j = 0;
+        if (q < n) {
+            j = work[q];
+            if (A.status[j] == S_UNDEC) {
+                if (A.blocked[j]) { A.blocked[j] = 0; keep = true; }
+                else A.status[j] = S_IN;
+            }
+        }
+        wave_append(keep, j, next, &cnt[CNT_A]);
     }
 }
 
@@ -124,14 +201,16 @@ __global__ void k_elect_flags(const uint8_t *__restrict__ kind, const uint8_t *_
 uint32_t lcve(Ctx &c)
 {
     const uint32_t nv = c.nvars;
-    hist(c);   // histSimp after prop (src/cpu/lcve.cpp:30)
+    if (!c.hist_live) hist(c);   // histSimp after prop (src/cpu/lcve.cpp:30)
     ensure(c, c.scores, nv + 1);
     ensure(c, c.eligible, std::max<uint64_t>(c.nlits2, nv + 1));
     ensure(c, c.rank, nv + 1);
-    ensure(c, c.tmpu8, 2ull * nv + 2);
+    ensure(c, c.tmpu8, 3ull * nv + 3);
     ensure(c, c.tmpu32a, nv + 1);
     ensure(c, c.tmpu32b, nv + 1);
     ensure(c, c.pvs, nv + 1);
+    ensure(c, c.tmpu64a, c.nrefs + 1);
+    ensure(c, c.tmpu64b, c.nrefs + 1);
     k_scores<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.hist.p, nv, c.scores.p, c.eligible.p);
     const uint32_t maxs = reduce_max_u32(c, c.scores.p, nv);
     int bits = 0;
@@ -141,29 +220,49 @@ uint32_t lcve(Ctx &c)
         radix_sort_u32_pairs(c, c.scores.p, c.eligible.p, nv, ((bits + 7) / 8) * 8);
     }
     k_rank<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, c.rank.p);
-    uint8_t *kind = c.tmpu8.p, *status = c.tmpu8.p + nv + 1;
+    uint8_t *kind = c.tmpu8.p, *status = kind + nv + 1, *blocked = status + nv + 1;
     const uint32_t pt = c.opts.mu_pos << c.mu_inc, nt = c.opts.mu_neg << c.mu_inc;
-    const unsigned long long init[2] = {0ull, (unsigned long long)nv};
-    PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_A, init, 16, hipMemcpyHostToDevice, c.stream));
+    const unsigned long long init[3] = {0ull, (unsigned long long)nv, 0ull};
+    PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_A, init, 24, hipMemcpyHostToDevice, c.stream));
     k_classify<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, c.hist.p, c.vstate.p, c.ot.p, pt, nt,
-                                                              kind, status, (unsigned long long *)c.cnt.p);
+                                                              kind, status, blocked, (unsigned long long *)c.cnt.p);
     read_counters(c, 3);
     const uint64_t b0 = c.hcnt[CNT_B];
-    k_work_init<<<grid_for(b0, 256, 8192), 256, 0, c.stream>>>(kind, b0, c.tmpu32a.p, (unsigned long long *)c.cnt.p);
-    read_counters(c, 2);
-    uint64_t nwork = c.hcnt[CNT_A];
-    uint32_t *work = c.tmpu32a.p, *next = c.tmpu32b.p;
+    uint32_t *work = c.tmpu32a.p, *work2 = c.tmpu32b.p;
+    uint64_t *act = c.tmpu64a.p, *act2 = c.tmpu64b.p;
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 24, c.stream));
+    k_work_init<<<grid_for(b0, 256, 8192), 256, 0, c.stream>>>(kind, b0, work, (unsigned long long *)c.cnt.p);
+    ElectArgs A{c.mem.p, c.rank.p, kind, status, blocked, b0};
+    if (c.nrefs && b0) {
+        KTimer kt(c, K_LCVE_MIS, 20.0 * c.nrefs + 8.0 * c.nliterals);
+        k_elect_clauses<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(A, c.refs.p, c.nrefs, act,
+                                                                             (unsigned long long *)c.cnt.p);
+    }
+    read_counters(c, 4);
+    uint64_t nwork = c.hcnt[CNT_A], nact = c.hcnt[CNT_C];
+    if (c.trace)
+        fprintf(stderr, "[pfg] LCVE b0 %llu candidates %llu clauses %llu\n", (unsigned long long)b0, (unsigned long long)nwork,
+                (unsigned long long)nact);
     int rounds = 0;
     while (nwork) {
-        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 8, c.stream));
-        KTimer kt(c, K_LCVE_MIS, 0.0);
-        k_mis_round<<<grid_for(nwork * 64, 256, 8192), 256, 0, c.stream>>>(work, nwork, c.eligible.p, c.rank.p, kind, status,
-                                                                            c.ot.p, c.occ.p, c.mem.p, next,
-                                                                            (unsigned long long *)c.cnt.p);
-        read_counters(c, 2);
+        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 24, c.stream));
+        if (nact) {
+            KTimer kt(c, K_LCVE_MIS, 0.0);
+            k_elect_pass_a<<<grid_for(nact, 256, 16384), 256, 0, c.stream>>>(A, act, nact, act2, (unsigned long long *)c.cnt.p);
+        }
+        {
+            KTimer kt(c, K_LCVE_MIS, 0.0);
+            k_elect_pass_b<<<grid_for(nwork, 256, 16384), 256, 0, c.stream>>>(A, work, nwork, work2, (unsigned long long *)c.cnt.p);
+        }
+        read_counters(c, 4);
+        if (c.trace) fprintf(stderr, "[pfg] LCVE round %d: undecided %llu -> %llu, clauses %llu -> %llu\n", rounds,
+                             (unsigned long long)nwork, (unsigned long long)c.hcnt[CNT_A], (unsigned long long)nact,
+                             (unsigned long long)c.hcnt[CNT_C]);
         nwork = c.hcnt[CNT_A];
-        std::swap(work, next);
-        if (++rounds > (1 << 22)) throw Error(PFG_EINTERNAL, "LCVE election did not converge");
+        nact = c.hcnt[CNT_C];
+        std::swap(work, work2);
+        std::swap(act, act2);
+        if (++rounds > (1 << 24)) throw Error(PFG_EINTERNAL, "LCVE election did not converge");
     }
     const unsigned long long bb = b0;
     PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_B, &bb, 8, hipMemcpyHostToDevice, c.stream));
@@ -172,6 +271,7 @@ uint32_t lcve(Ctx &c)
     const uint64_t brk = c.hcnt[CNT_B];
     k_elect_flags<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(kind, status, brk, nv, c.tmpu32a.p);
     c.npvs = (uint32_t)compact_u32(c, c.eligible.p, c.tmpu32a.p, nv, c.pvs.p);
+    c.hist_live = false;
     return c.npvs;
 }
 

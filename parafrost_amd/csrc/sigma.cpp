@@ -7,6 +7,7 @@
 #include <vector>
 #include <algorithm>
 #include <new>
+#include <cstdlib>
 
 namespace pfg {
 
@@ -99,7 +100,7 @@ struct Sigma {
         const int times = c.phase + 1;
         if (times > 1 && times != c.opts.phases && c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0) shrinkSimp();
     }
-    void shrinkSimp() { PassTimer t(c, &c.times.gc); shrink(c); }
+    void shrinkSimp() { PassTimer t(c, &c.times.gc); c.hist_live = false; shrink(c); }
     void createOT() { PassTimer t(c, &c.times.ot); create_ot(c); }
     bool propagate() { PassTimer t(c, &c.times.prop); return prop(c); }
     bool LCVE()
@@ -109,9 +110,9 @@ struct Sigma {
     }
     void sortOT() { PassTimer t(c, &c.times.sort_ot); sort_ot(c); }
     bool stop(int64_t lr) const { return (c.phase == c.opts.phases) || (lr <= c.opts.lits_min && c.phase > 2); }
-    void ERE() { PassTimer t(c, &c.times.ere); ere(c); }
-    void SUB() { if (c.opts.sub_en || c.opts.ve_en) { PassTimer t(c, &c.times.sub); sub(c); } }
-    void VE() { if (c.opts.ve_en) { PassTimer t(c, &c.times.ve); ve(c); } }
+    void ERE() { PassTimer t(c, &c.times.ere); c.hist_live = false; ere(c); }
+    void SUB() { c.hist_live = false; if (c.opts.sub_en || c.opts.ve_en) { PassTimer t(c, &c.times.sub); sub(c); } }
+    void VE() { c.hist_live = false; if (c.opts.ve_en) { PassTimer t(c, &c.times.ve); ve(c); } }
     void countAll(int64_t &cls, int64_t &lits) { PassTimer t(c, &c.times.count); count_all(c, cls, lits); }
     void filterPVs() { filter_pvs(c); }
     void nextPhase()
@@ -203,6 +204,7 @@ int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, c
     Ctx &c = ctx->c;
     c.device = device;
     c.budget = hbm_budget;
+    c.trace = getenv("PFG_TRACE") != nullptr;
     c.nvars = nvars;
     c.nlits2 = 2ull * ((uint64_t)nvars + 1);
     if (opts) c.opts = *opts; else pfg_default_opts(&c.opts);
@@ -243,7 +245,7 @@ int pfg_destroy(pfg_ctx *ctx)
     release(c, c.mem); release(c, c.mem2); release(c, c.refs); release(c, c.refs2);
     release(c, c.ot); release(c, c.occ); release(c, c.hist); release(c, c.value); release(c, c.vstate);
     release(c, c.trail); release(c, c.resolved); release(c, c.pvs); release(c, c.units);
-    release(c, c.scores); release(c, c.eligible); release(c, c.rank); release(c, c.tmpu32a); release(c, c.tmpu32b);
+    release(c, c.scores); release(c, c.eligible); release(c, c.rank); release(c, c.tmpu32a); release(c, c.tmpu32b); release(c, c.tmpu32c); release(c, c.tmpu32d);
     release(c, c.tmpu8); release(c, c.tmpu64a); release(c, c.tmpu64b); release(c, c.tmpu64c);
     release(c, c.owner); release(c, c.claim); release(c, c.ufirst); release(c, c.cnt);
     release(c, c.scan_tmp); release(c, c.radix_tmp);
@@ -502,6 +504,7 @@ int pfg_reset(pfg_ctx *ctx)
     c.nclauses = c.nliterals = 0;
     c.unsat = false;
     c.ot_valid = false;
+    c.hist_live = false;
     API_END
 }
 

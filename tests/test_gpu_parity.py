@@ -75,13 +75,19 @@ def test_lockstep_learnt_clauses(seed):
     hdr = np.where(rng.random(cnf.ncls) < 0.3, 1 | (rng.integers(1, 30, cnf.ncls) << 6) | (1 << 4), 0).astype(np.uint32)
     oo, go = ls._opts_pair(**kw)
     o = po.Oracle(cnf, oo, hdr0=hdr)
-    o.sigmify()
+    ro = o.sigmify()
     import parafrost_amd as pf
     g = pf.Simplifier(cnf.nvars, go)
     g.upload(cnf, hdr0=hdr)
     try:
-        g.sigmify()
-        ls.compare(o.state(), ls.state_gpu(g), "learnt sigmify")
+        try:
+            g.sigmify()
+            rg = 0
+        except pf.Unsat:
+            rg = -1
+        assert (ro != 0) == (rg != 0), f"unsat oracle {ro} gpu {rg}"
+        if ro == 0:
+            ls.compare(o.state(), ls.state_gpu(g), "learnt sigmify")
     finally:
         g.close()
 
