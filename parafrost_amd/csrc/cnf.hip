@@ -251,7 +251,7 @@ static void order_lists_by_ref(Ctx &c)
 
 void create_ot(Ctx &c)
 {
-    hist(c);
+    if (!c.hist_live) hist(c);
     ensure(c, c.ot, c.nlits2);
     ensure(c, c.tmpu64a, c.nlits2);
     scan_u32_to_u64(c, c.hist.p, c.tmpu64a.p, c.nlits2, c.cnt.p + CNT_TOTAL);
@@ -382,28 +382,43 @@ bool prop(Ctx &c)
 }
 
 // ------------------------------------------------------------------ counting
-__global__ void k_count(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n, unsigned long long *cnt)
+// countAll; optionally also the literal histogram of the live clauses, which the
+// next round's occurrence table then reuses
+__global__ void k_count(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n, unsigned long long *cnt,
+                        uint32_t *__restrict__ hist)
 {
     unsigned long long nc = 0, nl = 0;
     for (uint64_t i = gtid(); i < n; i += gstride()) {
         const uint32_t *c = mem + refs[i];
-        if (!c_del(c)) { nc++; nl += (unsigned)c_size(c); }
+        if (!c_del(c)) {
+            const int sz = c_size(c);
+            nc++;
+            nl += (unsigned)sz;
+            if (hist)
+                for (int k = 0; k < sz; k++) atomicAdd(&hist[c[3 + k]], 1u);
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) { nc += __shfl_down(nc, o, 64); nl += __shfl_down(nl, o, 64); }
     if (__lane_id() == 0) { atomicAdd(&cnt[CNT_A], nc); atomicAdd(&cnt[CNT_B], nl); }
 }
 
-void count_all(Ctx &c, int64_t &cls, int64_t &lits)
+void count_all(Ctx &c, int64_t &cls, int64_t &lits, bool with_hist)
 {
     PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));
+    if (with_hist) {
+        ensure(c, c.hist, c.nlits2);
+        PFG_HIP(hipMemsetAsync(c.hist.p, 0, c.nlits2 * 4, c.stream));
+    }
     if (c.nrefs) {
         KTimer kt(c, K_COUNT, 20.0 * c.nrefs + 4.0 * c.nliterals);
-        k_count<<<grid_for(c.nrefs, 256, 4096), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, (unsigned long long *)c.cnt.p);
+        k_count<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, (unsigned long long *)c.cnt.p,
+                                                                      with_hist ? c.hist.p : nullptr);
     }
     read_counters(c, 3);
     cls = (int64_t)c.hcnt[CNT_A];
     lits = (int64_t)c.hcnt[CNT_B];
+    if (with_hist) c.hist_live = true;
 }
 
 // ------------------------------------------------------------------ garbage collection
@@ -423,16 +438,14 @@ __global__ void k_gc_move(const uint32_t *__restrict__ mem, const uint64_t *__re
                           const uint64_t *__restrict__ woff, const uint64_t *__restrict__ idx,
                           const uint32_t *__restrict__ live, uint32_t *__restrict__ mem2, uint64_t *__restrict__ refs2)
 {
-    // one wave per clause slot group: lanes copy words
-    const int lane = __lane_id();
-    const uint64_t wid = gtid() >> 6, nw = gstride() >> 6;
-    for (uint64_t i = wid; i < n; i += nw) {
+    // one thread per clause: neighbouring threads copy neighbouring clauses
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
         if (!live[i]) continue;
         const uint32_t *c = mem + refs[i];
         const uint64_t words = 3ull + (uint64_t)c_size(c);
         uint32_t *d = mem2 + woff[i];
-        for (uint64_t k = lane; k < words; k += 64) d[k] = c[k];
-        if (lane == 0) refs2[idx[i]] = woff[i];
+        for (uint64_t k = 0; k < words; k++) d[k] = c[k];
+        refs2[idx[i]] = woff[i];
     }
 }
 
@@ -453,7 +466,7 @@ void shrink(Ctx &c)
     ensure(c, c.refs2, std::max<uint64_t>(c.refs.cap, live + 1));
     {
         KTimer kt(c, K_GC, 8.0 * words + 36.0 * n);
-        k_gc_move<<<grid_for(n * 64, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu64b.p, c.tmpu64c.p,
+        k_gc_move<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu64b.p, c.tmpu64c.p,
                                                                        c.tmpu32a.p, c.mem2.p, c.refs2.p);
     }
     std::swap(c.mem, c.mem2);

@@ -98,7 +98,7 @@ __device__ __forceinline__ void sort_u32_small(uint32_t *d, int n)
 }
 
 // device counter slots (Ctx::cnt) and flags
-enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_NTRAIL = 8, CNT_FLAGS = 9 };
+enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_NTRAIL = 8, CNT_FLAGS = 9 };
 enum { FLAG_UNSAT = 1, FLAG_INTERNAL = 2 };
 
 // returns 1 if lit became true, 0 if it was true already, -1 if it was false
@@ -140,6 +140,43 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t value, uint32_t 
     if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
     base = __shfl(base, leader, 64);
     if (flag) out[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+// Block-wide reservation of output slots: every thread asks for `want` slots and
+// gets the offset of its own run; one atomic per block and call. Every thread of
+// the block must call it (blockDim.x a multiple of 64, at most 1024).
+__device__ __forceinline__ uint64_t block_reserve(uint64_t want, unsigned long long *counter)
+{
+    __shared__ unsigned long long s_wave[16];
+    __shared__ unsigned long long s_base;
+    const int lane = __lane_id(), w = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    uint64_t incl = want;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) s_wave[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int q = 0; q < nwaves; q++) {
+            const unsigned long long t = s_wave[q];
+            s_wave[q] = run;
+            run += t;
+        }
+        s_base = run ? atomicAdd(counter, run) : 0ull;
+    }
+    __syncthreads();
+    const uint64_t off = s_base + s_wave[w] + incl - want;
+    __syncthreads();   // s_wave / s_base may be reused by the next call
+    return off;
+}
+
+__device__ __forceinline__ void block_append(bool flag, uint32_t value, uint32_t *out, unsigned long long *counter)
+{
+    const uint64_t o = block_reserve(flag ? 1ull : 0ull, counter);
+    if (flag) out[o] = value;
 }
 
 }  // namespace pfg

@@ -66,13 +66,13 @@ __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, co
     }
 }
 
-__global__ void __launch_bounds__(256) k_work_init(const uint8_t *__restrict__ kind, uint64_t b0, uint32_t *__restrict__ work,
+__global__ void __launch_bounds__(1024) k_work_init(const uint8_t *__restrict__ kind, uint64_t b0, uint32_t *__restrict__ work,
                                                    unsigned long long *cnt)
 {
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < b0; b += gstride()) {
         const uint64_t i = b + threadIdx.x;
         const uint8_t k = i < b0 ? kind[i] : (uint8_t)K_SKIP;
-        wave_append(k == K_CAND || k == K_DYNBREAK, (uint32_t)i, work, &cnt[CNT_A]);
+        block_append(k == K_CAND || k == K_DYNBREAK, (uint32_t)i, work, &cnt[CNT_A]);
     }
 }
 
@@ -89,37 +89,53 @@ __device__ __forceinline__ bool is_cand(const ElectArgs &A, uint32_t j)
     return j < A.b0 && (A.kind[j] == K_CAND || A.kind[j] == K_DYNBREAK);
 }
 
-// live clauses with at least two candidates take part in the election
-__global__ void __launch_bounds__(256) k_elect_clauses(ElectArgs A, const uint64_t *__restrict__ refs, uint64_t n,
-                                                       uint64_t *__restrict__ active, unsigned long long *cnt)
+// Live clauses with at least two candidates take part in the election. Each is
+// written once as a compact record [k, rank_1 .. rank_k] of its candidates, so the
+// rounds stream records instead of clauses and rank lookups. Round 0's pass A is
+// fused in: nothing is elected yet, so every candidate ranked after the clause's
+// earliest freezing candidate is blocked.
+__global__ void __launch_bounds__(1024) k_elect_init(ElectArgs A, const uint64_t *__restrict__ refs, uint64_t n,
+                                                    uint32_t *__restrict__ rec, uint64_t *__restrict__ active,
+                                                    unsigned long long *cnt)
 {
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
         const uint64_t i = b + threadIdx.x;
-        bool keep = false;
-        uint64_t r = 0;
+        const uint32_t *c = nullptr;
+        int sz = 0;
+        uint32_t k = 0, m = 0xFFFFFFFFu;
         if (i < n) {
-            r = refs[i];
-            const uint32_t *c = A.mem + r;
+            c = A.mem + refs[i];
             if (!c_del(c)) {
-                const int sz = c_size(c);
-                uint32_t k = 0;
-                for (int t = 0; t < sz; t++) k += is_cand(A, A.rank[c[3 + t] >> 1]) ? 1u : 0u;
-                keep = k >= 2u;
+                sz = c_size(c);
+                for (int t = 0; t < sz; t++) {
+                    const uint32_t j = A.rank[c[3 + t] >> 1];
+                    if (!is_cand(A, j)) continue;
+                    k++;
+                    if (A.kind[j] == K_CAND && j < m) m = j;
+                }
             }
         }
-        const uint64_t m = __ballot(keep);
-        if (m) {
-            const int lane = __lane_id(), leader = __ffsll((long long)m) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&cnt[CNT_C], (unsigned long long)__popcll(m));
-            base = __shfl(base, leader, 64);
-            if (keep) active[base + __popcll(m & ((1ull << lane) - 1ull))] = r;
+        const bool keep = k >= 2u;
+        const uint64_t off = block_reserve(keep ? 1ull + k : 0ull, &cnt[CNT_D]);
+        const uint64_t aoff = block_reserve(keep ? 1ull : 0ull, &cnt[CNT_C]);
+        if (keep) {
+            uint32_t *r = rec + off;
+            r[0] = k;
+            int w = 1;
+            for (int t = 0; t < sz; t++) {
+                const uint32_t j = A.rank[c[3 + t] >> 1];
+                if (!is_cand(A, j)) continue;
+                r[w++] = j;
+                if (j > m) A.blocked[j] = 1;
+            }
+            active[aoff] = off;
         }
     }
 }
 
-// pass A over the active clauses
-__global__ void __launch_bounds__(256) k_elect_pass_a(ElectArgs A, const uint64_t *__restrict__ active, uint64_t n,
+// pass A over the active clause records
+__global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32_t *__restrict__ rec,
+                                                      const uint64_t *__restrict__ active, uint64_t n,
                                                       uint64_t *__restrict__ next, unsigned long long *cnt)
 {
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
@@ -128,14 +144,13 @@ __global__ void __launch_bounds__(256) k_elect_pass_a(ElectArgs A, const uint64_
         uint64_t r = 0;
         if (i < n) {
             r = active[i];
-            const uint32_t *c = A.mem + r;
-            const int sz = c_size(c);
+            const uint32_t *rr = rec + r;
+            const uint32_t k = rr[0];
             bool in_freezer = false;
             uint32_t m = 0xFFFFFFFFu;
-            int undec = 0, alive = 0;
-            for (int t = 0; t < sz; t++) {
-                const uint32_t j = A.rank[c[3 + t] >> 1];
-                if (!is_cand(A, j)) continue;
+            uint32_t undec = 0, alive = 0;
+            for (uint32_t t = 1; t <= k; t++) {
+                const uint32_t j = rr[t];
                 const uint8_t st = A.status[j];
                 const bool freezer = A.kind[j] == K_CAND;
                 if (st == S_IN && freezer) in_freezer = true;
@@ -146,28 +161,22 @@ __global__ void __launch_bounds__(256) k_elect_pass_a(ElectArgs A, const uint64_
                 if (st == S_UNDEC || (st == S_IN && freezer)) alive++;
             }
             if (undec) {
-                for (int t = 0; t < sz; t++) {
-                    const uint32_t j = A.rank[c[3 + t] >> 1];
-                    if (!is_cand(A, j) || A.status[j] != S_UNDEC) continue;
+                for (uint32_t t = 1; t <= k; t++) {
+                    const uint32_t j = rr[t];
+                    if (A.status[j] != S_UNDEC) continue;
                     if (in_freezer) A.status[j] = S_OUT;
                     else if (j > m) A.blocked[j] = 1;
                 }
             }
-            keep = !in_freezer && undec >= 1 && alive >= 2;
+            keep = !in_freezer && undec >= 1u && alive >= 2u;
         }
-        const uint64_t mk = __ballot(keep);
-        if (mk) {
-            const int lane = __lane_id(), leader = __ffsll((long long)mk) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&cnt[CNT_C], (unsigned long long)__popcll(mk));
-            base = __shfl(base, leader, 64);
-            if (keep) next[base + __popcll(mk & ((1ull << lane) - 1ull))] = r;
-        }
+        const uint64_t o = block_reserve(keep ? 1ull : 0ull, &cnt[CNT_C]);
+        if (keep) next[o] = r;
     }
 }
 
 // pass B over the undecided candidates
-__global__ void __launch_bounds__(256) k_elect_pass_b(ElectArgs A, const uint32_t *__restrict__ work, uint64_t n,
+__global__ void __launch_bounds__(1024) k_elect_pass_b(ElectArgs A, const uint32_t *__restrict__ work, uint64_t n,
                                                       uint32_t *__restrict__ next, unsigned long long *cnt)
 {
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
@@ -181,7 +190,7 @@ __global__ void __launch_bounds__(256) k_elect_pass_b(ElectArgs A, const uint32_
                 else A.status[j] = S_IN;
             }
         }
-        wave_append(keep, j, next, &cnt[CNT_A]);
+        block_append(keep, j, next, &cnt[CNT_A]);
     }
 }
 
@@ -211,6 +220,7 @@ uint32_t lcve(Ctx &c)
     ensure(c, c.pvs, nv + 1);
     ensure(c, c.tmpu64a, c.nrefs + 1);
     ensure(c, c.tmpu64b, c.nrefs + 1);
+    ensure(c, c.erec, c.msz + 1);
     k_scores<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.hist.p, nv, c.scores.p, c.eligible.p);
     const uint32_t maxs = reduce_max_u32(c, c.scores.p, nv);
     int bits = 0;
@@ -230,38 +240,43 @@ uint32_t lcve(Ctx &c)
     const uint64_t b0 = c.hcnt[CNT_B];
     uint32_t *work = c.tmpu32a.p, *work2 = c.tmpu32b.p;
     uint64_t *act = c.tmpu64a.p, *act2 = c.tmpu64b.p;
-    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 24, c.stream));
-    k_work_init<<<grid_for(b0, 256, 8192), 256, 0, c.stream>>>(kind, b0, work, (unsigned long long *)c.cnt.p);
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 32, c.stream));
+    k_work_init<<<grid_for(b0, 1024, 4096), 1024, 0, c.stream>>>(kind, b0, work, (unsigned long long *)c.cnt.p);
     ElectArgs A{c.mem.p, c.rank.p, kind, status, blocked, b0};
     if (c.nrefs && b0) {
         KTimer kt(c, K_LCVE_MIS, 20.0 * c.nrefs + 8.0 * c.nliterals);
-        k_elect_clauses<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(A, c.refs.p, c.nrefs, act,
-                                                                             (unsigned long long *)c.cnt.p);
+        k_elect_init<<<grid_for(c.nrefs, 1024, 8192), 1024, 0, c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act,
+                                                                          (unsigned long long *)c.cnt.p);
     }
-    read_counters(c, 4);
+    read_counters(c, 5);
     uint64_t nwork = c.hcnt[CNT_A], nact = c.hcnt[CNT_C];
     if (c.trace)
         fprintf(stderr, "[pfg] LCVE b0 %llu candidates %llu clauses %llu\n", (unsigned long long)b0, (unsigned long long)nwork,
                 (unsigned long long)nact);
     int rounds = 0;
+    bool fused = true;   // round 0's pass A ran inside k_elect_init
     while (nwork) {
         PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 24, c.stream));
-        if (nact) {
+        if (!fused && nact) {
             KTimer kt(c, K_LCVE_MIS, 0.0);
-            k_elect_pass_a<<<grid_for(nact, 256, 16384), 256, 0, c.stream>>>(A, act, nact, act2, (unsigned long long *)c.cnt.p);
+            k_elect_pass_a<<<grid_for(nact, 1024, 8192), 1024, 0, c.stream>>>(A, c.erec.p, act, nact, act2,
+                                                                              (unsigned long long *)c.cnt.p);
         }
         {
             KTimer kt(c, K_LCVE_MIS, 0.0);
-            k_elect_pass_b<<<grid_for(nwork, 256, 16384), 256, 0, c.stream>>>(A, work, nwork, work2, (unsigned long long *)c.cnt.p);
+            k_elect_pass_b<<<grid_for(nwork, 1024, 8192), 1024, 0, c.stream>>>(A, work, nwork, work2, (unsigned long long *)c.cnt.p);
         }
         read_counters(c, 4);
         if (c.trace) fprintf(stderr, "[pfg] LCVE round %d: undecided %llu -> %llu, clauses %llu -> %llu\n", rounds,
                              (unsigned long long)nwork, (unsigned long long)c.hcnt[CNT_A], (unsigned long long)nact,
                              (unsigned long long)c.hcnt[CNT_C]);
         nwork = c.hcnt[CNT_A];
-        nact = c.hcnt[CNT_C];
+        if (!fused) {
+            nact = c.hcnt[CNT_C];
+            std::swap(act, act2);
+        }
+        fused = false;
         std::swap(work, work2);
-        std::swap(act, act2);
         if (++rounds > (1 << 24)) throw Error(PFG_EINTERNAL, "LCVE election did not converge");
     }
     const unsigned long long bb = b0;

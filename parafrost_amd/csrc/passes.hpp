@@ -11,7 +11,7 @@ void hist(Ctx &c);                       // c.hist = live literal counts
 void create_ot(Ctx &c);                  // histogram, list layout, fill in clause order
 void reduce_ot(Ctx &c);
 bool prop(Ctx &c);                       // false on conflict
-void count_all(Ctx &c, int64_t &cls, int64_t &lits);
+void count_all(Ctx &c, int64_t &cls, int64_t &lits, bool with_hist = false);
 void shrink(Ctx &c);                     // stable compaction of live clauses
 void filter_pvs(Ctx &c);
 void sort_ot(Ctx &c);                    // PV lists by the reference clause key

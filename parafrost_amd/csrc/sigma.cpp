@@ -100,7 +100,7 @@ struct Sigma {
         const int times = c.phase + 1;
         if (times > 1 && times != c.opts.phases && c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0) shrinkSimp();
     }
-    void shrinkSimp() { PassTimer t(c, &c.times.gc); c.hist_live = false; shrink(c); }
+    void shrinkSimp() { PassTimer t(c, &c.times.gc); shrink(c); }   // live counts do not change
     void createOT() { PassTimer t(c, &c.times.ot); create_ot(c); }
     bool propagate() { PassTimer t(c, &c.times.prop); return prop(c); }
     bool LCVE()
@@ -113,7 +113,11 @@ struct Sigma {
     void ERE() { PassTimer t(c, &c.times.ere); c.hist_live = false; ere(c); }
     void SUB() { c.hist_live = false; if (c.opts.sub_en || c.opts.ve_en) { PassTimer t(c, &c.times.sub); sub(c); } }
     void VE() { c.hist_live = false; if (c.opts.ve_en) { PassTimer t(c, &c.times.ve); ve(c); } }
-    void countAll(int64_t &cls, int64_t &lits) { PassTimer t(c, &c.times.count); count_all(c, cls, lits); }
+    void countAll(int64_t &cls, int64_t &lits, bool with_hist = false)
+    {
+        PassTimer t(c, &c.times.count);
+        count_all(c, cls, lits, with_hist);
+    }
     void filterPVs() { filter_pvs(c); }
     void nextPhase()
     {
@@ -139,7 +143,7 @@ struct Sigma {
             VE();
             if (c.unsat) return PFG_EUNSAT;
             int64_t nc, nl;
-            countAll(nc, nl);
+            countAll(nc, nl, true);   // also the next round's histogram
             filterPVs();
             c.nclauses = nc;
             c.nliterals = nl;
@@ -247,7 +251,7 @@ int pfg_destroy(pfg_ctx *ctx)
     release(c, c.trail); release(c, c.resolved); release(c, c.pvs); release(c, c.units);
     release(c, c.scores); release(c, c.eligible); release(c, c.rank); release(c, c.tmpu32a); release(c, c.tmpu32b); release(c, c.tmpu32c); release(c, c.tmpu32d);
     release(c, c.tmpu8); release(c, c.tmpu64a); release(c, c.tmpu64b); release(c, c.tmpu64c);
-    release(c, c.owner); release(c, c.claim); release(c, c.ufirst); release(c, c.cnt);
+    release(c, c.owner); release(c, c.claim); release(c, c.erec); release(c, c.ufirst); release(c, c.cnt);
     release(c, c.scan_tmp); release(c, c.radix_tmp);
     if (c.hcnt) (void)hipHostFree(c.hcnt);
     for (auto &e : c.kev) { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); }
