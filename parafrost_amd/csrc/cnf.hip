@@ -249,8 +249,100 @@ static void order_lists_by_ref(Ctx &c)
     }
 }
 
+// ---- occurrence table by a stable radix sort of (literal, reference) pairs:
+// pairs are laid out in clause order, so sorting them by literal leaves every list
+// in clause order (the createOT push order) without atomics; list bounds and the
+// live-literal histogram fall out of the sorted keys.
+__global__ void k_live_sizes(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                             uint32_t *__restrict__ out)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t *c = mem + refs[i];
+        out[i] = c_del(c) ? 0u : (uint32_t)c_size(c);
+    }
+}
+
+__global__ void k_ot_expand(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                            const uint64_t *__restrict__ pos, uint64_t *__restrict__ keys)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint64_t r = refs[i];
+        const uint32_t *c = mem + r;
+        if (c_del(c)) continue;
+        const int sz = c_size(c);
+        uint64_t *k = keys + pos[i];
+        for (int t = 0; t < sz; t++) k[t] = ((uint64_t)c[3 + t] << 32) | r;
+    }
+}
+
+// segment starts go to OL.off, segment ends (exclusive) to OL.cap for now
+__global__ void k_ot_bounds(const uint64_t *__restrict__ keys, uint64_t L, OL *__restrict__ ot, uint64_t *__restrict__ occ)
+{
+    for (uint64_t k = gtid(); k < L; k += gstride()) {
+        const uint64_t key = keys[k];
+        const uint32_t lit = (uint32_t)(key >> 32);
+        occ[k] = key & 0xFFFFFFFFull;
+        if (k == 0 || (uint32_t)(keys[k - 1] >> 32) != lit) ot[lit].off = k;
+        if (k + 1 == L || (uint32_t)(keys[k + 1] >> 32) != lit) ot[lit].cap = (uint32_t)(k + 1);
+    }
+}
+
+__global__ void k_ot_finish(OL *__restrict__ ot, uint32_t *__restrict__ hist, uint64_t n)
+{
+    for (uint64_t l = gtid(); l < n; l += gstride()) {
+        OL o = ot[l];
+        const uint32_t sz = o.cap ? (uint32_t)(o.cap - o.off) : 0u;
+        o.sz = o.cap = sz;
+        ot[l] = o;
+        hist[l] = sz;
+    }
+}
+
+static int bits_of(uint64_t v)
+{
+    int b = 0;
+    while (b < 64 && (v >> b)) b++;
+    return b;
+}
+
+static void create_ot_radix(Ctx &c)
+{
+    const uint64_t n = c.nrefs;
+    ensure(c, c.ot, c.nlits2);
+    ensure(c, c.hist, c.nlits2);
+    ensure(c, c.tmpu32a, n + 1);
+    ensure(c, c.tmpu64a, n + 1);
+    k_live_sizes<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu32a.p);
+    scan_u32_to_u64(c, c.tmpu32a.p, c.tmpu64a.p, n, c.cnt.p + CNT_TOTAL);
+    read_counters(c, 1);
+    const uint64_t L = c.hcnt[CNT_TOTAL];
+    ensure(c, c.occ, L + 1024);
+    ensure(c, c.tmpu64b, L + 1024);
+    PFG_HIP(hipMemsetAsync(c.ot.p, 0, c.nlits2 * sizeof(OL), c.stream));
+    if (L) {
+        {
+            KTimer kt(c, K_OT_FILL, 20.0 * n + 12.0 * L);
+            k_ot_expand<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu64a.p, c.tmpu64b.p);
+        }
+        {
+            const int lbits = bits_of(c.nlits2 - 1);
+            KTimer kt(c, K_OT_SORT, 32.0 * L * ((lbits + 7) / 8));
+            radix_sort_u64_keys(c, c.tmpu64b.p, L, lbits, 32);
+            k_ot_bounds<<<grid_for(L, 256, 16384), 256, 0, c.stream>>>(c.tmpu64b.p, L, c.ot.p, c.occ.p);
+        }
+    }
+    k_ot_finish<<<grid_for(c.nlits2, 256, 8192), 256, 0, c.stream>>>(c.ot.p, c.hist.p, c.nlits2);
+    c.ot_valid = true;
+    c.hist_live = true;
+}
+
 void create_ot(Ctx &c)
 {
+    if (c.msz < (1ull << 32) && c.msz + 3 * c.nrefs < (1ull << 32)) {
+        create_ot_radix(c);
+        return;
+    }
+    // references beyond 32 bits: histogram, atomic fill, per-list ordering
     if (!c.hist_live) hist(c);
     ensure(c, c.ot, c.nlits2);
     ensure(c, c.tmpu64a, c.nlits2);
@@ -398,9 +490,9 @@ __global__ void k_count(const uint32_t *__restrict__ mem, const uint64_t *__rest
                 for (int k = 0; k < sz; k++) atomicAdd(&hist[c[3 + k]], 1u);
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) { nc += __shfl_down(nc, o, 64); nl += __shfl_down(nl, o, 64); }
-    if (__lane_id() == 0) { atomicAdd(&cnt[CNT_A], nc); atomicAdd(&cnt[CNT_B], nl); }
+    nc = block_sum(nc);
+    nl = block_sum(nl);
+    if (threadIdx.x == 0) { atomicAdd(&cnt[CNT_A], nc); atomicAdd(&cnt[CNT_B], nl); }
 }
 
 void count_all(Ctx &c, int64_t &cls, int64_t &lits, bool with_hist)
@@ -412,7 +504,7 @@ void count_all(Ctx &c, int64_t &cls, int64_t &lits, bool with_hist)
     }
     if (c.nrefs) {
         KTimer kt(c, K_COUNT, 20.0 * c.nrefs + 4.0 * c.nliterals);
-        k_count<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, (unsigned long long *)c.cnt.p,
+        k_count<<<grid_for(c.nrefs, 256, 1024), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, (unsigned long long *)c.cnt.p,
                                                                       with_hist ? c.hist.p : nullptr);
     }
     read_counters(c, 3);

@@ -173,6 +173,21 @@ __device__ __forceinline__ uint64_t block_reserve(uint64_t want, unsigned long l
     return off;
 }
 
+// block-wide sum; the result is valid in thread 0
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v)
+{
+    __shared__ unsigned long long s_part[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (__lane_id() == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += s_part[q];
+    __syncthreads();
+    return t;
+}
+
 __device__ __forceinline__ void block_append(bool flag, uint32_t value, uint32_t *out, unsigned long long *counter)
 {
     const uint64_t o = block_reserve(flag ? 1ull : 0ull, counter);

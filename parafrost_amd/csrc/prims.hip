@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_radix_scatter(const K *__restrict_
 }
 
 template <typename K, bool V>
-static void radix_impl(Ctx &c, K *keys, uint32_t *vals, uint64_t n, int nbits)
+static void radix_impl(Ctx &c, K *keys, uint32_t *vals, uint64_t n, int nbits, int lo = 0)
 {
     if (n <= 1 || nbits <= 0) return;
     const uint64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
@@ -204,7 +204,7 @@ static void radix_impl(Ctx &c, K *keys, uint32_t *vals, uint64_t n, int nbits)
     uint64_t *offs = (uint64_t *)(t + bk + bv + bc);
     K *a = keys, *b = kt;
     uint32_t *va = vals, *vb = vt;
-    for (int shift = 0; shift < nbits; shift += 8) {
+    for (int shift = lo; shift < lo + nbits; shift += 8) {
         k_radix_hist<K><<<(unsigned)ntiles, RS_BLOCK, 0, c.stream>>>(a, n, shift, counts, (uint32_t)ntiles);
         scan_u32_to_u64(c, counts, offs, cnt_n, nullptr);
         k_radix_scatter<K, V><<<(unsigned)ntiles, RS_BLOCK, 0, c.stream>>>(a, va, b, vb, n, shift, offs, (uint32_t)ntiles);
@@ -222,24 +222,30 @@ void radix_sort_u32_pairs(Ctx &c, uint32_t *keys, uint32_t *vals, uint64_t n, in
     radix_impl<uint32_t, true>(c, keys, vals, n, nbits);
 }
 
-void radix_sort_u64_keys(Ctx &c, uint64_t *keys, uint64_t n, int nbits)
+void radix_sort_u64_keys(Ctx &c, uint64_t *keys, uint64_t n, int nbits, int lo)
 {
-    radix_impl<uint64_t, false>(c, keys, nullptr, n, nbits);
+    radix_impl<uint64_t, false>(c, keys, nullptr, n, nbits, lo);
 }
 
 __global__ void k_max_u32(const uint32_t *__restrict__ in, uint64_t n, unsigned long long *out)
 {
+    __shared__ uint32_t s_part[16];
     uint32_t m = 0;
     for (uint64_t i = gtid(); i < n; i += gstride()) m = max(m, in[i]);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_down(m, o, 64));
-    if (__lane_id() == 0) atomicMax(out, (unsigned long long)m);
+    if (__lane_id() == 0) s_part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = max(m, s_part[q]);
+        atomicMax(out, (unsigned long long)m);
+    }
 }
 
 uint32_t reduce_max_u32(Ctx &c, const uint32_t *in, uint64_t n)
 {
     PFG_HIP(hipMemsetAsync(c.cnt.p, 0, 8, c.stream));
-    k_max_u32<<<grid_for(n, 256, 2048), 256, 0, c.stream>>>(in, n, (unsigned long long *)c.cnt.p);
+    k_max_u32<<<grid_for(n, 256, 512), 256, 0, c.stream>>>(in, n, (unsigned long long *)c.cnt.p);
     read_counters(c, 1);
     return (uint32_t)c.hcnt[0];
 }

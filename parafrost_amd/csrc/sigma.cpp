@@ -143,7 +143,7 @@ struct Sigma {
             VE();
             if (c.unsat) return PFG_EUNSAT;
             int64_t nc, nl;
-            countAll(nc, nl, true);   // also the next round's histogram
+            countAll(nc, nl);
             filterPVs();
             c.nclauses = nc;
             c.nliterals = nl;
