@@ -40,17 +40,35 @@ def parse():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check", action="store_true", help="compare the final state with the oracle")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
+def copy_calibration():
+    """One plain device-to-device copy of 256 MiB, three times: the known-size kernel
+    against which HBM byte counters are scaled."""
+    import torch
+    n = 256 << 20
+    a = torch.ones(n // 4, dtype=torch.int32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 3}))
+
+
 def cpu_baseline(cfg: str):
-    """Oracle timed on a bounded sample: c1 (20k vars) full simplification, repeated,
-    plus the literal count of c2 scaled out by throughput (literals/s)."""
+    """The oracle (single-thread C restatement of the reference CPU simplifier) timed
+    on a bounded sample: the same generator at 1/10 scale, full simplification,
+    repeated until about 10 s of CPU time; literals/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     from parafrost_amd import gen
-    # sample: the c2 generator's distribution at 1/10 scale (200k vars, 840k clauses)
-    sample = gen.geometric(200_000, 840_000, 2, 8, seed=2) if cfg == "c2" else gen.config("c1")
+    sample = {"c1": lambda: gen.config("c1"),
+              "c2": lambda: gen.geometric(200_000, 840_000, 2, 8, seed=2),
+              "c3": lambda: gen.scalefree(500_000, 2_000_000, 2.5, 0.2, seed=3)}[cfg]()
+    scale = "full scale" if cfg == "c1" else "1/10 scale"
     reps, t_total, lits = 0, 0.0, 0
     while t_total < 10.0 and reps < 20:
         o = po.Oracle(sample, po.default_opts())
@@ -61,9 +79,34 @@ def cpu_baseline(cfg: str):
         reps += 1
         del o
     return {"value": lits / t_total, "unit": "literals/s", "cores": 1, "kind": "port",
-            "sample": f"{reps} x full sigmify of the same generator at 1/10 scale "
+            "sample": f"{reps} x full sigmify of the same generator at {scale} "
                       f"({sample.nvars} vars, {sample.ncls} clauses, {sample.nlits} literals), "
                       f"single thread, {t_total:.1f} s"}
+
+
+def roofline(kt: dict, ka: dict):
+    """Per-kernel HBM roofline from the event timers: achieved = algorithmic bytes per
+    launch / average launch duration. Only timers that bracket exactly one kernel and
+    have defined algorithmic bytes qualify; the dominant one is the largest total time."""
+    rows = {}
+    for name, k in kt.items():
+        a = ka.get(name)
+        if not k["kernel"] or not a or a["bytes"] <= 0 or not k["launches"]:
+            continue
+        avg_ms = k["ms"] / k["launches"]
+        bpl = a["bytes"] / a["launches"]
+        gbs = bpl / (avg_ms * 1e-3) / 1e9
+        rows[name] = {"kernel": k["kernel"], "ms_total": round(k["ms"], 3), "launches": k["launches"],
+                      "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(bpl),
+                      "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    if not rows:
+        return None, rows
+    dom = max(rows, key=lambda n: rows[n]["ms_total"])
+    r = rows[dom]
+    roof = {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac"],
+            "traffic": None, "kernel": r["kernel"], "avg_launch_ms": r["avg_launch_ms"],
+            "bytes_per_launch": r["bytes_per_launch"], "launches": r["launches"]}
+    return roof, rows
 
 
 def main():
@@ -72,6 +115,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
                                    "(see DESIGN.md), --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import torch
     import parafrost_amd as pf
     from parafrost_amd import gen
@@ -103,20 +149,17 @@ def main():
     kt = s.kernel_times()
     stats = s.stats()
 
-    # dominant kernel roofline: occurrence-table fill (createOT) -- bytes per launch
-    # = 20C + 12L + 32(V+1) over the live clauses of that launch
-    dom = max(kt, key=lambda k: kt[k]["ms"]) if kt else None
-    roof = None
-    if dom:
-        k = kt[dom]
-        avg_ms = k["ms"] / max(1, k["launches"])
-        gbs = (k["bytes"] / max(1, k["launches"])) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_launch_ms": round(avg_ms, 4), "launches": k["launches"]}
-
+    # algorithmic bytes per launch: one more, untimed step with accounting on
+    # (the step is deterministic, so every launch repeats with the same sizes)
+    s.reset_times()
+    s.set_accounting(True)
+    step()
+    s.set_accounting(False)
+    ka = s.kernel_times()
+    torch.cuda.synchronize()
+    roof, kernels = roofline(kt, ka)
     out = {
-        "metric": "simplified literals/sec on 1 MI355X (plain path)",
+        "metric": "simplified literals/sec on 1 MI355X (batched/served vs plain); p50/p99 latency",
         "value": L * args.steps / dt,
         "unit": "literals/s",
         "n_gpus": 1,
@@ -128,7 +171,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": {"c1": "uniform 3-SAT 20k vars / 85.2k clauses, seed 1",
+        "config": {"path": "plain (one instance per context)", "workload": {"c1": "uniform 3-SAT 20k vars / 85.2k clauses, seed 1",
                                 "c2": "mixed-width CNF 2M vars / 8.4M clauses, widths 2-8 geometric, seed 2",
                                 "c3": "scale-free CNF 5M vars / 20M clauses, 20% Tseitin gates, seed 3"}[args.config],
                    "input_literals": L, "variables": cnf.nvars, "clauses": cnf.ncls,
@@ -136,6 +179,7 @@ def main():
                    "clauses_out": stats["clauses"], "literals_out": stats["literals"],
                    "eliminated": stats["eliminated"], "pass_ms_per_step": {k: round(v / args.steps, 3) for k, v in times.items()}},
         "roofline": roof,
+        "kernels": kernels,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config)

@@ -113,14 +113,19 @@ int pfg_download_ot_list(pfg_ctx *ctx, uint32_t lit, int64_t *idx, uint32_t *n);
 
 int pfg_times_get(pfg_ctx *ctx, pfg_times *t);
 
-/* per-kernel device time (HIP events around each launch) and algorithmic HBM bytes */
+/* per-kernel device time (HIP events around each launch) and algorithmic HBM bytes.
+ * kernel = the one HIP kernel the timer brackets, or NULL when it spans several launches.
+ * Bytes of the per-variable kernels (sort_pv_lists, sub_range, ve_detect, ere_claim) are
+ * only computed while accounting is on (it adds a reduction before each of them). */
 typedef struct pfg_kstat {
     const char *name;
+    const char *kernel;
     double ms;
     uint64_t launches;
     double bytes;
 } pfg_kstat;
 int pfg_kernel_stats(pfg_ctx *ctx, pfg_kstat *out, int max, int *n);
+int pfg_set_accounting(pfg_ctx *ctx, int on);
 int pfg_reset_stats(pfg_ctx *ctx);
 /* forget the clause database and all variable state (allocations are kept); the
    context can then take a new upload of the same variable range */

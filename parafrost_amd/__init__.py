@@ -39,8 +39,8 @@ class Times(ctypes.Structure):
 
 
 class KStat(ctypes.Structure):
-    _fields_ = [("name", ctypes.c_char_p), ("ms", ctypes.c_double), ("launches", ctypes.c_uint64),
-                ("bytes", ctypes.c_double)]
+    _fields_ = [("name", ctypes.c_char_p), ("kernel", ctypes.c_char_p), ("ms", ctypes.c_double),
+                ("launches", ctypes.c_uint64), ("bytes", ctypes.c_double)]
 
 
 API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_upload_cnf_device",
@@ -48,7 +48,8 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_lcve", "pfg_sort_ot", "pfg_sub", "pfg_ve", "pfg_ere", "pfg_count", "pfg_filter_pvs", "pfg_gc",
        "pfg_stop", "pfg_next_phase", "pfg_hist", "pfg_download_cnf", "pfg_download_trail",
        "pfg_download_resolved", "pfg_download_vstate", "pfg_download_pvs", "pfg_download_ot_list",
-       "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset"]
+       "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
+       "pfg_set_accounting"]
 
 
 def lib_path() -> str:
@@ -85,6 +86,7 @@ def lib():
         L.pfg_times_get.argtypes = [vp, ctypes.POINTER(Times)]
         L.pfg_kernel_stats.argtypes = [vp, ctypes.POINTER(KStat), i32, ctypes.POINTER(i32)]
         L.pfg_reset_stats.argtypes = [vp]
+        L.pfg_set_accounting.argtypes = [vp, i32]
         L.pfg_reset.argtypes = [vp]
         L.pfg_last_error.argtypes = [vp]
         L.pfg_last_error.restype = ctypes.c_char_p
@@ -249,8 +251,13 @@ class Simplifier:
         arr = (KStat * 32)()
         n = ctypes.c_int()
         self._check(self.L.pfg_kernel_stats(self.h, arr, 32, ctypes.byref(n)))
-        return {arr[i].name.decode(): {"ms": arr[i].ms, "launches": int(arr[i].launches), "bytes": arr[i].bytes}
+        return {arr[i].name.decode(): {"kernel": arr[i].kernel.decode() if arr[i].kernel else None,
+                                       "ms": arr[i].ms, "launches": int(arr[i].launches), "bytes": arr[i].bytes}
                 for i in range(n.value)}
+
+    def set_accounting(self, on: bool):
+        """Compute algorithmic bytes of the per-variable kernels (adds a reduction before each)."""
+        self._check(self.L.pfg_set_accounting(self.h, 1 if on else 0))
 
     def reset_times(self):
         self._check(self.L.pfg_reset_stats(self.h))

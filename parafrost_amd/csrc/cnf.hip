@@ -866,12 +866,41 @@ __global__ void k_sort_pv_lists(const uint32_t *__restrict__ pvs, uint32_t npvs,
     }
 }
 
+__global__ void k_pv_bytes(const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e, const OL *__restrict__ ot,
+                           const uint64_t *__restrict__ occ, const uint32_t *__restrict__ mem, int with_clauses,
+                           unsigned long long *__restrict__ out)
+{
+    unsigned long long acc = 0;
+    for (uint64_t i = 2ull * b + gtid(); i < 2ull * e; i += gstride()) {
+        const uint32_t lit = 2u * pvs[i >> 1] + (uint32_t)(i & 1u);
+        const OL l = ot[lit];
+        acc += 16ull + 8ull * l.sz;
+        if (with_clauses)
+            for (uint32_t j = 0; j < l.sz; j++) acc += 12ull + 4ull * mem[occ[l.off + j] + 2];
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0 && acc) atomicAdd(out, acc);
+}
+
+double pv_bytes(Ctx &c, uint32_t b, uint32_t e, bool with_clauses)
+{
+    if (!c.account || e <= b) return 0.0;
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_ACCT, 0, 8, c.stream));
+    k_pv_bytes<<<grid_for(2ull * (e - b), 256, 4096), 256, 0, c.stream>>>(c.pvs.p, b, e, c.ot.p, c.occ.p, c.mem.p,
+                                                                         with_clauses ? 1 : 0,
+                                                                         (unsigned long long *)c.cnt.p + CNT_ACCT);
+    uint64_t v = 0;
+    PFG_HIP(hipMemcpyAsync(&v, c.cnt.p + CNT_ACCT, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    return (double)v;
+}
+
 void sort_ot(Ctx &c)
 {
     if (!c.npvs) return;
     uint64_t *tmp = nullptr;
     if (c.opts.stable_sort) { ensure(c, c.tmpu64b, c.occ.cap); tmp = c.tmpu64b.p; }
-    KTimer kt(c, K_SORT_PV, 0.0);
+    KTimer kt(c, K_SORT_PV, 2.0 * pv_bytes(c, 0, c.npvs, false));   // lists read and written once
     k_sort_pv_lists<<<grid_for(2ull * c.npvs, 64, 16384), 64, 0, c.stream>>>(c.pvs.p, c.npvs, c.ot.p, c.occ.p, c.mem.p,
                                                                               c.opts.stable_sort, tmp);
 }

@@ -36,12 +36,14 @@ struct DBuf {
     uint64_t cap = 0;
 };
 
-// kernels with their own event timers (the roofline reports the dominant one)
+// event timers: most wrap exactly one kernel (kernel_symbols names it), the
+// rest a short sequence of launches (kernel_symbols entry null)
 enum KId {
-    K_UPLOAD, K_HIST, K_OT_FILL, K_OT_SORT, K_PROP, K_LCVE_RADIX, K_LCVE_MIS, K_SORT_PV, K_SUB,
-    K_VE_DETECT, K_VE_APPLY, K_ERE, K_GC, K_COUNT, K_NUM
+    K_UPLOAD, K_HIST, K_OT_FILL, K_OT_SORT, K_PROP, K_LCVE_RADIX, K_LCVE_INIT, K_LCVE_PASS_A, K_LCVE_PASS_B,
+    K_SORT_PV, K_SUB, K_VE_DETECT, K_VE_APPLY, K_ERE, K_GC, K_COUNT, K_NUM
 };
 extern const char *const kernel_names[K_NUM];
+extern const char *const kernel_symbols[K_NUM];
 
 struct KStat {
     double ms = 0, bytes = 0;
@@ -69,6 +71,7 @@ struct Ctx {
     bool ot_valid = false;
     bool hist_live = false;     // hist holds the live-literal counts of the current database
     bool trace = false;         // PFG_TRACE set: per-round progress on stderr
+    bool account = false;       // compute algorithmic bytes of the per-variable kernels (extra reductions)
 
     // variables
     DBuf<int8_t> value;         // per literal: -1 unassigned, 0 false, 1 true

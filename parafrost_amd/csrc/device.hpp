@@ -98,7 +98,7 @@ __device__ __forceinline__ void sort_u32_small(uint32_t *d, int n)
 }
 
 // device counter slots (Ctx::cnt) and flags
-enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_NTRAIL = 8, CNT_FLAGS = 9 };
+enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_NTRAIL = 8, CNT_FLAGS = 9, CNT_ACCT = 10 };
 enum { FLAG_UNSAT = 1, FLAG_INTERNAL = 2 };
 
 // returns 1 if lit became true, 0 if it was true already, -1 if it was false

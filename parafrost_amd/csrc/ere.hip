@@ -145,7 +145,7 @@ void ere(Ctx &c)
               (unsigned long long *)c.cnt.p};
     PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));
     {
-        KTimer kt(c, K_ERE, 0.0);
+        KTimer kt(c, K_ERE, pv_bytes(c, 0, c.npvs, true));
         k_ere_claim<<<grid_for(c.npvs, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, c.npvs);
     }
     read_counters(c, 3);

@@ -40,10 +40,6 @@ __global__ void k_scores(const uint32_t *__restrict__ hist, uint32_t nvars, uint
     }
 }
 
-__global__ void k_rank(const uint32_t *__restrict__ elig, uint32_t nvars, uint32_t *__restrict__ rank)
-{
-    for (uint64_t i = gtid(); i < nvars; i += gstride()) rank[elig[i]] = (uint32_t)i;
-}
 
 __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, const uint32_t *__restrict__ hist,
                            const uint8_t *__restrict__ vstate, const OL *__restrict__ ot, uint32_t pt, uint32_t nt,
@@ -78,28 +74,42 @@ __global__ void __launch_bounds__(1024) k_work_init(const uint8_t *__restrict__ 
 
 struct ElectArgs {
     const uint32_t *mem;
-    const uint32_t *rank;
+    const uint32_t *crank;    // per variable: rank << 1 | freezer, or NO_CAND
     const uint8_t *kind;
     uint8_t *status, *blocked;
     uint64_t b0;
 };
 
-__device__ __forceinline__ bool is_cand(const ElectArgs &A, uint32_t j)
+constexpr uint32_t NO_CAND = 0xFFFFFFFFu;
+
+// per variable (indexed by variable, not rank): its rank among the candidates before
+// the static break and whether it freezes neighbours, so the clause pass needs one
+// gather per literal
+__global__ void k_crank(const uint32_t *__restrict__ elig, uint32_t nvars, const uint8_t *__restrict__ kind,
+                        uint64_t b0, uint32_t *__restrict__ crank)
 {
-    return j < A.b0 && (A.kind[j] == K_CAND || A.kind[j] == K_DYNBREAK);
+    for (uint64_t i = gtid(); i < nvars; i += gstride()) {
+        const uint8_t k = kind[i];
+        const bool cand = i < b0 && (k == K_CAND || k == K_DYNBREAK);
+        crank[elig[i]] = cand ? (((uint32_t)i << 1) | (k == K_CAND ? 1u : 0u)) : NO_CAND;
+    }
 }
 
 // Live clauses with at least two candidates take part in the election. Each is
 // written once as a compact record [k, rank_1 .. rank_k] of its candidates, so the
 // rounds stream records instead of clauses and rank lookups. Round 0's pass A is
 // fused in: nothing is elected yet, so every candidate ranked after the clause's
-// earliest freezing candidate is blocked.
-__global__ void __launch_bounds__(1024) k_elect_init(ElectArgs A, const uint64_t *__restrict__ refs, uint64_t n,
-                                                    uint32_t *__restrict__ rec, uint64_t *__restrict__ active,
-                                                    unsigned long long *cnt)
+// earliest freezing candidate is blocked. The first ELECT_CACHE candidate codes of
+// a clause stay in LDS between the counting and the writing loop.
+constexpr int ELECT_CACHE = 16;
+__global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t *__restrict__ refs, uint64_t n,
+                                                   uint32_t *__restrict__ rec, uint64_t *__restrict__ active,
+                                                   unsigned long long *cnt)
 {
+    __shared__ uint32_t s_cr[ELECT_CACHE][256];
+    const unsigned tid = threadIdx.x;
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
-        const uint64_t i = b + threadIdx.x;
+        const uint64_t i = b + tid;
         const uint32_t *c = nullptr;
         int sz = 0;
         uint32_t k = 0, m = 0xFFFFFFFFu;
@@ -108,10 +118,11 @@ __global__ void __launch_bounds__(1024) k_elect_init(ElectArgs A, const uint64_t
             if (!c_del(c)) {
                 sz = c_size(c);
                 for (int t = 0; t < sz; t++) {
-                    const uint32_t j = A.rank[c[3 + t] >> 1];
-                    if (!is_cand(A, j)) continue;
+                    const uint32_t x = A.crank[c[3 + t] >> 1];
+                    if (x == NO_CAND) continue;
+                    if (k < (uint32_t)ELECT_CACHE) s_cr[k][tid] = x;
                     k++;
-                    if (A.kind[j] == K_CAND && j < m) m = j;
+                    if ((x & 1u) && (x >> 1) < m) m = x >> 1;
                 }
             }
         }
@@ -121,12 +132,22 @@ __global__ void __launch_bounds__(1024) k_elect_init(ElectArgs A, const uint64_t
         if (keep) {
             uint32_t *r = rec + off;
             r[0] = k;
-            int w = 1;
-            for (int t = 0; t < sz; t++) {
-                const uint32_t j = A.rank[c[3 + t] >> 1];
-                if (!is_cand(A, j)) continue;
-                r[w++] = j;
+            const uint32_t kc = k < (uint32_t)ELECT_CACHE ? k : (uint32_t)ELECT_CACHE;
+            for (uint32_t w = 0; w < kc; w++) {
+                const uint32_t j = s_cr[w][tid] >> 1;
+                r[1 + w] = j;
                 if (j > m) A.blocked[j] = 1;
+            }
+            if (k > (uint32_t)ELECT_CACHE) {
+                uint32_t w = 0;
+                for (int t = 0; t < sz; t++) {
+                    const uint32_t x = A.crank[c[3 + t] >> 1];
+                    if (x == NO_CAND) continue;
+                    if (++w <= (uint32_t)ELECT_CACHE) continue;
+                    const uint32_t j = x >> 1;
+                    r[w] = j;
+                    if (j > m) A.blocked[j] = 1;
+                }
             }
             active[aoff] = off;
         }
@@ -229,7 +250,6 @@ uint32_t lcve(Ctx &c)
         KTimer kt(c, K_LCVE_RADIX, 16.0 * nv * ((bits + 7) / 8));
         radix_sort_u32_pairs(c, c.scores.p, c.eligible.p, nv, ((bits + 7) / 8) * 8);
     }
-    k_rank<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, c.rank.p);
     uint8_t *kind = c.tmpu8.p, *status = kind + nv + 1, *blocked = status + nv + 1;
     const uint32_t pt = c.opts.mu_pos << c.mu_inc, nt = c.opts.mu_neg << c.mu_inc;
     const unsigned long long init[3] = {0ull, (unsigned long long)nv, 0ull};
@@ -238,14 +258,15 @@ uint32_t lcve(Ctx &c)
                                                               kind, status, blocked, (unsigned long long *)c.cnt.p);
     read_counters(c, 3);
     const uint64_t b0 = c.hcnt[CNT_B];
+    k_crank<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, kind, b0, c.rank.p);
     uint32_t *work = c.tmpu32a.p, *work2 = c.tmpu32b.p;
     uint64_t *act = c.tmpu64a.p, *act2 = c.tmpu64b.p;
     PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 32, c.stream));
     k_work_init<<<grid_for(b0, 1024, 4096), 1024, 0, c.stream>>>(kind, b0, work, (unsigned long long *)c.cnt.p);
     ElectArgs A{c.mem.p, c.rank.p, kind, status, blocked, b0};
     if (c.nrefs && b0) {
-        KTimer kt(c, K_LCVE_MIS, 20.0 * c.nrefs + 8.0 * c.nliterals);
-        k_elect_init<<<grid_for(c.nrefs, 1024, 8192), 1024, 0, c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act,
+        KTimer kt(c, K_LCVE_INIT, 20.0 * c.nrefs + 8.0 * c.nliterals);
+        k_elect_init<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act,
                                                                           (unsigned long long *)c.cnt.p);
     }
     read_counters(c, 5);
@@ -258,12 +279,12 @@ uint32_t lcve(Ctx &c)
     while (nwork) {
         PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 24, c.stream));
         if (!fused && nact) {
-            KTimer kt(c, K_LCVE_MIS, 0.0);
+            KTimer kt(c, K_LCVE_PASS_A, 0.0);
             k_elect_pass_a<<<grid_for(nact, 1024, 8192), 1024, 0, c.stream>>>(A, c.erec.p, act, nact, act2,
                                                                               (unsigned long long *)c.cnt.p);
         }
         {
-            KTimer kt(c, K_LCVE_MIS, 0.0);
+            KTimer kt(c, K_LCVE_PASS_B, 0.0);
             k_elect_pass_b<<<grid_for(nwork, 1024, 8192), 1024, 0, c.stream>>>(A, work, nwork, work2, (unsigned long long *)c.cnt.p);
         }
         read_counters(c, 4);

@@ -18,6 +18,10 @@ void sort_ot(Ctx &c);                    // PV lists by the reference clause key
 uint64_t compact_u32(Ctx &c, const uint32_t *in, const uint32_t *flags, uint64_t n, uint32_t *out);
 void enqueue_units_host(Ctx &c, const uint32_t *lits, uint32_t n);
 void merge_units(Ctx &c, const uint32_t *d_units, uint64_t n);  // ordered dedupe into the trail
+// algorithmic bytes of reading the occurrence lists of elected variables pvs[b, e):
+// 16 per list header + 8 per entry (+ 12 + 4|c| per listed clause when with_clauses);
+// 0 unless accounting is on (synchronizes the stream)
+double pv_bytes(Ctx &c, uint32_t b, uint32_t e, bool with_clauses);
 
 // lcve.hip
 uint32_t lcve(Ctx &c);                   // returns #elected

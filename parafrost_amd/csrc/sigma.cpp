@@ -54,8 +54,11 @@ PassTimer::~PassTimer()
 }
 
 const char *const kernel_names[K_NUM] = {
-    "upload_prep", "hist", "ot_fill", "ot_sort", "prop", "lcve_radix", "lcve_mis", "sort_pv_lists", "sub",
-    "ve_detect", "ve_apply", "ere", "gc_move", "count"};
+    "upload_prep", "hist", "ot_fill", "ot_sort", "prop", "lcve_radix", "lcve_init", "lcve_pass_a", "lcve_pass_b",
+    "sort_pv_lists", "sub_range", "ve_detect", "ve_apply", "ere_claim", "gc_move", "count"};
+const char *const kernel_symbols[K_NUM] = {
+    "k_upload", nullptr, nullptr, nullptr, nullptr, nullptr, "k_elect_init", "k_elect_pass_a", "k_elect_pass_b",
+    "k_sort_pv_lists", "k_sub_range", "k_ve_detect", "k_ve_apply", "k_ere_claim", "k_gc_move", "k_count"};
 
 KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
 {
@@ -472,6 +475,7 @@ int pfg_kernel_stats(pfg_ctx *ctx, pfg_kstat *out, int max, int *n)
         if (!c.kstat[i].launches) continue;
         if (out) {
             out[k].name = kernel_names[i];
+            out[k].kernel = kernel_symbols[i];
             out[k].ms = c.kstat[i].ms;
             out[k].launches = c.kstat[i].launches;
             out[k].bytes = c.kstat[i].bytes;
@@ -479,6 +483,13 @@ int pfg_kernel_stats(pfg_ctx *ctx, pfg_kstat *out, int max, int *n)
         k++;
     }
     if (n) *n = k;
+    API_END
+}
+
+int pfg_set_accounting(pfg_ctx *ctx, int on)
+{
+    API_BEGIN
+    c.account = on != 0;
     API_END
 }
 

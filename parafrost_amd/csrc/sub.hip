@@ -230,7 +230,7 @@ void sub(Ctx &c)
     uint32_t b = 0;
     auto run_range = [&](uint32_t e) {
         if (e > b) {
-            KTimer kt(c, K_SUB, 0.0);
+            KTimer kt(c, K_SUB, pv_bytes(c, b, e, true));
             k_sub_range<<<grid_for(e - b, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, b, e);
         }
     };

@@ -28,14 +28,43 @@ struct VeArgs {
     int rlimit;
     int lbound;
     int xor_max_arity;
+    // per-lane LDS stage of the variable's own clauses (detect only; null = global)
+    uint32_t v, ps;
+    const uint32_t *tab;   // LDS word offsets of the staged clauses, positive list then negative
+    uint32_t *arena;       // LDS arena; a staged clause is [ref_lo, ref_hi, word0, sig, size, lits..]
 };
 
 __device__ __forceinline__ uint32_t *cl_at(const VeArgs &A, uint32_t lit, uint32_t i)
 {
+    if (A.tab && (lit >> 1) == A.v) return A.arena + A.tab[(lit & 1u) ? A.ps + i : i];
     return A.mem + A.occ[A.ot[lit].off + i];
 }
 
-__device__ int count_orgs(const VeArgs &A, uint32_t lit)
+constexpr uint32_t VE_ARENA_WORDS = 8192;   // LDS words per 64-lane block for staged clauses (32 KB)
+
+// Header word writes keep the staged copy and the pool copy equal: a staged clause
+// writes through to the pool; a pool clause that is also staged (reached through
+// another literal's list) updates its staged copy.
+__device__ __forceinline__ void set_w0(const VeArgs &A, uint32_t *c, uint32_t w)
+{
+    if (c[0] == w) return;
+    c[0] = w;
+    if (!A.tab) return;
+    if (c >= A.arena && c < A.arena + VE_ARENA_WORDS) {
+        A.mem[(uint64_t)c[-2] | ((uint64_t)c[-1] << 32)] = w;
+        return;
+    }
+    const uint64_t ref = (uint64_t)(c - A.mem);
+    const uint32_t n = A.ps + A.ot[2 * A.v + 1].sz;
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t *l = A.arena + A.tab[k];
+        if (l[-2] == (uint32_t)ref && l[-1] == (uint32_t)(ref >> 32)) { l[0] = w; return; }
+    }
+}
+__device__ __forceinline__ void melt(const VeArgs &A, uint32_t *c) { set_w0(A, c, c[0] | 4u); }
+__device__ __forceinline__ void freeze(const VeArgs &A, uint32_t *c) { set_w0(A, c, c[0] & ~4u); }
+
+__device__ __forceinline__ int count_orgs(const VeArgs &A, uint32_t lit)
 {
     const uint32_t sz = A.ot[lit].sz;
     int n = 0;
@@ -45,7 +74,7 @@ __device__ int count_orgs(const VeArgs &A, uint32_t lit)
 
 // countSubstituted / countResolvents (src/cpu/simplify.h:227-287); on success the
 // totals needed for placement are returned through words/units.
-__device__ bool count_subst(const VeArgs &A, uint32_t x, int clsbefore, uint32_t me, uint32_t other, bool molten_rule,
+__device__ __forceinline__ bool count_subst(const VeArgs &A, uint32_t x, int clsbefore, uint32_t me, uint32_t other, bool molten_rule,
                             int &nAdded, uint64_t &words, uint32_t &units)
 {
     int nAddedLits = 0;
@@ -78,17 +107,17 @@ __device__ bool count_subst(const VeArgs &A, uint32_t x, int clsbefore, uint32_t
     return false;
 }
 
-__device__ void freeze_binaries(const VeArgs &A, uint32_t lit)
+__device__ __forceinline__ void freeze_binaries(const VeArgs &A, uint32_t lit)
 {
     const uint32_t sz = A.ot[lit].sz;
     for (uint32_t i = 0; i < sz; i++) {
         uint32_t *c = cl_at(A, lit, i);
-        if (c_orig(c) && c_size(c) == 2) c_freeze(c);
+        if (c_orig(c) && c_size(c) == 2) freeze(A, c);
     }
 }
 
 // find_sfanin + find_BN_gate (equivalence.h:88-130)
-__device__ uint32_t find_bn_gate(const VeArgs &A, uint32_t p)
+__device__ __forceinline__ uint32_t find_bn_gate(const VeArgs &A, uint32_t p)
 {
     const uint32_t n = p ^ 1u;
     const uint32_t ps = A.ot[p].sz;
@@ -100,7 +129,7 @@ __device__ uint32_t find_bn_gate(const VeArgs &A, uint32_t p)
         uint32_t *c = cl_at(A, p, i);
         if (c_orig(c) && c_size(c) == 2) {
             imp = (c[3] ^ c[4] ^ p) ^ 1u;
-            c_melt(c);
+            melt(A, c);
             nImps++;
         }
         if (nImps > 1) { many = true; break; }
@@ -113,7 +142,7 @@ __device__ uint32_t find_bn_gate(const VeArgs &A, uint32_t p)
         for (uint32_t i = 0; i < ns; i++) {
             uint32_t *c = cl_at(A, n, i);
             if (c_orig(c) && c_size(c) == 2 && c[3] == first && c[4] == second) {
-                c_melt(c);
+                melt(A, c);
                 return def;
             }
         }
@@ -122,7 +151,7 @@ __device__ uint32_t find_bn_gate(const VeArgs &A, uint32_t p)
     return 0;
 }
 
-__device__ bool has_lit(const uint32_t *c, uint32_t lit)
+__device__ __forceinline__ bool has_lit(const uint32_t *c, uint32_t lit)
 {
     const int sz = c_size(c);
     for (int i = 0; i < sz; i++) if (c[3 + i] == lit) return true;
@@ -131,7 +160,7 @@ __device__ bool has_lit(const uint32_t *c, uint32_t lit)
 
 // find_AO_gate (and.h:25-85) without materialising the fan-in vector: the fan-out
 // clause must have |fanins|+1 distinct literals, all of them fan-ins or ~dx
-__device__ bool find_ao_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words, uint32_t &units)
+__device__ __forceinline__ bool find_ao_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words, uint32_t &units)
 {
     const uint32_t x = dx >> 1, f_dx = dx ^ 1u;
     const uint32_t is = A.ot[dx].sz;
@@ -143,7 +172,7 @@ __device__ bool find_ao_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAd
             const uint32_t imp = (c[3] ^ c[4] ^ dx) ^ 1u;
             sig |= maphash(imp);
             nbin++;
-            c_melt(c);
+            melt(A, c);
         }
     }
     if (nbin > 1) {
@@ -165,9 +194,9 @@ __device__ bool find_ao_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAd
                 eq = found;
             }
             if (!eq) continue;
-            c_melt(c);
+            melt(A, c);
             nAdded = 0;
-            if (count_subst(A, x, nOrgCls, dx, f_dx, true, nAdded, words, units)) { c_freeze(c); break; }
+            if (count_subst(A, x, nOrgCls, dx, f_dx, true, nAdded, words, units)) { freeze(A, c); break; }
             return true;
         }
     }
@@ -175,10 +204,30 @@ __device__ bool find_ao_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAd
     return false;
 }
 
+// Both searches below look for a clause holding a literal of the variable, so when
+// the variable is staged its own list holds every candidate. With exactly one match
+// (or none) the result equals the reference's walk of the shortest list; with
+// duplicates the reference's order decides, so that walk is taken.
+
 // fast_equality_check (ifthenelse.h:24-40)
-__device__ uint32_t *fast_equality_check(const VeArgs &A, uint32_t x, uint32_t y, uint32_t z)
+__device__ __forceinline__ uint32_t *fast_equality_check(const VeArgs &A, uint32_t x, uint32_t y, uint32_t z)
 {
     uint32_t t;
+    if (A.tab && (x >> 1) == A.v) {
+        uint32_t a = x, b = y, d = z;
+        if (b > d) { t = b; b = d; d = t; }
+        if (a > d) { t = a; a = d; d = t; }
+        if (a > b) { t = a; a = b; b = t; }
+        const uint32_t sz = A.ot[x].sz;
+        uint32_t *hit = nullptr;
+        int nhit = 0;
+        for (uint32_t i = 0; i < sz && nhit < 2; i++) {
+            uint32_t *c = cl_at(A, x, i);
+            if (c_molten(c)) continue;
+            if (c_orig(c) && c_size(c) == 3 && c[3] == a && c[4] == b && c[5] == d) { if (!hit) hit = c; nhit++; }
+        }
+        if (nhit < 2) return hit;
+    }
     if (A.ot[y].sz > A.ot[z].sz) { t = y; y = z; z = t; }
     if (A.ot[x].sz > A.ot[y].sz) { t = x; x = y; y = t; }
     const uint32_t lx = x;
@@ -195,10 +244,25 @@ __device__ uint32_t *fast_equality_check(const VeArgs &A, uint32_t x, uint32_t y
 }
 
 // find_ITE_gate (ifthenelse.h:42-95)
-__device__ bool find_ite_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words, uint32_t &units)
+// An ITE gate needs two original ternaries on dx's list and two original,
+// non-molten ternaries on ~dx's list; without them the reference's search finds
+// nothing and changes nothing.
+__device__ __forceinline__ uint32_t count_ternaries(const VeArgs &A, uint32_t lit, bool skip_molten)
+{
+    const uint32_t sz = A.ot[lit].sz;
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < sz && n < 2; i++) {
+        const uint32_t *c = cl_at(A, lit, i);
+        if (c_orig(c) && c_size(c) == 3 && !(skip_molten && c_molten(c))) n++;
+    }
+    return n;
+}
+
+__device__ __forceinline__ bool find_ite_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words, uint32_t &units)
 {
     const uint32_t fx = dx ^ 1u;
     const uint32_t is = A.ot[dx].sz;
+    if (count_ternaries(A, dx, false) < 2 || count_ternaries(A, fx, true) < 2) return false;
     for (uint32_t i = 0; i < is; i++) {
         uint32_t *ci = cl_at(A, dx, i);
         if (!(c_orig(ci) && c_size(ci) == 3)) continue;
@@ -218,10 +282,10 @@ __device__ bool find_ite_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nA
             if (!d1) continue;
             uint32_t *d2 = fast_equality_check(A, fx, yj, zj ^ 1u);
             if (!d2) continue;
-            c_melt(ci); c_melt(cj); c_melt(d1); c_melt(d2);
+            melt(A, ci); melt(A, cj); melt(A, d1); melt(A, d2);
             nAdded = 0;
             if (count_subst(A, dx >> 1, nOrgCls, dx, fx, true, nAdded, words, units)) {
-                c_freeze(ci); c_freeze(cj); c_freeze(d1); c_freeze(d2);
+                freeze(A, ci); freeze(A, cj); freeze(A, d1); freeze(A, d2);
                 return false;
             }
             return true;
@@ -231,24 +295,45 @@ __device__ bool find_ite_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nA
 }
 
 // XOR gates (xor.h:30-147)
-__device__ void freeze_arities(const VeArgs &A, uint32_t me, uint32_t other)
+__device__ __forceinline__ void freeze_arities(const VeArgs &A, uint32_t me, uint32_t other)
 {
     for (int s = 0; s < 2; s++) {
         const uint32_t l = s ? other : me, sz = A.ot[l].sz;
         for (uint32_t i = 0; i < sz; i++) {
             uint32_t *c = cl_at(A, l, i);
-            if (c_size(c) > 2 && c_molten(c)) c_freeze(c);
+            if (c_size(c) > 2 && c_molten(c)) freeze(A, c);
         }
     }
 }
 
-__device__ bool make_arity(const VeArgs &A, uint32_t &parity, uint32_t *lits, int size)
+__device__ __forceinline__ bool make_arity(const VeArgs &A, uint32_t &parity, uint32_t *lits, int size)
 {
     const uint32_t old = parity;
     while (__builtin_parity(++parity));
     for (int k = 0; k < size; k++) {
         const uint32_t bit = 1u << k;
         if ((parity & bit) != (old & bit)) lits[k] ^= 1u;
+    }
+    if (A.tab) {
+        uint32_t own = lits[0];
+        for (int k = 1; k < size; k++) if ((lits[k] >> 1) == A.v) own = lits[k];
+        const uint32_t sz = A.ot[own].sz;
+        uint32_t *hit = nullptr;
+        int nhit = 0;
+        for (uint32_t i = 0; i < sz && nhit < 2; i++) {
+            uint32_t *c = cl_at(A, own, i);
+            if (!(c_orig(c) && c_size(c) == size)) continue;
+            bool ok = true;
+            for (int k = 0; k < size && ok; k++) {
+                const uint32_t l = c[3 + k];
+                int j;
+                for (j = 0; j < size; j++) if (l == lits[j]) break;
+                ok = j < size;
+            }
+            if (ok) { if (!hit) hit = c; nhit++; }
+        }
+        if (nhit == 1) { melt(A, hit); return true; }
+        if (nhit == 0) return false;
     }
     uint32_t best = lits[0];
     uint32_t minsize = A.ot[best].sz;
@@ -267,28 +352,100 @@ __device__ bool make_arity(const VeArgs &A, uint32_t &parity, uint32_t *lits, in
             for (j = 0; j < size; j++) if (l == lits[j]) break;
             ok = j < size;
         }
-        if (ok) { c_melt(c); return true; }
+        if (ok) { melt(A, c); return true; }
     }
     return false;
 }
 
-__device__ bool find_xor_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words, uint32_t &units)
+// An XOR over a clause of `size` literals needs all 2^(size-1) parity variants,
+// each an original clause of that size holding a literal of the variable. With
+// fewer such clauses on the two lists the reference's attempt fails, and its only
+// lasting effect is freeze_arities, which the caller applies when anything is molten.
+__device__ __forceinline__ bool xor_possible(const VeArgs &A, uint32_t dx, uint32_t fx, int size)
+{
+    const uint32_t need = 1u << (size - 1);
+    uint32_t cnt = 0;
+    for (int s = 0; s < 2; s++) {
+        const uint32_t l = s ? fx : dx, sz = A.ot[l].sz;
+        for (uint32_t i = 0; i < sz; i++) {
+            const uint32_t *c = cl_at(A, l, i);
+            const int cs = c_size(c);
+            if (c_orig(c) && cs == size && ++cnt >= need) return true;
+        }
+    }
+    return false;
+}
+
+// Every parity variant of ci has ci's variables in ci's order (literals are
+// sorted and a literal orders like its variable), so fewer than 2^(size-1) such
+// original clauses on the two lists means the attempt fails.
+__device__ __forceinline__ bool xor_variants_possible(const VeArgs &A, uint32_t dx, const uint32_t *ci, int size)
+{
+    const uint32_t need = 1u << (size - 1);
+    uint32_t cnt = 0;
+    for (int s = 0; s < 2; s++) {
+        const uint32_t l = dx ^ (uint32_t)s, sz = A.ot[l].sz;
+        for (uint32_t i = 0; i < sz; i++) {
+            const uint32_t *c = cl_at(A, l, i);
+            if (!c_orig(c) || c_size(c) != size) continue;
+            bool same = true;
+            for (int k = 0; k < size && same; k++) same = (c[3 + k] >> 1) == (ci[3 + k] >> 1);
+            if (same && ++cnt >= need) return true;
+        }
+    }
+    return false;
+}
+
+// counts of original clauses of sizes 3..8 on both lists, 8 bits each (saturating;
+// the XOR thresholds up to size 8 are at most 128); *molten: a clause of size > 2 is molten
+__device__ __forceinline__ uint64_t xor_size_counts(const VeArgs &A, uint32_t dx, bool &molten)
+{
+    uint64_t packed = 0;
+    molten = false;
+    for (int s = 0; s < 2; s++) {
+        const uint32_t l = dx ^ (uint32_t)s, sz = A.ot[l].sz;
+        for (uint32_t i = 0; i < sz; i++) {
+            const uint32_t *c = cl_at(A, l, i);
+            const uint32_t cs = (uint32_t)c_size(c);
+            if (c_orig(c) && cs >= 3u && cs <= 8u) {
+                const uint32_t sh = (cs - 3u) * 8u;
+                if (((packed >> sh) & 0xFFu) != 0xFFu) packed += 1ull << sh;
+            }
+            if (cs > 2u && c_molten(c)) molten = true;
+        }
+    }
+    return packed;
+}
+
+__device__ __forceinline__ bool find_xor_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words, uint32_t &units)
 {
     const uint32_t fx = dx ^ 1u, v = dx >> 1;
     const uint32_t is = A.ot[dx].sz;
+    bool molten = false;
+    const uint64_t counts = xor_size_counts(A, dx, molten);
     uint32_t lits[21];
     for (uint32_t i = 0; i < is; i++) {
         uint32_t *ci = cl_at(A, dx, i);
         if (!c_orig(ci)) continue;
         const int size = c_size(ci), arity = size - 1;
         if (size < 3 || arity > A.xor_max_arity) continue;
+        bool possible;
+        if (size <= 8) possible = ((counts >> ((size - 3) * 8)) & 0xFFu) >= (1u << (size - 1));
+        else possible = xor_possible(A, dx, fx, size);
+        if (possible) possible = xor_variants_possible(A, dx, ci, size);
+        if (!possible) {
+            if (molten) freeze_arities(A, dx, fx);   // the failing attempt's only lasting effect
+            molten = false;
+            continue;
+        }
+        molten = false;
         for (int k = 0; k < size; k++) lits[k] = ci[3 + k];
         uint32_t parity = 0;
         int itargets = 1 << arity;
         while (--itargets && make_arity(A, parity, lits, size));
         if (itargets) freeze_arities(A, dx, fx);
         else {
-            c_melt(ci);
+            melt(A, ci);
             nAdded = 0;
             if (count_subst(A, v, nOrgCls, dx, fx, true, nAdded, words, units)) { freeze_arities(A, dx, fx); break; }
             return true;
@@ -298,7 +455,7 @@ __device__ bool find_xor_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nA
 }
 
 // model words written by toblivion / save_BN_gate for the smaller original side
-__device__ uint64_t model_words(const VeArgs &A, uint32_t p, int pOrgs, int nOrgs)
+__device__ __forceinline__ uint64_t model_words(const VeArgs &A, uint32_t p, int pOrgs, int nOrgs)
 {
     const uint32_t side = pOrgs > nOrgs ? (p ^ 1u) : p;
     const uint32_t sz = A.ot[side].sz;
@@ -310,7 +467,7 @@ __device__ uint64_t model_words(const VeArgs &A, uint32_t p, int pOrgs, int nOrg
     return w;
 }
 
-__device__ uint32_t bn_units(const VeArgs &A, uint32_t p, uint32_t def)
+__device__ __forceinline__ uint32_t bn_units(const VeArgs &A, uint32_t p, uint32_t def)
 {   // substitute_single (equivalence.h:42-86): clauses that collapse to a unit
     const uint32_t n = p ^ 1u, def_f = def ^ 1u;
     uint32_t u = 0;
@@ -328,12 +485,51 @@ __device__ uint32_t bn_units(const VeArgs &A, uint32_t p, uint32_t def)
     return u;
 }
 
-__global__ void k_ve_detect(VeArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs, uint32_t *__restrict__ type,
-                            uint32_t *__restrict__ aux, uint32_t *__restrict__ nres, uint64_t *__restrict__ nwords,
-                            uint32_t *__restrict__ nunits, uint64_t *__restrict__ nmodel)
+
+// Copy the clauses of both occurrence lists of variable v into the lane's arena
+// region [base, base + need): first the table (one word per list entry), then the
+// staged clauses. Returns the table pointer.
+__device__ __forceinline__ uint32_t *stage_var(const VeArgs &A, uint32_t *arena, uint32_t base, uint32_t v)
 {
-    for (uint64_t q = gtid(); q < npvs; q += gstride()) {
-        const uint32_t v = pvs[q], p = 2 * v, n = p + 1;
+    uint32_t *tab = arena + base;
+    const OL lp = A.ot[2 * v], ln = A.ot[2 * v + 1];
+    uint32_t w = base + lp.sz + ln.sz, k = 0;
+    for (int s = 0; s < 2; s++) {
+        const OL l = s ? ln : lp;
+        for (uint32_t i = 0; i < l.sz; i++) {
+            const uint64_t ref = A.occ[l.off + i];
+            const uint32_t *g = A.mem + ref;
+            const uint32_t sz = g[2];
+            arena[w] = (uint32_t)ref;
+            arena[w + 1] = (uint32_t)(ref >> 32);
+            for (uint32_t t = 0; t < 3 + sz; t++) arena[w + 2 + t] = g[t];
+            tab[k++] = w + 2;
+            w += 5 + sz;
+        }
+    }
+    return tab;
+}
+
+__device__ __forceinline__ uint32_t stage_need(const VeArgs &A, uint32_t v)
+{
+    const OL lp = A.ot[2 * v], ln = A.ot[2 * v + 1];
+    uint32_t need = lp.sz + ln.sz;
+    for (int s = 0; s < 2; s++) {
+        const OL l = s ? ln : lp;
+        for (uint32_t i = 0; i < l.sz; i++) need += 5 + A.mem[A.occ[l.off + i] + 2];
+    }
+    return need;
+}
+
+struct VeDetectOut {
+    uint32_t *type, *aux, *nres, *nunits;
+    uint64_t *nwords, *nmodel;
+};
+
+// detection of one elected variable (pvs[q] = v); A carries the lane's stage
+__device__ __forceinline__ void ve_detect_var(const VeArgs &A, const VeDetectOut &O, uint64_t q, uint32_t v)
+{
+        const uint32_t p = 2 * v, n = p + 1;
         const int pOrgs = count_orgs(A, p), nOrgs = count_orgs(A, n);
         uint32_t t = T_NONE, a = 0, nr = 0, nu = 0;
         uint64_t nw = 0, nm = 0;
@@ -377,12 +573,50 @@ __global__ void k_ve_detect(VeArgs A, const uint32_t *__restrict__ pvs, uint32_t
                 }
             }
         }
-        type[q] = t | ((t == T_BN ? (uint32_t)(pOrgs > nOrgs) : 0u) << 8);
-        aux[q] = a;
-        nres[q] = nr;
-        nwords[q] = nw;
-        nunits[q] = nu;
-        nmodel[q] = nm;
+        O.type[q] = t | ((t == T_BN ? (uint32_t)(pOrgs > nOrgs) : 0u) << 8);
+        O.aux[q] = a;
+        O.nres[q] = nr;
+        O.nwords[q] = nw;
+        O.nunits[q] = nu;
+        O.nmodel[q] = nm;
+}
+
+// One 64-lane block per 64 elected variables. The lanes' variables are staged in
+// LDS in rounds: each round takes the pending lanes (in lane order) whose clauses
+// fit in the block's arena together; a variable larger than the whole arena is
+// detected from the pool instead. Detection of one variable touches only its own
+// clauses, so the rounds do not change results.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_ve_detect(VeArgs G, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t npvs)
+{
+    __shared__ uint32_t arena[VE_ARENA_WORDS];
+    const int lane = threadIdx.x;
+    for (uint64_t qb = (uint64_t)blockIdx.x * 64; qb < npvs; qb += (uint64_t)gridDim.x * 64) {
+        const uint64_t q = qb + lane;
+        const bool active = q < npvs;
+        const uint32_t v = active ? pvs[q] : 0u;
+        const uint32_t need = active ? stage_need(G, v) : 0u;
+        bool pending = active;
+        while (__ballot(pending)) {
+            uint32_t incl = pending ? need : 0u;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += t;
+            }
+            const bool first = pending && incl == need;   // first pending lane of the round
+            if (pending && incl <= VE_ARENA_WORDS) {
+                const uint32_t *tab = stage_var(G, arena, incl - need, v);
+                const VeArgs A{G.mem, G.ot, G.occ, G.rlimit, G.lbound, G.xor_max_arity, v, G.ot[2 * v].sz, tab, arena};
+                ve_detect_var(A, O, q, v);
+                pending = false;
+            }
+            else if (first) {   // larger than the arena
+                ve_detect_var(G, O, q, v);
+                pending = false;
+            }
+            __syncthreads();   // the arena is reused by the next round
+        }
     }
 }
 
@@ -395,7 +629,7 @@ struct VeOut {
     uint64_t ref_base, data_base, model_base;
 };
 
-__device__ void save_side(const VeArgs &A, uint32_t *&rs, uint32_t p, bool pbig)
+__device__ __forceinline__ void save_side(const VeArgs &A, uint32_t *&rs, uint32_t p, bool pbig)
 {   // saveClause/saveWitness for the smaller side (simplify.h:299-331, model.cpp:263-307)
     const uint32_t n = p ^ 1u;
     const uint32_t side = pbig ? n : p;
@@ -415,7 +649,7 @@ __device__ void save_side(const VeArgs &A, uint32_t *&rs, uint32_t p, bool pbig)
     rs += 2;
 }
 
-__device__ void delete_lists(const VeArgs &A, uint32_t p)
+__device__ __forceinline__ void delete_lists(const VeArgs &A, uint32_t p)
 {   // toblivion: every clause on both lists is deleted, lists cleared
     for (int s = 0; s < 2; s++) {
         OL *o = A.ot + (p ^ (uint32_t)s);
@@ -425,7 +659,7 @@ __device__ void delete_lists(const VeArgs &A, uint32_t p)
     }
 }
 
-__device__ void substitute_one(uint32_t dx, uint32_t *org, uint32_t def, uint32_t *&us)
+__device__ __forceinline__ void substitute_one(uint32_t dx, uint32_t *org, uint32_t def, uint32_t *&us)
 {   // substitute_single(dx, org, def) (equivalence.h:24-40)
     uint32_t *lits = org + 3;
     const int sz = c_size(org);
@@ -530,8 +764,9 @@ void ve(Ctx &c)
     uint64_t *nwords = c.tmpu64a.p, *nmodel = c.tmpu64a.p + np + 1;
     uint64_t *roff = c.tmpu64b.p, *uoff = roff + np + 2, *woff = uoff + np + 2, *moff = woff + np + 2;
     {
-        KTimer kt(c, K_VE_DETECT, 0.0);
-        k_ve_detect<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, np, type, aux, nres, nwords, nunits, nmodel);
+        const VeDetectOut DO{type, aux, nres, nunits, nwords, nmodel};
+        KTimer kt(c, K_VE_DETECT, pv_bytes(c, 0, np, true));
+        k_ve_detect<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np);
     }
     // exclusive scans with the totals stored one past the end
     scan_u32_to_u64(c, nres, roff, np, roff + np);
