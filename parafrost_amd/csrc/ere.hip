@@ -12,6 +12,7 @@
 #include "device.hpp"
 #include "passes.hpp"
 #include "prims.hpp"
+#include "stage.hpp"
 
 namespace pfg {
 
@@ -28,7 +29,7 @@ struct EreArgs {
     unsigned long long *cnt;
 };
 
-__device__ bool merge_ere(uint32_t x, const uint32_t *c1, const uint32_t *c2, const uint32_t *target)
+__device__ __forceinline__ bool merge_ere(uint32_t x, const uint32_t *c1, const uint32_t *c2, const uint32_t *target)
 {   // merge_ere (redundancy.h:37-85)
     const int n1 = c_size(c1), n2 = c_size(c2), n = c_size(target);
     const uint32_t *a = c1 + 3, *b = c2 + 3, *t = target + 3;
@@ -48,9 +49,19 @@ __device__ bool merge_ere(uint32_t x, const uint32_t *c1, const uint32_t *c2, co
     return true;
 }
 
-// forward_equ (redundancy.h:87-143). seq: delete in place (sequential replay);
-// otherwise claim the target for variable q. Returns false on a claim collision.
-__device__ bool forward_equ(const EreArgs &A, uint32_t x, const uint32_t *c1, const uint32_t *c2, bool seq, uint32_t q)
+// The resolvent of c1 and c2 on x as forward_equ (redundancy.h:87-143) sees it:
+// its length, signature, and the literal with the shortest occurrence list among
+// those met before one antecedent runs out. False when the resolvent is a
+// tautology or outside the size bounds. ST: c1 and c2 are staged clauses carrying
+// the list size of each literal after their literals (sizes do not change during
+// the pass).
+struct EreRes {
+    int len;
+    uint32_t msig, best, minsize, type;
+};
+
+template <bool ST>
+__device__ __forceinline__ bool ere_resolvent(const EreArgs &A, uint32_t x, const uint32_t *c1, const uint32_t *c2, EreRes &R)
 {
     const int n1 = c_size(c1), n2 = c_size(c2);
     const uint32_t *a = c1 + 3, *b = c2 + 3;
@@ -59,26 +70,45 @@ __device__ bool forward_equ(const EreArgs &A, uint32_t x, const uint32_t *c1, co
     bool any = false;
     while (it1 < n1 && it2 < n2) {
         const uint32_t l1 = a[it1], l2 = b[it2], v1 = l1 >> 1, v2 = l2 >> 1;
-        uint32_t lit;
+        uint32_t lit, ls;
         if (v1 == x) { it1++; continue; }
         else if (v2 == x) { it2++; continue; }
-        else if ((l1 ^ l2) == 1u) return true;
-        else if (v1 < v2) { it1++; lit = l1; }
-        else if (v2 < v1) { it2++; lit = l2; }
-        else { it1++; it2++; len--; lit = l1; }
-        const uint32_t ls = A.ot[lit].sz;
+        else if ((l1 ^ l2) == 1u) return false;
+        else if (v1 < v2) { lit = l1; ls = ST ? a[n1 + it1] : A.ot[lit].sz; it1++; }
+        else if (v2 < v1) { lit = l2; ls = ST ? b[n2 + it2] : A.ot[lit].sz; it2++; }
+        else { lit = l1; ls = ST ? a[n1 + it1] : A.ot[lit].sz; it1++; it2++; len--; }
         if (!any || ls < minsize) { minsize = ls; best = lit; any = true; }
         msig |= maphash(lit);
     }
-    if (!(len > 1 && (!A.maxsize || len <= A.maxsize))) return true;
-    if (!any) return true;   // unreachable for clauses of size >= 2
-    const uint32_t type = (c_learnt(c1) || c_learnt(c2)) ? ST_LEARNT : ST_ORIGINAL;
-    const OL o = A.ot[best];
-    for (uint32_t i = 0; i < minsize; i++) {
-        const uint64_t r = A.occ[o.off + i];
+    if (!(len > 1 && (!A.maxsize || len <= A.maxsize))) return false;
+    if (!any) return false;   // unreachable for clauses of size >= 2
+    R.len = len;
+    R.msig = msig;
+    R.best = best;
+    R.minsize = minsize;
+    R.type = (c_learnt(c1) || c_learnt(c2)) ? ST_LEARNT : ST_ORIGINAL;
+    return true;
+}
+
+__device__ __forceinline__ bool ere_match(const EreArgs &A, uint32_t x, const uint32_t *c1, const uint32_t *c2,
+                                          const EreRes &R, const uint32_t *c)
+{
+    return R.len == c_size(c) && (c_learnt(c) || c_st(c) == R.type) && sig_sub(R.msig, c[1]) && merge_ere(x, c1, c2, c);
+}
+
+// forward_equ (redundancy.h:87-143). seq: delete in place (sequential replay);
+// otherwise claim the target for variable q. Returns false on a claim collision.
+template <bool ST>
+__device__ __forceinline__ bool forward_equ(const EreArgs &A, uint32_t x, const uint32_t *c1, const uint32_t *c2, bool seq,
+                                            uint32_t q)
+{
+    EreRes R;
+    if (!ere_resolvent<ST>(A, x, c1, c2, R)) return true;
+    const uint64_t off = A.ot[R.best].off;
+    for (uint32_t i = 0; i < R.minsize; i++) {
+        const uint64_t r = A.occ[off + i];
         uint32_t *c = A.mem + r;
-        if (!(len == c_size(c) && (c_learnt(c) || c_st(c) == type) && sig_sub(msig, c[1]) && merge_ere(x, c1, c2, c)))
-            continue;
+        if (!ere_match(A, x, c1, c2, R, c)) continue;
         if (seq) { c_markdel(c); return true; }
         uint32_t *slot = A.owner + (r >> 1);
         const uint32_t cur = __atomic_load_n(slot, __ATOMIC_RELAXED);
@@ -92,28 +122,79 @@ __device__ bool forward_equ(const EreArgs &A, uint32_t x, const uint32_t *c1, co
     return true;
 }
 
-__device__ bool ere_var(const EreArgs &A, uint32_t x, bool seq, uint32_t q)
+// tab: the variable's staged region (stage.hpp), or null to read the pool
+template <bool ST>
+__device__ __forceinline__ bool ere_var(const EreArgs &A, uint32_t x, bool seq, uint32_t q, const uint32_t *tab,
+                                        const uint32_t *arena)
 {
     uint32_t dx = 2 * x, fx = dx + 1;
     if (A.ot[dx].sz > A.ot[fx].sz) { dx = fx; fx = 2 * x; }
     const OL me = A.ot[dx], other = A.ot[fx];
     if (!(me.sz <= (uint32_t)A.ere_limit && other.sz <= (uint32_t)A.ere_limit)) return true;
+    const uint32_t ps = A.ot[2 * x].sz;
+    const uint32_t mb = (dx & 1u) ? ps : 0u, ob = (fx & 1u) ? ps : 0u;
     for (uint32_t i = 0; i < me.sz; i++) {
-        const uint32_t *ci = A.mem + A.occ[me.off + i];
+        const uint32_t *ci = ST ? arena + tab[mb + i] : A.mem + A.occ[me.off + i];
         if (c_del(ci)) continue;
         for (uint32_t j = 0; j < other.sz; j++) {
-            const uint32_t *cj = A.mem + A.occ[other.off + j];
+            const uint32_t *cj = ST ? arena + tab[ob + j] : A.mem + A.occ[other.off + j];
             if (c_del(cj)) continue;
-            if (!forward_equ(A, x, ci, cj, seq, q)) return false;
+            if (!forward_equ<ST>(A, x, ci, cj, seq, q)) return false;
         }
     }
     return true;
 }
 
-__global__ void k_ere_claim(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs)
+// ERE runs in two steps. The probe looks at every (variable, clause of the shorter
+// list, clause of the other list) pair independently, one thread each, and flags
+// the variable when the pair's shortest list holds a clause equal to the resolvent
+// at all. A variable without such a pair deletes nothing, so only flagged
+// variables run the claiming pass below, which is the reference's loop.
+
+// pairs of variable q (0 when a list exceeds ere_limit)
+__global__ void k_ere_pairs(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs, uint64_t *__restrict__ npairs)
+{
+    for (uint64_t q = gtid(); q < npvs; q += gstride()) {
+        const uint32_t x = pvs[q];
+        const uint32_t ps = A.ot[2 * x].sz, ns = A.ot[2 * x + 1].sz;
+        npairs[q] = (ps <= (uint32_t)A.ere_limit && ns <= (uint32_t)A.ere_limit) ? (uint64_t)ps * ns : 0ull;
+    }
+}
+
+__global__ void k_ere_probe(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs, const uint64_t *__restrict__ poff,
+                            uint64_t total, uint32_t *__restrict__ flag)
+{
+    for (uint64_t g = gtid(); g < total; g += gstride()) {
+        uint32_t lo = 0, hi = npvs;   // last q with poff[q] <= g
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (poff[mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t q = lo, x = pvs[q];
+        uint32_t dx = 2 * x, fx = dx + 1;
+        if (A.ot[dx].sz > A.ot[fx].sz) { dx = fx; fx = 2 * x; }
+        const OL me = A.ot[dx], other = A.ot[fx];
+        const uint64_t r = g - poff[q];
+        const uint32_t i = (uint32_t)(r / other.sz), j = (uint32_t)(r % other.sz);
+        const uint32_t *ci = A.mem + A.occ[me.off + i];
+        const uint32_t *cj = A.mem + A.occ[other.off + j];
+        if (c_del(ci) || c_del(cj)) continue;
+        EreRes R;
+        if (!ere_resolvent<false>(A, x, ci, cj, R)) continue;
+        const uint64_t off = A.ot[R.best].off;
+        for (uint32_t k = 0; k < R.minsize; k++) {
+            if (ere_match(A, x, ci, cj, R, A.mem + A.occ[off + k])) { flag[q] = 1u; break; }
+        }
+    }
+}
+
+// the claiming pass, thread per flagged variable
+__global__ void __launch_bounds__(64) k_ere_claim(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs,
+                                                 const uint32_t *__restrict__ flag)
 {
     for (uint64_t q = gtid(); q < npvs; q += gstride())
-        if (!ere_var(A, pvs[q], false, (uint32_t)q)) atomicOr(&A.cnt[CNT_B], 1ull);
+        if (flag[q] && !ere_var<false>(A, pvs[q], false, (uint32_t)q, nullptr, nullptr)) atomicOr(&A.cnt[CNT_B], 1ull);
 }
 
 __global__ void k_ere_commit(EreArgs A, uint64_t n, int del)
@@ -128,7 +209,7 @@ __global__ void k_ere_commit(EreArgs A, uint64_t n, int del)
 __global__ void k_ere_seq(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs)
 {
     if (gtid() != 0) return;
-    for (uint32_t q = 0; q < npvs; q++) ere_var(A, pvs[q], true, q);
+    for (uint32_t q = 0; q < npvs; q++) ere_var<false>(A, pvs[q], true, q, nullptr, nullptr);
 }
 
 void ere(Ctx &c)
@@ -144,10 +225,22 @@ void ere(Ctx &c)
     EreArgs A{c.mem.p, c.ot.p, c.occ.p, c.opts.ere_limit, c.opts.ere_max_resolvent, c.owner.p, c.tmpu64b.p,
               (unsigned long long *)c.cnt.p};
     PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));
-    {
-        KTimer kt(c, K_ERE, pv_bytes(c, 0, c.npvs, true));
-        k_ere_claim<<<grid_for(c.npvs, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, c.npvs);
+    const uint32_t np = c.npvs;
+    ensure(c, c.tmpu64a, 2ull * np + 2);
+    ensure(c, c.tmpu32c, np + 1);
+    uint64_t *npairs = c.tmpu64a.p, *poff = c.tmpu64a.p + np + 1;
+    uint32_t *flag = c.tmpu32c.p;
+    PFG_HIP(hipMemsetAsync(flag, 0, 4ull * np, c.stream));
+    k_ere_pairs<<<grid_for(np, 256, 4096), 256, 0, c.stream>>>(A, c.pvs.p, np, npairs);
+    scan_u64(c, npairs, poff, np, poff + np);
+    uint64_t total = 0;
+    PFG_HIP(hipMemcpyAsync(&total, poff + np, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    if (total) {
+        KTimer kt(c, K_ERE, pv_bytes(c, 0, np, true));
+        k_ere_probe<<<grid_for(total, 256, 65535), 256, 0, c.stream>>>(A, c.pvs.p, np, poff, total, flag);
     }
+    k_ere_claim<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, np, flag);
     read_counters(c, 3);
     const uint64_t nt = c.hcnt[CNT_A];
     const bool collision = c.hcnt[CNT_B] != 0;

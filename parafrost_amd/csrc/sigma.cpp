@@ -55,10 +55,10 @@ PassTimer::~PassTimer()
 
 const char *const kernel_names[K_NUM] = {
     "upload_prep", "hist", "ot_fill", "ot_sort", "prop", "lcve_radix", "lcve_init", "lcve_pass_a", "lcve_pass_b",
-    "sort_pv_lists", "sub_range", "ve_detect", "ve_apply", "ere_claim", "gc_move", "count"};
+    "sort_pv_lists", "sub_staged", "ve_detect", "ve_apply", "ere_probe", "gc_move", "count"};
 const char *const kernel_symbols[K_NUM] = {
     "k_upload", nullptr, nullptr, nullptr, nullptr, nullptr, "k_elect_init", "k_elect_pass_a", "k_elect_pass_b",
-    "k_sort_pv_lists", "k_sub_range", "k_ve_detect", "k_ve_apply", "k_ere_claim", "k_gc_move", "k_count"};
+    "k_sort_pv_lists", "k_sub_staged", "k_ve_detect", "k_ve_apply", "k_ere_probe", "k_gc_move", "k_count"};
 
 KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
 {

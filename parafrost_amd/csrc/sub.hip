@@ -11,6 +11,7 @@
 #include "device.hpp"
 #include "passes.hpp"
 #include "prims.hpp"
+#include "stage.hpp"
 #include <vector>
 
 namespace pfg {
@@ -28,7 +29,7 @@ struct SubArgs {
     int allow_units;
 };
 
-__device__ bool sub_cl(const uint32_t *sm, const uint32_t *lr)
+__device__ __forceinline__ bool sub_cl(const uint32_t *sm, const uint32_t *lr)
 {   // sub(SCLAUSE&, SCLAUSE&) (subsume.h:39-55)
     int it1 = 0, it2 = 0, sub = 0;
     const int n1 = c_size(sm), n2 = c_size(lr);
@@ -42,7 +43,7 @@ __device__ bool sub_cl(const uint32_t *sm, const uint32_t *lr)
     return sub == n1;
 }
 
-__device__ bool selfsub_cl(uint32_t x, uint32_t fx, const uint32_t *sm, const uint32_t *lr)
+__device__ __forceinline__ bool selfsub_cl(uint32_t x, uint32_t fx, const uint32_t *sm, const uint32_t *lr)
 {   // selfsub (subsume.h:72-99)
     int it1 = 0, it2 = 0, sub = 0;
     bool self = false;
@@ -63,7 +64,7 @@ __device__ bool selfsub_cl(uint32_t x, uint32_t fx, const uint32_t *sm, const ui
     return false;
 }
 
-__device__ void toblivion_list(const SubArgs &A, uint32_t lit)
+__device__ __forceinline__ void toblivion_list(const SubArgs &A, uint32_t lit)
 {   // toblivion(OL&) (src/cpu/simplify.h:291-297)
     OL *o = A.ot + lit;
     const uint64_t *d = A.occ + o->off;
@@ -71,7 +72,7 @@ __device__ void toblivion_list(const SubArgs &A, uint32_t lit)
     o->sz = 0;
 }
 
-__device__ void strengthen(const SubArgs &A, uint32_t *c, uint32_t me)
+__device__ __forceinline__ void strengthen(const SubArgs &A, uint32_t *c, uint32_t me)
 {   // strengthen (src/cpu/elimination.cpp:199-231)
     uint32_t sig = 0;
     int n = 0;
@@ -104,7 +105,7 @@ __device__ void strengthen(const SubArgs &A, uint32_t *c, uint32_t me)
     }
 }
 
-__device__ void update_ol(const SubArgs &A, uint32_t lit)
+__device__ __forceinline__ void update_ol(const SubArgs &A, uint32_t lit)
 {   // updateOL (subsume.h:27-37)
     OL *o = A.ot + lit;
     uint64_t *d = A.occ + o->off;
@@ -117,7 +118,7 @@ __device__ void update_ol(const SubArgs &A, uint32_t lit)
     o->sz = j;
 }
 
-__device__ void subsume_l(const SubArgs &A, const uint64_t *list, uint32_t end, uint32_t *cand)
+__device__ __forceinline__ void subsume_l(const SubArgs &A, const uint64_t *list, uint32_t end, uint32_t *cand)
 {   // subsume (subsume.h:101-117)
     const int candsz = c_size(cand);
     for (uint32_t j = 0; j < end; j++) {
@@ -132,7 +133,7 @@ __device__ void subsume_l(const SubArgs &A, const uint64_t *list, uint32_t end, 
     }
 }
 
-__device__ void selfsubsume_l(const SubArgs &A, uint32_t x, uint32_t fx, uint32_t listlit, uint32_t *cand)
+__device__ __forceinline__ void selfsubsume_l(const SubArgs &A, uint32_t x, uint32_t fx, uint32_t listlit, uint32_t *cand)
 {   // selfsubsume (subsume.h:119-138)
     const OL *o = A.ot + listlit;
     const uint64_t *d = A.occ + o->off;
@@ -151,7 +152,7 @@ __device__ void selfsubsume_l(const SubArgs &A, uint32_t x, uint32_t fx, uint32_
     }
 }
 
-__device__ void self_sub_x(const SubArgs &A, uint32_t p)
+__device__ __forceinline__ void self_sub_x(const SubArgs &A, uint32_t p)
 {   // self_sub_x (subsume.h:140-176)
     const uint32_t n = p ^ 1u;
     for (int side = 0; side < 2; side++) {
@@ -167,6 +168,104 @@ __device__ void self_sub_x(const SubArgs &A, uint32_t p)
             subsume_l(A, d, i, cand);
         }
         update_ol(A, me);
+    }
+}
+
+// The same pass over a staged variable (stage.hpp, with the original-order table):
+// lists are table ranges, modified clause copies are marked dirty, and at the end
+// the dirty clause images and the compacted lists are written back to the pool.
+__device__ __forceinline__ void subsume_staged(uint32_t *arena, const uint32_t *list, uint32_t end, uint32_t *cand)
+{   // subsume (subsume.h:101-117)
+    const int candsz = c_size(cand);
+    for (uint32_t j = 0; j < end; j++) {
+        uint32_t *sg = arena + list[j];
+        if (c_del(sg)) continue;
+        if (c_molten(cand) && c_size(sg) > candsz) continue;
+        if (c_size(sg) > 1 && sig_sub(sg[1], cand[1]) && sub_cl(sg, cand)) {
+            if (c_learnt(sg) && c_orig(cand)) { c_setst(sg, ST_ORIGINAL); staged_dirty(sg); }
+            if (!c_del(cand)) { c_markdel(cand); staged_dirty(cand); }
+            return;
+        }
+    }
+}
+
+__device__ __forceinline__ void selfsubsume_staged(const SubArgs &A, uint32_t x, uint32_t fx, uint32_t *arena,
+                                                   const uint32_t *list, uint32_t n, uint32_t *cand)
+{   // selfsubsume (subsume.h:119-138)
+    const int candsz = c_size(cand);
+    const uint32_t candsig = cand[1];
+    for (uint32_t j = 0; j < n; j++) {
+        const uint32_t *sg = arena + list[j];
+        const int subsize = c_size(sg);
+        if (subsize > candsz) break;
+        if (c_del(sg)) continue;
+        if (subsize > 1 && sig_selfsub(sg[1], candsig) && selfsub_cl(x, fx, sg, cand)) {
+            strengthen(A, cand, x);
+            c_melt(cand);
+            staged_dirty(cand);
+            return;
+        }
+    }
+}
+
+__device__ __forceinline__ void self_sub_x_staged(const SubArgs &A, uint32_t p, uint32_t *tab, uint32_t *arena)
+{   // self_sub_x (subsume.h:140-176)
+    uint32_t len[2] = {A.ot[p].sz, A.ot[p ^ 1u].sz};
+    const uint32_t base[2] = {0u, len[0]}, ne = len[0] + len[1];
+    for (int side = 0; side < 2; side++) {
+        const uint32_t me = p ^ (uint32_t)side, other = me ^ 1u;
+        uint32_t *mt = tab + base[side];
+        const uint32_t ms = len[side];
+        for (uint32_t i = 0; i < ms; i++) {
+            uint32_t *cand = arena + mt[i];
+            if (c_size(cand) > HSE_MAX_CL_SIZE) break;
+            if (c_del(cand)) continue;
+            selfsubsume_staged(A, me, other, arena, tab + base[1 - side], len[1 - side], cand);
+            subsume_staged(arena, mt, i, cand);
+        }
+        uint32_t j = 0;   // updateOL (subsume.h:27-37)
+        for (uint32_t i = 0; i < ms; i++) {
+            uint32_t *c = arena + mt[i];
+            if (c_molten(c)) c_freeze(c);
+            else if (!c_del(c)) mt[j++] = mt[i];
+        }
+        len[side] = j;
+    }
+    for (uint32_t k = 0; k < ne; k++) {
+        const uint32_t *c = arena + tab[ne + k];
+        if (!staged_is_dirty(c)) continue;
+        uint32_t *g = A.mem + staged_ref(c);
+        const int n = c_size(c);
+        for (int t = 0; t < 3 + n; t++) g[t] = c[t];
+    }
+    for (int side = 0; side < 2; side++) {
+        OL *o = A.ot + (p ^ (uint32_t)side);
+        if (len[side] == o->sz) continue;   // nothing left the list, so its order is unchanged
+        uint64_t *d = A.occ + o->off;
+        for (uint32_t k = 0; k < len[side]; k++) d[k] = staged_ref(arena + tab[base[side] + k]);
+        o->sz = len[side];
+    }
+}
+
+constexpr uint32_t SUB_ARENA_WORDS = 8192;   // LDS words per 64-lane block (32 KB)
+
+// parallel range (no unit can arise): one 64-lane block per 64 elected variables
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_sub_staged(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e)
+{
+    __shared__ uint32_t arena[SUB_ARENA_WORDS];
+    const StageSrc S{A.mem, A.ot, A.occ};
+    // elected variables come in increasing score order: the heaviest blocks go first
+    const uint64_t nblk = ((uint64_t)(e - b) + 63) / 64;
+    for (uint64_t it = blockIdx.x; it < nblk; it += gridDim.x) {
+        const uint64_t q = b + (nblk - 1 - it) * 64 + threadIdx.x;
+        const bool active = q < e;
+        const uint32_t v = active ? pvs[q] : 0u;
+        const bool take = active && A.ot[2 * v].sz <= (uint32_t)A.sub_limit && A.ot[2 * v + 1].sz <= (uint32_t)A.sub_limit;
+        stage_rounds(S, arena, SUB_ARENA_WORDS, take, v, STAGE_TAB2, [&](uint32_t *tab) {
+            if (tab) self_sub_x_staged(A, 2 * v, tab, arena);
+            else self_sub_x(A, 2 * v);
+        });
     }
 }
 
@@ -231,7 +330,7 @@ void sub(Ctx &c)
     auto run_range = [&](uint32_t e) {
         if (e > b) {
             KTimer kt(c, K_SUB, pv_bytes(c, b, e, true));
-            k_sub_range<<<grid_for(e - b, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, b, e);
+            k_sub_staged<<<grid_for(e - b, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, b, e);
         }
     };
     for (uint64_t k = 0; k < nu; k++) {

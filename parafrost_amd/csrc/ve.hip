@@ -16,6 +16,7 @@
 #include "device.hpp"
 #include "passes.hpp"
 #include "prims.hpp"
+#include "stage.hpp"
 
 namespace pfg {
 
@@ -51,7 +52,7 @@ __device__ __forceinline__ void set_w0(const VeArgs &A, uint32_t *c, uint32_t w)
     c[0] = w;
     if (!A.tab) return;
     if (c >= A.arena && c < A.arena + VE_ARENA_WORDS) {
-        A.mem[(uint64_t)c[-2] | ((uint64_t)c[-1] << 32)] = w;
+        A.mem[staged_ref(c)] = w;
         return;
     }
     const uint64_t ref = (uint64_t)(c - A.mem);
@@ -486,41 +487,6 @@ __device__ __forceinline__ uint32_t bn_units(const VeArgs &A, uint32_t p, uint32
 }
 
 
-// Copy the clauses of both occurrence lists of variable v into the lane's arena
-// region [base, base + need): first the table (one word per list entry), then the
-// staged clauses. Returns the table pointer.
-__device__ __forceinline__ uint32_t *stage_var(const VeArgs &A, uint32_t *arena, uint32_t base, uint32_t v)
-{
-    uint32_t *tab = arena + base;
-    const OL lp = A.ot[2 * v], ln = A.ot[2 * v + 1];
-    uint32_t w = base + lp.sz + ln.sz, k = 0;
-    for (int s = 0; s < 2; s++) {
-        const OL l = s ? ln : lp;
-        for (uint32_t i = 0; i < l.sz; i++) {
-            const uint64_t ref = A.occ[l.off + i];
-            const uint32_t *g = A.mem + ref;
-            const uint32_t sz = g[2];
-            arena[w] = (uint32_t)ref;
-            arena[w + 1] = (uint32_t)(ref >> 32);
-            for (uint32_t t = 0; t < 3 + sz; t++) arena[w + 2 + t] = g[t];
-            tab[k++] = w + 2;
-            w += 5 + sz;
-        }
-    }
-    return tab;
-}
-
-__device__ __forceinline__ uint32_t stage_need(const VeArgs &A, uint32_t v)
-{
-    const OL lp = A.ot[2 * v], ln = A.ot[2 * v + 1];
-    uint32_t need = lp.sz + ln.sz;
-    for (int s = 0; s < 2; s++) {
-        const OL l = s ? ln : lp;
-        for (uint32_t i = 0; i < l.sz; i++) need += 5 + A.mem[A.occ[l.off + i] + 2];
-    }
-    return need;
-}
-
 struct VeDetectOut {
     uint32_t *type, *aux, *nres, *nunits;
     uint64_t *nwords, *nmodel;
@@ -590,33 +556,20 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
 k_ve_detect(VeArgs G, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t npvs)
 {
     __shared__ uint32_t arena[VE_ARENA_WORDS];
-    const int lane = threadIdx.x;
-    for (uint64_t qb = (uint64_t)blockIdx.x * 64; qb < npvs; qb += (uint64_t)gridDim.x * 64) {
-        const uint64_t q = qb + lane;
+    const StageSrc S{G.mem, G.ot, G.occ};
+    // elected variables come in increasing score order: the heaviest blocks go first
+    const uint64_t nblk = (npvs + 63) / 64;
+    for (uint64_t it = blockIdx.x; it < nblk; it += gridDim.x) {
+        const uint64_t q = (nblk - 1 - it) * 64 + threadIdx.x;
         const bool active = q < npvs;
         const uint32_t v = active ? pvs[q] : 0u;
-        const uint32_t need = active ? stage_need(G, v) : 0u;
-        bool pending = active;
-        while (__ballot(pending)) {
-            uint32_t incl = pending ? need : 0u;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t t = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += t;
-            }
-            const bool first = pending && incl == need;   // first pending lane of the round
-            if (pending && incl <= VE_ARENA_WORDS) {
-                const uint32_t *tab = stage_var(G, arena, incl - need, v);
+        stage_rounds(S, arena, VE_ARENA_WORDS, active, v, 0, [&](const uint32_t *tab) {
+            if (tab) {
                 const VeArgs A{G.mem, G.ot, G.occ, G.rlimit, G.lbound, G.xor_max_arity, v, G.ot[2 * v].sz, tab, arena};
                 ve_detect_var(A, O, q, v);
-                pending = false;
             }
-            else if (first) {   // larger than the arena
-                ve_detect_var(G, O, q, v);
-                pending = false;
-            }
-            __syncthreads();   // the arena is reused by the next round
-        }
+            else ve_detect_var(G, O, q, v);   // larger than the arena: from the pool
+        });
     }
 }
 
