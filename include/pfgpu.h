@@ -126,6 +126,10 @@ typedef struct pfg_kstat {
 } pfg_kstat;
 int pfg_kernel_stats(pfg_ctx *ctx, pfg_kstat *out, int max, int *n);
 int pfg_set_accounting(pfg_ctx *ctx, int on);
+/* Tuning: cap the LDS words per block that the staged per-variable kernels (SUB,
+ * BVE detection) use for a variable's clauses; a variable that does not fit is
+ * processed from the pool. Results do not depend on it. Default: the whole arena. */
+int pfg_set_stage_words(pfg_ctx *ctx, uint32_t words);
 int pfg_reset_stats(pfg_ctx *ctx);
 /* forget the clause database and all variable state (allocations are kept); the
    context can then take a new upload of the same variable range */

@@ -49,7 +49,7 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_stop", "pfg_next_phase", "pfg_hist", "pfg_download_cnf", "pfg_download_trail",
        "pfg_download_resolved", "pfg_download_vstate", "pfg_download_pvs", "pfg_download_ot_list",
        "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
-       "pfg_set_accounting"]
+       "pfg_set_accounting", "pfg_set_stage_words"]
 
 
 def lib_path() -> str:
@@ -87,6 +87,7 @@ def lib():
         L.pfg_kernel_stats.argtypes = [vp, ctypes.POINTER(KStat), i32, ctypes.POINTER(i32)]
         L.pfg_reset_stats.argtypes = [vp]
         L.pfg_set_accounting.argtypes = [vp, i32]
+        L.pfg_set_stage_words.argtypes = [vp, u32]
         L.pfg_reset.argtypes = [vp]
         L.pfg_last_error.argtypes = [vp]
         L.pfg_last_error.restype = ctypes.c_char_p
@@ -254,6 +255,10 @@ class Simplifier:
         return {arr[i].name.decode(): {"kernel": arr[i].kernel.decode() if arr[i].kernel else None,
                                        "ms": arr[i].ms, "launches": int(arr[i].launches), "bytes": arr[i].bytes}
                 for i in range(n.value)}
+
+    def set_stage_words(self, words: int):
+        """Cap the LDS words per block of the staged kernels (results do not depend on it)."""
+        self._check(self.L.pfg_set_stage_words(self.h, int(words)))
 
     def set_accounting(self, on: bool):
         """Compute algorithmic bytes of the per-variable kernels (adds a reduction before each)."""

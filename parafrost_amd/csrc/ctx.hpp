@@ -72,6 +72,7 @@ struct Ctx {
     bool hist_live = false;     // hist holds the live-literal counts of the current database
     bool trace = false;         // PFG_TRACE set: per-round progress on stderr
     bool account = false;       // compute algorithmic bytes of the per-variable kernels (extra reductions)
+    uint32_t stage_words = 0xFFFFFFFFu;   // cap on the LDS words per block the staged kernels use
 
     // variables
     DBuf<int8_t> value;         // per literal: -1 unassigned, 0 false, 1 true

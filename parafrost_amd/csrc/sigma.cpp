@@ -486,6 +486,13 @@ int pfg_kernel_stats(pfg_ctx *ctx, pfg_kstat *out, int max, int *n)
     API_END
 }
 
+int pfg_set_stage_words(pfg_ctx *ctx, uint32_t words)
+{
+    API_BEGIN
+    c.stage_words = words;
+    API_END
+}
+
 int pfg_set_accounting(pfg_ctx *ctx, int on)
 {
     API_BEGIN

@@ -251,7 +251,7 @@ constexpr uint32_t SUB_ARENA_WORDS = 8192;   // LDS words per 64-lane block (32 
 
 // parallel range (no unit can arise): one 64-lane block per 64 elected variables
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_sub_staged(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e)
+k_sub_staged(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e, uint32_t arena_words)
 {
     __shared__ uint32_t arena[SUB_ARENA_WORDS];
     const StageSrc S{A.mem, A.ot, A.occ};
@@ -262,7 +262,7 @@ k_sub_staged(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e
         const bool active = q < e;
         const uint32_t v = active ? pvs[q] : 0u;
         const bool take = active && A.ot[2 * v].sz <= (uint32_t)A.sub_limit && A.ot[2 * v + 1].sz <= (uint32_t)A.sub_limit;
-        stage_rounds(S, arena, SUB_ARENA_WORDS, take, v, STAGE_TAB2, [&](uint32_t *tab) {
+        stage_rounds(S, arena, arena_words, take, v, STAGE_TAB2, [&](uint32_t *tab) {
             if (tab) self_sub_x_staged(A, 2 * v, tab, arena);
             else self_sub_x(A, 2 * v);
         });
@@ -330,7 +330,8 @@ void sub(Ctx &c)
     auto run_range = [&](uint32_t e) {
         if (e > b) {
             KTimer kt(c, K_SUB, pv_bytes(c, b, e, true));
-            k_sub_staged<<<grid_for(e - b, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, b, e);
+            k_sub_staged<<<grid_for(e - b, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, b, e,
+                                                                           std::min(SUB_ARENA_WORDS, c.stage_words));
         }
     };
     for (uint64_t k = 0; k < nu; k++) {

@@ -553,7 +553,7 @@ __device__ __forceinline__ void ve_detect_var(const VeArgs &A, const VeDetectOut
 // detected from the pool instead. Detection of one variable touches only its own
 // clauses, so the rounds do not change results.
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_ve_detect(VeArgs G, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t npvs)
+k_ve_detect(VeArgs G, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t npvs, uint32_t arena_words)
 {
     __shared__ uint32_t arena[VE_ARENA_WORDS];
     const StageSrc S{G.mem, G.ot, G.occ};
@@ -563,7 +563,7 @@ k_ve_detect(VeArgs G, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t 
         const uint64_t q = (nblk - 1 - it) * 64 + threadIdx.x;
         const bool active = q < npvs;
         const uint32_t v = active ? pvs[q] : 0u;
-        stage_rounds(S, arena, VE_ARENA_WORDS, active, v, 0, [&](const uint32_t *tab) {
+        stage_rounds(S, arena, arena_words, active, v, 0, [&](const uint32_t *tab) {
             if (tab) {
                 const VeArgs A{G.mem, G.ot, G.occ, G.rlimit, G.lbound, G.xor_max_arity, v, G.ot[2 * v].sz, tab, arena};
                 ve_detect_var(A, O, q, v);
@@ -719,7 +719,8 @@ void ve(Ctx &c)
     {
         const VeDetectOut DO{type, aux, nres, nunits, nwords, nmodel};
         KTimer kt(c, K_VE_DETECT, pv_bytes(c, 0, np, true));
-        k_ve_detect<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np);
+        k_ve_detect<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np,
+                                                                  std::min(VE_ARENA_WORDS, c.stage_words));
     }
     // exclusive scans with the totals stored one past the end
     scan_u32_to_u64(c, nres, roff, np, roff + np);

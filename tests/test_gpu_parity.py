@@ -65,6 +65,29 @@ def test_lockstep_long_lists(seed):
     ls.run_lockstep(cnf, lists=True, phases=5, mu_pos=int(rng.integers(64, 400)), mu_neg=int(rng.integers(64, 400)))
 
 
+@pytest.mark.parametrize("words", [0, 40, 400])
+@pytest.mark.parametrize("seed", [0, 1, 2, 4, 5, 7])
+def test_lockstep_stage_limits(seed, words):
+    """SUB and BVE detection with the LDS stage capped: 0 reads every variable from
+    the pool, small caps stage a block's variables over many rounds and leave the
+    larger ones unstaged. Results must not change."""
+    print(f"lockstep stage limit {words} seed={seed}", flush=True)
+    cnf, kw = _small(seed)
+    ls.run_lockstep(cnf, setup=lambda g: g.set_stage_words(words), **kw)
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_lockstep_long_lists_stage_limit(seed):
+    """Long lists with a stage cap below their size: the unstaged fallback on
+    variables with std::sort-ordered lists and gates."""
+    print(f"lockstep long lists stage limit seed={seed}", flush=True)
+    rng = np.random.default_rng(9000 + seed)
+    nv = int(rng.integers(3000, 12000))
+    cnf = gen.scalefree(nv, int(nv * rng.uniform(3.0, 5.0)), 2.5, 0.2, 9000 + seed)
+    ls.run_lockstep(cnf, lists=True, setup=lambda g: g.set_stage_words(300), phases=5,
+                    mu_pos=int(rng.integers(64, 400)), mu_neg=int(rng.integers(64, 400)))
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_lockstep_learnt_clauses(seed):
     """Learnt clauses (status LEARNT with an LBD): ERE types, subsumption promotion,
