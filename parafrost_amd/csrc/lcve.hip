@@ -63,12 +63,12 @@ __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, co
 }
 
 __global__ void __launch_bounds__(1024) k_work_init(const uint8_t *__restrict__ kind, uint64_t b0, uint32_t *__restrict__ work,
-                                                   unsigned long long *cnt)
+                                                   unsigned long long *nwork)
 {
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < b0; b += gstride()) {
         const uint64_t i = b + threadIdx.x;
         const uint8_t k = i < b0 ? kind[i] : (uint8_t)K_SKIP;
-        block_append(k == K_CAND || k == K_DYNBREAK, (uint32_t)i, work, &cnt[CNT_A]);
+        block_append(k == K_CAND || k == K_DYNBREAK, (uint32_t)i, work, nwork);
     }
 }
 
@@ -96,7 +96,8 @@ __global__ void k_crank(const uint32_t *__restrict__ elig, uint32_t nvars, const
 }
 
 // Live clauses with at least two candidates take part in the election. Each is
-// written once as a compact record [k, rank_1 .. rank_k] of its candidates, so the
+// written once as a compact record [k, code_1 .. code_k] of its candidates (code =
+// rank << 1 | freezes neighbours), so the
 // rounds stream records instead of clauses and rank lookups. Round 0's pass A is
 // fused in: nothing is elected yet, so every candidate ranked after the clause's
 // earliest freezing candidate is blocked. The first ELECT_CACHE candidate codes of
@@ -104,7 +105,7 @@ __global__ void k_crank(const uint32_t *__restrict__ elig, uint32_t nvars, const
 constexpr int ELECT_CACHE = 16;
 __global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t *__restrict__ refs, uint64_t n,
                                                    uint32_t *__restrict__ rec, uint64_t *__restrict__ active,
-                                                   unsigned long long *cnt)
+                                                   unsigned long long *nactive, unsigned long long *cnt)
 {
     __shared__ uint32_t s_cr[ELECT_CACHE][256];
     const unsigned tid = threadIdx.x;
@@ -128,14 +129,14 @@ __global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t 
         }
         const bool keep = k >= 2u;
         const uint64_t off = block_reserve(keep ? 1ull + k : 0ull, &cnt[CNT_D]);
-        const uint64_t aoff = block_reserve(keep ? 1ull : 0ull, &cnt[CNT_C]);
+        const uint64_t aoff = block_reserve(keep ? 1ull : 0ull, nactive);
         if (keep) {
             uint32_t *r = rec + off;
             r[0] = k;
             const uint32_t kc = k < (uint32_t)ELECT_CACHE ? k : (uint32_t)ELECT_CACHE;
             for (uint32_t w = 0; w < kc; w++) {
-                const uint32_t j = s_cr[w][tid] >> 1;
-                r[1 + w] = j;
+                const uint32_t x = s_cr[w][tid], j = x >> 1;
+                r[1 + w] = x;
                 if (j > m) A.blocked[j] = 1;
             }
             if (k > (uint32_t)ELECT_CACHE) {
@@ -145,7 +146,7 @@ __global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t 
                     if (x == NO_CAND) continue;
                     if (++w <= (uint32_t)ELECT_CACHE) continue;
                     const uint32_t j = x >> 1;
-                    r[w] = j;
+                    r[w] = x;
                     if (j > m) A.blocked[j] = 1;
                 }
             }
@@ -154,11 +155,14 @@ __global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t 
     }
 }
 
-// pass A over the active clause records
+// pass A over the active clause records (count read on the device: rounds are
+// launched back to back and the host looks at the counts every few rounds)
 __global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32_t *__restrict__ rec,
-                                                      const uint64_t *__restrict__ active, uint64_t n,
-                                                      uint64_t *__restrict__ next, unsigned long long *cnt)
+                                                      const uint64_t *__restrict__ active,
+                                                      const unsigned long long *__restrict__ nactive,
+                                                      uint64_t *__restrict__ next, unsigned long long *nnext)
 {
+    const uint64_t n = *nactive;
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
         const uint64_t i = b + threadIdx.x;
         bool keep = false;
@@ -171,9 +175,9 @@ __global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32
             uint32_t m = 0xFFFFFFFFu;
             uint32_t undec = 0, alive = 0;
             for (uint32_t t = 1; t <= k; t++) {
-                const uint32_t j = rr[t];
+                const uint32_t x = rr[t], j = x >> 1;
                 const uint8_t st = A.status[j];
-                const bool freezer = A.kind[j] == K_CAND;
+                const bool freezer = x & 1u;
                 if (st == S_IN && freezer) in_freezer = true;
                 if (st == S_UNDEC) {
                     undec++;
@@ -183,7 +187,7 @@ __global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32
             }
             if (undec) {
                 for (uint32_t t = 1; t <= k; t++) {
-                    const uint32_t j = rr[t];
+                    const uint32_t j = rr[t] >> 1;
                     if (A.status[j] != S_UNDEC) continue;
                     if (in_freezer) A.status[j] = S_OUT;
                     else if (j > m) A.blocked[j] = 1;
@@ -191,15 +195,17 @@ __global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32
             }
             keep = !in_freezer && undec >= 1u && alive >= 2u;
         }
-        const uint64_t o = block_reserve(keep ? 1ull : 0ull, &cnt[CNT_C]);
+        const uint64_t o = block_reserve(keep ? 1ull : 0ull, nnext);
         if (keep) next[o] = r;
     }
 }
 
 // pass B over the undecided candidates
-__global__ void __launch_bounds__(1024) k_elect_pass_b(ElectArgs A, const uint32_t *__restrict__ work, uint64_t n,
-                                                      uint32_t *__restrict__ next, unsigned long long *cnt)
+__global__ void __launch_bounds__(1024) k_elect_pass_b(ElectArgs A, const uint32_t *__restrict__ work,
+                                                      const unsigned long long *__restrict__ nwork,
+                                                      uint32_t *__restrict__ next, unsigned long long *nnext)
 {
+    const uint64_t n = *nwork;
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
         const uint64_t q = b + threadIdx.x;
         bool keep = false;
@@ -211,7 +217,7 @@ __global__ void __launch_bounds__(1024) k_elect_pass_b(ElectArgs A, const uint32
                 else A.status[j] = S_IN;
             }
         }
-        block_append(keep, j, next, &cnt[CNT_A]);
+        block_append(keep, j, next, nnext);
     }
 }
 
@@ -259,46 +265,55 @@ uint32_t lcve(Ctx &c)
     read_counters(c, 3);
     const uint64_t b0 = c.hcnt[CNT_B];
     k_crank<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, kind, b0, c.rank.p);
+    // per-round counts on the device: rw[r] = undecided candidates entering round r,
+    // ra[r] = active clause records entering round r's pass A (rounds 1..)
+    constexpr int MAXR = 256;
+    ensure(c, c.tmpu64c, 2ull * MAXR + 2);
+    unsigned long long *rw = (unsigned long long *)c.tmpu64c.p, *ra = rw + MAXR + 1;
+    PFG_HIP(hipMemsetAsync(rw, 0, (2ull * MAXR + 2) * 8, c.stream));
     uint32_t *work = c.tmpu32a.p, *work2 = c.tmpu32b.p;
     uint64_t *act = c.tmpu64a.p, *act2 = c.tmpu64b.p;
-    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 32, c.stream));
-    k_work_init<<<grid_for(b0, 1024, 4096), 1024, 0, c.stream>>>(kind, b0, work, (unsigned long long *)c.cnt.p);
+    k_work_init<<<grid_for(b0, 1024, 4096), 1024, 0, c.stream>>>(kind, b0, work, rw);
     ElectArgs A{c.mem.p, c.rank.p, kind, status, blocked, b0};
     if (c.nrefs && b0) {
         KTimer kt(c, K_LCVE_INIT, 20.0 * c.nrefs + 8.0 * c.nliterals);
+        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_D, 0, 8, c.stream));
         k_elect_init<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act,
-                                                                          (unsigned long long *)c.cnt.p);
+                                                                          ra + 1, (unsigned long long *)c.cnt.p);
     }
-    read_counters(c, 5);
-    uint64_t nwork = c.hcnt[CNT_A], nact = c.hcnt[CNT_C];
+    uint64_t hw[2] = {0, 0};
+    PFG_HIP(hipMemcpyAsync(hw, rw, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipMemcpyAsync(hw + 1, ra + 1, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    uint64_t nwork = hw[0], nact = hw[1];   // upper bounds of later rounds' counts (sizes the grids)
     if (c.trace)
         fprintf(stderr, "[pfg] LCVE b0 %llu candidates %llu clauses %llu\n", (unsigned long long)b0, (unsigned long long)nwork,
                 (unsigned long long)nact);
-    int rounds = 0;
-    bool fused = true;   // round 0's pass A ran inside k_elect_init
+    // round 0's pass A ran inside k_elect_init; rounds are launched in batches and
+    // the count of undecided candidates is checked after each batch
+    int r = 0;
     while (nwork) {
-        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 24, c.stream));
-        if (!fused && nact) {
-            KTimer kt(c, K_LCVE_PASS_A, 0.0);
-            k_elect_pass_a<<<grid_for(nact, 1024, 8192), 1024, 0, c.stream>>>(A, c.erec.p, act, nact, act2,
-                                                                              (unsigned long long *)c.cnt.p);
+        const int batch = r == 0 ? 2 : 4;
+        for (int k = 0; k < batch && r < MAXR - 1; k++, r++) {
+            if (r > 0) {
+                KTimer kt(c, K_LCVE_PASS_A, 0.0);
+                k_elect_pass_a<<<grid_for(nact, 1024, 8192), 1024, 0, c.stream>>>(A, c.erec.p, act, ra + r, act2, ra + r + 1);
+                std::swap(act, act2);
+            }
+            {
+                KTimer kt(c, K_LCVE_PASS_B, 0.0);
+                k_elect_pass_b<<<grid_for(nwork, 1024, 8192), 1024, 0, c.stream>>>(A, work, rw + r, work2, rw + r + 1);
+            }
+            std::swap(work, work2);
         }
-        {
-            KTimer kt(c, K_LCVE_PASS_B, 0.0);
-            k_elect_pass_b<<<grid_for(nwork, 1024, 8192), 1024, 0, c.stream>>>(A, work, nwork, work2, (unsigned long long *)c.cnt.p);
-        }
-        read_counters(c, 4);
-        if (c.trace) fprintf(stderr, "[pfg] LCVE round %d: undecided %llu -> %llu, clauses %llu -> %llu\n", rounds,
-                             (unsigned long long)nwork, (unsigned long long)c.hcnt[CNT_A], (unsigned long long)nact,
-                             (unsigned long long)c.hcnt[CNT_C]);
-        nwork = c.hcnt[CNT_A];
-        if (!fused) {
-            nact = c.hcnt[CNT_C];
-            std::swap(act, act2);
-        }
-        fused = false;
-        std::swap(work, work2);
-        if (++rounds > (1 << 24)) throw Error(PFG_EINTERNAL, "LCVE election did not converge");
+        if (r >= MAXR - 1) throw Error(PFG_EINTERNAL, "LCVE election did not converge");
+        PFG_HIP(hipMemcpyAsync(hw, rw + r, 8, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipMemcpyAsync(hw + 1, ra + r, 8, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+        if (c.trace) fprintf(stderr, "[pfg] LCVE after round %d: undecided %llu, clauses %llu\n", r - 1,
+                             (unsigned long long)hw[0], (unsigned long long)hw[1]);
+        nwork = hw[0];
+        nact = hw[1];
     }
     const unsigned long long bb = b0;
     PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_B, &bb, 8, hipMemcpyHostToDevice, c.stream));
