@@ -242,9 +242,9 @@ static void order_lists_by_ref(Ctx &c)
         PFG_HIP(hipMemcpyAsync(c.tmpu64c.p, base.data(), nb * 8, hipMemcpyHostToDevice, c.stream));
         k_big_gather<<<(unsigned)std::min<uint64_t>(nb, 4096), 256, 0, c.stream>>>(c.eligible.p, c.tmpu64c.p, (uint32_t)nb,
                                                                                  c.ot.p, c.occ.p, c.tmpu64b.p);
-        radix_sort_u64_keys(c, c.tmpu64b.p, tot, 32 + bits_for(nb));
+        const uint64_t *sorted = radix_sort_u64_keys(c, c.tmpu64b.p, tot, 32 + bits_for(nb));
         k_big_scatter<<<(unsigned)std::min<uint64_t>(nb, 4096), 256, 0, c.stream>>>(c.eligible.p, c.tmpu64c.p, (uint32_t)nb,
-                                                                                  c.ot.p, c.occ.p, c.tmpu64b.p);
+                                                                                  c.ot.p, c.occ.p, sorted);
         PFG_HIP(hipStreamSynchronize(c.stream));  // host vectors go out of scope
     }
 }
@@ -327,8 +327,8 @@ static void create_ot_radix(Ctx &c)
         {
             const int lbits = bits_of(c.nlits2 - 1);
             KTimer kt(c, K_OT_SORT, 32.0 * L * ((lbits + 7) / 8));
-            radix_sort_u64_keys(c, c.tmpu64b.p, L, lbits, 32);
-            k_ot_bounds<<<grid_for(L, 256, 16384), 256, 0, c.stream>>>(c.tmpu64b.p, L, c.ot.p, c.occ.p);
+            const uint64_t *sorted = radix_sort_u64_keys(c, c.tmpu64b.p, L, lbits, 32);
+            k_ot_bounds<<<grid_for(L, 256, 16384), 256, 0, c.stream>>>(sorted, L, c.ot.p, c.occ.p);
         }
     }
     k_ot_finish<<<grid_for(c.nlits2, 256, 8192), 256, 0, c.stream>>>(c.ot.p, c.hist.p, c.nlits2);

@@ -142,16 +142,32 @@ __global__ void __launch_bounds__(RS_BLOCK) k_radix_hist(const K *__restrict__ k
     counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Stable scatter of one tile: the tile's keys are ranked by digit (8 wave ballots
+// per round), reordered in LDS, and written out as contiguous runs per digit, so
+// global writes are coalesced. The tile's digit starts come from its histogram.
 template <typename K, bool V>
 __global__ void __launch_bounds__(RS_BLOCK) k_radix_scatter(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                             K *__restrict__ kout, uint32_t *__restrict__ vout, uint64_t n,
-                                                            int shift, const uint64_t *__restrict__ offs, uint32_t ntiles)
+                                                            int shift, const uint64_t *__restrict__ offs,
+                                                            const uint32_t *__restrict__ counts, uint32_t ntiles)
 {
-    __shared__ uint64_t run[256];
+    __shared__ K sk[RS_TILE];
+    __shared__ uint32_t sv[V ? RS_TILE : 1];
+    __shared__ uint32_t lbase[256], lrun[256], wsum[RS_BLOCK / 64];
+    __shared__ uint64_t gbase[256];
     __shared__ uint32_t wc[RS_BLOCK / 64][256];
     const int tid = threadIdx.x, w = tid >> 6, lane = __lane_id();
     const uint64_t lt = (1ull << lane) - 1ull;
-    run[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
+    // tile-local start of every digit: exclusive scan of the tile's digit counts
+    const uint32_t cnt = counts[(uint64_t)tid * ntiles + blockIdx.x];
+    const uint32_t incl = wave_incl_scan(cnt);
+    if (lane == 63) wsum[w] = incl;
+    gbase[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
+    __syncthreads();
+    uint32_t excl = incl - cnt;
+    for (int q = 0; q < w; q++) excl += wsum[q];
+    lbase[tid] = excl;
+    lrun[tid] = excl;
     const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
     for (int r = 0; r < RS_ITEMS; r++) {
 #pragma unroll
@@ -175,23 +191,34 @@ __global__ void __launch_bounds__(RS_BLOCK) k_radix_scatter(const K *__restrict_
         }
         __syncthreads();
         if (valid) {
-            uint64_t pos = run[d] + rank;
+            uint32_t pos = lrun[d] + rank;
             for (int q = 0; q < w; q++) pos += wc[q][d];
-            kout[pos] = key;
-            if (V) vout[pos] = vin[i];
+            sk[pos] = key;
+            if (V) sv[pos] = vin[i];
         }
         __syncthreads();
         uint32_t add = 0;
 #pragma unroll
         for (int q = 0; q < RS_BLOCK / 64; q++) add += wc[q][tid];
-        run[tid] += add;
+        lrun[tid] += add;
+    }
+    __syncthreads();
+    const uint32_t tn = (uint32_t)((n - base) < (uint64_t)RS_TILE ? (n - base) : (uint64_t)RS_TILE);
+    for (uint32_t p = tid; p < tn; p += RS_BLOCK) {
+        const K key = sk[p];
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const uint64_t g = gbase[d] + (p - lbase[d]);
+        kout[g] = key;
+        if (V) vout[g] = sv[p];
     }
 }
 
+// returns the buffer holding the sorted keys (keys or the context's temporary);
+// with copy_back the result is always moved into keys/vals
 template <typename K, bool V>
-static void radix_impl(Ctx &c, K *keys, uint32_t *vals, uint64_t n, int nbits, int lo = 0)
+static K *radix_impl(Ctx &c, K *keys, uint32_t *vals, uint64_t n, int nbits, int lo, bool copy_back)
 {
-    if (n <= 1 || nbits <= 0) return;
+    if (n <= 1 || nbits <= 0) return keys;
     const uint64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
     const uint64_t cnt_n = 256 * ntiles;
     auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
@@ -207,24 +234,27 @@ static void radix_impl(Ctx &c, K *keys, uint32_t *vals, uint64_t n, int nbits, i
     for (int shift = lo; shift < lo + nbits; shift += 8) {
         k_radix_hist<K><<<(unsigned)ntiles, RS_BLOCK, 0, c.stream>>>(a, n, shift, counts, (uint32_t)ntiles);
         scan_u32_to_u64(c, counts, offs, cnt_n, nullptr);
-        k_radix_scatter<K, V><<<(unsigned)ntiles, RS_BLOCK, 0, c.stream>>>(a, va, b, vb, n, shift, offs, (uint32_t)ntiles);
+        k_radix_scatter<K, V><<<(unsigned)ntiles, RS_BLOCK, 0, c.stream>>>(a, va, b, vb, n, shift, offs, counts,
+                                                                            (uint32_t)ntiles);
         K *tk = a; a = b; b = tk;
         uint32_t *tv = va; va = vb; vb = tv;
     }
-    if (a != keys) {
+    if (copy_back && a != keys) {
         PFG_HIP(hipMemcpyAsync(keys, a, n * sizeof(K), hipMemcpyDeviceToDevice, c.stream));
         if (V) PFG_HIP(hipMemcpyAsync(vals, va, n * 4, hipMemcpyDeviceToDevice, c.stream));
+        return keys;
     }
+    return a;
 }
 
 void radix_sort_u32_pairs(Ctx &c, uint32_t *keys, uint32_t *vals, uint64_t n, int nbits)
 {
-    radix_impl<uint32_t, true>(c, keys, vals, n, nbits);
+    radix_impl<uint32_t, true>(c, keys, vals, n, nbits, 0, true);
 }
 
-void radix_sort_u64_keys(Ctx &c, uint64_t *keys, uint64_t n, int nbits, int lo)
+uint64_t *radix_sort_u64_keys(Ctx &c, uint64_t *keys, uint64_t n, int nbits, int lo)
 {
-    radix_impl<uint64_t, false>(c, keys, nullptr, n, nbits, lo);
+    return radix_impl<uint64_t, false>(c, keys, nullptr, n, nbits, lo, false);
 }
 
 __global__ void k_max_u32(const uint32_t *__restrict__ in, uint64_t n, unsigned long long *out)
