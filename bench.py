@@ -42,7 +42,72 @@ def parse():
     ap.add_argument("--check", action="store_true", help="compare the final state with the oracle")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--summarize-trace", metavar="KERNEL_STATS_CSV",
+                    help="print a per-step kernel table from a rocprofv3 --stats kernel_stats.csv and exit")
+    ap.add_argument("--trace-steps", type=int, default=1, help="sigmify calls covered by the traced run")
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="print per-kernel HBM bytes per dispatch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
+                         "counter_collection.csv files, scaled by the copy calibration, and exit")
     return ap.parse_args()
+
+
+COPY_BYTES = 256 << 20
+
+
+def _kname(n: str) -> str:
+    n = n.split("(")[0].replace("void ", "").strip()
+    return n.split("::")[-1].split("<")[0]
+
+
+def summarize_trace(path: str, steps: int):
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    print(f"total kernel time per sigmify: {tot / steps / 1e6:.3f} ms ({steps} sigmify calls traced)")
+    print()
+    print("| kernel | launches / call | ms / call | avg us / launch | share |")
+    print("|---|---|---|---|---|")
+    for r in rows:
+        t = float(r["TotalDurationNs"])
+        if t / tot < 0.002:
+            continue
+        print(f"| {_kname(r['Name'])} | {int(r['Calls']) / steps:.1f} | {t / steps / 1e6:.3f} | "
+              f"{float(r['AverageNs']) / 1e3:.1f} | {100 * t / tot:.1f}% |")
+
+
+def _pmc(path: str):
+    import csv, collections
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        d[_kname(r["Kernel_Name"])].append(float(r["Counter_Value"]) * 1024.0)   # KiB per dispatch
+    return d
+
+
+def summarize_pmc(fetch: str, write: str, cfetch: str, cwrite: str):
+    f, w, cf, cw = _pmc(fetch), _pmc(write), _pmc(cfetch), _pmc(cwrite)
+    cf_b = [v for k, vs in cf.items() if "copyBuffer" in k for v in vs]
+    cw_b = [v for k, vs in cw.items() if "copyBuffer" in k for v in vs]
+    fs = COPY_BYTES / (sum(cf_b) / len(cf_b))
+    ws = COPY_BYTES / (sum(cw_b) / len(cw_b))
+    print(f"# copy calibration: 256 MiB device copy reads FETCH_SIZE x{1 / fs:.3f}, WRITE_SIZE x{1 / ws:.3f} "
+          f"of the true bytes; factors applied: fetch {fs:.4f}, write {ws:.4f}")
+    print("# kernel launches fetch_bytes_per_launch write_bytes_per_launch")
+    for k in sorted(f, key=lambda k: -sum(f[k])):
+        fb = sum(f[k]) / len(f[k]) * fs
+        wb = (sum(w[k]) / len(w[k]) * ws) if k in w else 0.0
+        print(f"{k} {len(f[k])} {fb:.0f} {wb:.0f}")
+
+
+def traffic_for(cfg: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed PMC summary of this workload."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{cfg}.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        t = line.split()
+        if len(t) == 4 and not line.startswith("#") and t[0] == kernel:
+            return int(float(t[2]) + float(t[3]))
+    return None
 
 
 def copy_calibration():
@@ -118,6 +183,12 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.trace_steps)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
     import torch
     import parafrost_amd as pf
     from parafrost_amd import gen
@@ -158,6 +229,10 @@ def main():
     ka = s.kernel_times()
     torch.cuda.synchronize()
     roof, kernels = roofline(kt, ka)
+    if roof:
+        roof["traffic"] = traffic_for(args.config, roof["kernel"])
+        if roof["traffic"] is not None:
+            roof["traffic_source"] = f"profiles/traffic_{args.config}.txt (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE)"
     out = {
         "metric": "simplified literals/sec on 1 MI355X (batched/served vs plain); p50/p99 latency",
         "value": L * args.steps / dt,

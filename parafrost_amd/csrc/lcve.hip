@@ -234,6 +234,30 @@ __global__ void k_elect_flags(const uint8_t *__restrict__ kind, const uint8_t *_
         flags[i] = (i < brk && kind[i] == K_CAND && status[i] == S_IN) ? 1u : 0u;
 }
 
+// algorithmic bytes of one pass A (accounting only): per active record its offset
+// (8), the record (4 (k + 1)) and one status byte per candidate
+__global__ void k_elect_bytes(const uint32_t *__restrict__ rec, const uint64_t *__restrict__ active,
+                              const unsigned long long *__restrict__ nactive, unsigned long long *__restrict__ out)
+{
+    const uint64_t n = *nactive;
+    unsigned long long acc = 0;
+    for (uint64_t i = gtid(); i < n; i += gstride()) acc += 12ull + 5ull * rec[active[i]];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0 && acc) atomicAdd(out, acc);
+}
+
+static double elect_bytes(Ctx &c, const uint64_t *active, const unsigned long long *nactive, uint64_t bound)
+{
+    if (!c.account || !bound) return 0.0;
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_ACCT, 0, 8, c.stream));
+    k_elect_bytes<<<grid_for(bound, 256, 4096), 256, 0, c.stream>>>(c.erec.p, active, nactive,
+                                                                     (unsigned long long *)c.cnt.p + CNT_ACCT);
+    uint64_t v = 0;
+    PFG_HIP(hipMemcpyAsync(&v, c.cnt.p + CNT_ACCT, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    return (double)v;
+}
+
 uint32_t lcve(Ctx &c)
 {
     const uint32_t nv = c.nvars;
@@ -296,7 +320,7 @@ uint32_t lcve(Ctx &c)
         const int batch = r == 0 ? 2 : 4;
         for (int k = 0; k < batch && r < MAXR - 1; k++, r++) {
             if (r > 0) {
-                KTimer kt(c, K_LCVE_PASS_A, 0.0);
+                KTimer kt(c, K_LCVE_PASS_A, elect_bytes(c, act, ra + r, nact));
                 k_elect_pass_a<<<grid_for(nact, 1024, 8192), 1024, 0, c.stream>>>(A, c.erec.p, act, ra + r, act2, ra + r + 1);
                 std::swap(act, act2);
             }
