@@ -432,7 +432,7 @@ __global__ void k_prop_apply(uint32_t *__restrict__ mem, const uint64_t *__restr
         }
         c[1] = sig;
         c[2] = (uint32_t)j;
-        if (j == 0) atomicOr(&cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT);
+        if (j == 0) report_conflict(cnt, lits[0] >> 1);   // every literal false: lits[] untouched
         else if (j == 1) enqueue_dev(value, vstate, trail, cnt, lits[0]);
     }
 }
@@ -641,11 +641,11 @@ __global__ void k_units_keep(const uint32_t *__restrict__ u, uint64_t n, const u
         const uint32_t l = u[i];
         const int8_t v = value[l];
         uint32_t k = 0;
-        if (v >= 0) { if (v == 0) atomicOr(&cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT); }
+        if (v >= 0) { if (v == 0) report_conflict(cnt, l >> 1); }
         else {
             const uint32_t f = first[l >> 1];
             if (f == (uint32_t)i) k = 1;
-            else if (u[f] != l) atomicOr(&cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT);
+            else if (u[f] != l) report_conflict(cnt, l >> 1);
         }
         keep[i] = k;
     }

@@ -36,6 +36,14 @@ struct DBuf {
     uint64_t cap = 0;
 };
 
+// device counter slots (Ctx::cnt) and flags. Slot CNT_CONF counts contradictions;
+// the variables involved follow it (at most CONF_CAP), so that a batch can tell
+// which instances were refuted.
+enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_NTRAIL = 8, CNT_FLAGS = 9, CNT_ACCT = 10,
+       CNT_CONF = 16 };
+enum { FLAG_UNSAT = 1, FLAG_INTERNAL = 2 };
+constexpr uint64_t CONF_CAP = 1ull << 20;
+
 // event timers: most wrap exactly one kernel (kernel_symbols names it), the
 // rest a short sequence of launches (kernel_symbols entry null)
 enum KId {
@@ -97,6 +105,17 @@ struct Ctx {
     uint64_t *hcnt = nullptr;   // pinned mirror
     DBuf<uint8_t> scan_tmp;     // scan partial sums
     DBuf<uint8_t> radix_tmp;    // radix sort temporaries
+
+    // batch of independent instances (ninst == 1: one instance). Instance i owns the
+    // global variables (hibase[i], hibase[i + 1]]; its literals are shifted by 2 hibase[i].
+    uint32_t ninst = 1;
+    std::vector<uint32_t> hibase;
+    DBuf<uint32_t> ibase;                 // device copy of hibase
+    std::vector<uint8_t> imu_host;        // per instance: mu_inc of this round, 0xFF = not electing
+    DBuf<uint8_t> imu;
+    DBuf<uint64_t> ib0;                   // per instance: static break, then dynamic break (2 ninst)
+    DBuf<uint64_t> inel;                  // per instance: elected variables
+    DBuf<uint64_t> icnt;                  // per instance: live clauses, live literals (2 ninst)
 
     // round state
     int32_t phase = 0;

@@ -97,9 +97,25 @@ __device__ __forceinline__ void sort_u32_small(uint32_t *d, int n)
     }
 }
 
-// device counter slots (Ctx::cnt) and flags
-enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_NTRAIL = 8, CNT_FLAGS = 9, CNT_ACCT = 10 };
-enum { FLAG_UNSAT = 1, FLAG_INTERNAL = 2 };
+
+__device__ __forceinline__ void report_conflict(unsigned long long *cnt, uint32_t var)
+{
+    atomicOr(&cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT);
+    const unsigned long long p = atomicAdd(&cnt[CNT_CONF], 1ull);
+    if (p < CONF_CAP) cnt[CNT_CONF + 1 + p] = var;
+}
+
+// instance owning global variable v: the last i with ibase[i] < v (ibase has ninst + 1 entries)
+__device__ __forceinline__ uint32_t find_inst(const uint32_t *ibase, uint32_t ninst, uint32_t v)
+{
+    uint32_t lo = 0, hi = ninst;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ibase[mid] < v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 // returns 1 if lit became true, 0 if it was true already, -1 if it was false
 __device__ __forceinline__ int try_assign(int8_t *value, uint32_t lit)
@@ -126,7 +142,7 @@ __device__ __forceinline__ void enqueue_dev(int8_t *value, uint8_t *vstate, uint
         const unsigned long long pos = atomicAdd(&cnt[CNT_NTRAIL], 1ull);
         trail[pos] = lit;
     }
-    else if (r < 0) atomicOr(&cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT);
+    else if (r < 0) report_conflict(cnt, lit >> 1);
 }
 
 // wave-aggregated append of flagged lanes (one atomic per wave and list kind)

@@ -30,6 +30,7 @@ namespace pfg {
 
 enum { K_SKIP = 0, K_CAND = 1, K_DYNBREAK = 2, K_BREAK = 3 };
 enum { S_UNDEC = 0, S_IN = 1, S_OUT = 2 };
+constexpr uint32_t NO_CAND = 0xFFFFFFFFu;
 
 __global__ void k_scores(const uint32_t *__restrict__ hist, uint32_t nvars, uint32_t *__restrict__ keys, uint32_t *__restrict__ vars)
 {
@@ -41,19 +42,34 @@ __global__ void k_scores(const uint32_t *__restrict__ hist, uint32_t nvars, uint
 }
 
 
+// Per-instance parameters of a batch (a single instance is the one-entry case):
+// instance i owns variables (ibase[i], ibase[i + 1]], elects with temperatures
+// mu << imu[i] (IMU_OFF: not electing this round), and gets its own static break
+// ib0, dynamic break ibrk and elected count inel.
+constexpr uint8_t IMU_OFF = 0xFFu;
+struct InstArgs {
+    const uint32_t *ibase;
+    uint32_t ninst;
+    const uint8_t *imu;
+    uint32_t mu_pos, mu_neg;
+    unsigned long long *ib0, *ibrk, *inel;
+};
+
 __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, const uint32_t *__restrict__ hist,
-                           const uint8_t *__restrict__ vstate, const OL *__restrict__ ot, uint32_t pt, uint32_t nt,
-                           uint8_t *__restrict__ kind, uint8_t *__restrict__ status, uint8_t *__restrict__ blocked,
-                           unsigned long long *cnt)
+                           const uint8_t *__restrict__ vstate, const OL *__restrict__ ot, InstArgs I,
+                           uint8_t *__restrict__ kind, uint8_t *__restrict__ status, uint8_t *__restrict__ blocked)
 {
     for (uint64_t i = gtid(); i < nvars; i += gstride()) {
         const uint32_t v = elig[i];
+        const uint32_t inst = find_inst(I.ibase, I.ninst, v);
+        const uint8_t mi = I.imu[inst];
         const uint32_t ps = hist[2 * v], ns = hist[2 * v + 1];
         uint8_t k = K_SKIP;
-        if (!vstate[v] && (ps || ns)) {
+        if (mi != IMU_OFF && !vstate[v] && (ps || ns)) {
+            const uint32_t pt = I.mu_pos << mi, nt = I.mu_neg << mi;
             const bool hot = ot[2 * v].sz >= pt && ot[2 * v + 1].sz >= nt;
             const bool freezable = ps < pt || ns < nt;
-            if (hot && !freezable) { k = K_BREAK; atomicMin(&cnt[CNT_B], (unsigned long long)i); }
+            if (hot && !freezable) { k = K_BREAK; atomicMin(&I.ib0[inst], (unsigned long long)i); }
             else k = hot ? K_DYNBREAK : K_CAND;
         }
         kind[i] = k;
@@ -62,36 +78,30 @@ __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, co
     }
 }
 
-__global__ void __launch_bounds__(1024) k_work_init(const uint8_t *__restrict__ kind, uint64_t b0, uint32_t *__restrict__ work,
-                                                   unsigned long long *nwork)
-{
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < b0; b += gstride()) {
-        const uint64_t i = b + threadIdx.x;
-        const uint8_t k = i < b0 ? kind[i] : (uint8_t)K_SKIP;
-        block_append(k == K_CAND || k == K_DYNBREAK, (uint32_t)i, work, nwork);
-    }
-}
-
 struct ElectArgs {
     const uint32_t *mem;
     const uint32_t *crank;    // per variable: rank << 1 | freezer, or NO_CAND
     const uint8_t *kind;
     uint8_t *status, *blocked;
-    uint64_t b0;
 };
 
-constexpr uint32_t NO_CAND = 0xFFFFFFFFu;
-
 // per variable (indexed by variable, not rank): its rank among the candidates before
-// the static break and whether it freezes neighbours, so the clause pass needs one
-// gather per literal
-__global__ void k_crank(const uint32_t *__restrict__ elig, uint32_t nvars, const uint8_t *__restrict__ kind,
-                        uint64_t b0, uint32_t *__restrict__ crank)
+// its instance's static break and whether it freezes neighbours, so the clause pass
+// needs one gather per literal; the candidates' ranks also form the first work list
+__global__ void __launch_bounds__(256) k_crank(const uint32_t *__restrict__ elig, uint32_t nvars,
+                                              const uint8_t *__restrict__ kind, InstArgs I, uint32_t *__restrict__ crank,
+                                              uint32_t *__restrict__ work, unsigned long long *nwork)
 {
-    for (uint64_t i = gtid(); i < nvars; i += gstride()) {
-        const uint8_t k = kind[i];
-        const bool cand = i < b0 && (k == K_CAND || k == K_DYNBREAK);
-        crank[elig[i]] = cand ? (((uint32_t)i << 1) | (k == K_CAND ? 1u : 0u)) : NO_CAND;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < nvars; b += gstride()) {
+        const uint64_t i = b + threadIdx.x;
+        bool cand = false;
+        if (i < nvars) {
+            const uint32_t v = elig[i];
+            const uint8_t k = kind[i];
+            if (k == K_CAND || k == K_DYNBREAK) cand = i < I.ib0[find_inst(I.ibase, I.ninst, v)];
+            crank[v] = cand ? (((uint32_t)i << 1) | (k == K_CAND ? 1u : 0u)) : NO_CAND;
+        }
+        block_append(cand, (uint32_t)i, work, nwork);
     }
 }
 
@@ -221,17 +231,35 @@ __global__ void __launch_bounds__(1024) k_elect_pass_b(ElectArgs A, const uint32
     }
 }
 
-__global__ void k_dynbreak(const uint8_t *__restrict__ kind, const uint8_t *__restrict__ status, uint64_t b0, unsigned long long *cnt)
-{
-    for (uint64_t i = gtid(); i < b0; i += gstride())
-        if (kind[i] == K_DYNBREAK && status[i] == S_IN) atomicMin(&cnt[CNT_B], (unsigned long long)i);
-}
-
-__global__ void k_elect_flags(const uint8_t *__restrict__ kind, const uint8_t *__restrict__ status, uint64_t brk,
-                              uint32_t nvars, uint32_t *__restrict__ flags)
+__global__ void k_dynbreak(const uint32_t *__restrict__ elig, uint32_t nvars, const uint8_t *__restrict__ kind,
+                           const uint8_t *__restrict__ status, InstArgs I)
 {
     for (uint64_t i = gtid(); i < nvars; i += gstride())
-        flags[i] = (i < brk && kind[i] == K_CAND && status[i] == S_IN) ? 1u : 0u;
+        if (kind[i] == K_DYNBREAK && status[i] == S_IN)
+            atomicMin(&I.ibrk[find_inst(I.ibase, I.ninst, elig[i])], (unsigned long long)i);
+}
+
+__global__ void k_elect_flags(const uint32_t *__restrict__ elig, const uint8_t *__restrict__ kind,
+                              const uint8_t *__restrict__ status, uint32_t nvars, InstArgs I, uint32_t *__restrict__ flags)
+{
+    for (uint64_t i = gtid(); i < nvars; i += gstride()) {
+        uint32_t f = 0;
+        if (kind[i] == K_CAND && status[i] == S_IN) {
+            const uint32_t inst = find_inst(I.ibase, I.ninst, elig[i]);
+            if (i < I.ibrk[inst]) {
+                f = 1;
+                if (I.ninst > 1) atomicAdd(&I.inel[inst], 1ull);
+            }
+        }
+        flags[i] = f;
+    }
+}
+
+// batch: order candidates by (instance, score, variable): a stable sort of the
+// (score, variable) order by instance
+__global__ void k_inst_keys(const uint32_t *__restrict__ elig, uint32_t nvars, InstArgs I, uint32_t *__restrict__ keys)
+{
+    for (uint64_t i = gtid(); i < nvars; i += gstride()) keys[i] = find_inst(I.ibase, I.ninst, elig[i]);
 }
 
 // algorithmic bytes of one pass A (accounting only): per active record its offset
@@ -258,6 +286,19 @@ static double elect_bytes(Ctx &c, const uint64_t *active, const unsigned long lo
     return (double)v;
 }
 
+static int bits_of32(uint32_t x)
+{
+    int b = 0;
+    while (b < 32 && (x >> b)) b++;
+    return b;
+}
+
+static InstArgs inst_args(Ctx &c)
+{
+    return InstArgs{c.ibase.p, c.ninst, c.imu.p, c.opts.mu_pos, c.opts.mu_neg, (unsigned long long *)c.ib0.p,
+                    (unsigned long long *)c.ib0.p + c.ninst, (unsigned long long *)c.inel.p};
+}
+
 uint32_t lcve(Ctx &c)
 {
     const uint32_t nv = c.nvars;
@@ -280,15 +321,18 @@ uint32_t lcve(Ctx &c)
         KTimer kt(c, K_LCVE_RADIX, 16.0 * nv * ((bits + 7) / 8));
         radix_sort_u32_pairs(c, c.scores.p, c.eligible.p, nv, ((bits + 7) / 8) * 8);
     }
+    const InstArgs I = inst_args(c);
+    if (c.ninst > 1) {   // (instance, score, variable) order
+        k_inst_keys<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, I, c.scores.p);
+        radix_sort_u32_pairs(c, c.scores.p, c.eligible.p, nv, bits_of32(c.ninst - 1));
+    }
+    else c.imu_host[0] = (uint8_t)c.mu_inc;
+    PFG_HIP(hipMemcpyAsync(c.imu.p, c.imu_host.data(), c.ninst, hipMemcpyHostToDevice, c.stream));
+    PFG_HIP(hipMemsetAsync(c.ib0.p, 0xFF, 16ull * c.ninst, c.stream));   // ib0 and ibrk: no break
+    PFG_HIP(hipMemsetAsync(c.inel.p, 0, 8ull * c.ninst, c.stream));
     uint8_t *kind = c.tmpu8.p, *status = kind + nv + 1, *blocked = status + nv + 1;
-    const uint32_t pt = c.opts.mu_pos << c.mu_inc, nt = c.opts.mu_neg << c.mu_inc;
-    const unsigned long long init[3] = {0ull, (unsigned long long)nv, 0ull};
-    PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_A, init, 24, hipMemcpyHostToDevice, c.stream));
-    k_classify<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, c.hist.p, c.vstate.p, c.ot.p, pt, nt,
-                                                              kind, status, blocked, (unsigned long long *)c.cnt.p);
-    read_counters(c, 3);
-    const uint64_t b0 = c.hcnt[CNT_B];
-    k_crank<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, kind, b0, c.rank.p);
+    k_classify<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, c.hist.p, c.vstate.p, c.ot.p, I,
+                                                              kind, status, blocked);
     // per-round counts on the device: rw[r] = undecided candidates entering round r,
     // ra[r] = active clause records entering round r's pass A (rounds 1..)
     constexpr int MAXR = 256;
@@ -297,9 +341,9 @@ uint32_t lcve(Ctx &c)
     PFG_HIP(hipMemsetAsync(rw, 0, (2ull * MAXR + 2) * 8, c.stream));
     uint32_t *work = c.tmpu32a.p, *work2 = c.tmpu32b.p;
     uint64_t *act = c.tmpu64a.p, *act2 = c.tmpu64b.p;
-    k_work_init<<<grid_for(b0, 1024, 4096), 1024, 0, c.stream>>>(kind, b0, work, rw);
-    ElectArgs A{c.mem.p, c.rank.p, kind, status, blocked, b0};
-    if (c.nrefs && b0) {
+    k_crank<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, kind, I, c.rank.p, work, rw);
+    ElectArgs A{c.mem.p, c.rank.p, kind, status, blocked};
+    if (c.nrefs) {
         KTimer kt(c, K_LCVE_INIT, 20.0 * c.nrefs + 8.0 * c.nliterals);
         PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_D, 0, 8, c.stream));
         k_elect_init<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act,
@@ -311,8 +355,7 @@ uint32_t lcve(Ctx &c)
     PFG_HIP(hipStreamSynchronize(c.stream));
     uint64_t nwork = hw[0], nact = hw[1];   // upper bounds of later rounds' counts (sizes the grids)
     if (c.trace)
-        fprintf(stderr, "[pfg] LCVE b0 %llu candidates %llu clauses %llu\n", (unsigned long long)b0, (unsigned long long)nwork,
-                (unsigned long long)nact);
+        fprintf(stderr, "[pfg] LCVE candidates %llu clauses %llu\n", (unsigned long long)nwork, (unsigned long long)nact);
     // round 0's pass A ran inside k_elect_init; rounds are launched in batches and
     // the count of undecided candidates is checked after each batch
     int r = 0;
@@ -339,12 +382,8 @@ uint32_t lcve(Ctx &c)
         nwork = hw[0];
         nact = hw[1];
     }
-    const unsigned long long bb = b0;
-    PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_B, &bb, 8, hipMemcpyHostToDevice, c.stream));
-    k_dynbreak<<<grid_for(b0, 256, 8192), 256, 0, c.stream>>>(kind, status, b0, (unsigned long long *)c.cnt.p);
-    read_counters(c, 3);
-    const uint64_t brk = c.hcnt[CNT_B];
-    k_elect_flags<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(kind, status, brk, nv, c.tmpu32a.p);
+    k_dynbreak<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, kind, status, I);
+    k_elect_flags<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, kind, status, nv, I, c.tmpu32a.p);
     c.npvs = (uint32_t)compact_u32(c, c.eligible.p, c.tmpu32a.p, nv, c.pvs.p);
     c.hist_live = false;
     return c.npvs;

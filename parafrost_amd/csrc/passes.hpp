@@ -23,6 +23,9 @@ void merge_units(Ctx &c, const uint32_t *d_units, uint64_t n);  // ordered dedup
 // 0 unless accounting is on (synchronizes the stream)
 double pv_bytes(Ctx &c, uint32_t b, uint32_t e, bool with_clauses);
 
+// sigma.cpp
+void set_instances(Ctx &c, uint32_t ninst, const uint32_t *nvars);
+
 // lcve.hip
 uint32_t lcve(Ctx &c);                   // returns #elected
 // sub.hip

@@ -92,6 +92,22 @@ void resolve_ktimers(Ctx &c)
     c.kpend.clear();
 }
 
+// instance layout of a context: consecutive variable ranges of the given sizes
+void set_instances(Ctx &c, uint32_t ninst, const uint32_t *nvars)
+{
+    c.ninst = ninst;
+    c.hibase.assign(ninst + 1, 0);
+    for (uint32_t i = 0; i < ninst; i++) c.hibase[i + 1] = c.hibase[i] + nvars[i];
+    c.imu_host.assign(ninst, 0);
+    ensure(c, c.ibase, ninst + 1);
+    ensure(c, c.imu, ninst + 8);
+    ensure(c, c.ib0, 2ull * ninst);
+    ensure(c, c.inel, ninst);
+    ensure(c, c.icnt, 2ull * ninst);
+    PFG_HIP(hipMemcpyAsync(c.ibase.p, c.hibase.data(), 4ull * (ninst + 1), hipMemcpyHostToDevice, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+}
+
 // The reference simplifier's round structure (src/cpu/simplify.cpp:150-214,
 // src/gpu/simplify.cu:155-234), one method per reference entry point.
 struct Sigma {
@@ -202,6 +218,7 @@ void pfg_default_opts(pfg_opts *o)
     o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
 }
 
+
 int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, const pfg_opts *opts)
 {
     if (!out || !nvars) return PFG_EINVAL;
@@ -219,9 +236,9 @@ int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, c
         PFG_HIP(hipSetDevice(device));
         PFG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         for (auto &e : c.ev) { PFG_HIP(hipEventCreate(&e[0])); PFG_HIP(hipEventCreate(&e[1])); }
-        PFG_HIP(hipHostMalloc((void **)&c.hcnt, 16 * sizeof(uint64_t), hipHostMallocDefault));
-        ensure(c, c.cnt, 16);
-        PFG_HIP(hipMemsetAsync(c.cnt.p, 0, 16 * 8, c.stream));
+        PFG_HIP(hipHostMalloc((void **)&c.hcnt, (CNT_CONF + 1) * sizeof(uint64_t), hipHostMallocDefault));
+        ensure(c, c.cnt, CNT_CONF + 1 + CONF_CAP);
+        PFG_HIP(hipMemsetAsync(c.cnt.p, 0, (CNT_CONF + 1) * 8, c.stream));
         ensure(c, c.value, c.nlits2 + 8);
         PFG_HIP(hipMemsetAsync(c.value.p, 0xFF, c.value.cap, c.stream));
         ensure(c, c.vstate, (uint64_t)nvars + 1);
@@ -231,6 +248,7 @@ int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, c
         PFG_HIP(hipMemsetAsync(c.ufirst.p, 0xFF, c.ufirst.cap * 4, c.stream));
         ensure(c, c.pvs, (uint64_t)nvars + 1);
         ensure(c, c.resolved, 1024);
+        set_instances(c, 1, &nvars);
         PFG_HIP(hipStreamSynchronize(c.stream));
     }
     catch (const Error &e) {
@@ -256,6 +274,7 @@ int pfg_destroy(pfg_ctx *ctx)
     release(c, c.tmpu8); release(c, c.tmpu64a); release(c, c.tmpu64b); release(c, c.tmpu64c);
     release(c, c.owner); release(c, c.claim); release(c, c.erec); release(c, c.ufirst); release(c, c.cnt);
     release(c, c.scan_tmp); release(c, c.radix_tmp);
+    release(c, c.ibase); release(c, c.imu); release(c, c.ib0); release(c, c.inel); release(c, c.icnt);
     if (c.hcnt) (void)hipHostFree(c.hcnt);
     for (auto &e : c.kev) { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); }
     for (auto &e : c.ev) {
@@ -514,7 +533,7 @@ int pfg_reset(pfg_ctx *ctx)
     API_BEGIN
     PFG_HIP(hipMemsetAsync(c.value.p, 0xFF, c.value.cap, c.stream));
     PFG_HIP(hipMemsetAsync(c.vstate.p, 0, c.vstate.cap, c.stream));
-    PFG_HIP(hipMemsetAsync(c.cnt.p, 0, 16 * 8, c.stream));
+    PFG_HIP(hipMemsetAsync(c.cnt.p, 0, (CNT_CONF + 1) * 8, c.stream));
     PFG_HIP(hipStreamSynchronize(c.stream));
     c.ntrail = c.propagated = 0;
     c.nres = 0;

@@ -94,7 +94,7 @@ __device__ __forceinline__ void strengthen(const SubArgs &A, uint32_t *c, uint32
             A.trail[pos] = unit;
             toblivion_list(A, unit);
         }
-        else if (r < 0) atomicOr(&A.cnt[CNT_FLAGS], (unsigned long long)FLAG_UNSAT);
+        else if (r < 0) report_conflict(A.cnt, unit >> 1);
     }
     else if (c_learnt(c)) {   // bumpShrunken (elimination.cpp:27-39)
         const uint32_t old = c[0] >> 6;
