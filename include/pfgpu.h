@@ -56,6 +56,28 @@ typedef struct pfg_opts {
     int32_t stable_sort;       /* 0: lists > 20 ordered as libstdc++ std::sort (reference) */
 } pfg_opts;
 
+/* ---- Batches (no counterpart call in the reference, which simplifies one formula per
+ * solver): many independent instances in one context, so that each kernel launch of a
+ * round serves all of them. Instance i owns its own variables 1..nvars[i]; inside the
+ * context they occupy consecutive ranges, each starting at a multiple of 16 (so that
+ * clause signatures, which hash lit & 31, do not change). Every instance gets exactly the result that
+ * pfg_sigmify gives it alone (each keeps its own round count, temperatures, LCVE
+ * breaks and stop decision, and a contradiction refutes only its own instance). */
+int pfg_create_batch(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t ninst, const uint32_t *nvars,
+                     const pfg_opts *opts);
+int pfg_batch_ninst(pfg_ctx *ctx, uint32_t *ninst);
+/* clauses in CSR form, each instance in its own numbering; instance i has clauses
+ * [inst_first[i], inst_first[i + 1]) (inst_first has ninst + 1 entries). pfg_upload_cnf
+ * and pfg_upload_cnf_device take the context's numbering (instance literals shifted by
+ * twice the instance's base: base[0] = 0, base[i + 1] = round_up_16(base[i] + nvars[i])). */
+int pfg_upload_batch(pfg_ctx *ctx, const uint32_t *lits, const uint64_t *offs, const uint64_t *inst_first);
+/* sigmify every instance; status[i] = PFG_OK or PFG_EUNSAT */
+int pfg_sigmify_batch(pfg_ctx *ctx, int32_t *status);
+/* one instance's clauses, model records, variable states (nvars[i] + 1 bytes) and units,
+ * in its own numbering; NULL buffers query the sizes */
+int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *nwords, uint64_t *ncls, uint32_t *resolved,
+                       uint64_t *nres, uint8_t *vstate, uint32_t *trail, uint32_t *ntrail);
+
 /* per-pass device time accumulated over the context's life, in milliseconds */
 typedef struct pfg_times {
     double upload, hist, ot, sort_ot, prop, lcve, sub, ve, ere, gc, count, total;

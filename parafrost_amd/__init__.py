@@ -49,7 +49,8 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_stop", "pfg_next_phase", "pfg_hist", "pfg_download_cnf", "pfg_download_trail",
        "pfg_download_resolved", "pfg_download_vstate", "pfg_download_pvs", "pfg_download_ot_list",
        "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
-       "pfg_set_accounting", "pfg_set_stage_words"]
+       "pfg_set_accounting", "pfg_set_stage_words", "pfg_create_batch", "pfg_batch_ninst", "pfg_upload_batch",
+       "pfg_sigmify_batch", "pfg_batch_download"]
 
 
 def lib_path() -> str:
@@ -88,6 +89,12 @@ def lib():
         L.pfg_reset_stats.argtypes = [vp]
         L.pfg_set_accounting.argtypes = [vp, i32]
         L.pfg_set_stage_words.argtypes = [vp, u32]
+        L.pfg_create_batch.argtypes = [ctypes.POINTER(vp), i32, u64, u32, vp, ctypes.POINTER(Opts)]
+        L.pfg_batch_ninst.argtypes = [vp, ctypes.POINTER(u32)]
+        L.pfg_upload_batch.argtypes = [vp, vp, vp, vp]
+        L.pfg_sigmify_batch.argtypes = [vp, vp]
+        L.pfg_batch_download.argtypes = [vp, u32, vp, ctypes.POINTER(u64), ctypes.POINTER(u64), vp,
+                                         ctypes.POINTER(u64), vp, vp, ctypes.POINTER(u32)]
         L.pfg_reset.argtypes = [vp]
         L.pfg_last_error.argtypes = [vp]
         L.pfg_last_error.restype = ctypes.c_char_p
@@ -276,3 +283,65 @@ class Simplifier:
 
     def is_unsat(self) -> bool:
         return bool(self.L.pfg_is_unsat(self.h))
+
+
+class BatchSimplifier(Simplifier):
+    """A batch of independent CNF instances in one context: every kernel launch of a
+    round serves all of them. Instance i owns its own variables 1..nvars[i]; results
+    equal simplifying each instance alone."""
+
+    def __init__(self, nvars, opts: Opts | None = None, device: int = 0, hbm_budget: int = 0):
+        self.L = lib()
+        self.inst_nvars = np.ascontiguousarray(nvars, np.uint32)
+        self.ninst = len(self.inst_nvars)
+        self.opts = opts or default_opts()
+        h = ctypes.c_void_p()
+        self._check(self.L.pfg_create_batch(ctypes.byref(h), device, hbm_budget, self.ninst, self.inst_nvars.ctypes.data,
+                                             ctypes.byref(self.opts)), None)
+        self.h = h
+        n = ctypes.c_uint32()
+        # instance i's variable v is the context's variable base[i] + v (bases are multiples of 16)
+        self.base = np.zeros(self.ninst + 1, np.uint64)
+        at = 0
+        for i, n in enumerate(self.inst_nvars):
+            self.base[i] = at
+            at += int(n)
+            if i + 1 < self.ninst:
+                at = (at + 15) & ~15
+        self.base[self.ninst] = at
+        self.nvars = at
+
+    def upload_instances(self, cnfs):
+        """Instances in their own numbering, concatenated in order."""
+        lits = np.concatenate([np.asarray(c.lits[:c.nlits], np.uint32) for c in cnfs]) if cnfs else np.zeros(0, np.uint32)
+        offs = [np.zeros(1, np.uint64)]
+        first = np.zeros(self.ninst + 1, np.uint64)
+        at, lat = 0, 0
+        for i, c in enumerate(cnfs):
+            offs.append(np.asarray(c.offs[1:c.ncls + 1], np.uint64) + np.uint64(lat))
+            at += c.ncls
+            lat += int(c.nlits)
+            first[i + 1] = at
+        offs = np.concatenate(offs)
+        self._keep = (lits, offs, first)
+        self._check(self.L.pfg_upload_batch(self.h, lits.ctypes.data, offs.ctypes.data, first.ctypes.data))
+
+    def sigmify_batch(self) -> np.ndarray:
+        """Per-instance status: 0 (simplified) or 4 (proven unsatisfiable)."""
+        st = np.zeros(self.ninst, np.int32)
+        self._check(self.L.pfg_sigmify_batch(self.h, st.ctypes.data))
+        return st
+
+    def instance(self, i: int):
+        """(clause words, trail, model records, variable states) of instance i in its own numbering."""
+        nw, nc, nr = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        nt = ctypes.c_uint32()
+        self._check(self.L.pfg_batch_download(self.h, i, None, ctypes.byref(nw), ctypes.byref(nc), None, ctypes.byref(nr),
+                                              None, None, ctypes.byref(nt)))
+        w = np.zeros(nw.value, np.uint32)
+        r = np.zeros(nr.value, np.uint32)
+        t = np.zeros(nt.value, np.uint32)
+        vs = np.zeros(int(self.inst_nvars[i]) + 1, np.uint8)
+        self._check(self.L.pfg_batch_download(self.h, i, w.ctypes.data, ctypes.byref(nw), ctypes.byref(nc), r.ctypes.data,
+                                              ctypes.byref(nr), vs.ctypes.data, t.ctypes.data, ctypes.byref(nt)))
+        return w, t, r, vs

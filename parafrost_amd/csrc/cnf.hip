@@ -466,11 +466,93 @@ bool prop(Ctx &c)
         read_counters(c, 10);
         c.propagated = b1;
         c.ntrail = (uint32_t)c.hcnt[CNT_NTRAIL];
-        if (c.hcnt[CNT_FLAGS] & FLAG_UNSAT) { c.unsat = true; return false; }
+        if (c.hcnt[CNT_FLAGS] & FLAG_UNSAT) {
+            c.unsat = true;
+            if (!c.batch) return false;   // a batch propagates the other instances to the end
+        }
     }
     if (c.propagated != before) c.hist_live = false;
     if (c.propagated != before && !c.opts.sub_en) reduce_ot(c);
-    return true;
+    return !(!c.batch && c.unsat);
+}
+
+// ------------------------------------------------------------------ batches
+// live clauses and literals per instance (instance of a clause = instance of its
+// first literal's variable); a wave whose clauses all belong to one instance adds once
+__global__ void k_count_inst(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                             const uint32_t *__restrict__ ibase, uint32_t ninst, unsigned long long *__restrict__ icnt)
+{
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
+        const uint64_t i = b + threadIdx.x;
+        unsigned long long nc = 0, nl = 0;
+        uint32_t inst = 0xFFFFFFFFu;
+        if (i < n) {
+            const uint32_t *cl = mem + refs[i];
+            if (!c_del(cl)) {
+                nc = 1;
+                nl = (unsigned)c_size(cl);
+                inst = find_inst(ibase, ninst, cl[3] >> 1);
+            }
+        }
+        const uint32_t lead = __shfl(inst, 0, 64);
+        if (__all(inst == lead || inst == 0xFFFFFFFFu)) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) { nc += __shfl_down(nc, o, 64); nl += __shfl_down(nl, o, 64); }
+            if (__lane_id() == 0 && lead != 0xFFFFFFFFu && nc) {
+                atomicAdd(&icnt[2 * lead], nc);
+                atomicAdd(&icnt[2 * lead + 1], nl);
+            }
+        }
+        else if (inst != 0xFFFFFFFFu) {
+            atomicAdd(&icnt[2 * inst], nc);
+            atomicAdd(&icnt[2 * inst + 1], nl);
+        }
+    }
+}
+
+void count_inst(Ctx &c, std::vector<int64_t> &cls, std::vector<int64_t> &lits)
+{
+    PFG_HIP(hipMemsetAsync(c.icnt.p, 0, 16ull * c.ninst, c.stream));
+    if (c.nrefs) {
+        KTimer kt(c, K_COUNT, 20.0 * c.nrefs + 4.0 * c.nliterals);
+        k_count_inst<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.ibase.p, c.ninst,
+                                                                           (unsigned long long *)c.icnt.p);
+    }
+    std::vector<uint64_t> h(2ull * c.ninst);
+    PFG_HIP(hipMemcpyAsync(h.data(), c.icnt.p, 16ull * c.ninst, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    cls.resize(c.ninst);
+    lits.resize(c.ninst);
+    for (uint32_t i = 0; i < c.ninst; i++) { cls[i] = (int64_t)h[2 * i]; lits[i] = (int64_t)h[2 * i + 1]; }
+}
+
+// keep the elected variables of instances with cls 1, then those with cls 2, each
+// group in elected order; returns the two group sizes
+__global__ void k_pv_class(const uint32_t *__restrict__ pvs, uint32_t n, const uint32_t *__restrict__ ibase, uint32_t ninst,
+                           const uint8_t *__restrict__ icls, uint32_t *__restrict__ fa, uint32_t *__restrict__ fb)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint8_t k = icls[find_inst(ibase, ninst, pvs[i])];
+        fa[i] = k == 1;
+        fb[i] = k == 2;
+    }
+}
+
+void partition_pvs(Ctx &c, const std::vector<uint8_t> &cls, uint32_t &na, uint32_t &nb)
+{
+    na = nb = 0;
+    if (!c.npvs) return;
+    const uint32_t n = c.npvs;
+    ensure(c, c.tmpu8, c.ninst + 8);
+    ensure(c, c.tmpu32a, 2ull * n + 2);
+    ensure(c, c.tmpu32b, 2ull * n + 2);
+    PFG_HIP(hipMemcpyAsync(c.tmpu8.p, cls.data(), c.ninst, hipMemcpyHostToDevice, c.stream));
+    uint32_t *fa = c.tmpu32a.p, *fb = c.tmpu32a.p + n + 1;
+    k_pv_class<<<grid_for(n, 256, 4096), 256, 0, c.stream>>>(c.pvs.p, n, c.ibase.p, c.ninst, c.tmpu8.p, fa, fb);
+    na = (uint32_t)compact_u32(c, c.pvs.p, fa, n, c.tmpu32b.p);
+    nb = (uint32_t)compact_u32(c, c.pvs.p, fb, n, c.tmpu32b.p + na);
+    if (na + nb) PFG_HIP(hipMemcpyAsync(c.pvs.p, c.tmpu32b.p, 4ull * (na + nb), hipMemcpyDeviceToDevice, c.stream));
+    c.npvs = na + nb;
 }
 
 // ------------------------------------------------------------------ counting

@@ -109,13 +109,18 @@ struct Ctx {
     // batch of independent instances (ninst == 1: one instance). Instance i owns the
     // global variables (hibase[i], hibase[i + 1]]; its literals are shifted by 2 hibase[i].
     uint32_t ninst = 1;
-    std::vector<uint32_t> hibase;
+    bool batch = false;                   // created as a batch (per-instance round control)
+    std::vector<uint32_t> hibase, inst_nvars;
     DBuf<uint32_t> ibase;                 // device copy of hibase
     std::vector<uint8_t> imu_host;        // per instance: mu_inc of this round, 0xFF = not electing
     DBuf<uint8_t> imu;
     DBuf<uint64_t> ib0;                   // per instance: static break, then dynamic break (2 ninst)
     DBuf<uint64_t> inel;                  // per instance: elected variables
     DBuf<uint64_t> icnt;                  // per instance: live clauses, live literals (2 ninst)
+    uint64_t batch_gen = 0, cache_gen = ~0ull;   // host copy of the batch state for per-instance downloads
+    std::vector<uint32_t> cache_mem, cache_res, cache_trail, cache_ctx_words;
+    std::vector<uint64_t> cache_cls;
+    std::vector<uint8_t> cache_vstate;
 
     // round state
     int32_t phase = 0;

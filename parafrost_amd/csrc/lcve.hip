@@ -52,7 +52,8 @@ struct InstArgs {
     uint32_t ninst;
     const uint8_t *imu;
     uint32_t mu_pos, mu_neg;
-    unsigned long long *ib0, *ibrk, *inel;
+    unsigned long long *ib0, *ibrk, *inel;   // inel is counted for batches only
+    int batch;
 };
 
 __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, const uint32_t *__restrict__ hist,
@@ -248,7 +249,7 @@ __global__ void k_elect_flags(const uint32_t *__restrict__ elig, const uint8_t *
             const uint32_t inst = find_inst(I.ibase, I.ninst, elig[i]);
             if (i < I.ibrk[inst]) {
                 f = 1;
-                if (I.ninst > 1) atomicAdd(&I.inel[inst], 1ull);
+                if (I.batch) atomicAdd(&I.inel[inst], 1ull);
             }
         }
         flags[i] = f;
@@ -296,7 +297,7 @@ static int bits_of32(uint32_t x)
 static InstArgs inst_args(Ctx &c)
 {
     return InstArgs{c.ibase.p, c.ninst, c.imu.p, c.opts.mu_pos, c.opts.mu_neg, (unsigned long long *)c.ib0.p,
-                    (unsigned long long *)c.ib0.p + c.ninst, (unsigned long long *)c.inel.p};
+                    (unsigned long long *)c.ib0.p + c.ninst, (unsigned long long *)c.inel.p, c.batch ? 1 : 0};
 }
 
 uint32_t lcve(Ctx &c)
@@ -326,7 +327,7 @@ uint32_t lcve(Ctx &c)
         k_inst_keys<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, I, c.scores.p);
         radix_sort_u32_pairs(c, c.scores.p, c.eligible.p, nv, bits_of32(c.ninst - 1));
     }
-    else c.imu_host[0] = (uint8_t)c.mu_inc;
+    if (!c.batch) c.imu_host[0] = (uint8_t)c.mu_inc;
     PFG_HIP(hipMemcpyAsync(c.imu.p, c.imu_host.data(), c.ninst, hipMemcpyHostToDevice, c.stream));
     PFG_HIP(hipMemsetAsync(c.ib0.p, 0xFF, 16ull * c.ninst, c.stream));   // ib0 and ibrk: no break
     PFG_HIP(hipMemsetAsync(c.inel.p, 0, 8ull * c.ninst, c.stream));

@@ -1,6 +1,7 @@
 // Host launchers of the simplifier passes (one translation unit per pass family).
 #pragma once
 #include "ctx.hpp"
+#include <vector>
 
 namespace pfg {
 
@@ -17,6 +18,9 @@ void filter_pvs(Ctx &c);
 void sort_ot(Ctx &c);                    // PV lists by the reference clause key
 uint64_t compact_u32(Ctx &c, const uint32_t *in, const uint32_t *flags, uint64_t n, uint32_t *out);
 void enqueue_units_host(Ctx &c, const uint32_t *lits, uint32_t n);
+// batches
+void count_inst(Ctx &c, std::vector<int64_t> &cls, std::vector<int64_t> &lits);
+void partition_pvs(Ctx &c, const std::vector<uint8_t> &cls, uint32_t &na, uint32_t &nb);
 void merge_units(Ctx &c, const uint32_t *d_units, uint64_t n);  // ordered dedupe into the trail
 // algorithmic bytes of reading the occurrence lists of elected variables pvs[b, e):
 // 16 per list header + 8 per entry (+ 12 + 4|c| per listed clause when with_clauses);
@@ -25,6 +29,7 @@ double pv_bytes(Ctx &c, uint32_t b, uint32_t e, bool with_clauses);
 
 // sigma.cpp
 void set_instances(Ctx &c, uint32_t ninst, const uint32_t *nvars);
+uint64_t instance_bases(uint32_t ninst, const uint32_t *nvars, std::vector<uint32_t> &base);
 
 // lcve.hip
 uint32_t lcve(Ctx &c);                   // returns #elected

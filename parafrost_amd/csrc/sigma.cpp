@@ -92,12 +92,28 @@ void resolve_ktimers(Ctx &c)
     c.kpend.clear();
 }
 
-// instance layout of a context: consecutive variable ranges of the given sizes
+// instance layout of a context: consecutive variable ranges of the given sizes, each
+// starting at a multiple of 16 so that shifted literals keep their signature bits
+// (lit & 31); the padding variables have no clauses
+uint64_t instance_bases(uint32_t ninst, const uint32_t *nvars, std::vector<uint32_t> &base)
+{
+    base.assign(ninst + 1, 0);
+    uint64_t at = 0;
+    for (uint32_t i = 0; i < ninst; i++) {
+        base[i] = (uint32_t)at;
+        at += nvars[i];
+        if (i + 1 < ninst) at = (at + 15) & ~15ull;
+        if (at >= (1ull << 31)) return at;
+    }
+    base[ninst] = (uint32_t)at;
+    return at;
+}
+
 void set_instances(Ctx &c, uint32_t ninst, const uint32_t *nvars)
 {
     c.ninst = ninst;
-    c.hibase.assign(ninst + 1, 0);
-    for (uint32_t i = 0; i < ninst; i++) c.hibase[i + 1] = c.hibase[i] + nvars[i];
+    instance_bases(ninst, nvars, c.hibase);
+    c.inst_nvars.assign(nvars, nvars + ninst);
     c.imu_host.assign(ninst, 0);
     ensure(c, c.ibase, ninst + 1);
     ensure(c, c.imu, ninst + 8);
@@ -179,6 +195,126 @@ struct Sigma {
         c.nclauses = nc;
         c.nliterals = nl;
         shrinkSimp();
+        return PFG_OK;
+    }
+
+    // ---- batches: the same round structure, decided per instance
+    // instance of global variable v (host side)
+    uint32_t inst_of(uint64_t v) const
+    {
+        const auto it = std::lower_bound(c.hibase.begin(), c.hibase.end(), (uint32_t)v);
+        const uint64_t i = (uint64_t)(it - c.hibase.begin());
+        return (uint32_t)(i == 0 ? 0 : std::min<uint64_t>(i - 1, c.ninst - 1));
+    }
+
+    // contradictions reported since the last call refute their instances
+    void collect_conflicts(std::vector<uint8_t> &st)
+    {
+        uint64_t h[2] = {0, 0};
+        PFG_HIP(hipMemcpyAsync(&h[0], c.cnt.p + CNT_FLAGS, 8, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipMemcpyAsync(&h[1], c.cnt.p + CNT_CONF, 8, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+        if (!(h[0] & FLAG_UNSAT)) return;
+        if (h[1] > CONF_CAP) throw Error(PFG_EINTERNAL, "more contradictions in one pass than the conflict list holds");
+        std::vector<uint64_t> vars(h[1]);
+        if (h[1]) PFG_HIP(hipMemcpyAsync(vars.data(), c.cnt.p + CNT_CONF + 1, 8 * h[1], hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_FLAGS, 0, 8, c.stream));
+        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_CONF, 0, 8, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+        for (const uint64_t v : vars) st[inst_of(v)] = B_REFUTED;
+        c.unsat = false;
+    }
+
+    enum { B_RUN = 0, B_DONE = 1, B_REFUTED = 2 };
+
+    // sigmifying (src/cpu/simplify.cpp:150-214) for every instance of the batch at
+    // once: an instance leaves the loop where the reference's loop would (no
+    // literals, fewer than lcve_min elected, or stop() after which it runs ERE) and
+    // is refuted where the reference would return UNSAT; the others go on.
+    int sigmifying_batch(int32_t *status)
+    {
+        const uint32_t ni = c.ninst;
+        std::vector<uint8_t> st(ni, B_RUN), cls(ni, 0);
+        std::vector<int32_t> phase(ni, 0);
+        std::vector<uint32_t> mu(ni, 0);
+        std::vector<int64_t> nc, nl;
+        {
+            PassTimer t(c, &c.times.count);
+            count_inst(c, nc, nl);
+        }
+        std::vector<int64_t> before = nl, diff(ni, INT64_MAX);
+        std::vector<uint64_t> nel(ni);
+        int round = 0;
+        for (;;) {
+            bool any = false;
+            for (uint32_t i = 0; i < ni; i++) {
+                if (st[i] == B_RUN && before[i] == 0) st[i] = B_DONE;
+                any |= st[i] == B_RUN;
+            }
+            if (!any) break;
+            const int times = round + 1;   // compaction keeps clause order, so its schedule is free
+            if (times > 1 && c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0) shrinkSimp();
+            createOT();
+            propagate();
+            collect_conflicts(st);
+            for (uint32_t i = 0; i < ni; i++) c.imu_host[i] = st[i] == B_RUN ? (uint8_t)std::min<uint32_t>(mu[i], 254u) : 0xFFu;
+            {
+                PassTimer t(c, &c.times.lcve);
+                lcve(c);
+            }
+            PFG_HIP(hipMemcpyAsync(nel.data(), c.inel.p, 8ull * ni, hipMemcpyDeviceToHost, c.stream));
+            PFG_HIP(hipStreamSynchronize(c.stream));
+            for (uint32_t i = 0; i < ni; i++) {
+                cls[i] = 0;
+                if (st[i] != B_RUN) continue;
+                if (nel[i] < c.opts.lcve_min) { st[i] = B_DONE; continue; }
+                const bool stop = (phase[i] == c.opts.phases) || (diff[i] <= c.opts.lits_min && phase[i] > 2);
+                cls[i] = stop ? 2 : 1;
+            }
+            uint32_t na = 0, nb = 0;
+            partition_pvs(c, cls, na, nb);
+            sortOT();
+            if (nb) {
+                uint32_t *base = c.pvs.p;
+                c.pvs.p = base + na;
+                c.npvs = nb;
+                ERE();
+                c.pvs.p = base;
+            }
+            c.npvs = na;
+            for (uint32_t i = 0; i < ni; i++) if (cls[i] == 2) st[i] = B_DONE;
+            if (na) {
+                SUB();
+                collect_conflicts(st);
+                VE();
+                collect_conflicts(st);
+            }
+            {
+                PassTimer t(c, &c.times.count);
+                count_inst(c, nc, nl);
+            }
+            for (uint32_t i = 0; i < ni; i++) {
+                if (cls[i] != 1 || st[i] != B_RUN) continue;
+                diff[i] = before[i] - nl[i];
+                before[i] = nl[i];
+                phase[i]++;
+                mu[i]++;
+                mu[i] += (phase[i] == c.opts.phases);
+            }
+            if (++round > (1 << 20)) throw Error(PFG_EINTERNAL, "batch did not finish");
+        }
+        if (c.ntrail > c.propagated) {
+            if (!c.ot_valid) createOT();
+            propagate();
+            collect_conflicts(st);
+        }
+        int64_t tc, tl;
+        countAll(tc, tl);
+        c.nclauses = tc;
+        c.nliterals = tl;
+        shrinkSimp();
+        for (uint32_t i = 0; i < ni; i++) status[i] = st[i] == B_REFUTED ? PFG_EUNSAT : PFG_OK;
+        c.batch_gen++;
         return PFG_OK;
     }
 };
@@ -305,6 +441,7 @@ int pfg_upload_cnf(pfg_ctx *ctx, const uint32_t *lits, const uint64_t *offs, uin
         PFG_HIP(hipMemcpyAsync(dh, hdr0, ncls * 4, hipMemcpyHostToDevice, c.stream));
     }
     upload(c, dl, doff, ncls, nlits, dh);
+    c.batch_gen++;
     PFG_HIP(hipStreamSynchronize(c.stream));
     dfree_raw(c, dl, nlits * 4 + 4);
     dfree_raw(c, doff, (ncls + 1) * 8);
@@ -318,6 +455,7 @@ int pfg_upload_cnf_device(pfg_ctx *ctx, const uint32_t *d_lits, const uint64_t *
     API_BEGIN
     PassTimer t(c, &c.times.upload);
     upload(c, d_lits, d_offs, ncls, nlits, nullptr);
+    c.batch_gen++;
     PFG_HIP(hipStreamSynchronize(c.stream));
     API_END
 }
@@ -399,6 +537,143 @@ int pfg_hist(pfg_ctx *ctx, uint32_t *h)
         PFG_HIP(hipMemcpyAsync(h, c.hist.p, c.nlits2 * 4, hipMemcpyDeviceToHost, c.stream));
         PFG_HIP(hipStreamSynchronize(c.stream));
     }
+    API_END
+}
+
+int pfg_create_batch(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t ninst, const uint32_t *nvars,
+                     const pfg_opts *opts)
+{
+    if (!out || !ninst || !nvars) return PFG_EINVAL;
+    for (uint32_t i = 0; i < ninst; i++)
+        if (!nvars[i]) return PFG_EINVAL;
+    std::vector<uint32_t> base;
+    const uint64_t total = instance_bases(ninst, nvars, base);
+    if (total >= (1ull << 31)) return PFG_EINVAL;
+    int rc = pfg_create(out, device, hbm_budget, (uint32_t)total, opts);
+    if (rc != PFG_OK) return rc;
+    pfg_ctx *ctx = *out;
+    API_BEGIN
+    set_instances(c, ninst, nvars);
+    c.batch = true;
+    API_END
+}
+
+int pfg_batch_ninst(pfg_ctx *ctx, uint32_t *ninst)
+{
+    if (!ctx || !ninst) return PFG_EINVAL;
+    *ninst = ctx->c.ninst;
+    return PFG_OK;
+}
+
+int pfg_upload_batch(pfg_ctx *ctx, const uint32_t *lits, const uint64_t *offs, const uint64_t *inst_first)
+{
+    if (!ctx || !offs || !inst_first) return PFG_EINVAL;
+    Ctx &c0 = ctx->c;
+    const uint64_t ncls = inst_first[c0.ninst];
+    std::vector<uint32_t> g(offs[ncls]);
+    for (uint32_t i = 0; i < c0.ninst; i++) {
+        if (inst_first[i + 1] < inst_first[i]) return PFG_EINVAL;
+        const uint32_t nv = c0.inst_nvars[i], shift = 2u * c0.hibase[i];
+        for (uint64_t k = offs[inst_first[i]]; k < offs[inst_first[i + 1]]; k++) {
+            const uint32_t l = lits[k];
+            if ((l >> 1) == 0 || (l >> 1) > nv) return PFG_EINVAL;
+            g[k] = l + shift;
+        }
+    }
+    return pfg_upload_cnf(ctx, g.data(), offs, ncls, nullptr);
+}
+
+int pfg_sigmify_batch(pfg_ctx *ctx, int32_t *status)
+{
+    if (!ctx || !status) return PFG_EINVAL;
+    API_BEGIN
+    double t0 = 0;
+    {
+        PassTimer t(c, &t0);
+        Sigma s(c);
+        const int rc = s.sigmifying_batch(status);
+        if (rc != PFG_OK) return rc;
+    }
+    c.times.total += t0;
+    API_END
+}
+
+// one instance's part of the batch state, literals in the instance's own numbering:
+// its clauses (SCLAUSE images, batch order), model records, variable states (nvars_i + 1
+// bytes) and units; a NULL buffer only queries the size
+int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *nwords, uint64_t *ncls, uint32_t *resolved,
+                       uint64_t *nres, uint8_t *vstate, uint32_t *trail, uint32_t *ntrail)
+{
+    API_BEGIN
+    if (inst >= c.ninst) return PFG_EINVAL;
+    if (c.cache_gen != c.batch_gen || c.cache_ctx_words.empty()) {
+        std::vector<uint64_t> refs(c.nrefs);
+        std::vector<uint32_t> mem(c.msz);
+        c.cache_res.resize(c.nres);
+        c.cache_vstate.resize((uint64_t)c.nvars + 1);
+        c.cache_trail.resize(c.ntrail);
+        if (c.nrefs) PFG_HIP(hipMemcpyAsync(refs.data(), c.refs.p, c.nrefs * 8, hipMemcpyDeviceToHost, c.stream));
+        if (c.msz) PFG_HIP(hipMemcpyAsync(mem.data(), c.mem.p, c.msz * 4, hipMemcpyDeviceToHost, c.stream));
+        if (c.nres) PFG_HIP(hipMemcpyAsync(c.cache_res.data(), c.resolved.p, c.nres * 4, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipMemcpyAsync(c.cache_vstate.data(), c.vstate.p, (uint64_t)c.nvars + 1, hipMemcpyDeviceToHost, c.stream));
+        if (c.ntrail) PFG_HIP(hipMemcpyAsync(c.cache_trail.data(), c.trail.p, 4ull * c.ntrail, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+        c.cache_ctx_words.assign(1, 0);
+        c.cache_cls.clear();
+        for (uint64_t i = 0; i < c.nrefs; i++) c.cache_cls.push_back(refs[i]);
+        c.cache_mem.swap(mem);
+        c.cache_gen = c.batch_gen;
+    }
+    Sigma S(c);
+    const uint32_t lo = c.hibase[inst], hi = lo + c.inst_nvars[inst], shift = 2u * lo;
+    uint64_t w = 0, n = 0;
+    for (const uint64_t r : c.cache_cls) {
+        const uint32_t *cl = c.cache_mem.data() + r;
+        const uint32_t sz = cl[2];
+        const uint32_t v = sz ? (cl[3] >> 1) : 0u;
+        if (!(v > lo && v <= hi)) continue;
+        if (words) {
+            words[w] = cl[0];
+            words[w + 1] = cl[1];
+            words[w + 2] = sz;
+            for (uint32_t k = 0; k < sz; k++) words[w + 3 + k] = cl[3 + k] - shift;
+        }
+        w += 3ull + sz;
+        n++;
+    }
+    if (nwords) *nwords = w;
+    if (ncls) *ncls = n;
+    // model records end with their size; all literals of a record belong to one instance
+    std::vector<std::pair<uint64_t, uint64_t>> recs;   // (start, length incl. size word), newest first
+    for (int64_t j = (int64_t)c.cache_res.size() - 1; j > 0;) {
+        const uint64_t k = c.cache_res[j];
+        const int64_t s0 = j - (int64_t)k;
+        if (k == 0 || s0 < 0) throw Error(PFG_EINTERNAL, "malformed model stack");
+        const uint32_t v = c.cache_res[s0] >> 1;
+        if (v > lo && v <= hi) recs.emplace_back((uint64_t)s0, k + 1);
+        j = s0 - 1;
+    }
+    uint64_t m = 0;
+    for (auto it = recs.rbegin(); it != recs.rend(); ++it) {
+        if (resolved) {
+            for (uint64_t t = 0; t + 1 < it->second; t++) resolved[m + t] = c.cache_res[it->first + t] - shift;
+            resolved[m + it->second - 1] = c.cache_res[it->first + it->second - 1];
+        }
+        m += it->second;
+    }
+    if (nres) *nres = m;
+    if (vstate) {
+        vstate[0] = 0;
+        for (uint32_t v = lo + 1; v <= hi; v++) vstate[v - lo] = c.cache_vstate[v];
+    }
+    uint32_t u = 0;
+    for (const uint32_t l : c.cache_trail) {
+        const uint32_t v = l >> 1;
+        if (!(v > lo && v <= hi)) continue;
+        if (trail) trail[u] = l - shift;
+        u++;
+    }
+    if (ntrail) *ntrail = u;
     API_END
 }
 

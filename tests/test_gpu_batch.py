@@ -1,0 +1,93 @@
+"""Batches of independent instances in one context: each instance must come out
+exactly as the oracle (and the single-instance GPU path) simplifies it alone --
+clause database, model stack, variable states, units (as a set) and the
+unsatisfiable verdict -- whatever the other instances in the batch do.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from parafrost_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+
+def _instance(rng, k):
+    nv = int(rng.integers(20, 600))
+    kind = k % 4
+    seed = int(rng.integers(1, 1 << 30))
+    if kind == 0:
+        return gen.uniform_ksat(nv, int(nv * rng.uniform(2.0, 4.6)), 3, seed)
+    if kind == 1:
+        return gen.geometric(nv, int(nv * rng.uniform(1.5, 4.0)), 2, 8, seed)
+    if kind == 2:
+        nv = max(nv, 64)
+        return gen.scalefree(nv, int(nv * rng.uniform(2.0, 5.0)), 2.5, 0.3, seed)
+    # dense binary-heavy scale-free instances: often refuted by the simplifier
+    nv = max(nv, 64)
+    return gen.scalefree(nv, int(nv * rng.uniform(4.0, 8.0)), 2.5, 0.1, seed)
+
+
+def _full_3cnf():
+    """All eight clauses over three variables: unsatisfiable."""
+    lits, offs = [], [0]
+    for m in range(8):
+        for v in range(3):
+            lits.append(2 * (v + 1) + ((m >> v) & 1))
+        offs.append(len(lits))
+    return gen.Cnf(3, np.array(lits, np.uint32), np.array(offs, np.uint64))
+
+
+def _check(cnfs, **kw):
+    import parafrost_amd as pf
+    g = pf.BatchSimplifier([c.nvars for c in cnfs], pf.default_opts(**kw))
+    try:
+        g.upload_instances(cnfs)
+        status = g.sigmify_batch()
+        n_unsat = 0
+        for i, cnf in enumerate(cnfs):
+            o = po.Oracle(cnf, po.default_opts(**kw))
+            ro = o.sigmify()
+            assert (ro != 0) == (status[i] != 0), f"instance {i}: unsat oracle {ro} batch {status[i]}"
+            if ro != 0:
+                n_unsat += 1
+                continue
+            so = o.state()
+            w, t, r, vs = g.instance(i)
+            assert np.array_equal(so.cnf, w), f"instance {i}: clause database differs"
+            assert np.array_equal(np.sort(so.trail), np.sort(t)), f"instance {i}: units differ"
+            assert np.array_equal(so.resolved, r), f"instance {i}: model stack differs"
+            assert np.array_equal(so.vstate, vs), f"instance {i}: variable states differ"
+        return n_unsat
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_batch_matches_each_instance_alone(seed):
+    print(f"batch seed={seed}", flush=True)
+    rng = np.random.default_rng(4400 + seed)
+    cnfs = [_instance(rng, k) for k in range(int(rng.integers(8, 24)))]
+    if seed % 2 == 0:
+        cnfs.insert(int(rng.integers(0, len(cnfs))), _full_3cnf())
+    kw = dict(phases=int(rng.integers(1, 7)), mu_pos=int(rng.integers(10, 60)), mu_neg=int(rng.integers(10, 60)))
+    _check(cnfs, **kw)
+
+
+def test_batch_with_refuted_instances():
+    rng = np.random.default_rng(4499)
+    cnfs = [_instance(rng, 3) for _ in range(10)] + [_full_3cnf(), _instance(rng, 0)]
+    assert _check(cnfs) >= 1
+
+
+def test_batch_of_one_equals_single():
+    rng = np.random.default_rng(4500)
+    _check([_instance(rng, 1)])
+
+
+def test_batch_option_variants():
+    rng = np.random.default_rng(4501)
+    cnfs = [_instance(rng, k) for k in range(9)]
+    for kw in (dict(ere_en=0), dict(sub_en=0), dict(lcve_min=30), dict(phases=8, lits_min=1 << 40)):
+        print(f"batch options {kw}", flush=True)
+        _check(cnfs, **kw)
