@@ -37,7 +37,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--batch-lits", type=float, default=1.0e9,
+                    help="c4: literal budget of one batch (instances are admitted in order until it is reached)")
+    ap.add_argument("--c4-instances", type=int, default=4096, help="c4: number of instances")
+    ap.add_argument("--plain-sample", type=int, default=64, help="c4: instances timed on the plain path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check", action="store_true", help="compare the final state with the oracle")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -174,6 +178,141 @@ def roofline(kt: dict, ka: dict):
     return roof, rows
 
 
+def run_c4(args):
+    """Batching workload: 4,096 independent random 3-SAT / 4-SAT instances (variables
+    log-uniform in [1,000, 200,000]; ratios U[4.0, 4.3] / U[9.5, 10.0]; seed 4). The
+    instances are admitted in order into batches of at most --batch-lits literals; each
+    batch is one context whose kernel launches serve all of its instances. A step = every
+    batch simplified once (inputs resident in HBM). The plain path (one context per
+    instance) is timed on an evenly spread sample of instances."""
+    import torch
+    import parafrost_amd as pf
+    from parafrost_amd import gen
+    t0 = time.perf_counter()
+    B = gen.c4(ninst=args.c4_instances)
+    gen_s = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    d_lits = torch.from_numpy(B.lits.view(np.int32)).to(dev)
+    torch.cuda.synchronize()
+    # batches by literal budget, in instance order
+    lit_first = B.offs[B.cls_first.astype(np.int64)]
+    groups, a = [], 0
+    while a < B.ninst:
+        b = a + 1
+        while b < B.ninst and int(lit_first[b + 1] - lit_first[a]) <= args.batch_lits:
+            b += 1
+        groups.append((a, b))
+        a = b
+    # group inputs in batch numbering relative to the group's first instance, resident in HBM
+    inputs = []
+    for (a, b) in groups:
+        ca, cb = int(B.cls_first[a]), int(B.cls_first[b])
+        la, lb = int(B.offs[ca]), int(B.offs[cb])
+        lits = torch.from_numpy((B.lits[la:lb] - np.uint32(2 * int(B.base[a]))).view(np.int32)).to(dev)
+        offs = torch.from_numpy((B.offs[ca:cb + 1] - np.uint64(la)).view(np.int64)).to(dev)
+        inputs.append((B.inst_nvars[a:b].copy(), lits, offs, cb - ca, lb - la))
+    torch.cuda.synchronize()
+    cap = max(int(gen.batch_bases(nv)[-1]) for nv, *_ in inputs)
+    # one context sized for the largest group, laid out again for every group
+    probe = np.zeros(1, np.uint32)
+    probe[0] = cap
+    s = pf.BatchSimplifier(probe, pf.default_opts())
+
+    def step(account=False):
+        st = []
+        s.set_accounting(account)
+        for (nv, lits, offs, ncls, nl) in inputs:
+            s.relayout(nv)
+            s.upload_device(lits.data_ptr(), offs.data_ptr(), ncls, nl)
+            st.append(s.sigmify_batch())
+        s.set_accounting(False)
+        return st
+
+    for _ in range(args.warmup):
+        step()
+    s.reset_times()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    kt = s.kernel_times()
+    s.reset_times()
+    step(account=True)
+    ka = s.kernel_times()
+    roof, kernels = roofline(kt, ka)
+    unsat = int(sum(int((x != 0).sum()) for x in st))
+    s.close()
+    del inputs
+    # plain path on a spread sample
+    pick = np.unique(np.linspace(0, B.ninst - 1, min(args.plain_sample, B.ninst)).round().astype(np.int64))
+    plain_t, plain_l = 0.0, 0
+    for i in pick:
+        c = B.instance(int(i))
+        dl = torch.from_numpy(c.lits.view(np.int32)).to(dev)
+        do = torch.from_numpy(c.offs.view(np.int64)).to(dev)
+        s = pf.Simplifier(c.nvars, pf.default_opts())
+        for rep in range(2):   # the second run is timed
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            s.reset_for_upload()
+            s.upload_device(dl.data_ptr(), do.data_ptr(), c.ncls, c.nlits)
+            try:
+                s.sigmify()
+            except pf.Unsat:
+                pass
+            torch.cuda.synchronize()
+            if rep:
+                plain_t += time.perf_counter() - t1
+                plain_l += c.nlits
+        s.close()
+    plain = plain_l / plain_t
+    value = B.nlits / dt
+    out = {
+        "metric": "simplified literals/sec on 1 MI355X (batched/served vs plain); p50/p99 latency",
+        "value": value, "unit": "literals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic",
+        "config": {"path": "batched", "workload": f"{B.ninst} random 3-SAT/4-SAT instances, vars log-uniform "
+                   "[1000, 200000], ratios U[4.0,4.3]/U[9.5,10.0], seed 4",
+                   "instances": B.ninst, "variables": B.nvars, "clauses": B.ncls, "input_literals": B.nlits,
+                   "batches": len(groups), "batch_literal_budget": args.batch_lits, "refuted_instances": unsat,
+                   "generation_s": round(gen_s, 1)},
+        "plain": {"value": plain, "unit": "literals/s",
+                  "sample": f"{len(pick)} instances spread over the batch, {plain_l} literals, one context each, "
+                            "second of two runs timed"},
+        "vs_plain": value / plain,
+        "roofline": roof, "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_instances(B)
+    print(json.dumps(out))
+
+
+def cpu_baseline_instances(B):
+    """The oracle on instances of the batch (spread over sizes) until about 10 s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    order = np.argsort(B.inst_nvars)
+    pick = order[np.linspace(0, len(order) // 2, 40).round().astype(np.int64)]
+    t_total, lits, n = 0.0, 0, 0
+    for i in pick:
+        c = B.instance(int(i))
+        o = po.Oracle(c, po.default_opts())
+        t0 = time.perf_counter()
+        o.sigmify()
+        t_total += time.perf_counter() - t0
+        lits += c.nlits
+        n += 1
+        del o
+        if t_total > 10.0:
+            break
+    return {"value": lits / t_total, "unit": "literals/s", "cores": 1, "kind": "port",
+            "sample": f"{n} instances of the batch (smaller half, spread), full sigmify each, single thread, "
+                      f"{lits} literals in {t_total:.1f} s"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -188,6 +327,9 @@ def main():
         return
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
+        return
+    if args.config == "c4":
+        run_c4(args)
         return
     import torch
     import parafrost_amd as pf

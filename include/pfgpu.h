@@ -66,6 +66,10 @@ typedef struct pfg_opts {
 int pfg_create_batch(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t ninst, const uint32_t *nvars,
                      const pfg_opts *opts);
 int pfg_batch_ninst(pfg_ctx *ctx, uint32_t *ninst);
+/* lay the batch context out for another set of instances (whose aligned variable total
+ * fits the context's), forgetting the clause database and all variable state; the
+ * device allocations are kept, so a stream of batches is admitted without reallocating */
+int pfg_batch_relayout(pfg_ctx *ctx, uint32_t ninst, const uint32_t *nvars);
 /* clauses in CSR form, each instance in its own numbering; instance i has clauses
  * [inst_first[i], inst_first[i + 1]) (inst_first has ninst + 1 entries). pfg_upload_cnf
  * and pfg_upload_cnf_device take the context's numbering (instance literals shifted by

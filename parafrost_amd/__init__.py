@@ -49,7 +49,7 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_stop", "pfg_next_phase", "pfg_hist", "pfg_download_cnf", "pfg_download_trail",
        "pfg_download_resolved", "pfg_download_vstate", "pfg_download_pvs", "pfg_download_ot_list",
        "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
-       "pfg_set_accounting", "pfg_set_stage_words", "pfg_create_batch", "pfg_batch_ninst", "pfg_upload_batch",
+       "pfg_set_accounting", "pfg_set_stage_words", "pfg_create_batch", "pfg_batch_ninst", "pfg_batch_relayout", "pfg_upload_batch",
        "pfg_sigmify_batch", "pfg_batch_download"]
 
 
@@ -91,6 +91,7 @@ def lib():
         L.pfg_set_stage_words.argtypes = [vp, u32]
         L.pfg_create_batch.argtypes = [ctypes.POINTER(vp), i32, u64, u32, vp, ctypes.POINTER(Opts)]
         L.pfg_batch_ninst.argtypes = [vp, ctypes.POINTER(u32)]
+        L.pfg_batch_relayout.argtypes = [vp, u32, vp]
         L.pfg_upload_batch.argtypes = [vp, vp, vp, vp]
         L.pfg_sigmify_batch.argtypes = [vp, vp]
         L.pfg_batch_download.argtypes = [vp, u32, vp, ctypes.POINTER(u64), ctypes.POINTER(u64), vp,
@@ -309,6 +310,23 @@ class BatchSimplifier(Simplifier):
             if i + 1 < self.ninst:
                 at = (at + 15) & ~15
         self.base[self.ninst] = at
+        self.nvars = at
+
+    def relayout(self, nvars):
+        """Re-lay the context out for another set of instances (allocations are kept)."""
+        nv = np.ascontiguousarray(nvars, np.uint32)
+        self._check(self.L.pfg_batch_relayout(self.h, len(nv), nv.ctypes.data))
+        self.inst_nvars = nv
+        self.ninst = len(nv)
+        base = np.zeros(self.ninst + 1, np.uint64)
+        at = 0
+        for i, n in enumerate(nv):
+            base[i] = at
+            at += int(n)
+            if i + 1 < self.ninst:
+                at = (at + 15) & ~15
+        base[self.ninst] = at
+        self.base = base
         self.nvars = at
 
     def upload_instances(self, cnfs):

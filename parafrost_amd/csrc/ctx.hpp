@@ -63,7 +63,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     uint64_t budget = 0, allocated = 0;
     pfg_opts opts{};
-    uint32_t nvars = 0;
+    uint32_t nvars = 0, nvars_cap = 0;   // variables in use; per-variable buffers are sized for nvars_cap
     uint64_t nlits2 = 0;  // 2 * (nvars + 1)
 
     // clause pool (SCLAUSE images) and references in clause order

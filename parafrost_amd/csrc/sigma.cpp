@@ -366,6 +366,7 @@ int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, c
     c.budget = hbm_budget;
     c.trace = getenv("PFG_TRACE") != nullptr;
     c.nvars = nvars;
+    c.nvars_cap = nvars;
     c.nlits2 = 2ull * ((uint64_t)nvars + 1);
     if (opts) c.opts = *opts; else pfg_default_opts(&c.opts);
     try {
@@ -555,6 +556,37 @@ int pfg_create_batch(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t ni
     API_BEGIN
     set_instances(c, ninst, nvars);
     c.batch = true;
+    API_END
+}
+
+int pfg_batch_relayout(pfg_ctx *ctx, uint32_t ninst, const uint32_t *nvars)
+{
+    if (!ctx || !ninst || !nvars) return PFG_EINVAL;
+    API_BEGIN
+    if (!c.batch) return PFG_EINVAL;
+    for (uint32_t i = 0; i < ninst; i++)
+        if (!nvars[i]) return PFG_EINVAL;
+    std::vector<uint32_t> base;
+    const uint64_t total = instance_bases(ninst, nvars, base);
+    if (total > c.nvars_cap) return PFG_EINVAL;
+    c.nvars = (uint32_t)total;
+    c.nlits2 = 2ull * (total + 1);
+    set_instances(c, ninst, nvars);
+    // forget the previous clause database and variable state
+    PFG_HIP(hipMemsetAsync(c.value.p, 0xFF, c.value.cap, c.stream));
+    PFG_HIP(hipMemsetAsync(c.vstate.p, 0, c.vstate.cap, c.stream));
+    PFG_HIP(hipMemsetAsync(c.cnt.p, 0, (CNT_CONF + 1) * 8, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    c.ntrail = c.propagated = 0;
+    c.nres = 0;
+    c.npvs = 0;
+    c.msz = c.nrefs = 0;
+    c.phase = 0;
+    c.mu_inc = 0;
+    c.nclauses = c.nliterals = 0;
+    c.unsat = false;
+    c.ot_valid = c.hist_live = false;
+    c.batch_gen++;
     API_END
 }
 

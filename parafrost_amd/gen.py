@@ -99,3 +99,103 @@ def config(name: str) -> Cnf:
     if name == "c3":   # scale-free 5M vars, 20M clauses, beta 2.5, 20% Tseitin AND/OR gates, seed 3
         return scalefree(5_000_000, 20_000_000, 2.5, 0.2, seed=3)
     raise KeyError(name)
+
+
+# ---------------------------------------------------------------- batches
+@dataclass
+class Batch:
+    """Independent instances in batch numbering: instance i's variable v is the batch
+    variable base[i] + v (bases are multiples of 16, as the simplifier lays them out);
+    its clauses are [cls_first[i], cls_first[i + 1])."""
+    inst_nvars: np.ndarray  # uint32 per instance
+    base: np.ndarray        # uint64, ninst + 1 (base[ninst] = batch variable count)
+    cls_first: np.ndarray   # uint64, ninst + 1
+    lits: np.ndarray        # uint32, batch numbering
+    offs: np.ndarray        # uint64, ncls + 1
+    kinds: np.ndarray       # clause width per instance
+
+    @property
+    def ninst(self) -> int:
+        return len(self.inst_nvars)
+
+    @property
+    def nvars(self) -> int:
+        return int(self.base[-1])
+
+    @property
+    def ncls(self) -> int:
+        return len(self.offs) - 1
+
+    @property
+    def nlits(self) -> int:
+        return int(self.offs[-1])
+
+    def instance(self, i: int) -> Cnf:
+        """Instance i in its own numbering."""
+        a, b = int(self.cls_first[i]), int(self.cls_first[i + 1])
+        la, lb = int(self.offs[a]), int(self.offs[b])
+        return Cnf(int(self.inst_nvars[i]), self.lits[la:lb] - np.uint32(2 * int(self.base[i])),
+                   self.offs[a:b + 1] - np.uint64(la))
+
+
+def batch_bases(nvars) -> np.ndarray:
+    base = np.zeros(len(nvars) + 1, np.uint64)
+    at = 0
+    for i, n in enumerate(nvars):
+        base[i] = at
+        at += int(n)
+        if i + 1 < len(nvars):
+            at = (at + 15) & ~15
+    base[len(nvars)] = at
+    return base
+
+
+def batch_ksat(nvars, ncls, ks, seeds, threads: int = 16) -> Batch:
+    """Uniform random k-SAT instances (k distinct variables per clause, fair signs),
+    generated in parallel straight into batch numbering."""
+    import concurrent.futures as cf
+    nvars = np.asarray(nvars, np.uint32)
+    ncls = np.asarray(ncls, np.uint64)
+    ks = np.asarray(ks, np.int64)
+    n = len(nvars)
+    base = batch_bases(nvars)
+    cls_first = np.zeros(n + 1, np.uint64)
+    cls_first[1:] = np.cumsum(ncls)
+    lit_first = np.zeros(n + 1, np.uint64)
+    lit_first[1:] = np.cumsum(ncls * ks.astype(np.uint64))
+    lits = np.empty(int(lit_first[-1]), np.uint32)
+    offs = np.empty(int(cls_first[-1]) + 1, np.uint64)
+    offs[0] = 0
+    lib = _load()
+
+    def one(i):
+        la, lb = int(lit_first[i]), int(lit_first[i + 1])
+        ca, cb = int(cls_first[i]), int(cls_first[i + 1])
+        tmp = np.empty(cb - ca + 1, np.uint64)
+        part = lits[la:lb]
+        got = lib.pfgen_uniform_ksat(int(nvars[i]), cb - ca, int(ks[i]), int(seeds[i]), part, tmp)
+        if got != lb - la:
+            raise ValueError(f"instance {i}: generator returned {got} literals, expected {lb - la}")
+        part += np.uint32(2 * int(base[i]))
+        offs[ca + 1:cb + 1] = tmp[1:] + np.uint64(la)
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(one, range(n)))
+    return Batch(nvars, base, cls_first, lits, offs, ks)
+
+
+def c4_specs(ninst: int = 4096, vmin: int = 1000, vmax: int = 200_000, seed: int = 4):
+    """The batching workload: variable counts log-uniform in [vmin, vmax]; each instance
+    is random 3-SAT at clause ratio U[4.0, 4.3] or 4-SAT at U[9.5, 10.0], 50/50."""
+    rng = np.random.default_rng(seed)
+    nv = np.exp(rng.uniform(np.log(vmin), np.log(vmax), ninst)).round().astype(np.uint32)
+    four = rng.random(ninst) < 0.5
+    ratio = np.where(four, rng.uniform(9.5, 10.0, ninst), rng.uniform(4.0, 4.3, ninst))
+    ncls = np.maximum(1, (ratio * nv).round()).astype(np.uint64)
+    ks = np.where(four, 4, 3).astype(np.int64)
+    seeds = rng.integers(1, 1 << 62, ninst, dtype=np.int64).astype(np.uint64)
+    return nv, ncls, ks, seeds
+
+
+def c4(ninst: int = 4096, vmax: int = 200_000, seed: int = 4, threads: int = 16) -> Batch:
+    return batch_ksat(*c4_specs(ninst, 1000, vmax, seed), threads=threads)

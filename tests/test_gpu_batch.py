@@ -38,9 +38,13 @@ def _full_3cnf():
     return gen.Cnf(3, np.array(lits, np.uint32), np.array(offs, np.uint64))
 
 
-def _check(cnfs, **kw):
+def _check(cnfs, g=None, **kw):
     import parafrost_amd as pf
-    g = pf.BatchSimplifier([c.nvars for c in cnfs], pf.default_opts(**kw))
+    own = g is None
+    if own:
+        g = pf.BatchSimplifier([c.nvars for c in cnfs], pf.default_opts(**kw))
+    else:
+        g.relayout([c.nvars for c in cnfs])
     try:
         g.upload_instances(cnfs)
         status = g.sigmify_batch()
@@ -60,7 +64,8 @@ def _check(cnfs, **kw):
             assert np.array_equal(so.vstate, vs), f"instance {i}: variable states differ"
         return n_unsat
     finally:
-        g.close()
+        if own:
+            g.close()
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -91,3 +96,31 @@ def test_batch_option_variants():
     for kw in (dict(ere_en=0), dict(sub_en=0), dict(lcve_min=30), dict(phases=8, lits_min=1 << 40)):
         print(f"batch options {kw}", flush=True)
         _check(cnfs, **kw)
+
+
+def test_batch_relayout_reuses_context():
+    """One context laid out again for another set of instances gives the same results."""
+    import parafrost_amd as pf
+    from parafrost_amd import gen as G
+    rng = np.random.default_rng(4502)
+    g1 = [_instance(rng, k) for k in range(7)]
+    g2 = [_instance(rng, k + 1) for k in range(11)] + [_full_3cnf()]
+    cap = max(int(G.batch_bases([c.nvars for c in grp])[-1]) for grp in (g1, g2))
+    g = pf.BatchSimplifier([cap], pf.default_opts())
+    try:
+        _check(g1, g)
+        _check(g2, g)
+        _check(g1, g)
+    finally:
+        g.close()
+
+
+def test_batch_generator_instances_roundtrip():
+    """The batch generator's instance view is the instance in its own numbering."""
+    from parafrost_amd import gen as G
+    b = G.c4(ninst=12, vmax=3000, threads=4)
+    for i in range(b.ninst):
+        c = b.instance(i)
+        assert c.nvars == int(b.inst_nvars[i])
+        assert c.lits.min() >= 2 and c.lits.max() <= 2 * c.nvars + 1
+        assert c.nlits == c.ncls * int(b.kinds[i])
