@@ -311,6 +311,7 @@ class BatchSimplifier(Simplifier):
                 at = (at + 15) & ~15
         self.base[self.ninst] = at
         self.nvars = at
+        self.nvars_cap = at   # relayout() accepts instance sets up to this many (aligned) variables
 
     def relayout(self, nvars):
         """Re-lay the context out for another set of instances (allocations are kept)."""

@@ -118,8 +118,9 @@ struct Ctx {
     DBuf<uint64_t> inel;                  // per instance: elected variables
     DBuf<uint64_t> icnt;                  // per instance: live clauses, live literals (2 ninst)
     uint64_t batch_gen = 0, cache_gen = ~0ull;   // host copy of the batch state for per-instance downloads
-    std::vector<uint32_t> cache_mem, cache_res, cache_trail, cache_ctx_words;
-    std::vector<uint64_t> cache_cls;
+    std::vector<uint32_t> cache_mem;
+    std::vector<std::vector<uint64_t>> cache_cls;   // per instance: clause references
+    std::vector<std::vector<uint32_t>> cache_res, cache_trail;
     std::vector<uint8_t> cache_vstate;
 
     // round state

@@ -259,3 +259,70 @@ int pfgen_write_dimacs(const char *path, uint32_t nvars, uint64_t ncls,
     }
     return fclose(f);
 }
+
+/* BMC-like requests: a random and-inverter graph (AIG) unrolled over time frames.
+ * Nodes: nl latches, ni primary inputs, ng 2-input AND gates; gate j takes two earlier
+ * nodes, each edge inverted with probability pinv; every latch's next state is a random
+ * node literal. Frame 0 has a variable per latch; in frame t > 0 a latch is the literal
+ * of its next-state node in frame t - 1. Every frame has a variable per input and gate:
+ *   var(latch i, frame 0) = 1 + i
+ *   var(input i, frame t) = 1 + nl + t (ni + ng) + i,  var(gate j, t) = ... + ni + j
+ * A gate g = a & b is Tseitin-encoded as (~g | a), (~g | b), (g | ~a | ~b); a clause
+ * whose literals collapse (a == b, a == ~b) is written the way it comes out, with a
+ * repeated literal dropped and a tautology skipped. Only frames [t0, t1) are emitted,
+ * so an instance grows by generating its new frames. The structure depends on the
+ * seed only. Returns the literal count (offs gets ncls + 1 entries); *ncls_out and
+ * *nvars_out receive the clause count and the variable count after t1 frames. */
+typedef struct { uint32_t a, b; } aig_gate;
+
+int64_t pfgen_aig_bmc(uint64_t seed, uint32_t ng, uint32_t nl, uint32_t ni, double pinv, uint32_t t0, uint32_t t1,
+                      uint32_t *lits, uint64_t *offs, uint64_t *ncls_out, uint32_t *nvars_out)
+{
+    if (t1 < t0 || ng == 0) return -1;
+    rng_t r;
+    rng_seed(&r, seed);
+    const uint32_t nn = nl + ni + ng;   /* node literal = 2 node + sign */
+    aig_gate *g = (aig_gate *)malloc(sizeof(aig_gate) * ng);
+    uint32_t *next = (uint32_t *)malloc(sizeof(uint32_t) * (nl ? nl : 1));
+    uint32_t *lat = (uint32_t *)malloc(sizeof(uint32_t) * (nl ? nl : 1));    /* latch literals of the frame */
+    uint32_t *nlat = (uint32_t *)malloc(sizeof(uint32_t) * (nl ? nl : 1));
+    if (!g || !next || !lat || !nlat) { free(g); free(next); free(lat); free(nlat); return -1; }
+    for (uint32_t j = 0; j < ng; j++) {
+        const uint32_t lim = nl + ni + j;
+        const uint32_t x = (uint32_t)rng_below(&r, lim ? lim : 1), y = (uint32_t)rng_below(&r, lim ? lim : 1);
+        g[j].a = 2 * x + (rng_unit(&r) < pinv);
+        g[j].b = 2 * y + (rng_unit(&r) < pinv);
+    }
+    for (uint32_t i = 0; i < nl; i++) next[i] = 2 * (uint32_t)rng_below(&r, nn) + (rng_unit(&r) < pinv);
+    for (uint32_t i = 0; i < nl; i++) lat[i] = 2 * (1 + i);
+    const uint32_t per = ni + ng;
+    uint64_t w = 0, c = 0;
+    offs[0] = 0;
+    for (uint32_t t = 0; t < t1; t++) {
+        const uint32_t fb = 1 + nl + t * per;   /* first variable of frame t */
+        /* literal of node-literal x in frame t */
+#define NODE_LIT(x) (((x) >> 1) < nl ? (lat[(x) >> 1] ^ ((x) & 1u)) : (2 * (fb + ((x) >> 1) - nl) + ((x) & 1u)))
+        if (t >= t0) {
+            for (uint32_t j = 0; j < ng; j++) {
+                const uint32_t gv = 2 * (fb + ni + j);
+                const uint32_t a = NODE_LIT(g[j].a), b = NODE_LIT(g[j].b);
+                /* (~g | a) */
+                lits[w++] = gv ^ 1u; lits[w++] = a; offs[++c] = w;
+                /* (~g | b) */
+                if (b != a) { lits[w++] = gv ^ 1u; lits[w++] = b; offs[++c] = w; }
+                /* (g | ~a | ~b) */
+                if (a == (b ^ 1u)) continue;   /* tautology */
+                lits[w++] = gv; lits[w++] = a ^ 1u;
+                if (b != a) lits[w++] = b ^ 1u;
+                offs[++c] = w;
+            }
+        }
+        for (uint32_t i = 0; i < nl; i++) nlat[i] = NODE_LIT(next[i]);
+        for (uint32_t i = 0; i < nl; i++) lat[i] = nlat[i];
+#undef NODE_LIT
+    }
+    free(g); free(next); free(lat); free(nlat);
+    if (ncls_out) *ncls_out = c;
+    if (nvars_out) *nvars_out = nl + t1 * per;
+    return (int64_t)w;
+}

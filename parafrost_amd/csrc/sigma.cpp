@@ -638,32 +638,47 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
 {
     API_BEGIN
     if (inst >= c.ninst) return PFG_EINVAL;
-    if (c.cache_gen != c.batch_gen || c.cache_ctx_words.empty()) {
+    if (c.cache_gen != c.batch_gen) {
+        // one download of the batch state, bucketed by instance
         std::vector<uint64_t> refs(c.nrefs);
-        std::vector<uint32_t> mem(c.msz);
-        c.cache_res.resize(c.nres);
+        std::vector<uint32_t> res(c.nres), trail(c.ntrail);
+        c.cache_mem.resize(c.msz);
         c.cache_vstate.resize((uint64_t)c.nvars + 1);
-        c.cache_trail.resize(c.ntrail);
         if (c.nrefs) PFG_HIP(hipMemcpyAsync(refs.data(), c.refs.p, c.nrefs * 8, hipMemcpyDeviceToHost, c.stream));
-        if (c.msz) PFG_HIP(hipMemcpyAsync(mem.data(), c.mem.p, c.msz * 4, hipMemcpyDeviceToHost, c.stream));
-        if (c.nres) PFG_HIP(hipMemcpyAsync(c.cache_res.data(), c.resolved.p, c.nres * 4, hipMemcpyDeviceToHost, c.stream));
+        if (c.msz) PFG_HIP(hipMemcpyAsync(c.cache_mem.data(), c.mem.p, c.msz * 4, hipMemcpyDeviceToHost, c.stream));
+        if (c.nres) PFG_HIP(hipMemcpyAsync(res.data(), c.resolved.p, c.nres * 4, hipMemcpyDeviceToHost, c.stream));
         PFG_HIP(hipMemcpyAsync(c.cache_vstate.data(), c.vstate.p, (uint64_t)c.nvars + 1, hipMemcpyDeviceToHost, c.stream));
-        if (c.ntrail) PFG_HIP(hipMemcpyAsync(c.cache_trail.data(), c.trail.p, 4ull * c.ntrail, hipMemcpyDeviceToHost, c.stream));
+        if (c.ntrail) PFG_HIP(hipMemcpyAsync(trail.data(), c.trail.p, 4ull * c.ntrail, hipMemcpyDeviceToHost, c.stream));
         PFG_HIP(hipStreamSynchronize(c.stream));
-        c.cache_ctx_words.assign(1, 0);
-        c.cache_cls.clear();
-        for (uint64_t i = 0; i < c.nrefs; i++) c.cache_cls.push_back(refs[i]);
-        c.cache_mem.swap(mem);
+        Sigma S(c);
+        c.cache_cls.assign(c.ninst, {});
+        c.cache_res.assign(c.ninst, {});
+        c.cache_trail.assign(c.ninst, {});
+        for (const uint64_t r : refs) {
+            const uint32_t *cl = c.cache_mem.data() + r;
+            if (cl[2]) c.cache_cls[S.inst_of(cl[3] >> 1)].push_back(r);
+        }
+        // model records end with their size; all literals of a record belong to one
+        // instance; walked newest first, kept oldest first
+        std::vector<std::vector<std::pair<uint64_t, uint64_t>>> recs(c.ninst);
+        for (int64_t j = (int64_t)res.size() - 1; j > 0;) {
+            const uint64_t k = res[j];
+            const int64_t s0 = j - (int64_t)k;
+            if (k == 0 || s0 < 0) throw Error(PFG_EINTERNAL, "malformed model stack");
+            recs[S.inst_of(res[s0] >> 1)].emplace_back((uint64_t)s0, k + 1);
+            j = s0 - 1;
+        }
+        for (uint32_t i = 0; i < c.ninst; i++)
+            for (auto it = recs[i].rbegin(); it != recs[i].rend(); ++it)
+                c.cache_res[i].insert(c.cache_res[i].end(), res.begin() + it->first, res.begin() + it->first + it->second);
+        for (const uint32_t l : trail) c.cache_trail[S.inst_of(l >> 1)].push_back(l);
         c.cache_gen = c.batch_gen;
     }
-    Sigma S(c);
-    const uint32_t lo = c.hibase[inst], hi = lo + c.inst_nvars[inst], shift = 2u * lo;
-    uint64_t w = 0, n = 0;
-    for (const uint64_t r : c.cache_cls) {
+    const uint32_t lo = c.hibase[inst], shift = 2u * lo;
+    uint64_t w = 0;
+    for (const uint64_t r : c.cache_cls[inst]) {
         const uint32_t *cl = c.cache_mem.data() + r;
         const uint32_t sz = cl[2];
-        const uint32_t v = sz ? (cl[3] >> 1) : 0u;
-        if (!(v > lo && v <= hi)) continue;
         if (words) {
             words[w] = cl[0];
             words[w + 1] = cl[1];
@@ -671,41 +686,26 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
             for (uint32_t k = 0; k < sz; k++) words[w + 3 + k] = cl[3 + k] - shift;
         }
         w += 3ull + sz;
-        n++;
     }
     if (nwords) *nwords = w;
-    if (ncls) *ncls = n;
-    // model records end with their size; all literals of a record belong to one instance
-    std::vector<std::pair<uint64_t, uint64_t>> recs;   // (start, length incl. size word), newest first
-    for (int64_t j = (int64_t)c.cache_res.size() - 1; j > 0;) {
-        const uint64_t k = c.cache_res[j];
-        const int64_t s0 = j - (int64_t)k;
-        if (k == 0 || s0 < 0) throw Error(PFG_EINTERNAL, "malformed model stack");
-        const uint32_t v = c.cache_res[s0] >> 1;
-        if (v > lo && v <= hi) recs.emplace_back((uint64_t)s0, k + 1);
-        j = s0 - 1;
-    }
-    uint64_t m = 0;
-    for (auto it = recs.rbegin(); it != recs.rend(); ++it) {
-        if (resolved) {
-            for (uint64_t t = 0; t + 1 < it->second; t++) resolved[m + t] = c.cache_res[it->first + t] - shift;
-            resolved[m + it->second - 1] = c.cache_res[it->first + it->second - 1];
+    if (ncls) *ncls = c.cache_cls[inst].size();
+    const std::vector<uint32_t> &rs = c.cache_res[inst];
+    if (resolved) {
+        // every word but the sizes that end the records is a literal
+        for (int64_t j = (int64_t)rs.size() - 1; j >= 0;) {
+            const uint64_t k = rs[j];
+            resolved[j] = (uint32_t)k;
+            for (int64_t t = j - (int64_t)k; t < j; t++) resolved[t] = rs[t] - shift;
+            j -= (int64_t)k + 1;
         }
-        m += it->second;
     }
-    if (nres) *nres = m;
+    if (nres) *nres = rs.size();
     if (vstate) {
         vstate[0] = 0;
-        for (uint32_t v = lo + 1; v <= hi; v++) vstate[v - lo] = c.cache_vstate[v];
+        for (uint32_t v = 1; v <= c.inst_nvars[inst]; v++) vstate[v] = c.cache_vstate[lo + v];
     }
-    uint32_t u = 0;
-    for (const uint32_t l : c.cache_trail) {
-        const uint32_t v = l >> 1;
-        if (!(v > lo && v <= hi)) continue;
-        if (trail) trail[u] = l - shift;
-        u++;
-    }
-    if (ntrail) *ntrail = u;
+    if (trail) for (size_t k = 0; k < c.cache_trail[inst].size(); k++) trail[k] = c.cache_trail[inst][k] - shift;
+    if (ntrail) *ntrail = (uint32_t)c.cache_trail[inst].size();
     API_END
 }
 

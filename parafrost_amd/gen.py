@@ -33,6 +33,10 @@ def _load():
         lib.pfgen_scalefree.restype = ctypes.c_int64
         lib.pfgen_scalefree.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_uint64, u32p, u64p]
+        lib.pfgen_aig_bmc.restype = ctypes.c_int64
+        lib.pfgen_aig_bmc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, u32p, u64p,
+                                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32)]
         lib.pfgen_write_dimacs.restype = ctypes.c_int
         lib.pfgen_write_dimacs.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint64, u32p, u64p]
         _lib = lib
@@ -199,3 +203,19 @@ def c4_specs(ninst: int = 4096, vmin: int = 1000, vmax: int = 200_000, seed: int
 
 def c4(ninst: int = 4096, vmax: int = 200_000, seed: int = 4, threads: int = 16) -> Batch:
     return batch_ksat(*c4_specs(ninst, 1000, vmax, seed), threads=threads)
+
+
+# ---------------------------------------------------------------- BMC-like requests
+def aig_bmc(seed: int, ngates: int, t0: int, t1: int, nlatches: int = 64, ninputs: int = 32, pinv: float = 0.3):
+    """Frames [t0, t1) of a random AIG (see csrc/gen.c) Tseitin-encoded: returns
+    (Cnf of those frames' clauses, variable count after t1 frames). The clauses of
+    frames [0, t) followed by those of [t, t') are the clauses of frames [0, t')."""
+    nf = max(0, t1 - t0)
+    lits = np.empty(max(1, 8 * ngates * nf), np.uint32)
+    offs = np.empty(3 * ngates * nf + 1, np.uint64)
+    nc = ctypes.c_uint64()
+    nv = ctypes.c_uint32()
+    n = _load().pfgen_aig_bmc(seed, ngates, nlatches, ninputs, pinv, t0, t1, lits, offs, ctypes.byref(nc), ctypes.byref(nv))
+    if n < 0:
+        raise ValueError("generator rejected its arguments")
+    return Cnf(int(nv.value), lits[:n].copy(), offs[:nc.value + 1].copy()), int(nv.value)
