@@ -60,6 +60,13 @@ def lib_path() -> str:
 def lib():
     global _LIB
     if _LIB is None:
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 / libhsa-runtime64
+        # with the same sonames as ROCm's. Loaded first, they also serve this library; if
+        # this library loaded ROCm's copy first, PyTorch could no longer open the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         path = lib_path()
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")

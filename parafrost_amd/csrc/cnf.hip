@@ -478,35 +478,46 @@ bool prop(Ctx &c)
 
 // ------------------------------------------------------------------ batches
 // live clauses and literals per instance (instance of a clause = instance of its
-// first literal's variable); a wave whose clauses all belong to one instance adds once
-__global__ void k_count_inst(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
-                             const uint32_t *__restrict__ ibase, uint32_t ninst, unsigned long long *__restrict__ icnt)
+// first literal's variable; empty clauses, left only in refuted instances, are not counted). A block takes a tile of consecutive clauses; counts of
+// clauses of the tile's first instance are summed over the block and added once,
+// the others (a tile crossing an instance boundary) are added per clause.
+constexpr int CI_BLOCK = 256, CI_ITEMS = 16;
+__global__ void __launch_bounds__(CI_BLOCK) k_count_inst(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs,
+                                                        uint64_t n, const uint32_t *__restrict__ ibase, uint32_t ninst,
+                                                        unsigned long long *__restrict__ icnt)
 {
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
-        const uint64_t i = b + threadIdx.x;
+    __shared__ uint32_t s_first;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * CI_BLOCK * CI_ITEMS; t0 < n; t0 += (uint64_t)gridDim.x * CI_BLOCK * CI_ITEMS) {
+        if (threadIdx.x == 0) {
+            uint32_t f = 0xFFFFFFFFu;
+            for (uint64_t i = t0; i < n && i < t0 + CI_BLOCK * CI_ITEMS; i++) {   // first live clause of the tile
+                const uint32_t *cl = mem + refs[i];
+                if (!c_del(cl) && c_size(cl)) { f = find_inst(ibase, ninst, cl[3] >> 1); break; }
+            }
+            s_first = f;
+        }
+        __syncthreads();
+        const uint32_t first = s_first;
         unsigned long long nc = 0, nl = 0;
-        uint32_t inst = 0xFFFFFFFFu;
-        if (i < n) {
+        for (int k = 0; k < CI_ITEMS; k++) {
+            const uint64_t i = t0 + (uint64_t)k * CI_BLOCK + threadIdx.x;
+            if (i >= n) break;
             const uint32_t *cl = mem + refs[i];
-            if (!c_del(cl)) {
-                nc = 1;
-                nl = (unsigned)c_size(cl);
-                inst = find_inst(ibase, ninst, cl[3] >> 1);
+            if (c_del(cl) || !c_size(cl)) continue;   // an empty clause (refuted instance) names no instance
+            const uint32_t inst = find_inst(ibase, ninst, cl[3] >> 1);
+            if (inst == first) { nc++; nl += (unsigned)c_size(cl); }
+            else {
+                atomicAdd(&icnt[2 * inst], 1ull);
+                atomicAdd(&icnt[2 * inst + 1], (unsigned long long)c_size(cl));
             }
         }
-        const uint32_t lead = __shfl(inst, 0, 64);
-        if (__all(inst == lead || inst == 0xFFFFFFFFu)) {
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) { nc += __shfl_down(nc, o, 64); nl += __shfl_down(nl, o, 64); }
-            if (__lane_id() == 0 && lead != 0xFFFFFFFFu && nc) {
-                atomicAdd(&icnt[2 * lead], nc);
-                atomicAdd(&icnt[2 * lead + 1], nl);
-            }
+        nc = block_sum(nc);
+        nl = block_sum(nl);
+        if (threadIdx.x == 0 && nc) {
+            atomicAdd(&icnt[2 * first], nc);
+            atomicAdd(&icnt[2 * first + 1], nl);
         }
-        else if (inst != 0xFFFFFFFFu) {
-            atomicAdd(&icnt[2 * inst], nc);
-            atomicAdd(&icnt[2 * inst + 1], nl);
-        }
+        __syncthreads();
     }
 }
 
@@ -515,8 +526,8 @@ void count_inst(Ctx &c, std::vector<int64_t> &cls, std::vector<int64_t> &lits)
     PFG_HIP(hipMemsetAsync(c.icnt.p, 0, 16ull * c.ninst, c.stream));
     if (c.nrefs) {
         KTimer kt(c, K_COUNT, 20.0 * c.nrefs + 4.0 * c.nliterals);
-        k_count_inst<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.ibase.p, c.ninst,
-                                                                           (unsigned long long *)c.icnt.p);
+        k_count_inst<<<grid_for((c.nrefs + CI_ITEMS - 1) / CI_ITEMS, CI_BLOCK, 8192), CI_BLOCK, 0, c.stream>>>(
+            c.mem.p, c.refs.p, c.nrefs, c.ibase.p, c.ninst, (unsigned long long *)c.icnt.p);
     }
     std::vector<uint64_t> h(2ull * c.ninst);
     PFG_HIP(hipMemcpyAsync(h.data(), c.icnt.p, 16ull * c.ninst, hipMemcpyDeviceToHost, c.stream));
@@ -524,6 +535,86 @@ void count_inst(Ctx &c, std::vector<int64_t> &cls, std::vector<int64_t> &lits)
     cls.resize(c.ninst);
     lits.resize(c.ninst);
     for (uint32_t i = 0; i < c.ninst; i++) { cls[i] = (int64_t)h[2 * i]; lits[i] = (int64_t)h[2 * i + 1]; }
+}
+
+// Clause images grouped by instance (clause order kept within an instance), literals
+// in each instance's own numbering: instance keys per clause, a stable sort of the
+// clause indices by key, the image sizes scanned in that order, then the copy.
+// An empty clause (left only in a refuted instance) has no literal to name its
+// instance: it takes key ninst and sorts after every instance's range.
+__global__ void k_clause_inst(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                              const uint32_t *__restrict__ ibase, uint32_t ninst, uint32_t *__restrict__ keys,
+                              uint32_t *__restrict__ idx)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t *cl = mem + refs[i];
+        keys[i] = cl[2] ? find_inst(ibase, ninst, cl[3] >> 1) : ninst;
+        idx[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_image_sizes(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs,
+                              const uint32_t *__restrict__ idx, uint64_t n, uint32_t *__restrict__ sizes)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) sizes[i] = 3u + mem[refs[idx[i]] + 2];
+}
+
+__global__ void k_image_write(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs,
+                              const uint32_t *__restrict__ idx, const uint32_t *__restrict__ keys,
+                              const uint64_t *__restrict__ woff, uint64_t n, const uint32_t *__restrict__ ibase,
+                              uint32_t *__restrict__ out)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t *cl = mem + refs[idx[i]];
+        uint32_t *o = out + woff[i];
+        const uint32_t sz = cl[2], shift = sz ? 2u * ibase[keys[i]] : 0u;
+        o[0] = cl[0];
+        o[1] = cl[1];
+        o[2] = sz;
+        for (uint32_t k = 0; k < sz; k++) o[3 + k] = cl[3 + k] - shift;
+    }
+}
+
+// start[k] = word offset of instance k's first image (first sorted position with key >= k)
+__global__ void k_image_starts(const uint32_t *__restrict__ keys, const uint64_t *__restrict__ woff, uint64_t n,
+                               uint32_t ninst, uint64_t total, uint64_t *__restrict__ start)
+{
+    for (uint64_t k = gtid(); k <= ninst; k += gstride()) {
+        uint64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < k) lo = mid + 1;
+            else hi = mid;
+        }
+        start[k] = lo < n ? woff[lo] : total;
+    }
+}
+
+// out: device words (at least the live words); start: per-instance word starts (ninst + 1)
+void batch_images(Ctx &c, uint32_t *out, std::vector<uint64_t> &start)
+{
+    const uint64_t n = c.nrefs;
+    start.assign(c.ninst + 1, 0);
+    if (!n) return;
+    ensure(c, c.tmpu32a, n + 1);
+    ensure(c, c.tmpu32b, n + 1);
+    ensure(c, c.tmpu32c, n + 1);
+    ensure(c, c.tmpu64a, n + 1);
+    ensure(c, c.tmpu64c, (uint64_t)c.ninst + 2);
+    uint32_t *keys = c.tmpu32a.p, *idx = c.tmpu32b.p, *sizes = c.tmpu32c.p;
+    k_clause_inst<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.ibase.p, c.ninst, keys, idx);
+    int bits = 0;
+    while (bits < 32 && (c.ninst >> bits)) bits++;   // keys 0..ninst
+    if (bits) radix_sort_u32_pairs(c, keys, idx, n, ((bits + 7) / 8) * 8);
+    k_image_sizes<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.refs.p, idx, n, sizes);
+    scan_u32_to_u64(c, sizes, c.tmpu64a.p, n, c.cnt.p + CNT_TOTAL);
+    k_image_write<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.refs.p, idx, keys, c.tmpu64a.p, n, c.ibase.p, out);
+    read_counters(c, 1);
+    const uint64_t total = c.hcnt[CNT_TOTAL];
+    k_image_starts<<<grid_for((uint64_t)c.ninst + 1, 256, 1024), 256, 0, c.stream>>>(keys, c.tmpu64a.p, n, c.ninst, total,
+                                                                                      c.tmpu64c.p);
+    PFG_HIP(hipMemcpyAsync(start.data(), c.tmpu64c.p, 8ull * (c.ninst + 1), hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
 }
 
 // keep the elected variables of instances with cls 1, then those with cls 2, each

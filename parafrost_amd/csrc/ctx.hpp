@@ -29,6 +29,29 @@ struct Error : std::runtime_error {
             throw ::pfg::Error(PFG_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Growable pinned host buffer of T (download staging).
+template <typename T>
+struct PinBuf {
+    T *p = nullptr;
+    uint64_t cap = 0;
+    void ensure(uint64_t n)
+    {
+        if (n <= cap) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        uint64_t nc = n + n / 4 + 1024;
+        PFG_HIP(hipHostMalloc((void **)&p, nc * sizeof(T), hipHostMallocDefault));
+        cap = nc;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 // Growable device buffer of T.
 template <typename T>
 struct DBuf {
@@ -118,10 +141,11 @@ struct Ctx {
     DBuf<uint64_t> inel;                  // per instance: elected variables
     DBuf<uint64_t> icnt;                  // per instance: live clauses, live literals (2 ninst)
     uint64_t batch_gen = 0, cache_gen = ~0ull;   // host copy of the batch state for per-instance downloads
-    std::vector<uint32_t> cache_mem;
-    std::vector<std::vector<uint64_t>> cache_cls;   // per instance: clause references
+    PinBuf<uint32_t> cache_mem, pin_res, pin_trail;   // pinned staging of the downloads
+    PinBuf<uint64_t> pin_refs;
+    PinBuf<uint8_t> pin_vstate;
+    std::vector<uint64_t> cache_start;              // per instance: first word of its clause images
     std::vector<std::vector<uint32_t>> cache_res, cache_trail;
-    std::vector<uint8_t> cache_vstate;
 
     // round state
     int32_t phase = 0;

@@ -247,12 +247,30 @@ __global__ void k_elect_flags(const uint32_t *__restrict__ elig, const uint8_t *
         uint32_t f = 0;
         if (kind[i] == K_CAND && status[i] == S_IN) {
             const uint32_t inst = find_inst(I.ibase, I.ninst, elig[i]);
-            if (i < I.ibrk[inst]) {
-                f = 1;
-                if (I.batch) atomicAdd(&I.inel[inst], 1ull);
-            }
+            if (i < I.ibrk[inst]) f = 1;
         }
         flags[i] = f;
+    }
+}
+
+// batch: elected variables per instance. The elected list is in (instance, score,
+// variable) order, so an instance's variables form one run: found by two binary searches.
+__global__ void k_inst_elected(const uint32_t *__restrict__ pvs, uint32_t npvs, InstArgs I)
+{
+    for (uint64_t inst = gtid(); inst < I.ninst; inst += gstride()) {
+        uint32_t lo = 0, hi = npvs;   // first position of an instance >= inst
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (find_inst(I.ibase, I.ninst, pvs[mid]) < inst) lo = mid + 1;
+            else hi = mid;
+        }
+        uint32_t lo2 = lo, hi2 = npvs;   // first position of an instance > inst
+        while (lo2 < hi2) {
+            const uint32_t mid = (lo2 + hi2) >> 1;
+            if (find_inst(I.ibase, I.ninst, pvs[mid]) <= inst) lo2 = mid + 1;
+            else hi2 = mid;
+        }
+        I.inel[inst] = lo2 - lo;
     }
 }
 
@@ -386,6 +404,7 @@ uint32_t lcve(Ctx &c)
     k_dynbreak<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, kind, status, I);
     k_elect_flags<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, kind, status, nv, I, c.tmpu32a.p);
     c.npvs = (uint32_t)compact_u32(c, c.eligible.p, c.tmpu32a.p, nv, c.pvs.p);
+    if (c.batch) k_inst_elected<<<grid_for(c.ninst, 256, 1024), 256, 0, c.stream>>>(c.pvs.p, c.npvs, I);
     c.hist_live = false;
     return c.npvs;
 }

@@ -252,6 +252,16 @@ struct Sigma {
                 any |= st[i] == B_RUN;
             }
             if (!any) break;
+            if (c.trace) {
+                uint32_t run = 0;
+                int64_t lits_run = 0, lits_all = 0;
+                for (uint32_t i = 0; i < ni; i++) {
+                    lits_all += before[i];
+                    if (st[i] == B_RUN) { run++; lits_run += before[i]; }
+                }
+                fprintf(stderr, "[pfg] batch round %d: %u of %u instances running, literals %lld of %lld\n", round, run, ni,
+                        (long long)lits_run, (long long)lits_all);
+            }
             const int times = round + 1;   // compaction keeps clause order, so its schedule is free
             if (times > 1 && c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0) shrinkSimp();
             createOT();
@@ -412,6 +422,7 @@ int pfg_destroy(pfg_ctx *ctx)
     release(c, c.owner); release(c, c.claim); release(c, c.erec); release(c, c.ufirst); release(c, c.cnt);
     release(c, c.scan_tmp); release(c, c.radix_tmp);
     release(c, c.ibase); release(c, c.imu); release(c, c.ib0); release(c, c.inel); release(c, c.icnt);
+    c.cache_mem.release(); c.pin_res.release(); c.pin_trail.release(); c.pin_refs.release(); c.pin_vstate.release();
     if (c.hcnt) (void)hipHostFree(c.hcnt);
     for (auto &e : c.kev) { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); }
     for (auto &e : c.ev) {
@@ -639,29 +650,29 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
     API_BEGIN
     if (inst >= c.ninst) return PFG_EINVAL;
     if (c.cache_gen != c.batch_gen) {
-        // one download of the batch state, bucketed by instance
-        std::vector<uint64_t> refs(c.nrefs);
-        std::vector<uint32_t> res(c.nres), trail(c.ntrail);
-        c.cache_mem.resize(c.msz);
-        c.cache_vstate.resize((uint64_t)c.nvars + 1);
-        if (c.nrefs) PFG_HIP(hipMemcpyAsync(refs.data(), c.refs.p, c.nrefs * 8, hipMemcpyDeviceToHost, c.stream));
-        if (c.msz) PFG_HIP(hipMemcpyAsync(c.cache_mem.data(), c.mem.p, c.msz * 4, hipMemcpyDeviceToHost, c.stream));
-        if (c.nres) PFG_HIP(hipMemcpyAsync(res.data(), c.resolved.p, c.nres * 4, hipMemcpyDeviceToHost, c.stream));
-        PFG_HIP(hipMemcpyAsync(c.cache_vstate.data(), c.vstate.p, (uint64_t)c.nvars + 1, hipMemcpyDeviceToHost, c.stream));
-        if (c.ntrail) PFG_HIP(hipMemcpyAsync(trail.data(), c.trail.p, 4ull * c.ntrail, hipMemcpyDeviceToHost, c.stream));
+        // clause images grouped by instance on the device, then one pinned download;
+        // model records and units are bucketed on the host
+        ensure(c, c.mem2, c.msz + 1);
+        batch_images(c, c.mem2.p, c.cache_start);
+        const uint64_t nw = c.cache_start[c.ninst];
+        c.cache_mem.ensure(nw + 1);
+        c.pin_res.ensure(c.nres + 1);
+        c.pin_trail.ensure((uint64_t)c.ntrail + 1);
+        c.pin_vstate.ensure((uint64_t)c.nvars + 1);
+        if (nw) PFG_HIP(hipMemcpyAsync(c.cache_mem.p, c.mem2.p, nw * 4, hipMemcpyDeviceToHost, c.stream));
+        if (c.nres) PFG_HIP(hipMemcpyAsync(c.pin_res.p, c.resolved.p, c.nres * 4, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipMemcpyAsync(c.pin_vstate.p, c.vstate.p, (uint64_t)c.nvars + 1, hipMemcpyDeviceToHost, c.stream));
+        if (c.ntrail) PFG_HIP(hipMemcpyAsync(c.pin_trail.p, c.trail.p, 4ull * c.ntrail, hipMemcpyDeviceToHost, c.stream));
         PFG_HIP(hipStreamSynchronize(c.stream));
+        const uint32_t *res = c.pin_res.p, *trail = c.pin_trail.p;
+        const uint64_t nres = c.nres, ntr = c.ntrail;
         Sigma S(c);
-        c.cache_cls.assign(c.ninst, {});
         c.cache_res.assign(c.ninst, {});
         c.cache_trail.assign(c.ninst, {});
-        for (const uint64_t r : refs) {
-            const uint32_t *cl = c.cache_mem.data() + r;
-            if (cl[2]) c.cache_cls[S.inst_of(cl[3] >> 1)].push_back(r);
-        }
         // model records end with their size; all literals of a record belong to one
         // instance; walked newest first, kept oldest first
         std::vector<std::vector<std::pair<uint64_t, uint64_t>>> recs(c.ninst);
-        for (int64_t j = (int64_t)res.size() - 1; j > 0;) {
+        for (int64_t j = (int64_t)nres - 1; j > 0;) {
             const uint64_t k = res[j];
             const int64_t s0 = j - (int64_t)k;
             if (k == 0 || s0 < 0) throw Error(PFG_EINTERNAL, "malformed model stack");
@@ -670,25 +681,19 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
         }
         for (uint32_t i = 0; i < c.ninst; i++)
             for (auto it = recs[i].rbegin(); it != recs[i].rend(); ++it)
-                c.cache_res[i].insert(c.cache_res[i].end(), res.begin() + it->first, res.begin() + it->first + it->second);
-        for (const uint32_t l : trail) c.cache_trail[S.inst_of(l >> 1)].push_back(l);
+                c.cache_res[i].insert(c.cache_res[i].end(), res + it->first, res + it->first + it->second);
+        for (uint64_t k = 0; k < ntr; k++) c.cache_trail[S.inst_of(trail[k] >> 1)].push_back(trail[k]);
         c.cache_gen = c.batch_gen;
     }
     const uint32_t lo = c.hibase[inst], shift = 2u * lo;
-    uint64_t w = 0;
-    for (const uint64_t r : c.cache_cls[inst]) {
-        const uint32_t *cl = c.cache_mem.data() + r;
-        const uint32_t sz = cl[2];
-        if (words) {
-            words[w] = cl[0];
-            words[w + 1] = cl[1];
-            words[w + 2] = sz;
-            for (uint32_t k = 0; k < sz; k++) words[w + 3 + k] = cl[3 + k] - shift;
-        }
-        w += 3ull + sz;
+    const uint64_t w0 = c.cache_start[inst], w1 = c.cache_start[inst + 1];
+    if (words && w1 > w0) std::memcpy(words, c.cache_mem.p + w0, (w1 - w0) * 4);
+    if (nwords) *nwords = w1 - w0;
+    if (ncls) {   // count the images
+        uint64_t n = 0;
+        for (uint64_t w = w0; w < w1; w += 3ull + c.cache_mem.p[w + 2]) n++;
+        *ncls = n;
     }
-    if (nwords) *nwords = w;
-    if (ncls) *ncls = c.cache_cls[inst].size();
     const std::vector<uint32_t> &rs = c.cache_res[inst];
     if (resolved) {
         // every word but the sizes that end the records is a literal
@@ -702,7 +707,7 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
     if (nres) *nres = rs.size();
     if (vstate) {
         vstate[0] = 0;
-        for (uint32_t v = 1; v <= c.inst_nvars[inst]; v++) vstate[v] = c.cache_vstate[lo + v];
+        std::memcpy(vstate + 1, c.pin_vstate.p + lo + 1, c.inst_nvars[inst]);
     }
     if (trail) for (size_t k = 0; k < c.cache_trail[inst].size(); k++) trail[k] = c.cache_trail[inst][k] - shift;
     if (ntrail) *ntrail = (uint32_t)c.cache_trail[inst].size();

@@ -77,6 +77,8 @@ class Server:
         import torch
         self.torch = torch
         self.dev = torch.device("cuda:0")
+        torch.cuda.init()   # in this thread: the workers must not be the first to touch the device
+        torch.zeros(1, device=self.dev)
         self.slots = slots
         self.batch_lits = batch_lits
         self.store_bytes = store_bytes
@@ -88,6 +90,8 @@ class Server:
         self.cv = threading.Condition()
         self.closed = False
         self.finished = []
+        self.phase_s = {"intake": 0.0, "assemble": 0.0, "upload": 0.0, "simplify": 0.0, "results": 0.0}
+        self.batches = 0
 
     # ---- resident instances
     def _resident(self, r: Request):
@@ -118,6 +122,7 @@ class Server:
         with self.cv:
             r.arrival = time.perf_counter()
             lits, offs = self._resident(r)
+            self.phase_s["intake"] += time.perf_counter() - r.arrival
             r.nlits = int(lits.numel())
             self.pending.append((r, lits, offs))
             self.cv.notify()
@@ -150,6 +155,7 @@ class Server:
                 batch = self._admit()
                 if batch is None:
                     break
+                ta = time.perf_counter()
                 nv = np.array([e[0].nvars for e in batch], np.uint32)
                 base = gen.batch_bases(nv)
                 if int(base[-1]) > ctx.nvars_cap:
@@ -163,17 +169,24 @@ class Server:
                 d_l = torch.cat(parts_l)
                 d_o = torch.cat(parts_o)
                 torch.cuda.current_stream().synchronize()   # not the device: the other slot keeps running
+                tb = time.perf_counter()
                 ctx.relayout(nv)
                 ctx.upload_device(d_l.data_ptr(), d_o.data_ptr(), d_o.numel() - 1, d_l.numel())
+                tc = time.perf_counter()
                 st = ctx.sigmify_batch()
+                td = time.perf_counter()
                 for i, (r, _, _) in enumerate(batch):
                     res = ctx.instance(i)
                     r.status = int(st[i])
                     if self.keep:
                         r.result = res
                     r.done = time.perf_counter()
+                te = time.perf_counter()
                 with self.cv:
                     self.finished.extend(e[0] for e in batch)
+                    self.batches += 1
+                    for k, dt in (("assemble", tb - ta), ("upload", tc - tb), ("simplify", td - tc), ("results", te - td)):
+                        self.phase_s[k] += dt
         finally:
             ctx.close()
 
@@ -201,4 +214,5 @@ class Server:
         return {"requests": len(requests), "offered_rate": rate, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                 "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
                 "literals": int(sum(r.nlits for r in requests)),
-                "literals_per_s": float(sum(r.nlits for r in requests) / span), "span_s": float(span)}
+                "literals_per_s": float(sum(r.nlits for r in requests) / span), "span_s": float(span),
+                "batches": self.batches, "phase_s": {k: round(v, 3) for k, v in self.phase_s.items()}}
