@@ -617,6 +617,79 @@ void batch_images(Ctx &c, uint32_t *out, std::vector<uint64_t> &start)
     PFG_HIP(hipStreamSynchronize(c.stream));
 }
 
+// Model records grouped by instance (stack order kept within an instance), literals in
+// each instance's own numbering. The unit is an eliminated variable's chunk of records
+// (contiguous, one instance, starting with a literal); the chunk list holds their starts
+// in stack order, so a stable sort of chunk indices by instance gives every instance's
+// records oldest first.
+__global__ void k_chunk_inst(const uint32_t *__restrict__ res, const uint64_t *__restrict__ chunk, uint64_t n,
+                             const uint32_t *__restrict__ ibase, uint32_t ninst, uint32_t *__restrict__ keys,
+                             uint32_t *__restrict__ idx)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        keys[i] = find_inst(ibase, ninst, res[chunk[i]] >> 1);
+        idx[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_chunk_sizes(const uint64_t *__restrict__ chunk, const uint32_t *__restrict__ idx, uint64_t n,
+                              uint64_t nres, uint32_t *__restrict__ sizes)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t j = idx[i];
+        sizes[i] = (uint32_t)((j + 1 < n ? chunk[j + 1] : nres) - chunk[j]);
+    }
+}
+
+// a chunk is copied from its end: the last word of a record is its literal count
+__global__ void k_chunk_write(const uint32_t *__restrict__ res, const uint64_t *__restrict__ chunk,
+                              const uint32_t *__restrict__ idx, const uint32_t *__restrict__ keys,
+                              const uint64_t *__restrict__ woff, uint64_t n, uint64_t nres,
+                              const uint32_t *__restrict__ ibase, uint32_t *__restrict__ out)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t j = idx[i];
+        const uint64_t s0 = chunk[j], s1 = j + 1 < n ? chunk[j + 1] : nres;
+        const uint32_t *src = res + s0;
+        uint32_t *o = out + woff[i];
+        const uint32_t shift = 2u * ibase[keys[i]];
+        int64_t e = (int64_t)(s1 - s0) - 1;
+        while (e >= 0) {
+            const uint32_t k = src[e];
+            o[e] = k;
+            const int64_t b = e - (int64_t)k;
+            for (int64_t t = b < 0 ? 0 : b; t < e; t++) o[t] = src[t] - shift;
+            e = b - 1;
+        }
+    }
+}
+
+// out: device words (at least nres); start: per-instance word starts (ninst + 1)
+void batch_records(Ctx &c, uint32_t *out, std::vector<uint64_t> &start)
+{
+    const uint64_t n = c.nchunk;
+    start.assign(c.ninst + 1, 0);
+    if (!n) return;
+    ensure(c, c.tmpu32a, n + 1);
+    ensure(c, c.tmpu32b, n + 1);
+    ensure(c, c.tmpu32c, n + 1);
+    ensure(c, c.tmpu64a, n + 1);
+    ensure(c, c.tmpu64c, (uint64_t)c.ninst + 2);
+    uint32_t *keys = c.tmpu32a.p, *idx = c.tmpu32b.p, *sizes = c.tmpu32c.p;
+    k_chunk_inst<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.resolved.p, c.mchunk.p, n, c.ibase.p, c.ninst, keys, idx);
+    int bits = 0;
+    while (bits < 32 && ((c.ninst - 1) >> bits)) bits++;
+    if (bits) radix_sort_u32_pairs(c, keys, idx, n, ((bits + 7) / 8) * 8);
+    k_chunk_sizes<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.mchunk.p, idx, n, c.nres, sizes);
+    scan_u32_to_u64(c, sizes, c.tmpu64a.p, n, c.cnt.p + CNT_TOTAL);
+    k_chunk_write<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.resolved.p, c.mchunk.p, idx, keys, c.tmpu64a.p, n,
+                                                                c.nres, c.ibase.p, out);
+    k_image_starts<<<grid_for((uint64_t)c.ninst + 1, 256, 1024), 256, 0, c.stream>>>(keys, c.tmpu64a.p, n, c.ninst,
+                                                                                      c.nres, c.tmpu64c.p);
+    PFG_HIP(hipMemcpyAsync(start.data(), c.tmpu64c.p, 8ull * (c.ninst + 1), hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+}
+
 // keep the elected variables of instances with cls 1, then those with cls 2, each
 // group in elected order; returns the two group sizes
 __global__ void k_pv_class(const uint32_t *__restrict__ pvs, uint32_t n, const uint32_t *__restrict__ ibase, uint32_t ninst,

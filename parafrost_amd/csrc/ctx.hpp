@@ -112,6 +112,8 @@ struct Ctx {
     uint32_t ntrail = 0, propagated = 0;
     DBuf<uint32_t> resolved;    // model stack
     uint64_t nres = 0;
+    DBuf<uint64_t> mchunk;      // batch: first word of each eliminated variable's records, in stack order
+    uint64_t nchunk = 0;
     DBuf<uint32_t> pvs;         // elected variables
     DBuf<uint32_t> units;       // units produced by one BVE pass, in elected order
     uint32_t npvs = 0;
@@ -145,7 +147,8 @@ struct Ctx {
     PinBuf<uint64_t> pin_refs;
     PinBuf<uint8_t> pin_vstate;
     std::vector<uint64_t> cache_start;              // per instance: first word of its clause images
-    std::vector<std::vector<uint32_t>> cache_res, cache_trail;
+    std::vector<uint64_t> cache_rstart;             // per instance: first word of its model records (pin_res)
+    std::vector<std::vector<uint32_t>> cache_trail;
 
     // round state
     int32_t phase = 0;

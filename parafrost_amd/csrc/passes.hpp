@@ -22,6 +22,7 @@ void enqueue_units_host(Ctx &c, const uint32_t *lits, uint32_t n);
 void count_inst(Ctx &c, std::vector<int64_t> &cls, std::vector<int64_t> &lits);
 void partition_pvs(Ctx &c, const std::vector<uint8_t> &cls, uint32_t &na, uint32_t &nb);
 void batch_images(Ctx &c, uint32_t *out, std::vector<uint64_t> &start);
+void batch_records(Ctx &c, uint32_t *out, std::vector<uint64_t> &start);
 void merge_units(Ctx &c, const uint32_t *d_units, uint64_t n);  // ordered dedupe into the trail
 // algorithmic bytes of reading the occurrence lists of elected variables pvs[b, e):
 // 16 per list header + 8 per entry (+ 12 + 4|c| per listed clause when with_clauses);

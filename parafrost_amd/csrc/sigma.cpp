@@ -3,6 +3,7 @@
 // device passes, and the C ABI declared in include/pfgpu.h.
 #include "ctx.hpp"
 #include "passes.hpp"
+#include <chrono>
 #include <cstring>
 #include <vector>
 #include <algorithm>
@@ -416,7 +417,7 @@ int pfg_destroy(pfg_ctx *ctx)
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     release(c, c.mem); release(c, c.mem2); release(c, c.refs); release(c, c.refs2);
     release(c, c.ot); release(c, c.occ); release(c, c.hist); release(c, c.value); release(c, c.vstate);
-    release(c, c.trail); release(c, c.resolved); release(c, c.pvs); release(c, c.units);
+    release(c, c.trail); release(c, c.resolved); release(c, c.mchunk); release(c, c.pvs); release(c, c.units);
     release(c, c.scores); release(c, c.eligible); release(c, c.rank); release(c, c.tmpu32a); release(c, c.tmpu32b); release(c, c.tmpu32c); release(c, c.tmpu32d);
     release(c, c.tmpu8); release(c, c.tmpu64a); release(c, c.tmpu64b); release(c, c.tmpu64c);
     release(c, c.owner); release(c, c.claim); release(c, c.erec); release(c, c.ufirst); release(c, c.cnt);
@@ -589,7 +590,7 @@ int pfg_batch_relayout(pfg_ctx *ctx, uint32_t ninst, const uint32_t *nvars)
     PFG_HIP(hipMemsetAsync(c.cnt.p, 0, (CNT_CONF + 1) * 8, c.stream));
     PFG_HIP(hipStreamSynchronize(c.stream));
     c.ntrail = c.propagated = 0;
-    c.nres = 0;
+    c.nres = c.nchunk = 0;
     c.npvs = 0;
     c.msz = c.nrefs = 0;
     c.phase = 0;
@@ -650,40 +651,46 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
     API_BEGIN
     if (inst >= c.ninst) return PFG_EINVAL;
     if (c.cache_gen != c.batch_gen) {
-        // clause images grouped by instance on the device, then one pinned download;
-        // model records and units are bucketed on the host
+        // clause images and model records grouped by instance on the device, then pinned
+        // downloads; units are bucketed on the host
+        const auto tq0 = std::chrono::steady_clock::now();
         ensure(c, c.mem2, c.msz + 1);
         batch_images(c, c.mem2.p, c.cache_start);
         const uint64_t nw = c.cache_start[c.ninst];
+        if (c.batch) {
+            ensure(c, c.tmpu32d, c.nres + 1);
+            batch_records(c, c.tmpu32d.p, c.cache_rstart);
+        }
+        else   // one instance: the stack as it is
+            c.cache_rstart.assign({0, c.nres});
+        const uint64_t nr = c.cache_rstart[c.ninst];
+        const auto tq1 = std::chrono::steady_clock::now();
         c.cache_mem.ensure(nw + 1);
-        c.pin_res.ensure(c.nres + 1);
+        c.pin_res.ensure(nr + 1);
         c.pin_trail.ensure((uint64_t)c.ntrail + 1);
         c.pin_vstate.ensure((uint64_t)c.nvars + 1);
+        const auto tq2 = std::chrono::steady_clock::now();
         if (nw) PFG_HIP(hipMemcpyAsync(c.cache_mem.p, c.mem2.p, nw * 4, hipMemcpyDeviceToHost, c.stream));
-        if (c.nres) PFG_HIP(hipMemcpyAsync(c.pin_res.p, c.resolved.p, c.nres * 4, hipMemcpyDeviceToHost, c.stream));
+        if (nr)
+            PFG_HIP(hipMemcpyAsync(c.pin_res.p, c.batch ? c.tmpu32d.p : c.resolved.p, nr * 4, hipMemcpyDeviceToHost,
+                                   c.stream));
         PFG_HIP(hipMemcpyAsync(c.pin_vstate.p, c.vstate.p, (uint64_t)c.nvars + 1, hipMemcpyDeviceToHost, c.stream));
         if (c.ntrail) PFG_HIP(hipMemcpyAsync(c.pin_trail.p, c.trail.p, 4ull * c.ntrail, hipMemcpyDeviceToHost, c.stream));
         PFG_HIP(hipStreamSynchronize(c.stream));
-        const uint32_t *res = c.pin_res.p, *trail = c.pin_trail.p;
-        const uint64_t nres = c.nres, ntr = c.ntrail;
+        const auto tq3 = std::chrono::steady_clock::now();
+        const uint32_t *trail = c.pin_trail.p;
+        const uint64_t ntr = c.ntrail;
         Sigma S(c);
-        c.cache_res.assign(c.ninst, {});
         c.cache_trail.assign(c.ninst, {});
-        // model records end with their size; all literals of a record belong to one
-        // instance; walked newest first, kept oldest first
-        std::vector<std::vector<std::pair<uint64_t, uint64_t>>> recs(c.ninst);
-        for (int64_t j = (int64_t)nres - 1; j > 0;) {
-            const uint64_t k = res[j];
-            const int64_t s0 = j - (int64_t)k;
-            if (k == 0 || s0 < 0) throw Error(PFG_EINTERNAL, "malformed model stack");
-            recs[S.inst_of(res[s0] >> 1)].emplace_back((uint64_t)s0, k + 1);
-            j = s0 - 1;
-        }
-        for (uint32_t i = 0; i < c.ninst; i++)
-            for (auto it = recs[i].rbegin(); it != recs[i].rend(); ++it)
-                c.cache_res[i].insert(c.cache_res[i].end(), res + it->first, res + it->first + it->second);
         for (uint64_t k = 0; k < ntr; k++) c.cache_trail[S.inst_of(trail[k] >> 1)].push_back(trail[k]);
         c.cache_gen = c.batch_gen;
+        if (c.trace) {
+            const auto tq4 = std::chrono::steady_clock::now();
+            auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+            fprintf(stderr, "[pfg] batch results: group %.2f ms, pin %.2f ms, copy %.2f ms (%llu words, %llu records), "
+                            "units %.2f ms\n", ms(tq0, tq1), ms(tq1, tq2), ms(tq2, tq3), (unsigned long long)nw,
+                    (unsigned long long)nr, ms(tq3, tq4));
+        }
     }
     const uint32_t lo = c.hibase[inst], shift = 2u * lo;
     const uint64_t w0 = c.cache_start[inst], w1 = c.cache_start[inst + 1];
@@ -694,17 +701,9 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
         for (uint64_t w = w0; w < w1; w += 3ull + c.cache_mem.p[w + 2]) n++;
         *ncls = n;
     }
-    const std::vector<uint32_t> &rs = c.cache_res[inst];
-    if (resolved) {
-        // every word but the sizes that end the records is a literal
-        for (int64_t j = (int64_t)rs.size() - 1; j >= 0;) {
-            const uint64_t k = rs[j];
-            resolved[j] = (uint32_t)k;
-            for (int64_t t = j - (int64_t)k; t < j; t++) resolved[t] = rs[t] - shift;
-            j -= (int64_t)k + 1;
-        }
-    }
-    if (nres) *nres = rs.size();
+    const uint64_t r0 = c.cache_rstart[inst], r1 = c.cache_rstart[inst + 1];
+    if (resolved && r1 > r0) std::memcpy(resolved, c.pin_res.p + r0, (r1 - r0) * 4);
+    if (nres) *nres = r1 - r0;
     if (vstate) {
         vstate[0] = 0;
         std::memcpy(vstate + 1, c.pin_vstate.p + lo + 1, c.inst_nvars[inst]);
@@ -848,7 +847,7 @@ int pfg_reset(pfg_ctx *ctx)
     PFG_HIP(hipMemsetAsync(c.cnt.p, 0, (CNT_CONF + 1) * 8, c.stream));
     PFG_HIP(hipStreamSynchronize(c.stream));
     c.ntrail = c.propagated = 0;
-    c.nres = 0;
+    c.nres = c.nchunk = 0;
     c.npvs = 0;
     c.msz = 0;
     c.nrefs = 0;

@@ -702,6 +702,20 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
     }
 }
 
+// batch: the starts of the non-empty record chunks of one pass, appended in elected
+// order (= stack order) to the chunk list
+__global__ void k_chunk_flags(const uint64_t *__restrict__ moff, uint32_t np, uint32_t *__restrict__ flags)
+{
+    for (uint64_t q = gtid(); q < np; q += gstride()) flags[q] = moff[q + 1] > moff[q];
+}
+
+__global__ void k_chunk_append(const uint64_t *__restrict__ moff, const uint64_t *__restrict__ pos, uint32_t np,
+                               uint64_t model_base, uint64_t *__restrict__ out)
+{
+    for (uint64_t q = gtid(); q < np; q += gstride())
+        if (moff[q + 1] > moff[q]) out[pos[q]] = model_base + moff[q];
+}
+
 void ve(Ctx &c)
 {
     if (!c.opts.ve_en || !c.npvs) return;
@@ -744,6 +758,17 @@ void ve(Ctx &c)
     {
         KTimer kt(c, K_VE_APPLY, 4.0 * add_words + 8.0 * add_refs + 4.0 * add_model);
         k_ve_apply<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, O, type, aux, np, roff, woff, uoff, moff);
+    }
+    if (c.batch && add_model) {
+        ensure(c, c.mchunk, c.nchunk + np + 1, true);
+        ensure(c, c.tmpu32c, (uint64_t)np + 1);
+        ensure(c, c.tmpu64c, (uint64_t)np + 2);
+        k_chunk_flags<<<grid_for(np, 256, 8192), 256, 0, c.stream>>>(moff, np, c.tmpu32c.p);
+        scan_u32_to_u64(c, c.tmpu32c.p, c.tmpu64c.p, np, c.cnt.p + CNT_TOTAL);
+        k_chunk_append<<<grid_for(np, 256, 8192), 256, 0, c.stream>>>(moff, c.tmpu64c.p, np, c.nres,
+                                                                       c.mchunk.p + c.nchunk);
+        read_counters(c, CNT_TOTAL + 1);
+        c.nchunk += c.hcnt[CNT_TOTAL];
     }
     c.nrefs += add_refs;
     c.msz += add_words;
