@@ -81,6 +81,13 @@ int pfg_sigmify_batch(pfg_ctx *ctx, int32_t *status);
  * in its own numbering; NULL buffers query the sizes */
 int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *nwords, uint64_t *ncls, uint32_t *resolved,
                        uint64_t *nres, uint8_t *vstate, uint32_t *trail, uint32_t *ntrail);
+/* Write-back for incremental calls, kept in HBM (cacheCNF/writeBack, src/gpu/transfer.cu:44-179,
+ * without the host copy): one instance's clauses after pfg_sigmify_batch as CSR in its own
+ * numbering, into caller-owned device memory: d_offs gets ncls + 1 offsets (from 0), d_lits
+ * nlits literals, clause order as in pfg_batch_download. Valid until the next upload or
+ * sigmify on the context; NULL buffers query the sizes. The clauses can be given back to
+ * pfg_upload_cnf_device / pfg_upload_batch together with new clauses. */
+int pfg_batch_export(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint64_t *d_offs, uint64_t *nlits, uint64_t *ncls);
 
 /* per-pass device time accumulated over the context's life, in milliseconds */
 typedef struct pfg_times {
@@ -103,6 +110,13 @@ int pfg_upload_cnf_device(pfg_ctx *ctx, const uint32_t *d_lits, const uint64_t *
                           uint64_t nlits);
 /* root-level units to propagate (ParaFROST trail entries before sigmify) */
 int pfg_enqueue_units(pfg_ctx *ctx, const uint32_t *lits, uint32_t n);
+/* ParaFROST::ifreeze / iunfreeze (src/cpu/assume.cpp:45-61; iassume marks the same way,
+ * assume.cpp:68-84): on = 1 freezes, on = 0 melts the listed variables. A frozen variable is
+ * never elected by LCVE (iassumed, src/cpu/lcve.cpp:57; src/gpu/lcve.cu:62), so it is never
+ * eliminated. The marks persist across sigmify calls and uploads; pfg_reset and
+ * pfg_batch_relayout clear them. In a batch context variables are in the global numbering
+ * (instance i's variable v is hibase[i] + v). */
+int pfg_ifreeze(pfg_ctx *ctx, const uint32_t *vars, uint32_t n, int on);
 
 /* ParaFROST::sigmify (src/gpu/simplify.cu:75-89 / src/cpu/simplify.cpp:115-214): the whole
  * round loop on the device, ending with GC of deleted clauses. */

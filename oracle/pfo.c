@@ -49,6 +49,7 @@ struct pfo {
     uint32_t *ps, *ns;
     int8_t *value;
     uint8_t *vstate, *frozen;
+    uint8_t *ifrozen;          /* ifreeze/iassume marks (solve.h:841), skipped by LCVE */
     uint32_t *trail; uint32_t ntrail, propagated;
     uint32_t *resolved; uint64_t nres, rescap;
     uint32_t *pvs; uint32_t npvs;
@@ -350,6 +351,7 @@ pfo *pfo_create(uint32_t nvars, const pfo_opts *o, const uint32_t *lits, const u
     memset(s->value, -1, nlit);
     s->vstate = (uint8_t *)calloc((size_t)nvars + 1, 1);
     s->frozen = (uint8_t *)calloc((size_t)nvars + 1, 1);
+    s->ifrozen = (uint8_t *)calloc((size_t)nvars + 1, 1);
     s->trail = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)nvars + 1));
     s->pvs = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)nvars + 1));
     s->eligible = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)nvars + 1));
@@ -376,7 +378,7 @@ void pfo_destroy(pfo *s)
     if (!s) return;
     const size_t nlit = 2 * ((size_t)s->nvars + 1);
     for (size_t i = 0; i < nlit; i++) free(s->ot[i].d);
-    free(s->ot); free(s->ps); free(s->ns); free(s->value); free(s->vstate); free(s->frozen);
+    free(s->ot); free(s->ps); free(s->ns); free(s->value); free(s->vstate); free(s->frozen); free(s->ifrozen);
     free(s->trail); free(s->pvs); free(s->eligible); free(s->scores); free(s->fstack);
     free(s->mem); free(s->scnf); free(s->resolved); free(s->outc);
     free(s);
@@ -389,6 +391,16 @@ static void enqueue_unit(pfo *s, uint32_t lit)
     s->value[lit] = 1;
     s->value[FLIP(lit)] = 0;
     s->trail[s->ntrail++] = lit;
+}
+
+/* ifreeze / iunfreeze (src/cpu/assume.cpp:45-61); iassume marks its variables the
+ * same way (assume.cpp:68-84). In incremental mode iassumed(v) (solve.h:864-867) makes
+ * LCVE pass over the variable (lcve.cpp:57): it is never elected and never freezes. */
+int pfo_ifreeze(pfo *s, uint32_t v, int on)
+{
+    if (v == 0 || v > s->nvars) return -1;
+    s->ifrozen[v] = on ? 1 : 0;
+    return 0;
 }
 
 int pfo_enqueue(pfo *s, uint32_t lit)
@@ -578,6 +590,7 @@ int pfo_lcve(pfo *s)
     const uint32_t pt = s->opts.mu_pos << s->mu_inc, nt = s->opts.mu_neg << s->mu_inc;
     for (uint32_t i = 0; i < s->nvars; i++) {
         const uint32_t cand = s->eligible[i];
+        if (s->ifrozen[cand]) continue;
         if (s->vstate[cand]) continue;
         if (s->frozen[cand]) continue;
         const uint32_t p = 2 * cand, n = p + 1;

@@ -49,6 +49,10 @@ pfo *pfo_create(uint32_t nvars, const pfo_opts *o, const uint32_t *lits, const u
                 uint64_t ncls, const uint32_t *hdr0);
 void pfo_destroy(pfo *s);
 
+/* freeze (on = 1) or melt (on = 0) variable v for LCVE across later sigmify calls:
+ * the incremental ifreeze/iunfreeze (src/cpu/assume.cpp:45-61); 0 ok, -1 bad variable */
+int pfo_ifreeze(pfo *s, uint32_t v, int on);
+
 /* assign units before simplification (they are propagated by the first prop()). */
 int pfo_enqueue(pfo *s, uint32_t lit);
 

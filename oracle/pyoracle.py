@@ -44,6 +44,9 @@ def lib():
         L.pfo_create.argtypes = [ctypes.c_uint32, ctypes.POINTER(Opts), u32p, u64p, ctypes.c_uint64, vp]
         L.pfo_destroy.argtypes = [vp]
         L.pfo_enqueue.argtypes = [vp, ctypes.c_uint32]
+        L.pfo_enqueue.restype = ctypes.c_int
+        L.pfo_ifreeze.argtypes = [vp, ctypes.c_uint32, ctypes.c_int]
+        L.pfo_ifreeze.restype = ctypes.c_int
         for f in ("pfo_sigmify", "pfo_prop", "pfo_lcve", "pfo_unsat", "pfo_phase"):
             getattr(L, f).restype = ctypes.c_int
             getattr(L, f).argtypes = [vp]
@@ -124,6 +127,19 @@ class Oracle:
     def sigmify(self) -> int:
         return self.L.pfo_sigmify(self.h)
 
+    def ifreeze(self, vars, on: bool = True):
+        """ifreeze / iunfreeze (src/cpu/assume.cpp:45-61): LCVE passes over these variables."""
+        for v in np.asarray(vars, np.uint32).ravel():
+            if self.L.pfo_ifreeze(self.h, int(v), 1 if on else 0) != 0:
+                raise ValueError(f"variable {int(v)} out of range")
+
+    def enqueue(self, lits) -> bool:
+        """assign units before sigmify; False on a contradiction"""
+        ok = True
+        for l in np.asarray(lits, np.uint32).ravel():
+            ok &= self.L.pfo_enqueue(self.h, int(l)) == 0
+        return ok
+
     def state(self, lists=False) -> State:
         L, h = self.L, self.h
         nw = L.pfo_dump_cnf(h, None)
@@ -199,10 +215,15 @@ def ref_opts_args(o: Opts) -> list[str]:
 
 
 def run_reference(cnf_path: str, out_path: str, o: Opts, trace_prefix: str | None = None,
-                  timeout: float = 600) -> State:
+                  timeout: float = 600, freeze=None) -> State:
+    """freeze: variables frozen as by ifreeze() in incremental mode"""
     env = dict(os.environ)
     if trace_prefix:
         env["PFREF_TRACE"] = trace_prefix
+    if freeze is not None:
+        fz = out_path + ".freeze"
+        np.asarray(freeze, np.uint32).tofile(fz)
+        env["PFREF_FREEZE"] = fz
     if os.path.exists(out_path):
         os.remove(out_path)
     r = subprocess.run([REF_BIN, cnf_path, out_path] + ref_opts_args(o), env=env,

@@ -9,6 +9,9 @@
  * usage: pfref <input.cnf> <out.bin> [reference options, e.g. --phases=3]
  * env PFREF_TRACE=<prefix>: also dump the state after every pass to
  *   <prefix>.<round>.<pass>.bin  (pass in prop, lcve, sort, ere, sub, ve)
+ * env PFREF_FREEZE=<file>: u32 variables (little endian) frozen as by ifreeze()
+ *   (src/cpu/assume.cpp:45-52) with the solver in incremental mode, so that
+ *   iassumed() (src/cpu/solve.h:864-867) holds for them during LCVE
  *
  * Dump format (little endian):
  *   u32 'PFRD', u32 nvars, u64 nclauses, u64 words, words[] (per clause: hdr0, sig, size, lits)
@@ -102,6 +105,18 @@ int main(int argc, char** argv)
 	verbose = 0;
 	ParaFROST* s = new ParaFROST(argv[1]);
 	pfrost = s;
+	if (const char* fz = getenv("PFREF_FREEZE")) {
+		FILE* f = fopen(fz, "rb");
+		if (!f) { perror(fz); return 3; }
+		s->incremental = true;
+		s->ifrozen.expand(inf.maxVar + 1, 0);
+		uint32_t v;
+		while (fread(&v, 4, 1, f) == 1) {
+			if (v == 0 || v > inf.maxVar) { fprintf(stderr, "bad frozen variable %u\n", v); return 3; }
+			s->ifrozen[v] = 1;
+		}
+		fclose(f);
+	}
 	const std::string out = argv[2];
 	int round = 0;
 	auto tdump = [&](const char* pass) {

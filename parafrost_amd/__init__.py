@@ -50,7 +50,7 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_download_resolved", "pfg_download_vstate", "pfg_download_pvs", "pfg_download_ot_list",
        "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
        "pfg_set_accounting", "pfg_set_stage_words", "pfg_create_batch", "pfg_batch_ninst", "pfg_batch_relayout", "pfg_upload_batch",
-       "pfg_sigmify_batch", "pfg_batch_download"]
+       "pfg_sigmify_batch", "pfg_batch_download", "pfg_ifreeze", "pfg_batch_export"]
 
 
 def lib_path() -> str:
@@ -77,6 +77,7 @@ def lib():
         L.pfg_upload_cnf.argtypes = [vp, vp, vp, u64, vp]
         L.pfg_upload_cnf_device.argtypes = [vp, vp, vp, u64, u64]
         L.pfg_enqueue_units.argtypes = [vp, vp, u32]
+        L.pfg_ifreeze.argtypes = [vp, vp, u32, i32]
         for f in ("pfg_destroy", "pfg_sigmify", "pfg_resize_cnf", "pfg_create_ot", "pfg_reduce_ot", "pfg_prop",
                   "pfg_sort_ot", "pfg_sub", "pfg_ve", "pfg_ere", "pfg_filter_pvs", "pfg_gc", "pfg_next_phase",
                   "pfg_is_unsat"):
@@ -103,6 +104,7 @@ def lib():
         L.pfg_sigmify_batch.argtypes = [vp, vp]
         L.pfg_batch_download.argtypes = [vp, u32, vp, ctypes.POINTER(u64), ctypes.POINTER(u64), vp,
                                          ctypes.POINTER(u64), vp, vp, ctypes.POINTER(u32)]
+        L.pfg_batch_export.argtypes = [vp, u32, vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.pfg_reset.argtypes = [vp]
         L.pfg_last_error.argtypes = [vp]
         L.pfg_last_error.restype = ctypes.c_char_p
@@ -180,6 +182,11 @@ class Simplifier:
     def enqueue_units(self, lits):
         a = np.ascontiguousarray(lits, np.uint32)
         self._check(self.L.pfg_enqueue_units(self.h, a.ctypes.data, len(a)))
+
+    def ifreeze(self, vars, on: bool = True):
+        """ifreeze / iunfreeze (src/cpu/assume.cpp:45-61): LCVE never elects these variables."""
+        a = np.ascontiguousarray(vars, np.uint32)
+        self._check(self.L.pfg_ifreeze(self.h, a.ctypes.data, len(a), 1 if on else 0))
 
     # ---- reference entry points
     def sigmify(self):
@@ -357,6 +364,28 @@ class BatchSimplifier(Simplifier):
         st = np.zeros(self.ninst, np.int32)
         self._check(self.L.pfg_sigmify_batch(self.h, st.ctypes.data))
         return st
+
+    def ifreeze_instance(self, i: int, vars, on: bool = True):
+        """freeze instance i's variables (its own numbering)"""
+        self.ifreeze(np.asarray(vars, np.uint64) + self.base[i], on)
+
+    def enqueue_instance(self, i: int, lits):
+        """root-level units of instance i (its own numbering)"""
+        self.enqueue_units(np.asarray(lits, np.uint64) + 2 * self.base[i])
+
+    def export(self, i: int, device=None):
+        """Instance i's clauses after sigmify_batch as device CSR in its own numbering:
+        (lits int32 tensor, offs int64 tensor of ncls + 1 entries), kept in HBM."""
+        import torch
+        nl, nc = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.L.pfg_batch_export(self.h, i, None, None, ctypes.byref(nl), ctypes.byref(nc)))
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        lits = torch.empty(max(1, nl.value), dtype=torch.int32, device=dev)
+        offs = torch.empty(nc.value + 1, dtype=torch.int64, device=dev)
+        torch.cuda.current_stream(dev).synchronize()   # the allocations are ready before the library writes
+        self._check(self.L.pfg_batch_export(self.h, i, lits.data_ptr(), offs.data_ptr(), ctypes.byref(nl),
+                                            ctypes.byref(nc)))
+        return lits[:nl.value], offs
 
     def instance(self, i: int):
         """(clause words, trail, model records, variable states) of instance i in its own numbering."""

@@ -575,9 +575,10 @@ __global__ void k_image_write(const uint32_t *__restrict__ mem, const uint64_t *
     }
 }
 
-// start[k] = word offset of instance k's first image (first sorted position with key >= k)
+// first sorted position with key >= k, for k = 0..ninst: start[k] = woff at that
+// position (total past the end); cstart[k] = the position itself
 __global__ void k_image_starts(const uint32_t *__restrict__ keys, const uint64_t *__restrict__ woff, uint64_t n,
-                               uint32_t ninst, uint64_t total, uint64_t *__restrict__ start)
+                               uint32_t ninst, uint64_t total, uint64_t *__restrict__ start, uint64_t *__restrict__ cstart)
 {
     for (uint64_t k = gtid(); k <= ninst; k += gstride()) {
         uint64_t lo = 0, hi = n;
@@ -587,33 +588,68 @@ __global__ void k_image_starts(const uint32_t *__restrict__ keys, const uint64_t
             else hi = mid;
         }
         start[k] = lo < n ? woff[lo] : total;
+        if (cstart) cstart[k] = lo;
     }
 }
 
-// out: device words (at least the live words); start: per-instance word starts (ninst + 1)
-void batch_images(Ctx &c, uint32_t *out, std::vector<uint64_t> &start)
+// out: device words (at least the live words); start: per-instance word starts (ninst + 1);
+// cstart: per-instance first image index (ninst + 1). c.img_off keeps each image's word
+// offset in out (grouped order) for exports.
+void batch_images(Ctx &c, uint32_t *out, std::vector<uint64_t> &start, std::vector<uint64_t> &cstart)
 {
     const uint64_t n = c.nrefs;
     start.assign(c.ninst + 1, 0);
+    cstart.assign(c.ninst + 1, 0);
     if (!n) return;
     ensure(c, c.tmpu32a, n + 1);
     ensure(c, c.tmpu32b, n + 1);
     ensure(c, c.tmpu32c, n + 1);
-    ensure(c, c.tmpu64a, n + 1);
-    ensure(c, c.tmpu64c, (uint64_t)c.ninst + 2);
+    ensure(c, c.img_off, n + 1);
+    ensure(c, c.tmpu64c, 2ull * c.ninst + 4);
     uint32_t *keys = c.tmpu32a.p, *idx = c.tmpu32b.p, *sizes = c.tmpu32c.p;
     k_clause_inst<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.ibase.p, c.ninst, keys, idx);
     int bits = 0;
     while (bits < 32 && (c.ninst >> bits)) bits++;   // keys 0..ninst
     if (bits) radix_sort_u32_pairs(c, keys, idx, n, ((bits + 7) / 8) * 8);
     k_image_sizes<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.refs.p, idx, n, sizes);
-    scan_u32_to_u64(c, sizes, c.tmpu64a.p, n, c.cnt.p + CNT_TOTAL);
-    k_image_write<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.refs.p, idx, keys, c.tmpu64a.p, n, c.ibase.p, out);
+    scan_u32_to_u64(c, sizes, c.img_off.p, n, c.cnt.p + CNT_TOTAL);
+    k_image_write<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.refs.p, idx, keys, c.img_off.p, n, c.ibase.p, out);
     read_counters(c, 1);
     const uint64_t total = c.hcnt[CNT_TOTAL];
-    k_image_starts<<<grid_for((uint64_t)c.ninst + 1, 256, 1024), 256, 0, c.stream>>>(keys, c.tmpu64a.p, n, c.ninst, total,
-                                                                                      c.tmpu64c.p);
-    PFG_HIP(hipMemcpyAsync(start.data(), c.tmpu64c.p, 8ull * (c.ninst + 1), hipMemcpyDeviceToHost, c.stream));
+    uint64_t *d_start = c.tmpu64c.p, *d_cstart = d_start + c.ninst + 1;
+    k_image_starts<<<grid_for((uint64_t)c.ninst + 1, 256, 1024), 256, 0, c.stream>>>(keys, c.img_off.p, n, c.ninst, total,
+                                                                                      d_start, d_cstart);
+    PFG_HIP(hipMemcpyAsync(start.data(), d_start, 8ull * (c.ninst + 1), hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipMemcpyAsync(cstart.data(), d_cstart, 8ull * (c.ninst + 1), hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+}
+
+// Instance clauses [c0, c0 + ncls) of the grouped images as CSR in the instance's numbering:
+// clause q's literals at offs[q] = (its image offset - the first image's) - 3 q.
+__global__ void k_export_csr(const uint32_t *__restrict__ img, const uint64_t *__restrict__ img_off, uint64_t c0,
+                             uint64_t ncls, uint64_t nlits, uint32_t *__restrict__ lits, uint64_t *__restrict__ offs)
+{
+    const uint64_t w0 = img_off[c0];
+    for (uint64_t q = gtid(); q < ncls; q += gstride()) {
+        const uint64_t w = img_off[c0 + q];
+        const uint32_t *cl = img + w;
+        const uint64_t o = (w - w0) - 3 * q;
+        const uint32_t sz = cl[2];
+        offs[q] = o;
+        for (uint32_t k = 0; k < sz; k++) lits[o + k] = cl[3 + k];
+        if (q + 1 == ncls) offs[ncls] = nlits;
+    }
+}
+
+void batch_export(Ctx &c, const uint32_t *img, uint64_t c0, uint64_t ncls, uint64_t nlits, uint32_t *d_lits,
+                  uint64_t *d_offs)
+{
+    if (!ncls) {
+        const uint64_t zero = 0;
+        PFG_HIP(hipMemcpyAsync(d_offs, &zero, 8, hipMemcpyHostToDevice, c.stream));
+    }
+    else
+        k_export_csr<<<grid_for(ncls, 256, 8192), 256, 0, c.stream>>>(img, c.img_off.p, c0, ncls, nlits, d_lits, d_offs);
     PFG_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -685,7 +721,7 @@ void batch_records(Ctx &c, uint32_t *out, std::vector<uint64_t> &start)
     k_chunk_write<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(c.resolved.p, c.mchunk.p, idx, keys, c.tmpu64a.p, n,
                                                                 c.nres, c.ibase.p, out);
     k_image_starts<<<grid_for((uint64_t)c.ninst + 1, 256, 1024), 256, 0, c.stream>>>(keys, c.tmpu64a.p, n, c.ninst,
-                                                                                      c.nres, c.tmpu64c.p);
+                                                                                      c.nres, c.tmpu64c.p, nullptr);
     PFG_HIP(hipMemcpyAsync(start.data(), c.tmpu64c.p, 8ull * (c.ninst + 1), hipMemcpyDeviceToHost, c.stream));
     PFG_HIP(hipStreamSynchronize(c.stream));
 }
@@ -858,6 +894,21 @@ __global__ void k_assign_host_units(const uint32_t *__restrict__ lits, uint32_t 
 {
     if (gtid() != 0) return;
     for (uint32_t i = 0; i < n; i++) enqueue_dev(value, vstate, trail, cnt, lits[i]);
+}
+
+__global__ void k_set_ifrozen(const uint32_t *__restrict__ vars, uint32_t n, uint8_t on, uint8_t *__restrict__ ifrozen)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) ifrozen[vars[i]] = on;
+}
+
+// ifreeze / iunfreeze (src/cpu/assume.cpp:45-61) of host-given variables
+void set_ifrozen(Ctx &c, const uint32_t *vars, uint32_t n, bool on)
+{
+    if (!n) return;
+    ensure(c, c.tmpu32a, n);
+    PFG_HIP(hipMemcpyAsync(c.tmpu32a.p, vars, n * 4ull, hipMemcpyHostToDevice, c.stream));
+    k_set_ifrozen<<<grid_for(n, 256, 1024), 256, 0, c.stream>>>(c.tmpu32a.p, n, on ? 1 : 0, c.ifrozen.p);
+    PFG_HIP(hipStreamSynchronize(c.stream));
 }
 
 void enqueue_units_host(Ctx &c, const uint32_t *lits, uint32_t n)

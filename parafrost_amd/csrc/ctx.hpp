@@ -108,6 +108,7 @@ struct Ctx {
     // variables
     DBuf<int8_t> value;         // per literal: -1 unassigned, 0 false, 1 true
     DBuf<uint8_t> vstate;       // per variable: MELTED 1, FROZEN 2
+    DBuf<uint8_t> ifrozen;      // per variable: frozen for LCVE by the caller (incremental ifreeze)
     DBuf<uint32_t> trail;       // units (nvars + 1)
     uint32_t ntrail = 0, propagated = 0;
     DBuf<uint32_t> resolved;    // model stack
@@ -147,6 +148,8 @@ struct Ctx {
     PinBuf<uint64_t> pin_refs;
     PinBuf<uint8_t> pin_vstate;
     std::vector<uint64_t> cache_start;              // per instance: first word of its clause images
+    std::vector<uint64_t> cache_cstart;             // per instance: its first clause image (grouped order)
+    DBuf<uint64_t> img_off;                         // per grouped image: word offset in mem2
     std::vector<uint64_t> cache_rstart;             // per instance: first word of its model records (pin_res)
     std::vector<std::vector<uint32_t>> cache_trail;
 

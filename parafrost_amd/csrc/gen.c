@@ -272,11 +272,13 @@ int pfgen_write_dimacs(const char *path, uint32_t nvars, uint64_t ncls,
  * repeated literal dropped and a tautology skipped. Only frames [t0, t1) are emitted,
  * so an instance grows by generating its new frames. The structure depends on the
  * seed only. Returns the literal count (offs gets ncls + 1 entries); *ncls_out and
- * *nvars_out receive the clause count and the variable count after t1 frames. */
+ * *nvars_out receive the clause count and the variable count after t1 frames.
+ * iface (nullable, nl entries) receives the variables of the latch literals at the
+ * start of frame t1: the only earlier variables that frames t1, t1 + 1, .. refer to. */
 typedef struct { uint32_t a, b; } aig_gate;
 
 int64_t pfgen_aig_bmc(uint64_t seed, uint32_t ng, uint32_t nl, uint32_t ni, double pinv, uint32_t t0, uint32_t t1,
-                      uint32_t *lits, uint64_t *offs, uint64_t *ncls_out, uint32_t *nvars_out)
+                      uint32_t *lits, uint64_t *offs, uint64_t *ncls_out, uint32_t *nvars_out, uint32_t *iface)
 {
     if (t1 < t0 || ng == 0) return -1;
     rng_t r;
@@ -321,6 +323,8 @@ int64_t pfgen_aig_bmc(uint64_t seed, uint32_t ng, uint32_t nl, uint32_t ni, doub
         for (uint32_t i = 0; i < nl; i++) lat[i] = nlat[i];
 #undef NODE_LIT
     }
+    if (iface)
+        for (uint32_t i = 0; i < nl; i++) iface[i] = lat[i] >> 1;
     free(g); free(next); free(lat); free(nlat);
     if (ncls_out) *ncls_out = c;
     if (nvars_out) *nvars_out = nl + t1 * per;

@@ -1,7 +1,8 @@
 // LCVE election on the device.
 //
 // The reference elects variables greedily in (score, var) order, score = ps*ns
-// (uint32), skipping frozen / assigned / eliminated / absent variables, stopping at
+// (uint32), skipping caller-frozen (incremental ifreeze), round-frozen, assigned,
+// eliminated and absent variables, stopping at
 // the first candidate whose both occurrence lists reach the temperatures, and
 // freezing every cold neighbour of an elected variable (src/cpu/lcve.cpp:46-99).
 // That sequential greedy is reproduced exactly by a deterministic parallel
@@ -57,7 +58,8 @@ struct InstArgs {
 };
 
 __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, const uint32_t *__restrict__ hist,
-                           const uint8_t *__restrict__ vstate, const OL *__restrict__ ot, InstArgs I,
+                           const uint8_t *__restrict__ vstate, const uint8_t *__restrict__ ifrozen,
+                           const OL *__restrict__ ot, InstArgs I,
                            uint8_t *__restrict__ kind, uint8_t *__restrict__ status, uint8_t *__restrict__ blocked)
 {
     for (uint64_t i = gtid(); i < nvars; i += gstride()) {
@@ -66,7 +68,8 @@ __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, co
         const uint8_t mi = I.imu[inst];
         const uint32_t ps = hist[2 * v], ns = hist[2 * v + 1];
         uint8_t k = K_SKIP;
-        if (mi != IMU_OFF && !vstate[v] && (ps || ns)) {
+        // iassumed (lcve.cpp:57) and eliminated / assigned (lcve.cpp:58) variables are passed over
+        if (mi != IMU_OFF && !ifrozen[v] && !vstate[v] && (ps || ns)) {
             const uint32_t pt = I.mu_pos << mi, nt = I.mu_neg << mi;
             const bool hot = ot[2 * v].sz >= pt && ot[2 * v + 1].sz >= nt;
             const bool freezable = ps < pt || ns < nt;
@@ -350,7 +353,7 @@ uint32_t lcve(Ctx &c)
     PFG_HIP(hipMemsetAsync(c.ib0.p, 0xFF, 16ull * c.ninst, c.stream));   // ib0 and ibrk: no break
     PFG_HIP(hipMemsetAsync(c.inel.p, 0, 8ull * c.ninst, c.stream));
     uint8_t *kind = c.tmpu8.p, *status = kind + nv + 1, *blocked = status + nv + 1;
-    k_classify<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, c.hist.p, c.vstate.p, c.ot.p, I,
+    k_classify<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, c.hist.p, c.vstate.p, c.ifrozen.p, c.ot.p, I,
                                                               kind, status, blocked);
     // per-round counts on the device: rw[r] = undecided candidates entering round r,
     // ra[r] = active clause records entering round r's pass A (rounds 1..)

@@ -21,8 +21,11 @@ void enqueue_units_host(Ctx &c, const uint32_t *lits, uint32_t n);
 // batches
 void count_inst(Ctx &c, std::vector<int64_t> &cls, std::vector<int64_t> &lits);
 void partition_pvs(Ctx &c, const std::vector<uint8_t> &cls, uint32_t &na, uint32_t &nb);
-void batch_images(Ctx &c, uint32_t *out, std::vector<uint64_t> &start);
+void batch_images(Ctx &c, uint32_t *out, std::vector<uint64_t> &start, std::vector<uint64_t> &cstart);
+void batch_export(Ctx &c, const uint32_t *img, uint64_t c0, uint64_t ncls, uint64_t nlits, uint32_t *d_lits,
+                  uint64_t *d_offs);
 void batch_records(Ctx &c, uint32_t *out, std::vector<uint64_t> &start);
+void set_ifrozen(Ctx &c, const uint32_t *vars, uint32_t n, bool on);
 void merge_units(Ctx &c, const uint32_t *d_units, uint64_t n);  // ordered dedupe into the trail
 // algorithmic bytes of reading the occurrence lists of elected variables pvs[b, e):
 // 16 per list header + 8 per entry (+ 12 + 4|c| per listed clause when with_clauses);

@@ -391,6 +391,8 @@ int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, c
         PFG_HIP(hipMemsetAsync(c.value.p, 0xFF, c.value.cap, c.stream));
         ensure(c, c.vstate, (uint64_t)nvars + 1);
         PFG_HIP(hipMemsetAsync(c.vstate.p, 0, c.vstate.cap, c.stream));
+        ensure(c, c.ifrozen, (uint64_t)nvars + 1);
+        PFG_HIP(hipMemsetAsync(c.ifrozen.p, 0, c.ifrozen.cap, c.stream));
         ensure(c, c.trail, (uint64_t)nvars + 1);
         ensure(c, c.ufirst, (uint64_t)nvars + 1);
         PFG_HIP(hipMemsetAsync(c.ufirst.p, 0xFF, c.ufirst.cap * 4, c.stream));
@@ -416,8 +418,8 @@ int pfg_destroy(pfg_ctx *ctx)
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     release(c, c.mem); release(c, c.mem2); release(c, c.refs); release(c, c.refs2);
-    release(c, c.ot); release(c, c.occ); release(c, c.hist); release(c, c.value); release(c, c.vstate);
-    release(c, c.trail); release(c, c.resolved); release(c, c.mchunk); release(c, c.pvs); release(c, c.units);
+    release(c, c.ot); release(c, c.occ); release(c, c.hist); release(c, c.value); release(c, c.vstate); release(c, c.ifrozen);
+    release(c, c.trail); release(c, c.resolved); release(c, c.mchunk); release(c, c.img_off); release(c, c.pvs); release(c, c.units);
     release(c, c.scores); release(c, c.eligible); release(c, c.rank); release(c, c.tmpu32a); release(c, c.tmpu32b); release(c, c.tmpu32c); release(c, c.tmpu32d);
     release(c, c.tmpu8); release(c, c.tmpu64a); release(c, c.tmpu64b); release(c, c.tmpu64c);
     release(c, c.owner); release(c, c.claim); release(c, c.erec); release(c, c.ufirst); release(c, c.cnt);
@@ -480,6 +482,16 @@ int pfg_enqueue_units(pfg_ctx *ctx, const uint32_t *lits, uint32_t n)
     for (uint32_t i = 0; i < n; i++)
         if ((lits[i] >> 1) == 0 || (lits[i] >> 1) > c.nvars) throw Error(PFG_EINVAL, "literal out of range");
     enqueue_units_host(c, lits, n);
+    API_END
+}
+
+int pfg_ifreeze(pfg_ctx *ctx, const uint32_t *vars, uint32_t n, int on)
+{
+    if (!ctx || (n && !vars)) return PFG_EINVAL;
+    API_BEGIN
+    for (uint32_t i = 0; i < n; i++)
+        if (vars[i] == 0 || vars[i] > c.nvars) throw Error(PFG_EINVAL, "variable out of range");
+    set_ifrozen(c, vars, n, on != 0);
     API_END
 }
 
@@ -587,6 +599,7 @@ int pfg_batch_relayout(pfg_ctx *ctx, uint32_t ninst, const uint32_t *nvars)
     // forget the previous clause database and variable state
     PFG_HIP(hipMemsetAsync(c.value.p, 0xFF, c.value.cap, c.stream));
     PFG_HIP(hipMemsetAsync(c.vstate.p, 0, c.vstate.cap, c.stream));
+    PFG_HIP(hipMemsetAsync(c.ifrozen.p, 0, c.ifrozen.cap, c.stream));
     PFG_HIP(hipMemsetAsync(c.cnt.p, 0, (CNT_CONF + 1) * 8, c.stream));
     PFG_HIP(hipStreamSynchronize(c.stream));
     c.ntrail = c.propagated = 0;
@@ -642,6 +655,54 @@ int pfg_sigmify_batch(pfg_ctx *ctx, int32_t *status)
     API_END
 }
 
+// Host copy of the batch state for per-instance downloads, rebuilt once per batch:
+// clause images and model records grouped by instance on the device, then pinned
+// downloads; units are bucketed on the host. The grouped images stay in mem2 for exports.
+static void batch_cache(Ctx &c)
+{
+    if (c.cache_gen == c.batch_gen) return;
+    // clause images and model records grouped by instance on the device, then pinned
+    // downloads; units are bucketed on the host
+    const auto tq0 = std::chrono::steady_clock::now();
+    ensure(c, c.mem2, c.msz + 1);
+    batch_images(c, c.mem2.p, c.cache_start, c.cache_cstart);
+    const uint64_t nw = c.cache_start[c.ninst];
+    if (c.batch) {
+        ensure(c, c.tmpu32d, c.nres + 1);
+        batch_records(c, c.tmpu32d.p, c.cache_rstart);
+    }
+    else   // one instance: the stack as it is
+        c.cache_rstart.assign({0, c.nres});
+    const uint64_t nr = c.cache_rstart[c.ninst];
+    const auto tq1 = std::chrono::steady_clock::now();
+    c.cache_mem.ensure(nw + 1);
+    c.pin_res.ensure(nr + 1);
+    c.pin_trail.ensure((uint64_t)c.ntrail + 1);
+    c.pin_vstate.ensure((uint64_t)c.nvars + 1);
+    const auto tq2 = std::chrono::steady_clock::now();
+    if (nw) PFG_HIP(hipMemcpyAsync(c.cache_mem.p, c.mem2.p, nw * 4, hipMemcpyDeviceToHost, c.stream));
+    if (nr)
+        PFG_HIP(hipMemcpyAsync(c.pin_res.p, c.batch ? c.tmpu32d.p : c.resolved.p, nr * 4, hipMemcpyDeviceToHost,
+                               c.stream));
+    PFG_HIP(hipMemcpyAsync(c.pin_vstate.p, c.vstate.p, (uint64_t)c.nvars + 1, hipMemcpyDeviceToHost, c.stream));
+    if (c.ntrail) PFG_HIP(hipMemcpyAsync(c.pin_trail.p, c.trail.p, 4ull * c.ntrail, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    const auto tq3 = std::chrono::steady_clock::now();
+    const uint32_t *trail = c.pin_trail.p;
+    const uint64_t ntr = c.ntrail;
+    Sigma S(c);
+    c.cache_trail.assign(c.ninst, {});
+    for (uint64_t k = 0; k < ntr; k++) c.cache_trail[S.inst_of(trail[k] >> 1)].push_back(trail[k]);
+    c.cache_gen = c.batch_gen;
+    if (c.trace) {
+        const auto tq4 = std::chrono::steady_clock::now();
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        fprintf(stderr, "[pfg] batch results: group %.2f ms, pin %.2f ms, copy %.2f ms (%llu words, %llu records), "
+                        "units %.2f ms\n", ms(tq0, tq1), ms(tq1, tq2), ms(tq2, tq3), (unsigned long long)nw,
+                (unsigned long long)nr, ms(tq3, tq4));
+    }
+}
+
 // one instance's part of the batch state, literals in the instance's own numbering:
 // its clauses (SCLAUSE images, batch order), model records, variable states (nvars_i + 1
 // bytes) and units; a NULL buffer only queries the size
@@ -650,57 +711,12 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
 {
     API_BEGIN
     if (inst >= c.ninst) return PFG_EINVAL;
-    if (c.cache_gen != c.batch_gen) {
-        // clause images and model records grouped by instance on the device, then pinned
-        // downloads; units are bucketed on the host
-        const auto tq0 = std::chrono::steady_clock::now();
-        ensure(c, c.mem2, c.msz + 1);
-        batch_images(c, c.mem2.p, c.cache_start);
-        const uint64_t nw = c.cache_start[c.ninst];
-        if (c.batch) {
-            ensure(c, c.tmpu32d, c.nres + 1);
-            batch_records(c, c.tmpu32d.p, c.cache_rstart);
-        }
-        else   // one instance: the stack as it is
-            c.cache_rstart.assign({0, c.nres});
-        const uint64_t nr = c.cache_rstart[c.ninst];
-        const auto tq1 = std::chrono::steady_clock::now();
-        c.cache_mem.ensure(nw + 1);
-        c.pin_res.ensure(nr + 1);
-        c.pin_trail.ensure((uint64_t)c.ntrail + 1);
-        c.pin_vstate.ensure((uint64_t)c.nvars + 1);
-        const auto tq2 = std::chrono::steady_clock::now();
-        if (nw) PFG_HIP(hipMemcpyAsync(c.cache_mem.p, c.mem2.p, nw * 4, hipMemcpyDeviceToHost, c.stream));
-        if (nr)
-            PFG_HIP(hipMemcpyAsync(c.pin_res.p, c.batch ? c.tmpu32d.p : c.resolved.p, nr * 4, hipMemcpyDeviceToHost,
-                                   c.stream));
-        PFG_HIP(hipMemcpyAsync(c.pin_vstate.p, c.vstate.p, (uint64_t)c.nvars + 1, hipMemcpyDeviceToHost, c.stream));
-        if (c.ntrail) PFG_HIP(hipMemcpyAsync(c.pin_trail.p, c.trail.p, 4ull * c.ntrail, hipMemcpyDeviceToHost, c.stream));
-        PFG_HIP(hipStreamSynchronize(c.stream));
-        const auto tq3 = std::chrono::steady_clock::now();
-        const uint32_t *trail = c.pin_trail.p;
-        const uint64_t ntr = c.ntrail;
-        Sigma S(c);
-        c.cache_trail.assign(c.ninst, {});
-        for (uint64_t k = 0; k < ntr; k++) c.cache_trail[S.inst_of(trail[k] >> 1)].push_back(trail[k]);
-        c.cache_gen = c.batch_gen;
-        if (c.trace) {
-            const auto tq4 = std::chrono::steady_clock::now();
-            auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
-            fprintf(stderr, "[pfg] batch results: group %.2f ms, pin %.2f ms, copy %.2f ms (%llu words, %llu records), "
-                            "units %.2f ms\n", ms(tq0, tq1), ms(tq1, tq2), ms(tq2, tq3), (unsigned long long)nw,
-                    (unsigned long long)nr, ms(tq3, tq4));
-        }
-    }
+    batch_cache(c);
     const uint32_t lo = c.hibase[inst], shift = 2u * lo;
     const uint64_t w0 = c.cache_start[inst], w1 = c.cache_start[inst + 1];
     if (words && w1 > w0) std::memcpy(words, c.cache_mem.p + w0, (w1 - w0) * 4);
     if (nwords) *nwords = w1 - w0;
-    if (ncls) {   // count the images
-        uint64_t n = 0;
-        for (uint64_t w = w0; w < w1; w += 3ull + c.cache_mem.p[w + 2]) n++;
-        *ncls = n;
-    }
+    if (ncls) *ncls = c.cache_cstart[inst + 1] - c.cache_cstart[inst];
     const uint64_t r0 = c.cache_rstart[inst], r1 = c.cache_rstart[inst + 1];
     if (resolved && r1 > r0) std::memcpy(resolved, c.pin_res.p + r0, (r1 - r0) * 4);
     if (nres) *nres = r1 - r0;
@@ -710,6 +726,21 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
     }
     if (trail) for (size_t k = 0; k < c.cache_trail[inst].size(); k++) trail[k] = c.cache_trail[inst][k] - shift;
     if (ntrail) *ntrail = (uint32_t)c.cache_trail[inst].size();
+    API_END
+}
+
+// one instance's clauses as CSR in caller-owned device memory, in the instance's
+// numbering (clause order of the download); NULL buffers only query the sizes
+int pfg_batch_export(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint64_t *d_offs, uint64_t *nlits, uint64_t *ncls)
+{
+    API_BEGIN
+    if (inst >= c.ninst) return PFG_EINVAL;
+    batch_cache(c);
+    const uint64_t c0 = c.cache_cstart[inst], nc = c.cache_cstart[inst + 1] - c0;
+    const uint64_t nl = (c.cache_start[inst + 1] - c.cache_start[inst]) - 3 * nc;
+    if (nlits) *nlits = nl;
+    if (ncls) *ncls = nc;
+    if (d_offs) batch_export(c, c.mem2.p, c0, nc, nl, d_lits, d_offs);
     API_END
 }
 
@@ -844,6 +875,7 @@ int pfg_reset(pfg_ctx *ctx)
     API_BEGIN
     PFG_HIP(hipMemsetAsync(c.value.p, 0xFF, c.value.cap, c.stream));
     PFG_HIP(hipMemsetAsync(c.vstate.p, 0, c.vstate.cap, c.stream));
+    PFG_HIP(hipMemsetAsync(c.ifrozen.p, 0, c.ifrozen.cap, c.stream));
     PFG_HIP(hipMemsetAsync(c.cnt.p, 0, (CNT_CONF + 1) * 8, c.stream));
     PFG_HIP(hipStreamSynchronize(c.stream));
     c.ntrail = c.propagated = 0;

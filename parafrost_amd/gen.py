@@ -36,7 +36,8 @@ def _load():
         lib.pfgen_aig_bmc.restype = ctypes.c_int64
         lib.pfgen_aig_bmc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, u32p, u64p,
-                                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32)]
+                                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.c_void_p]
         lib.pfgen_write_dimacs.restype = ctypes.c_int
         lib.pfgen_write_dimacs.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint64, u32p, u64p]
         _lib = lib
@@ -215,7 +216,23 @@ def aig_bmc(seed: int, ngates: int, t0: int, t1: int, nlatches: int = 64, ninput
     offs = np.empty(3 * ngates * nf + 1, np.uint64)
     nc = ctypes.c_uint64()
     nv = ctypes.c_uint32()
-    n = _load().pfgen_aig_bmc(seed, ngates, nlatches, ninputs, pinv, t0, t1, lits, offs, ctypes.byref(nc), ctypes.byref(nv))
+    n = _load().pfgen_aig_bmc(seed, ngates, nlatches, ninputs, pinv, t0, t1, lits, offs, ctypes.byref(nc), ctypes.byref(nv),
+                              None)
     if n < 0:
         raise ValueError("generator rejected its arguments")
     return Cnf(int(nv.value), lits[:n].copy(), offs[:nc.value + 1].copy()), int(nv.value)
+
+
+def aig_bmc_interface(seed: int, ngates: int, t: int, nlatches: int = 64, ninputs: int = 32, pinv: float = 0.3):
+    """Variables of the same AIG's latches at the start of frame t (sorted, distinct):
+    the only variables of frames [0, t) that the clauses of frames t, t + 1, .. contain.
+    An instance that may be extended keeps them frozen (ifreeze)."""
+    out = np.zeros(max(1, nlatches), np.uint32)
+    lits = np.zeros(1, np.uint32)
+    offs = np.zeros(1, np.uint64)
+    nc = ctypes.c_uint64()
+    nv = ctypes.c_uint32()
+    if _load().pfgen_aig_bmc(seed, ngates, nlatches, ninputs, pinv, t, t, lits, offs, ctypes.byref(nc), ctypes.byref(nv),
+                             out.ctypes.data) < 0:
+        raise ValueError("generator rejected its arguments")
+    return np.unique(out[:nlatches])

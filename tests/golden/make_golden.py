@@ -4,7 +4,8 @@ Runs the reference simplifier, compiled from the reference's own sources by
 oracle/Makefile (target `ref`), on small seeded instances and stores, per case,
 the input clauses and the reference's final state (clause images in clause
 order, unit trail, model stack, variable states) plus per-pass dumps of the
-first rounds. Output: tests/golden/reference_vectors.npz (data only).
+first rounds. Cases f* run in incremental mode with variables frozen by
+ifreeze() and, for some, units given as unit clauses. Output: tests/golden/reference_vectors.npz (data only).
 
 usage: python tests/golden/make_golden.py
 """
@@ -41,7 +42,29 @@ def cases():
             cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 4.0)), 2.5, 0.3, 1000 + seed)
         opts = dict(phases=int(rng.integers(1, 6)), mu_pos=int(rng.integers(10, 60)),
                     mu_neg=int(rng.integers(10, 60)), shrink_rate=int(rng.integers(1, 4)))
-        out.append((f"g{seed}", cnf, opts))
+        out.append((f"g{seed}", cnf, opts, None, None))
+    # incremental mode: variables frozen by ifreeze(), half of the cases with units
+    # given as unit clauses (the reference parser enqueues them; the oracle gets them
+    # through pfo_enqueue)
+    for seed in range(16):
+        rng = np.random.default_rng(3000 + seed)
+        nv = int(rng.integers(30, 250))
+        kind = seed % 3
+        if kind == 0:
+            cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(2.0, 4.3)), 3, 3000 + seed)
+        elif kind == 1:
+            cnf = gen.geometric(nv, int(nv * rng.uniform(1.5, 4.0)), 2, 8, 3000 + seed)
+        else:
+            nv = max(nv, 64)
+            cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 4.0)), 2.5, 0.3, 3000 + seed)
+        opts = dict(phases=int(rng.integers(1, 6)), mu_pos=int(rng.integers(10, 60)),
+                    mu_neg=int(rng.integers(10, 60)), shrink_rate=int(rng.integers(1, 4)))
+        freeze = np.unique(rng.integers(1, nv + 1, int(nv * rng.uniform(0.05, 0.4)))).astype(np.uint32)
+        units = None
+        if seed % 2:
+            vs = rng.choice(np.arange(1, nv + 1), 3, replace=False)
+            units = (2 * vs + rng.integers(0, 2, 3)).astype(np.uint32)
+        out.append((f"f{seed}", cnf, opts, freeze, units))
     return out
 
 
@@ -50,11 +73,20 @@ def main():
     arrays = {}
     names = []
     with tempfile.TemporaryDirectory() as td:
-        for name, cnf, opts in cases():
+        for name, cnf, opts, freeze, units in cases():
             path = os.path.join(td, name + ".cnf")
-            cnf.write_dimacs(path)
+            if units is not None:
+                lits = np.concatenate([cnf.lits, units])
+                offs = np.concatenate([cnf.offs, cnf.offs[-1] + 1 + np.arange(len(units), dtype=np.uint64)])
+                gen.Cnf(cnf.nvars, lits, offs).write_dimacs(path)
+                arrays[f"{name}/units"] = units
+            else:
+                cnf.write_dimacs(path)
             o = po.default_opts(**opts)
-            st = po.run_reference(path, os.path.join(td, name + ".bin"), o, trace_prefix=os.path.join(td, name))
+            st = po.run_reference(path, os.path.join(td, name + ".bin"), o, trace_prefix=os.path.join(td, name),
+                                  freeze=freeze)
+            if freeze is not None:
+                arrays[f"{name}/freeze"] = freeze
             arrays[f"{name}/lits"] = cnf.lits
             arrays[f"{name}/offs"] = cnf.offs
             arrays[f"{name}/meta"] = np.array([cnf.nvars, opts["phases"], opts["mu_pos"], opts["mu_neg"],

@@ -65,10 +65,11 @@ def compare(o: po.State, g: po.State, where: str, lists: bool = False):
                 fail(f"occurrence list of literal {2 * int(o.pvs[k // 2]) + (k & 1)}", a, b)
 
 
-def run_lockstep(cnf, lists: bool = True, units=None, setup=None, **kw):
+def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, **kw):
     """Run both engines pass by pass; raise Mismatch at the first difference.
-    setup(g), if given, configures the GPU simplifier before the upload.
-    Returns the number of rounds and the final state."""
+    setup(g), if given, configures the GPU simplifier before the upload; freeze lists
+    variables frozen by ifreeze (incremental mode). Returns the number of rounds and
+    the final state."""
     oo, go = _opts_pair(**kw)
     o = po.Oracle(cnf, oo)
     g = pf.Simplifier(cnf.nvars, go)
@@ -76,6 +77,9 @@ def run_lockstep(cnf, lists: bool = True, units=None, setup=None, **kw):
         setup(g)
     g.upload(cnf)
     L = o.L
+    if freeze is not None and len(freeze):
+        o.ifreeze(freeze)
+        g.ifreeze(freeze)
     if units is not None and len(units):
         for u in units:
             L.pfo_enqueue(o.h, int(u))

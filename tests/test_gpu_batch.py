@@ -38,7 +38,8 @@ def _full_3cnf():
     return gen.Cnf(3, np.array(lits, np.uint32), np.array(offs, np.uint64))
 
 
-def _check(cnfs, g=None, **kw):
+def _check(cnfs, g=None, freezes=None, units=None, **kw):
+    """freezes / units: per instance (own numbering), or None"""
     import parafrost_amd as pf
     own = g is None
     if own:
@@ -47,10 +48,19 @@ def _check(cnfs, g=None, **kw):
         g.relayout([c.nvars for c in cnfs])
     try:
         g.upload_instances(cnfs)
+        for i in range(len(cnfs)):
+            if freezes is not None and len(freezes[i]):
+                g.ifreeze_instance(i, freezes[i])
+            if units is not None and len(units[i]):
+                g.enqueue_instance(i, units[i])
         status = g.sigmify_batch()
         n_unsat = 0
         for i, cnf in enumerate(cnfs):
             o = po.Oracle(cnf, po.default_opts(**kw))
+            if freezes is not None:
+                o.ifreeze(freezes[i])
+            if units is not None and len(units[i]):
+                assert o.enqueue(units[i])
             ro = o.sigmify()
             assert (ro != 0) == (status[i] != 0), f"instance {i}: unsat oracle {ro} batch {status[i]}"
             if ro != 0:
@@ -77,6 +87,23 @@ def test_batch_matches_each_instance_alone(seed):
         cnfs.insert(int(rng.integers(0, len(cnfs))), _full_3cnf())
     kw = dict(phases=int(rng.integers(1, 7)), mu_pos=int(rng.integers(10, 60)), mu_neg=int(rng.integers(10, 60)))
     _check(cnfs, **kw)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_batch_frozen_and_units(seed):
+    """Incremental mode in a batch: per-instance frozen variables and root units."""
+    print(f"batch frozen seed={seed}", flush=True)
+    rng = np.random.default_rng(4600 + seed)
+    cnfs = [_instance(rng, k) for k in range(int(rng.integers(6, 16)))]
+    freezes, units = [], []
+    for c in cnfs:
+        freezes.append(np.unique(rng.integers(1, c.nvars + 1, max(1, int(c.nvars * rng.uniform(0.0, 0.4))))).astype(np.uint32))
+        if rng.random() < 0.5:
+            vs = rng.choice(np.arange(1, c.nvars + 1), size=max(1, c.nvars // 50), replace=False)
+            units.append((2 * vs + rng.integers(0, 2, len(vs))).astype(np.uint32))
+        else:
+            units.append(np.zeros(0, np.uint32))
+    _check(cnfs, freezes=freezes, units=units, phases=int(rng.integers(2, 6)))
 
 
 def test_batch_with_refuted_instances():
@@ -124,3 +151,42 @@ def test_batch_generator_instances_roundtrip():
         assert c.nvars == int(b.inst_nvars[i])
         assert c.lits.min() >= 2 and c.lits.max() <= 2 * c.nvars + 1
         assert c.nlits == c.ncls * int(b.kinds[i])
+
+
+def _csr(words):
+    """clause images -> (literals, offsets)"""
+    lits, offs, i = [], [0], 0
+    while i < len(words):
+        n = int(words[i + 2])
+        lits.append(words[i + 3:i + 3 + n])
+        offs.append(offs[-1] + n)
+        i += 3 + n
+    return (np.concatenate(lits) if lits else np.zeros(0, np.uint32)), np.array(offs, np.uint64)
+
+
+def test_batch_export_matches_download():
+    """The HBM write-back of every instance is the CSR form of its downloaded clauses."""
+    import torch
+    import parafrost_amd as pf
+    rng = np.random.default_rng(4700)
+    cnfs = [_instance(rng, k) for k in range(9)] + [_full_3cnf()]
+    g = pf.BatchSimplifier([c.nvars for c in cnfs], pf.default_opts())
+    try:
+        g.upload_instances(cnfs)
+        st = g.sigmify_batch()
+        for i in range(len(cnfs)):
+            w, _, _, _ = g.instance(i)
+            dl, do = g.export(i)
+            el, eo = _csr(w)
+            assert np.array_equal(dl.cpu().numpy().view(np.uint32), el), f"instance {i}: literals"
+            assert np.array_equal(do.cpu().numpy().view(np.uint64), eo), f"instance {i}: offsets"
+            if st[i] == 0 and len(eo) > 1:
+                # what was exported uploads again as the same clauses
+                h = pf.Simplifier(cnfs[i].nvars, pf.default_opts(phases=0))
+                try:
+                    h.upload_device(dl.data_ptr(), do.data_ptr(), len(eo) - 1, len(el))
+                    assert np.array_equal(_csr(h.cnf_words())[0], el)
+                finally:
+                    h.close()
+    finally:
+        g.close()

@@ -126,6 +126,61 @@ def test_lockstep_root_units(seed):
     ls.run_lockstep(cnf, units=units, **kw)
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_lockstep_frozen(seed):
+    """Incremental mode: variables frozen by ifreeze are passed over by LCVE (lcve.cpp:57);
+    odd seeds also put units on the trail first."""
+    print(f"lockstep frozen seed={seed}", flush=True)
+    cnf, kw = _small(seed + 300)
+    rng = np.random.default_rng(77 + seed)
+    freeze = np.unique(rng.integers(1, cnf.nvars + 1, max(1, int(cnf.nvars * rng.uniform(0.05, 0.4))))).astype(np.uint32)
+    units = None
+    if seed % 2:
+        vs = rng.choice(np.arange(1, cnf.nvars + 1), size=max(1, cnf.nvars // 40), replace=False)
+        units = (2 * vs + rng.integers(0, 2, len(vs))).astype(np.uint32)
+    ls.run_lockstep(cnf, units=units, freeze=freeze, **kw)
+
+
+def test_frozen_marks_persist_and_clear():
+    """Marks set before an upload hold for the sigmify after it, melting undoes them,
+    and pfg_reset clears them."""
+    import parafrost_amd as pf
+    cnf, kw = _small(311)
+    frz = np.arange(1, cnf.nvars + 1, 2, dtype=np.uint32)
+    g = pf.Simplifier(cnf.nvars, pf.default_opts(**kw))
+    try:
+        def run(fz=None, melt=False):
+            g.reset_for_upload()
+            if fz is not None:
+                g.ifreeze(fz)
+                if melt:
+                    g.ifreeze(fz, on=False)
+            g.upload(cnf)
+            try:
+                g.sigmify()
+            except pf.Unsat:
+                return None
+            return g.cnf_words(), g.vstate()
+        base = run()
+        frozen = run(frz)
+        melted = run(frz, melt=True)
+        g.reset_for_upload()
+        g.ifreeze(frz)
+        cleared = run()
+        o = po.Oracle(cnf, po.default_opts(**kw))
+        o.ifreeze(frz)
+        ro = o.sigmify()
+        assert (ro != 0) == (frozen is None)
+        if frozen is not None:
+            assert np.array_equal(frozen[0], o.state().cnf)
+            assert not np.any(frozen[1][frz] == 1)
+        if base is not None:
+            assert np.array_equal(melted[0], base[0]) and np.array_equal(cleared[0], base[0])
+            assert frozen is None or not np.array_equal(frozen[0], base[0])
+    finally:
+        g.close()
+
+
 def test_unsat_by_propagation():
     import parafrost_amd as pf
     lits = np.array([2 * 1 + 1, 2 * 2, 2 * 2 + 1, 2 * 3, 2 * 3 + 1, 2 * 1], np.uint32)  # (~1|2) (~2|3) (~3|1)... closed by units

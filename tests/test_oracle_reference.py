@@ -13,12 +13,14 @@ from parafrost_amd import gen
 pytestmark = pytest.mark.skipif(not os.path.exists(po.REF_BIN), reason="reference build absent")
 
 
-def _run(cnf, opts):
+def _run(cnf, opts, freeze=None):
     with tempfile.TemporaryDirectory() as td:
         p = os.path.join(td, "f.cnf")
         cnf.write_dimacs(p)
-        ref = po.run_reference(p, os.path.join(td, "f.bin"), opts)
+        ref = po.run_reference(p, os.path.join(td, "f.bin"), opts, freeze=freeze)
     o = po.Oracle(cnf, opts)
+    if freeze is not None:
+        o.ifreeze(freeze)
     rc = o.sigmify()
     st = o.state()
     if ref.unsat:
@@ -46,3 +48,14 @@ def test_mid_size(seed):
     opts = po.default_opts(phases=int(rng.integers(2, 7)), mu_pos=int(rng.integers(10, 200)),
                            mu_neg=int(rng.integers(10, 200)), shrink_rate=int(rng.integers(1, 4)))
     _run(cnf, opts)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_mid_size_frozen(seed):
+    """incremental mode: a random set of variables frozen by ifreeze()"""
+    rng = np.random.default_rng(700 + seed)
+    nv = int(rng.integers(2000, 20000))
+    cnf = (gen.uniform_ksat(nv, int(nv * rng.uniform(3.0, 4.3)), 3, 700 + seed) if seed % 2 else
+           gen.scalefree(nv, int(nv * rng.uniform(2.0, 5.0)), 2.5, 0.2, 700 + seed))
+    freeze = np.unique(rng.integers(1, nv + 1, int(nv * rng.uniform(0.05, 0.4)))).astype(np.uint32)
+    _run(cnf, po.default_opts(phases=int(rng.integers(2, 7))), freeze)
