@@ -889,11 +889,14 @@ void filter_pvs(Ctx &c)
 }
 
 // ------------------------------------------------------------------ units from passes
+// root units given by the caller: each is assigned by an atomic exchange on its
+// variable's value, so a repeated unit is kept once and a unit whose negation is also
+// given reports a contradiction, as in the sequential enqueue; only the order in
+// which they enter the trail differs (no pass reads it)
 __global__ void k_assign_host_units(const uint32_t *__restrict__ lits, uint32_t n, int8_t *value, uint8_t *vstate,
                                     uint32_t *trail, unsigned long long *cnt)
 {
-    if (gtid() != 0) return;
-    for (uint32_t i = 0; i < n; i++) enqueue_dev(value, vstate, trail, cnt, lits[i]);
+    for (uint64_t i = gtid(); i < n; i += gstride()) enqueue_dev(value, vstate, trail, cnt, lits[i]);
 }
 
 __global__ void k_set_ifrozen(const uint32_t *__restrict__ vars, uint32_t n, uint8_t on, uint8_t *__restrict__ ifrozen)
@@ -917,7 +920,8 @@ void enqueue_units_host(Ctx &c, const uint32_t *lits, uint32_t n)
     ensure(c, c.tmpu32a, n);
     PFG_HIP(hipMemcpyAsync(c.tmpu32a.p, lits, n * 4, hipMemcpyHostToDevice, c.stream));
     c.hist_live = false;
-    k_assign_host_units<<<1, 64, 0, c.stream>>>(c.tmpu32a.p, n, c.value.p, c.vstate.p, c.trail.p, (unsigned long long *)c.cnt.p);
+    k_assign_host_units<<<grid_for(n, 256, 4096), 256, 0, c.stream>>>(c.tmpu32a.p, n, c.value.p, c.vstate.p, c.trail.p,
+                                                                      (unsigned long long *)c.cnt.p);
     read_counters(c, 10);
     c.ntrail = (uint32_t)c.hcnt[CNT_NTRAIL];
     if (c.hcnt[CNT_FLAGS] & FLAG_UNSAT) c.unsat = true;

@@ -330,3 +330,23 @@ int64_t pfgen_aig_bmc(uint64_t seed, uint32_t ng, uint32_t nl, uint32_t ni, doub
     if (nvars_out) *nvars_out = nl + t1 * per;
     return (int64_t)w;
 }
+
+/* SCLAUSE images ([word0, sig, size, lits..] per clause) to CSR: the host-side
+ * write-back of a downloaded clause database. offs gets ncls + 1 entries (at most
+ * nwords / 3 + 1). Returns the literal count, -1 on a malformed image. */
+int64_t pfgen_images_to_csr(const uint32_t *words, uint64_t nwords, uint32_t *lits, uint64_t *offs, uint64_t *ncls_out)
+{
+    uint64_t w = 0, c = 0, n = 0;
+    offs[0] = 0;
+    while (w < nwords) {
+        if (w + 3 > nwords) return -1;
+        const uint32_t sz = words[w + 2];
+        if (w + 3 + sz > nwords) return -1;
+        memcpy(lits + n, words + w + 3, sizeof(uint32_t) * sz);
+        n += sz;
+        offs[++c] = n;
+        w += 3ull + sz;
+    }
+    if (ncls_out) *ncls_out = c;
+    return (int64_t)n;
+}

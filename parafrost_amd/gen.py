@@ -33,6 +33,8 @@ def _load():
         lib.pfgen_scalefree.restype = ctypes.c_int64
         lib.pfgen_scalefree.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_uint64, u32p, u64p]
+        lib.pfgen_images_to_csr.restype = ctypes.c_int64
+        lib.pfgen_images_to_csr.argtypes = [u32p, ctypes.c_uint64, u32p, u64p, ctypes.POINTER(ctypes.c_uint64)]
         lib.pfgen_aig_bmc.restype = ctypes.c_int64
         lib.pfgen_aig_bmc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, u32p, u64p,
@@ -236,3 +238,15 @@ def aig_bmc_interface(seed: int, ngates: int, t: int, nlatches: int = 64, ninput
                              out.ctypes.data) < 0:
         raise ValueError("generator rejected its arguments")
     return np.unique(out[:nlatches])
+
+
+def images_to_csr(words, nvars: int) -> Cnf:
+    """Clause images ([word0, sig, size, lits..] per clause) as a Cnf (host write-back)."""
+    w = np.ascontiguousarray(words, np.uint32)
+    lits = np.empty(max(1, len(w)), np.uint32)
+    offs = np.empty(len(w) // 3 + 2, np.uint64)
+    nc = ctypes.c_uint64()
+    n = _load().pfgen_images_to_csr(w, len(w), lits, offs, ctypes.byref(nc))
+    if n < 0:
+        raise ValueError("malformed clause images")
+    return Cnf(nvars, lits[:n].copy(), offs[:nc.value + 1].copy())
