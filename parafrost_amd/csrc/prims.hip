@@ -1,15 +1,16 @@
 // Exclusive scan and stable LSD radix sort for wave64 (CDNA4).
 //  scan : reduce-then-scan over 2048-element tiles (256 threads x 8 items),
 //         coalesced tile loads through LDS, wave scans by __shfl_up.
-//  radix: 8-bit digits; per-tile LDS histograms, digit-major global scan,
-//         stable scatter ranked with 8 wave ballots per round (match-any on the digit).
+//  radix: 8-bit digits over 4096-element tiles; per-tile LDS histograms (wave-aggregated),
+//         stored tile-major, column scan to per-tile digit offsets, stable scatter ranked
+//         per wave with match-any ballots, tile held in registers.
 #include "prims.hpp"
 #include "device.hpp"
 
 namespace pfg {
 
 constexpr int SCAN_BLOCK = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
-constexpr int RS_BLOCK = 256, RS_ITEMS = 8, RS_TILE = RS_BLOCK * RS_ITEMS;
+constexpr int RS_BLOCK = 256, RS_ITEMS = 16, RS_TILE = RS_BLOCK * RS_ITEMS;
 
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v)
@@ -125,84 +126,186 @@ void scan_u64(Ctx &c, const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *d
     scan_rec<uint64_t>(c, in, out, n, d_total, (uint64_t *)c.scan_tmp.p);
 }
 
+// match-any on an 8-bit digit: the valid lanes of the wave holding the same digit
+__device__ __forceinline__ uint64_t match_digit(bool valid, uint32_t d)
+{
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(valid && bit);
+        m &= bit ? bal : ~bal;
+    }
+    return m;
+}
+
+// Per-tile digit counts. All loads of the tile are issued first; lanes of a wave that
+// share a digit are counted by their lowest lane with one LDS atomic (skewed digits
+// would otherwise serialize on one counter).
 template <typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_radix_hist(const K *__restrict__ keys, uint64_t n, int shift,
-                                                         uint32_t *__restrict__ counts, uint32_t ntiles)
+                                                         uint32_t *__restrict__ counts, uint32_t ch,
+                                                         unsigned long long *__restrict__ part)
 {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
-    __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    K kk[RS_ITEMS];
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; k++) {
         const uint64_t i = base + (uint64_t)k * RS_BLOCK + threadIdx.x;
-        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+        kk[k] = i < n ? keys[i] : (K)0;
     }
     __syncthreads();
-    counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        const bool valid = base + (uint64_t)k * RS_BLOCK + threadIdx.x < n;
+        const uint32_t d = (uint32_t)(kk[k] >> shift) & 255u;
+        const uint64_t m = match_digit(valid, d);
+        if (valid && (m & lt) == 0) atomicAdd(&h[d], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    const uint32_t c = h[threadIdx.x];
+    counts[(uint64_t)blockIdx.x * 256 + threadIdx.x] = c;
+    if (c) atomicAdd(&part[(uint64_t)(blockIdx.x / ch) * 256 + threadIdx.x], (unsigned long long)c);
 }
 
-// Stable scatter of one tile: the tile's keys are ranked by digit (8 wave ballots
-// per round), reordered in LDS, and written out as contiguous runs per digit, so
-// global writes are coalesced. The tile's digit starts come from its histogram.
+// Digit offsets of every tile from the tile-major counts (counts[t * 256 + d]):
+// offs[t * 256 + d] = (keys of digits < d) + (keys of digit d in tiles < t).
+// Tiles form at most 32 chunks of ch tiles; the histogram kernel adds each tile's
+// counts into its chunk's row (part), one block scans the chunk rows, and one block
+// per chunk walks its tiles (thread d down column d, one contiguous 1 KB row a step).
+// one block: chunk sums -> chunk starts (per digit, then the digit base added)
+__global__ void __launch_bounds__(256) k_radix_coltop(uint64_t *__restrict__ part, uint32_t nchunks)
+{
+    __shared__ uint64_t wsum[4];
+    const uint32_t d = threadIdx.x, w = d >> 6;
+    uint64_t s = 0;
+    for (uint32_t b = 0; b < nchunks; b++) {
+        const uint64_t v = part[(uint64_t)b * 256 + d];
+        part[(uint64_t)b * 256 + d] = s;
+        s += v;
+    }
+    const uint64_t incl = wave_incl_scan(s);
+    if (__lane_id() == 63) wsum[w] = incl;
+    __syncthreads();
+    uint64_t base = incl - s;
+    for (uint32_t q = 0; q < w; q++) base += wsum[q];
+    for (uint32_t b = 0; b < nchunks; b++) part[(uint64_t)b * 256 + d] += base;
+}
+
+__global__ void __launch_bounds__(256) k_radix_colapply(const uint32_t *__restrict__ counts, uint32_t ntiles, uint32_t ch,
+                                                        const uint64_t *__restrict__ part, uint64_t *__restrict__ offs)
+{
+    const uint32_t d = threadIdx.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * ch, t1 = t0 + ch < ntiles ? t0 + ch : ntiles;
+    uint64_t run = part[(uint64_t)blockIdx.x * 256 + d];
+    uint64_t t = t0;
+    for (; t + 4 <= t1; t += 4) {   // four independent loads in flight
+        const uint32_t c0 = counts[t * 256 + d], c1 = counts[(t + 1) * 256 + d];
+        const uint32_t c2 = counts[(t + 2) * 256 + d], c3 = counts[(t + 3) * 256 + d];
+        offs[t * 256 + d] = run;
+        offs[(t + 1) * 256 + d] = run + c0;
+        offs[(t + 2) * 256 + d] = run + c0 + c1;
+        offs[(t + 3) * 256 + d] = run + c0 + c1 + c2;
+        run += (uint64_t)c0 + c1 + c2 + c3;
+    }
+    for (; t < t1; t++) {
+        offs[t * 256 + d] = run;
+        run += counts[t * 256 + d];
+    }
+}
+
+// orders a wave's LDS accesses across its lanes (the wave runs in lockstep; this keeps
+// the compiler from moving LDS accesses across the point)
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stable scatter of one tile. Wave w owns the contiguous quarter [w WT, (w + 1) WT) of
+// the tile and holds it in registers. (1) each wave counts its digits (match-any
+// ballots, one LDS add per distinct digit, wave-private counters); (2) per digit, the
+// tile start (exclusive scan of the tile histogram) plus the counts of the earlier
+// waves give every wave its first slot; (3) each wave places its elements in LDS in
+// order, advancing its own counters; (4) the tile goes out as contiguous runs per
+// digit, so global writes are coalesced. Three block barriers in all; elements of
+// wave w precede those of wave w + 1 within a digit, so the pass is stable.
 template <typename K, bool V>
 __global__ void __launch_bounds__(RS_BLOCK) k_radix_scatter(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                             K *__restrict__ kout, uint32_t *__restrict__ vout, uint64_t n,
                                                             int shift, const uint64_t *__restrict__ offs,
                                                             const uint32_t *__restrict__ counts, uint32_t ntiles)
 {
+    constexpr int NW = RS_BLOCK / 64, WT = RS_TILE / NW, WR = WT / 64;
     __shared__ K sk[RS_TILE];
     __shared__ uint32_t sv[V ? RS_TILE : 1];
-    __shared__ uint32_t lbase[256], lrun[256], wsum[RS_BLOCK / 64];
+    __shared__ uint32_t wcnt[NW][256];   // per wave: digit counts, then next slot per digit
+    __shared__ uint32_t lbase[256], wsum[NW];
     __shared__ uint64_t gbase[256];
-    __shared__ uint32_t wc[RS_BLOCK / 64][256];
     const int tid = threadIdx.x, w = tid >> 6, lane = __lane_id();
     const uint64_t lt = (1ull << lane) - 1ull;
-    // tile-local start of every digit: exclusive scan of the tile's digit counts
-    const uint32_t cnt = counts[(uint64_t)tid * ntiles + blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE, wbase = base + (uint64_t)w * WT;
+    K kk[WR];
+    uint32_t vv[V ? WR : 1];
+#pragma unroll
+    for (int r = 0; r < WR; r++) {
+        const uint64_t i = wbase + (uint64_t)r * 64 + lane;
+        kk[r] = i < n ? kin[i] : (K)0;
+        if (V) vv[r] = i < n ? vin[i] : 0u;
+    }
+    const uint32_t cnt = counts[(uint64_t)blockIdx.x * 256 + tid];
+    gbase[tid] = offs[(uint64_t)blockIdx.x * 256 + tid];
+#pragma unroll
+    for (int q = lane; q < 256; q += 64) wcnt[w][q] = 0;
+    wave_sync();
+    // (1) wave histogram; per round one word: the lane's digit (256 = no element, bits
+    // 0-8), its rank among equal digits (bits 9-15) and, for the lowest such lane, the
+    // number of lanes with that digit (bits 16-23)
+    uint32_t pk[WR];
+#pragma unroll
+    for (int r = 0; r < WR; r++) {
+        const bool valid = wbase + (uint64_t)r * 64 + lane < n;
+        const uint32_t d = (uint32_t)(kk[r] >> shift) & 255u;
+        const uint64_t m = match_digit(valid, d);
+        const uint32_t rank = (uint32_t)__popcll(m & lt);
+        const uint32_t lead = (valid && rank == 0) ? (uint32_t)__popcll(m) : 0u;
+        pk[r] = (valid ? d : 256u) | (rank << 9) | (lead << 16);
+        if (lead) wcnt[w][d] += lead;
+        wave_sync();
+    }
+    // (2) digit tid: tile start, then each wave's first slot
     const uint32_t incl = wave_incl_scan(cnt);
     if (lane == 63) wsum[w] = incl;
-    gbase[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
     __syncthreads();
-    uint32_t excl = incl - cnt;
-    for (int q = 0; q < w; q++) excl += wsum[q];
-    lbase[tid] = excl;
-    lrun[tid] = excl;
-    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_ITEMS; r++) {
+    uint32_t slot = incl - cnt;
+    for (int q = 0; q < w; q++) slot += wsum[q];
+    lbase[tid] = slot;
 #pragma unroll
-        for (int q = 0; q < RS_BLOCK / 64; q++) wc[q][tid] = 0;
-        __syncthreads();
-        const uint64_t i = base + (uint64_t)r * RS_BLOCK + tid;
-        const bool valid = i < n;
-        const K key = valid ? kin[i] : (K)0;
-        const uint32_t d = valid ? ((uint32_t)(key >> shift) & 255u) : 0u;
-        uint64_t m = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(valid && bit);
-            m &= bit ? bal : ~bal;
-        }
-        uint32_t rank = 0;
-        if (valid) {
-            rank = (uint32_t)__popcll(m & lt);
-            if (rank == 0) wc[w][d] = (uint32_t)__popcll(m);
-        }
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = lrun[d] + rank;
-            for (int q = 0; q < w; q++) pos += wc[q][d];
-            sk[pos] = key;
-            if (V) sv[pos] = vin[i];
-        }
-        __syncthreads();
-        uint32_t add = 0;
-#pragma unroll
-        for (int q = 0; q < RS_BLOCK / 64; q++) add += wc[q][tid];
-        lrun[tid] += add;
+    for (int q = 0; q < NW; q++) {
+        const uint32_t c = wcnt[q][tid];
+        wcnt[q][tid] = slot;
+        slot += c;
     }
     __syncthreads();
+    // (3) place the wave's elements in order
+#pragma unroll
+    for (int r = 0; r < WR; r++) {
+        const uint32_t d = pk[r] & 511u, lead = pk[r] >> 16;
+        if (d < 256u) {
+            const uint32_t pos = wcnt[w][d] + ((pk[r] >> 9) & 127u);
+            sk[pos] = kk[r];
+            if (V) sv[pos] = vv[r];
+        }
+        wave_sync();
+        if (lead) wcnt[w][d] += lead;
+        wave_sync();
+    }
+    __syncthreads();
+    // (4) contiguous runs per digit
     const uint32_t tn = (uint32_t)((n - base) < (uint64_t)RS_TILE ? (n - base) : (uint64_t)RS_TILE);
     for (uint32_t p = tid; p < tn; p += RS_BLOCK) {
         const K key = sk[p];
@@ -222,18 +325,24 @@ static K *radix_impl(Ctx &c, K *keys, uint32_t *vals, uint64_t n, int nbits, int
     const uint64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
     const uint64_t cnt_n = 256 * ntiles;
     auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
-    const uint64_t bk = al(n * sizeof(K)), bv = V ? al(n * 4) : 0, bc = al(cnt_n * 4), bo = al(cnt_n * 8);
-    ensure(c, c.radix_tmp, bk + bv + bc + bo);
+    const uint32_t ch = (uint32_t)((ntiles + 31) / 32), nchunks = (uint32_t)((ntiles + ch - 1) / ch);
+    const uint64_t bk = al(n * sizeof(K)), bv = V ? al(n * 4) : 0, bc = al(cnt_n * 4), bo = al(cnt_n * 8),
+                   bp = al(256ull * nchunks * 8);
+    ensure(c, c.radix_tmp, bk + bv + bc + bo + bp);
     uint8_t *t = c.radix_tmp.p;
     K *kt = (K *)t;
     uint32_t *vt = V ? (uint32_t *)(t + bk) : nullptr;
     uint32_t *counts = (uint32_t *)(t + bk + bv);
     uint64_t *offs = (uint64_t *)(t + bk + bv + bc);
+    uint64_t *part = (uint64_t *)(t + bk + bv + bc + bo);
     K *a = keys, *b = kt;
     uint32_t *va = vals, *vb = vt;
     for (int shift = lo; shift < lo + nbits; shift += 8) {
-        k_radix_hist<K><<<(unsigned)ntiles, RS_BLOCK, 0, c.stream>>>(a, n, shift, counts, (uint32_t)ntiles);
-        scan_u32_to_u64(c, counts, offs, cnt_n, nullptr);
+        PFG_HIP(hipMemsetAsync(part, 0, 256ull * nchunks * 8, c.stream));
+        k_radix_hist<K><<<(unsigned)ntiles, RS_BLOCK, 0, c.stream>>>(a, n, shift, counts, ch,
+                                                                     (unsigned long long *)part);
+        k_radix_coltop<<<1, 256, 0, c.stream>>>(part, nchunks);
+        k_radix_colapply<<<nchunks, 256, 0, c.stream>>>(counts, (uint32_t)ntiles, ch, part, offs);
         k_radix_scatter<K, V><<<(unsigned)ntiles, RS_BLOCK, 0, c.stream>>>(a, va, b, vb, n, shift, offs, counts,
                                                                             (uint32_t)ntiles);
         K *tk = a; a = b; b = tk;
