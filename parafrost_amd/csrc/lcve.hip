@@ -182,25 +182,38 @@ __global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32
         bool keep = false;
         uint64_t r = 0;
         if (i < n) {
-            r = active[i];
+            // the records stream through once (non-temporal loads), so the status bytes
+            // they gather stay cached
+            r = __builtin_nontemporal_load(active + i);
             const uint32_t *rr = rec + r;
-            const uint32_t k = rr[0];
+            const uint32_t k = __builtin_nontemporal_load(rr);
             bool in_freezer = false;
             uint32_t m = 0xFFFFFFFFu;
             uint32_t undec = 0, alive = 0;
+            constexpr uint32_t KR = 8;   // undecided candidates of the first KR entries are kept in registers
+            uint32_t und[KR];
+            uint32_t nund = 0;
             for (uint32_t t = 1; t <= k; t++) {
-                const uint32_t x = rr[t], j = x >> 1;
+                const uint32_t x = __builtin_nontemporal_load(rr + t), j = x >> 1;
                 const uint8_t st = A.status[j];
                 const bool freezer = x & 1u;
                 if (st == S_IN && freezer) in_freezer = true;
                 if (st == S_UNDEC) {
                     undec++;
                     if (freezer && j < m) m = j;
+                    if (t <= KR) und[nund++] = j;
                 }
                 if (st == S_UNDEC || (st == S_IN && freezer)) alive++;
             }
             if (undec) {
-                for (uint32_t t = 1; t <= k; t++) {
+                // a candidate seen undecided above that another record has marked out since
+                // gets the same mark again, or a blocked mark that pass B ignores
+                for (uint32_t q = 0; q < nund; q++) {
+                    const uint32_t j = und[q];
+                    if (in_freezer) A.status[j] = S_OUT;
+                    else if (j > m) A.blocked[j] = 1;
+                }
+                for (uint32_t t = KR + 1; t <= k; t++) {
                     const uint32_t j = rr[t] >> 1;
                     if (A.status[j] != S_UNDEC) continue;
                     if (in_freezer) A.status[j] = S_OUT;
