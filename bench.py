@@ -408,10 +408,15 @@ def main():
 
     s = pf.Simplifier(cnf.nvars, pf.default_opts())
 
+    outcome = {"refuted": False}
+
     def step():
         s.reset_for_upload()
         s.upload_device(d_lits.data_ptr(), d_offs.data_ptr(), cnf.ncls, L)
-        s.sigmify()
+        try:
+            s.sigmify()
+        except pf.Unsat:   # c3: the simplifier proves the formula unsatisfiable mid-way
+            outcome["refuted"] = True
 
     for _ in range(args.warmup):
         step()
@@ -424,7 +429,10 @@ def main():
     dt = time.perf_counter() - t0
     times = s.times()
     kt = s.kernel_times()
-    stats = s.stats()
+    try:
+        stats = s.stats()
+    except pf.Unsat:
+        stats = {"clauses": None, "literals": None, "eliminated": None}
 
     # algorithmic bytes per launch: one more, untimed step with accounting on
     # (the step is deterministic, so every launch repeats with the same sizes)
@@ -458,7 +466,9 @@ def main():
                    "input_literals": L, "variables": cnf.nvars, "clauses": cnf.ncls,
                    "passes": "OT+prop+LCVE+sortOT+SUB+BVE(gates)+ERE+GC, 5 phases",
                    "clauses_out": stats["clauses"], "literals_out": stats["literals"],
-                   "eliminated": stats["eliminated"], "pass_ms_per_step": {k: round(v / args.steps, 3) for k, v in times.items()}},
+                   "eliminated": stats["eliminated"], "pass_ms_per_step": {k: round(v / args.steps, 3) for k, v in times.items()},
+                   "outcome": ("refuted: the formula is unsatisfiable; a step is the rounds up to the refutation"
+                               if outcome["refuted"] else "simplified")},
         "roofline": roof,
         "kernels": kernels,
     }

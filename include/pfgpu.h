@@ -164,6 +164,9 @@ typedef struct pfg_kstat {
     uint64_t launches;
     double bytes;
 } pfg_kstat;
+/* Statistics and settings (pfg_kernel_stats, pfg_set_accounting, pfg_set_stage_words,
+ * pfg_reset_stats) return PFG_OK also after the formula was refuted; every other call
+ * reports PFG_EUNSAT from then on. */
 int pfg_kernel_stats(pfg_ctx *ctx, pfg_kstat *out, int max, int *n);
 int pfg_set_accounting(pfg_ctx *ctx, int on);
 /* Tuning: cap the LDS words per block that the staged per-variable kernels (SUB,

@@ -62,7 +62,7 @@ struct DBuf {
 // device counter slots (Ctx::cnt) and flags. Slot CNT_CONF counts contradictions;
 // the variables involved follow it (at most CONF_CAP), so that a batch can tell
 // which instances were refuted.
-enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_NTRAIL = 8, CNT_FLAGS = 9, CNT_ACCT = 10,
+enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_SUBFROM = 5, CNT_NTRAIL = 8, CNT_FLAGS = 9, CNT_ACCT = 10,
        CNT_CONF = 16 };
 enum { FLAG_UNSAT = 1, FLAG_INTERNAL = 2 };
 constexpr uint64_t CONF_CAP = 1ull << 20;

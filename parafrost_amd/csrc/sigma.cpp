@@ -354,6 +354,18 @@ struct pfg_ctx {
         return PFG_ENOMEM;                       \
     }                                            \
     return c.unsat ? PFG_EUNSAT : PFG_OK;
+// bookkeeping calls (statistics, settings) succeed whatever the formula's state
+#define API_END_OK                               \
+    }                                            \
+    catch (const Error &e) {                     \
+        c.last_error = e.what();                 \
+        return e.code;                           \
+    }                                            \
+    catch (const std::bad_alloc &) {             \
+        c.last_error = "host allocation failed"; \
+        return PFG_ENOMEM;                       \
+    }                                            \
+    return PFG_OK;
 
 extern "C" {
 
@@ -844,21 +856,21 @@ int pfg_kernel_stats(pfg_ctx *ctx, pfg_kstat *out, int max, int *n)
         k++;
     }
     if (n) *n = k;
-    API_END
+    API_END_OK
 }
 
 int pfg_set_stage_words(pfg_ctx *ctx, uint32_t words)
 {
     API_BEGIN
     c.stage_words = words;
-    API_END
+    API_END_OK
 }
 
 int pfg_set_accounting(pfg_ctx *ctx, int on)
 {
     API_BEGIN
     c.account = on != 0;
-    API_END
+    API_END_OK
 }
 
 int pfg_reset_stats(pfg_ctx *ctx)
@@ -867,7 +879,7 @@ int pfg_reset_stats(pfg_ctx *ctx)
     resolve_ktimers(c);
     for (auto &k : c.kstat) k = KStat{};
     c.times = pfg_times{};
-    API_END
+    API_END_OK
 }
 
 int pfg_reset(pfg_ctx *ctx)

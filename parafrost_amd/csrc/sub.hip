@@ -12,6 +12,7 @@
 #include "passes.hpp"
 #include "prims.hpp"
 #include "stage.hpp"
+#include <cstdio>
 #include <vector>
 
 namespace pfg {
@@ -27,6 +28,7 @@ struct SubArgs {
     int sub_limit;
     int lbd_tier1;
     int allow_units;
+    int defer_oblivion;   // a unit's clauses are deleted by the caller (k_sub_oblivion)
 };
 
 __device__ __forceinline__ bool sub_cl(const uint32_t *sm, const uint32_t *lr)
@@ -72,7 +74,8 @@ __device__ __forceinline__ void toblivion_list(const SubArgs &A, uint32_t lit)
     o->sz = 0;
 }
 
-__device__ __forceinline__ void strengthen(const SubArgs &A, uint32_t *c, uint32_t me)
+// returns the unit it assigned (0 if none)
+__device__ __forceinline__ uint32_t strengthen(const SubArgs &A, uint32_t *c, uint32_t me)
 {   // strengthen (src/cpu/elimination.cpp:199-231)
     uint32_t sig = 0;
     int n = 0;
@@ -85,14 +88,15 @@ __device__ __forceinline__ void strengthen(const SubArgs &A, uint32_t *c, uint32
     c[1] = sig;
     c[2] = (uint32_t)(sz - 1);
     if (n == 1) {
-        if (!A.allow_units) { atomicOr(&A.cnt[CNT_FLAGS], (unsigned long long)FLAG_INTERNAL); return; }
+        if (!A.allow_units) { atomicOr(&A.cnt[CNT_FLAGS], (unsigned long long)FLAG_INTERNAL); return 0; }
         const uint32_t unit = lits[0];
         const int r = try_assign(A.value, unit);
         if (r == 1) {
             A.vstate[unit >> 1] = FROZEN_M;
             const unsigned long long pos = atomicAdd(&A.cnt[CNT_NTRAIL], 1ull);
             A.trail[pos] = unit;
-            toblivion_list(A, unit);
+            if (!A.defer_oblivion) toblivion_list(A, unit);
+            return unit;
         }
         else if (r < 0) report_conflict(A.cnt, unit >> 1);
     }
@@ -103,6 +107,7 @@ __device__ __forceinline__ void strengthen(const SubArgs &A, uint32_t *c, uint32
             if (nl < old) c[0] = (c[0] & 0xFu) | (1u << 4) | (nl << 6);
         }
     }
+    return 0;
 }
 
 __device__ __forceinline__ void update_ol(const SubArgs &A, uint32_t lit)
@@ -269,6 +274,190 @@ k_sub_staged(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e
     }
 }
 
+// One unit-capable variable by one wave: self_sub_x (subsume.h:140-176) with its list
+// scans split across the 64 lanes. Per list entry of the side's two lists, LDS holds
+// the size with a deleted flag, the signature and, for a binary, its other literal;
+// the current clause's literals are in LDS too. A binary (x|b) subsumes the clause iff
+// b is in it, and a binary (~x|b) strengthens it iff b is in it, so binaries are
+// checked in LDS; longer clauses are checked from the pool. Decisions are those of the
+// sequential loop: a clause is strengthened if some strengthening clause lies before
+// the size break, it is deleted by the first subsuming earlier clause (whose learnt
+// status it promotes), and lane 0 applies every change before the next clause. A unit
+// deletes the clauses of its literal: the wave deletes those on its own lists at once
+// (the only ones its later steps can see), k_sub_oblivion the rest after the variable.
+constexpr uint32_t SW_CAP = 3000;   // lists up to this length (the default sub_limit)
+constexpr uint32_t SW_CL = 128;     // clause literals held in LDS
+constexpr uint16_t SW_DEL = 0x8000u, SW_BIG = 0x7FFFu;
+constexpr uint32_t SW_NONE = 0xFFFFFFFFu;
+
+__device__ __forceinline__ bool sw_has(const uint32_t *lits, uint32_t n, uint32_t x)
+{   // binary search of a sorted literal array
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (lits[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && lits[lo] == x;
+}
+
+__global__ void __launch_bounds__(64) k_sub_wave(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t q)
+{
+    __shared__ uint16_t msz[SW_CAP], osz[SW_CAP];
+    __shared__ uint32_t msig[SW_CAP], osig[SW_CAP], mbin[SW_CAP], obin[SW_CAP];
+    __shared__ uint32_t clit[SW_CL];
+    __shared__ uint32_t s_unit;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t v = pvs[q], p = 2 * v;
+    if (A.ot[p].sz > (uint32_t)A.sub_limit || A.ot[p + 1].sz > (uint32_t)A.sub_limit) return;
+    for (int side = 0; side < 2; side++) {
+        const uint32_t me = side ? p + 1 : p, other = me ^ 1u;
+        const OL om = A.ot[me], oo = A.ot[other];
+        const uint64_t *d = A.occ + om.off, *od = A.occ + oo.off;
+        auto entry = [](const uint32_t *c, uint32_t lit, uint16_t &sz, uint32_t &sig, uint32_t &bin) {
+            const uint32_t n = c[2];
+            sz = (uint16_t)((n < SW_BIG ? n : SW_BIG) | (c_del(c) ? SW_DEL : 0u));
+            sig = c[1];
+            bin = (n == 2 && !c_molten(c)) ? (c[3] ^ c[4] ^ lit) : SW_NONE;   // a strengthened clause lost lit
+        };
+        auto stage = [&]() {
+            for (uint32_t j = lane; j < om.sz; j += 64) entry(A.mem + d[j], me, msz[j], msig[j], mbin[j]);
+            for (uint32_t j = lane; j < oo.sz; j += 64) entry(A.mem + od[j], other, osz[j], osig[j], obin[j]);
+        };
+        stage();
+        if (lane == 0) s_unit = 0;
+        __syncthreads();
+        for (uint32_t i = 0; i < om.sz; i++) {
+            const uint32_t mi = msz[i];
+            if ((mi & SW_BIG) > (uint32_t)HSE_MAX_CL_SIZE) break;
+            if (mi & SW_DEL) continue;
+            uint32_t *cand = A.mem + d[i];
+            uint32_t csz = mi & SW_BIG;
+            for (uint32_t k = lane; k < csz && k < SW_CL; k += 64) clit[k] = cand[3 + k];
+            __syncthreads();
+            // membership of a literal in the current clause
+            auto in_cand = [&](uint32_t b, uint32_t n) {
+                return n <= SW_CL ? sw_has(clit, n, b) : sw_has(cand + 3, n, b);
+            };
+            {   // selfsubsume (subsume.h:119-138): any hit before the first longer clause
+                const uint32_t candsig = msig[i];
+                const bool me_in = in_cand(me, csz);
+                bool found = false;
+                // four chunks of 64 entries per step of the loop, judged in order
+                for (uint32_t b = 0; b < oo.sz && !found; b += 256) {
+                    bool brk[4], hit[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const uint32_t j = b + 64u * u + lane;
+                        brk[u] = hit[u] = false;
+                        if (j < oo.sz) {
+                            const uint32_t sj = osz[j], subsize = sj & SW_BIG;
+                            if (subsize > csz) brk[u] = true;
+                            else if (!(sj & SW_DEL) && subsize > 1 && sig_selfsub(osig[j], candsig))
+                                hit[u] = obin[j] != SW_NONE ? (me_in && in_cand(obin[j], csz))
+                                                            : selfsub_cl(me, other, A.mem + od[j], cand);
+                        }
+                    }
+                    bool stop = false;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        if (stop) continue;
+                        const uint64_t mb = __ballot(brk[u]), mh = __ballot(hit[u]);
+                        const uint64_t before = mb ? ((1ull << (__ffsll((long long)mb) - 1)) - 1ull) : ~0ull;
+                        if (mh & before) { found = true; stop = true; }
+                        else if (mb) stop = true;
+                    }
+                    if (stop) break;
+                }
+                if (found) {
+                    if (lane == 0) {
+                        s_unit = strengthen(A, cand, me);
+                        c_melt(cand);
+                        msz[i] = (uint16_t)c_size(cand);
+                        msig[i] = cand[1];
+                        mbin[i] = SW_NONE;   // it no longer holds me
+                    }
+                    __syncthreads();
+                    csz = msz[i] & SW_BIG;
+                    for (uint32_t k = lane; k < csz && k < SW_CL; k += 64) clit[k] = cand[3 + k];
+                    const uint32_t ul = s_unit;
+                    if (ul) {
+                        // toblivion(ot[unit]) as far as this variable sees it: its clauses that
+                        // hold the unit (the unit's variable is never elected, so these are
+                        // exactly its clauses on that list); k_sub_oblivion deletes the rest
+                        for (uint32_t j = lane; j < om.sz; j += 64) {
+                            uint32_t *c = A.mem + d[j];
+                            if (!c_del(c) && sw_has(c + 3, c_size(c), ul)) { c_markdel(c); msz[j] |= SW_DEL; }
+                        }
+                        for (uint32_t j = lane; j < oo.sz; j += 64) {
+                            uint32_t *c = A.mem + od[j];
+                            if (!c_del(c) && sw_has(c + 3, c_size(c), ul)) { c_markdel(c); osz[j] |= SW_DEL; }
+                        }
+                    }
+                    __syncthreads();
+                    if (lane == 0) s_unit = 0;
+                }
+            }
+            {   // subsume (subsume.h:101-117): the first earlier clause that subsumes it
+                const uint32_t ci = msz[i];
+                const uint32_t candsig = msig[i];
+                const bool molten = c_molten(cand);
+                const bool me_in = in_cand(me, csz);
+                for (uint32_t b = 0; b < i; b += 256) {
+                    bool hit[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const uint32_t j = b + 64u * u + lane;
+                        hit[u] = false;
+                        if (j < i) {
+                            const uint32_t sj = msz[j], sz = sj & SW_BIG;
+                            if (!(sj & SW_DEL) && !(molten && sz > csz) && sz > 1 && sig_sub(msig[j], candsig))
+                                hit[u] = mbin[j] != SW_NONE ? (me_in && in_cand(mbin[j], csz))
+                                                            : sub_cl(A.mem + d[j], cand);
+                        }
+                    }
+                    int first = -1;
+                    uint64_t fm = 0;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const uint64_t mh = __ballot(hit[u]);
+                        if (first < 0 && mh) { first = u; fm = mh; }
+                    }
+                    if (first >= 0) {
+                        if (lane == (uint32_t)(__ffsll((long long)fm) - 1)) {
+                            uint32_t *sg = A.mem + d[b + 64u * (uint32_t)first + lane];
+                            if (c_learnt(sg) && c_orig(cand)) c_setst(sg, ST_ORIGINAL);
+                            if (!c_del(cand)) c_markdel(cand);
+                            msz[i] = (uint16_t)(ci | SW_DEL);
+                        }
+                        break;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (lane == 0) update_ol(A, me);
+        __syncthreads();
+    }
+}
+
+// the rest of toblivion(ot[unit]) for the units assigned since CNT_SUBFROM: every
+// clause on the unit's list is deleted, then the lists are emptied
+__global__ void k_sub_oblivion(SubArgs A)
+{
+    const uint64_t from = A.cnt[CNT_SUBFROM], to = A.cnt[CNT_NTRAIL];
+    for (uint64_t t = from; t < to; t++) {
+        const OL o = A.ot[A.trail[t]];
+        for (uint64_t e = gtid(); e < o.sz; e += gstride()) c_markdel(A.mem + A.occ[o.off + e]);
+    }
+}
+
+__global__ void k_sub_oblivion_sz(SubArgs A)
+{
+    const uint64_t from = A.cnt[CNT_SUBFROM], to = A.cnt[CNT_NTRAIL];
+    for (uint64_t t = from + threadIdx.x; t < to; t += blockDim.x) A.ot[A.trail[t]].sz = 0;
+}
+
 __global__ void k_sub_range(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e)
 {
     for (uint64_t q = gtid() + b; q < e; q += gstride()) {
@@ -276,6 +465,19 @@ __global__ void k_sub_range(SubArgs A, const uint32_t *__restrict__ pvs, uint32_
         const uint32_t p = 2 * v;
         if (A.ot[p].sz <= (uint32_t)A.sub_limit && A.ot[p + 1].sz <= (uint32_t)A.sub_limit) self_sub_x(A, p);
     }
+}
+
+// next live binary at or after position i of a list (its other literal in *a); the
+// binaries of an elected variable's sorted list form its prefix
+__device__ __forceinline__ uint32_t next_binary(const uint64_t *list, uint32_t sz, uint32_t i, const uint32_t *mem,
+                                                uint32_t lit, uint32_t *a)
+{
+    for (; i < sz; i++) {
+        const uint32_t *c = mem + list[i];
+        if (c_size(c) > 2) return sz;
+        if (!c_del(c) && c_size(c) == 2) { *a = c[3] ^ c[4] ^ lit; return i; }
+    }
+    return sz;
 }
 
 // unit-capable: a live binary (x|a) on the positive list and a live binary (~x|a) on the negative list
@@ -288,7 +490,23 @@ __global__ void k_sub_unitcap(const uint32_t *__restrict__ pvs, uint32_t npvs, c
         const OL op = ot[p], on = ot[n];
         uint32_t f = 0;
         if (op.sz <= (uint32_t)sub_limit && on.sz <= (uint32_t)sub_limit) {
-            for (uint32_t i = 0; i < op.sz && !f; i++) {
+            // Lists sorted by (size, first, last literal) order the binaries (x|a) by a,
+            // so a merge of the two binary prefixes finds a common a. If a prefix is out
+            // of that order, every pair is compared instead.
+            const uint64_t *lp = occ + op.off, *ln = occ + on.off;
+            uint32_t a = 0, b = 0, pa = 0, pb = 0;
+            bool ordered = true;
+            uint32_t i = next_binary(lp, op.sz, 0, mem, p, &a), j = next_binary(ln, on.sz, 0, mem, n, &b);
+            bool firsta = true, firstb = true;
+            while (i < op.sz && j < on.sz) {
+                if ((!firsta && a < pa) || (!firstb && b < pb)) { ordered = false; break; }
+                pa = a; pb = b; firsta = firstb = false;
+                if (a == b) { f = 1; break; }
+                if (a < b) i = next_binary(lp, op.sz, i + 1, mem, p, &a);
+                else j = next_binary(ln, on.sz, j + 1, mem, n, &b);
+            }
+            if (!ordered) f = 0;
+            for (uint32_t i = 0; !ordered && i < op.sz && !f; i++) {
                 const uint32_t *c = mem + occ[op.off + i];
                 if (c_del(c) || c_size(c) != 2) continue;
                 const uint32_t a = c[3] ^ c[4] ^ p;
@@ -312,7 +530,7 @@ void sub(Ctx &c)
 {
     if (!c.npvs || !(c.opts.sub_en || c.opts.ve_en)) return;
     SubArgs A{c.mem.p, c.ot.p, c.occ.p, c.value.p, c.vstate.p, c.trail.p, (unsigned long long *)c.cnt.p,
-              c.opts.sub_limit, c.opts.lbd_tier1, 0};
+              c.opts.sub_limit, c.opts.lbd_tier1, 0, 0};
     const uint32_t np = c.npvs;
     ensure(c, c.tmpu32a, np);
     ensure(c, c.tmpu32b, np);
@@ -322,6 +540,7 @@ void sub(Ctx &c)
     k_iota<<<grid_for(np, 256, 4096), 256, 0, c.stream>>>(c.scores.p, np);
     const uint64_t nu = compact_u32(c, c.scores.p, c.tmpu32a.p, np, c.tmpu32b.p);
     std::vector<uint32_t> ucap(nu);
+    if (c.trace) fprintf(stderr, "[pfg] SUB: %u elected, %llu unit-capable\n", np, (unsigned long long)nu);
     if (nu) {
         PFG_HIP(hipMemcpyAsync(ucap.data(), c.tmpu32b.p, nu * 4, hipMemcpyDeviceToHost, c.stream));
         PFG_HIP(hipStreamSynchronize(c.stream));
@@ -339,7 +558,15 @@ void sub(Ctx &c)
         A.allow_units = 0;
         run_range(u);
         A.allow_units = 1;
-        k_sub_range<<<1, 1, 0, c.stream>>>(A, c.pvs.p, u, u + 1);
+        if (c.opts.sub_limit <= (int)SW_CAP) {
+            PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_SUBFROM, c.cnt.p + CNT_NTRAIL, 8, hipMemcpyDeviceToDevice, c.stream));
+            A.defer_oblivion = 1;
+            k_sub_wave<<<1, 64, 0, c.stream>>>(A, c.pvs.p, u);
+            k_sub_oblivion<<<256, 256, 0, c.stream>>>(A);
+            k_sub_oblivion_sz<<<1, 64, 0, c.stream>>>(A);
+            A.defer_oblivion = 0;
+        }
+        else k_sub_range<<<1, 1, 0, c.stream>>>(A, c.pvs.p, u, u + 1);
         b = u + 1;
     }
     A.allow_units = 0;
