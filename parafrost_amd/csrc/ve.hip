@@ -397,9 +397,39 @@ __device__ __forceinline__ bool xor_variants_possible(const VeArgs &A, uint32_t 
     return false;
 }
 
+// Original clauses of size >= 3 on both lists counted per hash bucket of (size,
+// variable sequence): 64 buckets of 3-bit saturating counters, bit-sliced. A parity
+// variant of a clause has its size and variable sequence, so a bucket count is at
+// least the number of variants of any clause hashed to it.
+struct SeqCounts {
+    uint64_t b0 = 0, b1 = 0, b2 = 0;
+    __device__ __forceinline__ void add(uint32_t b)
+    {
+        const uint64_t m = 1ull << b;
+        if (b0 & b1 & b2 & m) return;   // saturated at 7
+        const uint64_t c0 = b0 & m;
+        b0 ^= m;
+        const uint64_t c1 = b1 & c0;
+        b1 ^= c0;
+        b2 ^= c1;
+    }
+    __device__ __forceinline__ uint32_t get(uint32_t b) const
+    {
+        return (uint32_t)(((b0 >> b) & 1u) | (((b1 >> b) & 1u) << 1) | (((b2 >> b) & 1u) << 2));
+    }
+};
+
+__device__ __forceinline__ uint32_t seq_bucket(const uint32_t *c, int size)
+{
+    uint32_t h = (uint32_t)size * 0x9E3779B1u;
+    for (int k = 0; k < size; k++) h = (h ^ (c[3 + k] >> 1)) * 0x85EBCA6Bu;
+    return h >> 26;
+}
+
 // counts of original clauses of sizes 3..8 on both lists, 8 bits each (saturating;
-// the XOR thresholds up to size 8 are at most 128); *molten: a clause of size > 2 is molten
-__device__ __forceinline__ uint64_t xor_size_counts(const VeArgs &A, uint32_t dx, bool &molten)
+// the XOR thresholds up to size 8 are at most 128), the per-sequence bucket counts;
+// *molten: a clause of size > 2 is molten
+__device__ __forceinline__ uint64_t xor_size_counts(const VeArgs &A, uint32_t dx, bool &molten, SeqCounts &seq)
 {
     uint64_t packed = 0;
     molten = false;
@@ -408,9 +438,12 @@ __device__ __forceinline__ uint64_t xor_size_counts(const VeArgs &A, uint32_t dx
         for (uint32_t i = 0; i < sz; i++) {
             const uint32_t *c = cl_at(A, l, i);
             const uint32_t cs = (uint32_t)c_size(c);
-            if (c_orig(c) && cs >= 3u && cs <= 8u) {
-                const uint32_t sh = (cs - 3u) * 8u;
-                if (((packed >> sh) & 0xFFu) != 0xFFu) packed += 1ull << sh;
+            if (c_orig(c) && cs >= 3u) {
+                seq.add(seq_bucket(c, (int)cs));
+                if (cs <= 8u) {
+                    const uint32_t sh = (cs - 3u) * 8u;
+                    if (((packed >> sh) & 0xFFu) != 0xFFu) packed += 1ull << sh;
+                }
             }
             if (cs > 2u && c_molten(c)) molten = true;
         }
@@ -423,7 +456,8 @@ __device__ __forceinline__ bool find_xor_gate(const VeArgs &A, uint32_t dx, int 
     const uint32_t fx = dx ^ 1u, v = dx >> 1;
     const uint32_t is = A.ot[dx].sz;
     bool molten = false;
-    const uint64_t counts = xor_size_counts(A, dx, molten);
+    SeqCounts seq;
+    const uint64_t counts = xor_size_counts(A, dx, molten, seq);
     uint32_t lits[21];
     for (uint32_t i = 0; i < is; i++) {
         uint32_t *ci = cl_at(A, dx, i);
@@ -433,6 +467,10 @@ __device__ __forceinline__ bool find_xor_gate(const VeArgs &A, uint32_t dx, int 
         bool possible;
         if (size <= 8) possible = ((counts >> ((size - 3) * 8)) & 0xFFu) >= (1u << (size - 1));
         else possible = xor_possible(A, dx, fx, size);
+        if (possible) {   // the bucket of ci's sequence bounds its variants' count from above
+            const uint32_t bc = seq.get(seq_bucket(ci, size));
+            if (bc < 7u && bc < (1u << (size - 1))) possible = false;
+        }
         if (possible) possible = xor_variants_possible(A, dx, ci, size);
         if (!possible) {
             if (molten) freeze_arities(A, dx, fx);   // the failing attempt's only lasting effect
