@@ -62,23 +62,37 @@ __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, co
                            const OL *__restrict__ ot, InstArgs I,
                            uint8_t *__restrict__ kind, uint8_t *__restrict__ status, uint8_t *__restrict__ blocked)
 {
-    for (uint64_t i = gtid(); i < nvars; i += gstride()) {
-        const uint32_t v = elig[i];
-        const uint32_t inst = find_inst(I.ibase, I.ninst, v);
-        const uint8_t mi = I.imu[inst];
-        const uint32_t ps = hist[2 * v], ns = hist[2 * v + 1];
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < nvars; b += gstride()) {
+        const uint64_t i = b + threadIdx.x;
         uint8_t k = K_SKIP;
-        // iassumed (lcve.cpp:57) and eliminated / assigned (lcve.cpp:58) variables are passed over
-        if (mi != IMU_OFF && !ifrozen[v] && !vstate[v] && (ps || ns)) {
-            const uint32_t pt = I.mu_pos << mi, nt = I.mu_neg << mi;
-            const bool hot = ot[2 * v].sz >= pt && ot[2 * v + 1].sz >= nt;
-            const bool freezable = ps < pt || ns < nt;
-            if (hot && !freezable) { k = K_BREAK; atomicMin(&I.ib0[inst], (unsigned long long)i); }
-            else k = hot ? K_DYNBREAK : K_CAND;
+        uint32_t inst = 0;
+        if (i < nvars) {
+            const uint32_t v = elig[i];
+            inst = find_inst(I.ibase, I.ninst, v);
+            const uint8_t mi = I.imu[inst];
+            const uint32_t ps = hist[2 * v], ns = hist[2 * v + 1];
+            // iassumed (lcve.cpp:57) and eliminated / assigned (lcve.cpp:58) variables are passed over
+            if (mi != IMU_OFF && !ifrozen[v] && !vstate[v] && (ps || ns)) {
+                const uint32_t pt = I.mu_pos << mi, nt = I.mu_neg << mi;
+                const bool hot = ot[2 * v].sz >= pt && ot[2 * v + 1].sz >= nt;
+                const bool freezable = ps < pt || ns < nt;
+                k = (hot && !freezable) ? K_BREAK : hot ? K_DYNBREAK : K_CAND;
+            }
+            kind[i] = k;
+            status[i] = S_UNDEC;
+            blocked[i] = 0;
         }
-        kind[i] = k;
-        status[i] = S_UNDEC;
-        blocked[i] = 0;
+        // static break = first K_BREAK position per instance: lanes hold consecutive
+        // positions, so the lowest such lane of each instance in the wave takes the
+        // atomic for its group
+        uint64_t m = __ballot(k == K_BREAK);
+        while (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            const uint32_t li = __shfl(inst, leader, 64);
+            const uint64_t grp = __ballot(k == K_BREAK && inst == li);
+            if (__lane_id() == leader) atomicMin(&I.ib0[li], (unsigned long long)i);
+            m &= ~grp;
+        }
     }
 }
 
