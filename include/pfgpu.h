@@ -144,6 +144,16 @@ int pfg_hist(pfg_ctx *ctx, uint32_t *hist);
 /* state download (cacheCNF / cacheUnits / cacheResolved / cacheEliminated, transfer.cu:104-179).
  * Sizes are returned through the size pointers; a NULL buffer only queries the size. */
 int pfg_download_cnf(pfg_ctx *ctx, uint32_t *words, uint64_t *nwords, uint64_t *ncls);
+/* ParaFROST::map(true) after sigmify (src/cpu/vmap.cpp:64-131, vmap.h:44-171; write-back
+ * through newBeginning, simplify.cpp:247-265): the variable mapping computed on the device.
+ * Variables without a state keep their order; the first variable that has a state and a
+ * value (a level-0 unit) is kept too and stands for every assigned literal; the rest are
+ * dropped. words: the clauses as SCLAUSE images in clause order with renumbered literals
+ * and their signatures; vorg[nvars_new + 1]: new variable -> context variable; trail: the
+ * mapped units; vstate[nvars_new + 1]. The context itself is not changed. NULL buffers
+ * query the sizes. Single-instance contexts only. */
+int pfg_download_mapped(pfg_ctx *ctx, uint32_t *words, uint64_t *nwords, uint64_t *ncls, uint32_t *vorg,
+                        uint32_t *nvars_new, uint32_t *trail, uint32_t *ntrail, uint8_t *vstate);
 int pfg_download_trail(pfg_ctx *ctx, uint32_t *lits, uint32_t *n);
 int pfg_download_resolved(pfg_ctx *ctx, uint32_t *words, uint64_t *n);
 int pfg_download_vstate(pfg_ctx *ctx, uint8_t *vstate);          /* nvars + 1 bytes */

@@ -1410,6 +1410,65 @@ uint64_t pfo_dump_cnf(const pfo *s, uint32_t *out)
     return w;
 }
 
+/* Variable mapping with write-back after the simplifier: VMAP::initiate / mapLit /
+ * mapShrinkLits / mapShrinkVars (src/cpu/vmap.h:44-170) as ParaFROST::map(true) applies
+ * them (vmap.cpp:64-131; clauses through newBeginning -> newClause -> mapClause,
+ * simplify.cpp:247-265, sclause.cpp:22-33). Variables without a state are kept in
+ * order; the first variable that has a state and a value (a level-0 unit) is kept as
+ * well and stands for every assigned literal. The state itself is not changed.
+ * Outputs: live clauses in clause order (lits, offs[ncls + 1]), vorg[nvars_new + 1]
+ * (new variable -> variable), the mapped trail, vstate[nvars_new + 1].
+ * Returns the clause count. */
+uint64_t pfo_vmap(const pfo *s, uint32_t *lits, uint64_t *offs, uint32_t *vorg, uint32_t *nvars_new, uint32_t *trail,
+                  uint32_t *ntrail, uint8_t *vstate)
+{
+    uint32_t *mapped = (uint32_t *)calloc((size_t)s->nvars + 1, sizeof(uint32_t));
+    uint32_t nv = 0, first = 0, mfirst = 0;
+    int8_t vfirst = -1;
+    for (uint32_t v = 1; v <= s->nvars; v++) {   /* initiate (vmap.h:61-72) */
+        if (!s->vstate[v]) mapped[v] = ++nv;
+        else if (!first && s->value[2 * v] >= 0) {
+            first = v;
+            vfirst = s->value[2 * v];
+            mapped[v] = ++nv;
+            mfirst = nv;
+        }
+    }
+    uint64_t n = 0, ncls = 0;
+    offs[0] = 0;
+    for (uint64_t i = 0; i < s->nscnf; i++) {   /* mapClause over the written-back clauses */
+        const uint32_t *c = CL(s, s->scnf[i]);
+        if (c_del(c)) continue;
+        for (int k = 0; k < c_size(c); k++) {
+            const uint32_t lit = c[3 + k], v = lit >> 1;
+            uint32_t out;
+            if (mapped[v] && v != first) out = 2 * mapped[v] + (lit & 1u);   /* mapLit (vmap.h:44-60) */
+            else if (s->value[lit] < 0) out = 0;
+            else {
+                out = 2 * mfirst;
+                if (vfirst != s->value[lit]) out ^= 1u;
+            }
+            lits[n++] = out;
+        }
+        offs[++ncls] = n;
+    }
+    vorg[0] = 0;   /* mapShrinkVars(vorg) with vorg the identity */
+    for (uint32_t v = 1; v <= s->nvars; v++)
+        if (mapped[v]) vorg[mapped[v]] = v;
+    uint32_t nt = 0;   /* mapShrinkLits(trail) (vmap.h:162-171) */
+    for (uint32_t t = 0; t < s->ntrail; t++) {
+        const uint32_t lit = s->trail[t], m = mapped[lit >> 1];
+        if (m) trail[nt++] = 2 * m + (lit & 1u);
+    }
+    vstate[0] = 0;   /* mapSP: vstate of the kept variables */
+    for (uint32_t v = 1; v <= s->nvars; v++)
+        if (mapped[v]) vstate[mapped[v]] = s->vstate[v];
+    free(mapped);
+    *nvars_new = nv;
+    *ntrail = nt;
+    return ncls;
+}
+
 uint32_t pfo_trail(const pfo *s, uint32_t *out)
 {
     if (out) memcpy(out, s->trail, sizeof(uint32_t) * s->ntrail);

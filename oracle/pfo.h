@@ -75,6 +75,13 @@ void pfo_shrink(pfo *s);              /* shrinkSimp (src/cpu/simplify.cpp:216-22
 void pfo_hist(pfo *s, uint32_t *hist);/* histSimp   (src/cpu/histogram.cpp:80-95) per literal */
 void pfo_next_phase(pfo *s);          /* phase++, mu_inc update (simplify.cpp:197-199) */
 
+/* variable mapping with write-back after the simplifier (ParaFROST::map(true),
+ * src/cpu/vmap.cpp:64-131, vmap.h:44-171); the state is not changed. Buffers: lits
+ * (live literals), offs (live clauses + 1), vorg and vstate (nvars + 1), trail (ntrail).
+ * Returns the mapped clause count. */
+uint64_t pfo_vmap(const pfo *s, uint32_t *lits, uint64_t *offs, uint32_t *vorg, uint32_t *nvars_new, uint32_t *trail,
+                  uint32_t *ntrail, uint8_t *vstate);
+
 /* state access */
 int      pfo_unsat(const pfo *s);
 int32_t  pfo_phase(const pfo *s);

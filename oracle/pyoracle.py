@@ -6,6 +6,7 @@ benchmark's CPU-baseline leg; never by the product.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import os
 import struct
 import subprocess
@@ -47,6 +48,8 @@ def lib():
         L.pfo_enqueue.restype = ctypes.c_int
         L.pfo_ifreeze.argtypes = [vp, ctypes.c_uint32, ctypes.c_int]
         L.pfo_ifreeze.restype = ctypes.c_int
+        L.pfo_vmap.restype = ctypes.c_uint64
+        L.pfo_vmap.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint32), vp, ctypes.POINTER(ctypes.c_uint32), vp]
         for f in ("pfo_sigmify", "pfo_prop", "pfo_lcve", "pfo_unsat", "pfo_phase"):
             getattr(L, f).restype = ctypes.c_int
             getattr(L, f).argtypes = [vp]
@@ -110,6 +113,44 @@ class State:
         return out
 
 
+@dataclasses.dataclass
+class Mapped:
+    """a mapped write-back: clauses (CSR) over variables 1..nvars, vorg[new] = old"""
+    nvars: int
+    lits: np.ndarray
+    offs: np.ndarray
+    vorg: np.ndarray
+    trail: np.ndarray
+    vstate: np.ndarray
+
+
+def read_mapped(path: str) -> Mapped:
+    """the harness's mapped dump (PFREF_MAP)"""
+    b = open(path, "rb").read()
+    off = 0
+
+    def take(fmt, n=1):
+        nonlocal off
+        a = np.frombuffer(b, dtype=fmt, count=n, offset=off)
+        off += a.nbytes
+        return a
+    assert int(take(np.uint32)[0]) == 0x4d524650
+    nv = int(take(np.uint32)[0])
+    nc = int(take(np.uint64)[0])
+    lits, offs = [], [0]
+    for _ in range(nc):
+        k = int(take(np.uint32)[0])
+        lits.append(take(np.uint32, k))
+        offs.append(offs[-1] + k)
+    nvo = int(take(np.uint32)[0])
+    vorg = take(np.uint32, nvo).copy()
+    nt = int(take(np.uint32)[0])
+    trail = take(np.uint32, nt).copy()
+    vs = take(np.uint8, nv + 1).copy()
+    return Mapped(nv, np.concatenate(lits) if lits else np.zeros(0, np.uint32), np.array(offs, np.uint64), vorg,
+                  trail, vs)
+
+
 class Oracle:
     def __init__(self, cnf, opts: Opts | None = None, hdr0=None):
         self.L = lib()
@@ -132,6 +173,22 @@ class Oracle:
         for v in np.asarray(vars, np.uint32).ravel():
             if self.L.pfo_ifreeze(self.h, int(v), 1 if on else 0) != 0:
                 raise ValueError(f"variable {int(v)} out of range")
+
+    def vmap(self) -> "Mapped":
+        """variable mapping with write-back after sigmify (ParaFROST::map(true))"""
+        st = self.state()
+        nl = int(sum(len(c[2]) for c in st.clauses()))
+        lits = np.zeros(max(1, nl), np.uint32)
+        offs = np.zeros(len(st.clauses()) + 1, np.uint64)
+        vorg = np.zeros(self.nvars + 1, np.uint32)
+        trail = np.zeros(max(1, len(st.trail)), np.uint32)
+        vs = np.zeros(self.nvars + 1, np.uint8)
+        nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+        nc = self.L.pfo_vmap(self.h, lits.ctypes.data, offs.ctypes.data, vorg.ctypes.data, ctypes.byref(nv),
+                             trail.ctypes.data, ctypes.byref(nt), vs.ctypes.data)
+        n = int(nv.value)
+        return Mapped(n, lits[:int(offs[nc])].copy(), offs[:nc + 1].copy(), vorg[:n + 1].copy(),
+                      trail[:nt.value].copy(), vs[:n + 1].copy())
 
     def enqueue(self, lits) -> bool:
         """assign units before sigmify; False on a contradiction"""
@@ -215,9 +272,13 @@ def ref_opts_args(o: Opts) -> list[str]:
 
 
 def run_reference(cnf_path: str, out_path: str, o: Opts, trace_prefix: str | None = None,
-                  timeout: float = 600, freeze=None) -> State:
-    """freeze: variables frozen as by ifreeze() in incremental mode"""
+                  timeout: float = 600, freeze=None, mapped: bool = False):
+    """freeze: variables frozen as by ifreeze() in incremental mode; mapped: run the
+    reference's variable mapping with write-back after the simplifier and return its
+    Mapped dump (None when the formula is refuted)"""
     env = dict(os.environ)
+    if mapped:
+        env["PFREF_MAP"] = "1"
     if trace_prefix:
         env["PFREF_TRACE"] = trace_prefix
     if freeze is not None:
@@ -234,4 +295,8 @@ def run_reference(cnf_path: str, out_path: str, o: Opts, trace_prefix: str | Non
             return State(0, np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint32),
                          np.zeros(0, np.uint8), np.zeros(0, np.uint32), unsat=1)
         raise RuntimeError(f"reference run failed ({r.returncode}): {r.stdout[-500:]!r} {r.stderr[-500:]!r}")
+    if mapped:
+        with open(out_path, "rb") as f:
+            magic = f.read(4)
+        return read_mapped(out_path) if magic == b"PFRM" else None
     return read_dump(out_path)

@@ -9,6 +9,12 @@
  * usage: pfref <input.cnf> <out.bin> [reference options, e.g. --phases=3]
  * env PFREF_TRACE=<prefix>: also dump the state after every pass to
  *   <prefix>.<round>.<pass>.bin  (pass in prop, lcve, sort, ere, sub, ve)
+ * env PFREF_MAP=1: after the simplifier, run the reference's variable mapping with
+ *   write-back (ParaFROST::map(true), src/cpu/vmap.cpp:64-131; newBeginning,
+ *   simplify.cpp:247-265) and write a mapped dump instead:
+ *   u32 'PFRM', u32 nvars_new, u64 nclauses, per clause (u32 size, lits[]),
+ *   u32 nvorg, vorg[] (new variable -> original), u32 ntrail, trail[],
+ *   u8 vstate[nvars_new + 1]
  * env PFREF_FREEZE=<file>: u32 variables (little endian) frozen as by ifreeze()
  *   (src/cpu/assume.cpp:45-52) with the solver in incremental mode, so that
  *   iassumed() (src/cpu/solve.h:864-867) holds for them during LCVE
@@ -90,6 +96,29 @@ static void dump(ParaFROST* s, const std::string& path, bool with_lists, uint32_
 	fclose(f);
 }
 
+static void dump_mapped(ParaFROST* s, const std::string& path)
+{
+	FILE* f = fopen(path.c_str(), "wb");
+	if (!f) { perror(path.c_str()); exit(3); }
+	put32(f, 0x4d524650u);
+	put32(f, inf.maxVar);
+	put64(f, s->orgs.size());
+	for (uint32_t i = 0; i < s->orgs.size(); i++) {
+		CLAUSE& c = s->cm[s->orgs[i]];
+		put32(f, uint32_t(c.size()));
+		for (int k = 0; k < c.size(); k++) put32(f, c[k]);
+	}
+	put32(f, s->vorg.size());
+	for (uint32_t i = 0; i < s->vorg.size(); i++) put32(f, s->vorg[i]);
+	put32(f, s->trail.size());
+	for (uint32_t i = 0; i < s->trail.size(); i++) put32(f, s->trail[i]);
+	for (uint32_t v = 0; v <= inf.maxVar; v++) {
+		uint8_t st = v ? uint8_t(s->sp->vstate[v].state) : 0;
+		fwrite(&st, 1, 1, f);
+	}
+	fclose(f);
+}
+
 int main(int argc, char** argv)
 {
 	if (argc < 3) { fprintf(stderr, "usage: pfref <input.cnf> <out.bin> [options]\n"); return 2; }
@@ -155,6 +184,11 @@ int main(int argc, char** argv)
 	if (!s->prop()) { dump(s, out, false, 1); return 0; }
 	s->countFinal();
 	s->shrinkSimp();
+	if (getenv("PFREF_MAP")) {
+		s->map(true);
+		dump_mapped(s, out);
+		return 0;
+	}
 	dump(s, out, false, 0);
 	return 0;
 }

@@ -50,7 +50,7 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_download_resolved", "pfg_download_vstate", "pfg_download_pvs", "pfg_download_ot_list",
        "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
        "pfg_set_accounting", "pfg_set_stage_words", "pfg_create_batch", "pfg_batch_ninst", "pfg_batch_relayout", "pfg_upload_batch",
-       "pfg_sigmify_batch", "pfg_batch_download", "pfg_ifreeze", "pfg_batch_export"]
+       "pfg_sigmify_batch", "pfg_batch_download", "pfg_ifreeze", "pfg_batch_export", "pfg_download_mapped"]
 
 
 def lib_path() -> str:
@@ -78,6 +78,8 @@ def lib():
         L.pfg_upload_cnf_device.argtypes = [vp, vp, vp, u64, u64]
         L.pfg_enqueue_units.argtypes = [vp, vp, u32]
         L.pfg_ifreeze.argtypes = [vp, vp, u32, i32]
+        L.pfg_download_mapped.argtypes = [vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64), vp, ctypes.POINTER(u32), vp,
+                                          ctypes.POINTER(u32), vp]
         for f in ("pfg_destroy", "pfg_sigmify", "pfg_resize_cnf", "pfg_create_ot", "pfg_reduce_ot", "pfg_prop",
                   "pfg_sort_ot", "pfg_sub", "pfg_ve", "pfg_ere", "pfg_filter_pvs", "pfg_gc", "pfg_next_phase",
                   "pfg_is_unsat"):
@@ -182,6 +184,21 @@ class Simplifier:
     def enqueue_units(self, lits):
         a = np.ascontiguousarray(lits, np.uint32)
         self._check(self.L.pfg_enqueue_units(self.h, a.ctypes.data, len(a)))
+
+    def mapped(self):
+        """ParaFROST::map(true) write-back after sigmify, computed on the device: (clause
+        images with renumbered literals, vorg[new] = variable, mapped trail, vstate)."""
+        nw, nc = ctypes.c_uint64(), ctypes.c_uint64()
+        nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self.L.pfg_download_mapped(self.h, None, ctypes.byref(nw), ctypes.byref(nc), None, ctypes.byref(nv),
+                                               None, ctypes.byref(nt), None))
+        w = np.zeros(max(1, nw.value), np.uint32)
+        vo = np.zeros(nv.value + 1, np.uint32)
+        tr = np.zeros(max(1, nt.value), np.uint32)
+        vs = np.zeros(nv.value + 1, np.uint8)
+        self._check(self.L.pfg_download_mapped(self.h, w.ctypes.data, ctypes.byref(nw), ctypes.byref(nc), vo.ctypes.data,
+                                               ctypes.byref(nv), tr.ctypes.data, ctypes.byref(nt), vs.ctypes.data))
+        return w[:nw.value], vo, tr[:nt.value], vs
 
     def ifreeze(self, vars, on: bool = True):
         """ifreeze / iunfreeze (src/cpu/assume.cpp:45-61): LCVE never elects these variables."""

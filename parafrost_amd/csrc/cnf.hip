@@ -726,6 +726,159 @@ void batch_records(Ctx &c, uint32_t *out, std::vector<uint64_t> &start)
     PFG_HIP(hipStreamSynchronize(c.stream));
 }
 
+// ------------------------------------------------------------------ variable mapping
+// ParaFROST::map(true) after the simplifier (src/cpu/vmap.cpp:64-131, vmap.h:44-171):
+// variables without a state keep their order; the first variable that has a state
+// and a value (a level-0 unit) is kept too and stands for every assigned literal.
+__global__ void k_vmap_flags(const uint8_t *__restrict__ vstate, const int8_t *__restrict__ value, uint32_t nvars,
+                             uint32_t *__restrict__ flag, unsigned long long *first)
+{
+    for (uint64_t i = gtid(); i < nvars; i += gstride()) {
+        const uint32_t v = (uint32_t)i + 1;
+        const bool keep = !vstate[v];
+        flag[i] = keep;
+        if (!keep && value[2 * v] >= 0) atomicMin(first, (unsigned long long)v);
+    }
+}
+
+__global__ void k_vmap_first(uint32_t *__restrict__ flag, const unsigned long long *first)
+{
+    if (gtid() == 0 && *first != ~0ull) flag[*first - 1] = 1;
+}
+
+// mapped[v] = new variable (0: dropped); vorg[new] = v; vstate of the kept variables
+__global__ void k_vmap_ids(const uint32_t *__restrict__ flag, const uint64_t *__restrict__ pos, uint32_t nvars,
+                           const uint8_t *__restrict__ vstate, uint32_t *__restrict__ mapped, uint32_t *__restrict__ vorg,
+                           uint8_t *__restrict__ vs_out)
+{
+    for (uint64_t i = gtid(); i < nvars; i += gstride()) {
+        const uint32_t v = (uint32_t)i + 1;
+        if (flag[i]) {
+            const uint32_t m = (uint32_t)pos[i] + 1;
+            mapped[v] = m;
+            vorg[m] = v;
+            vs_out[m] = vstate[v];
+        }
+        else mapped[v] = 0;
+    }
+}
+
+// mapLit (vmap.h:44-60)
+__device__ __forceinline__ uint32_t vmap_lit(uint32_t lit, const uint32_t *mapped, const int8_t *value, uint32_t first,
+                                             uint32_t mfirst, int8_t vfirst)
+{
+    const uint32_t v = lit >> 1, m = mapped[v];
+    if (m && v != first) return 2 * m + (lit & 1u);
+    const int8_t val = value[lit];
+    if (val < 0) return 0;
+    return (2 * mfirst) ^ (vfirst != val ? 1u : 0u);
+}
+
+__global__ void k_vmap_sizes(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                             uint32_t *__restrict__ sizes)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t *c = mem + refs[i];
+        sizes[i] = c_del(c) ? 0u : 3u + (uint32_t)c_size(c);
+    }
+}
+
+// mapped clause images in clause order (mapClause; signature of the new literals)
+__global__ void k_vmap_write(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                             const uint64_t *__restrict__ woff, const uint32_t *__restrict__ mapped,
+                             const int8_t *__restrict__ value, const unsigned long long *first_p,
+                             uint32_t *__restrict__ out)
+{
+    const uint32_t first = *first_p == ~0ull ? 0u : (uint32_t)*first_p;
+    const uint32_t mfirst = first ? mapped[first] : 0u;
+    const int8_t vfirst = first ? value[2 * first] : (int8_t)-1;
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t *c = mem + refs[i];
+        if (c_del(c)) continue;
+        uint32_t *o = out + woff[i];
+        const int sz = c_size(c);
+        uint32_t sig = 0;
+        for (int k = 0; k < sz; k++) {
+            const uint32_t l = vmap_lit(c[3 + k], mapped, value, first, mfirst, vfirst);
+            o[3 + k] = l;
+            sig |= maphash(l);
+        }
+        o[0] = c[0];
+        o[1] = sig;
+        o[2] = (uint32_t)sz;
+    }
+}
+
+// mapShrinkLits (vmap.h:162-171) of the trail: the trail holds assigned variables, of
+// which only the first one with a state is kept, so at most one literal survives and
+// its position needs no order
+__global__ void k_vmap_trail(const uint32_t *__restrict__ trail, uint32_t n, const uint32_t *__restrict__ mapped,
+                             uint32_t *__restrict__ out, unsigned long long *cnt)
+{
+    for (uint64_t t = gtid(); t < n; t += gstride()) {
+        const uint32_t lit = trail[t], m = mapped[lit >> 1];
+        if (m) out[atomicAdd(cnt, 1ull)] = 2 * m + (lit & 1u);
+    }
+}
+
+void vmap_download(Ctx &c, std::vector<uint32_t> &words, uint64_t &ncls, std::vector<uint32_t> &vorg, uint32_t &nvars_new,
+                   std::vector<uint32_t> &trail, std::vector<uint8_t> &vstate)
+{
+    const uint32_t nv = c.nvars;
+    const uint64_t n = c.nrefs;
+    c.cache_gen = ~0ull;   // mem2 is reused: per-instance downloads must regroup
+    ensure(c, c.tmpu32a, (uint64_t)nv + 2);
+    ensure(c, c.tmpu32b, (uint64_t)nv + 2);
+    ensure(c, c.tmpu32c, (uint64_t)nv + 2 + c.ntrail);
+    ensure(c, c.tmpu8, (uint64_t)nv + 2);
+    ensure(c, c.tmpu64a, (uint64_t)std::max<uint64_t>(nv, n) + 2);
+    ensure(c, c.tmpu64c, 4);
+    unsigned long long *first = (unsigned long long *)c.tmpu64c.p;
+    PFG_HIP(hipMemsetAsync(first, 0xFF, 8, c.stream));
+    uint32_t *flag = c.tmpu32a.p, *mapped = c.tmpu32b.p, *d_vorg = c.tmpu32c.p, *d_trail = c.tmpu32c.p + nv + 2;
+    uint8_t *d_vs = c.tmpu8.p;
+    PFG_HIP(hipMemsetAsync(d_vorg, 0, 4, c.stream));
+    PFG_HIP(hipMemsetAsync(d_vs, 0, 1, c.stream));
+    PFG_HIP(hipMemsetAsync(mapped, 0, 4, c.stream));
+    if (nv) {
+        k_vmap_flags<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.vstate.p, c.value.p, nv, flag, first);
+        k_vmap_first<<<1, 64, 0, c.stream>>>(flag, first);
+        scan_u32_to_u64(c, flag, c.tmpu64a.p, nv, c.cnt.p + CNT_TOTAL);
+        k_vmap_ids<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(flag, c.tmpu64a.p, nv, c.vstate.p, mapped, d_vorg, d_vs);
+    }
+    read_counters(c, 1);
+    nvars_new = nv ? (uint32_t)c.hcnt[CNT_TOTAL] : 0u;
+    // clause images
+    ensure(c, c.tmpu32d, n + 1);
+    ensure(c, c.mem2, c.msz + 1);
+    uint64_t nw = 0;
+    ncls = 0;
+    if (n) {
+        k_vmap_sizes<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu32d.p);
+        scan_u32_to_u64(c, c.tmpu32d.p, c.tmpu64a.p, n, c.cnt.p + CNT_TOTAL);
+        k_vmap_write<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu64a.p, mapped, c.value.p,
+                                                                    first, c.mem2.p);
+        read_counters(c, 1);
+        nw = c.hcnt[CNT_TOTAL];
+    }
+    unsigned long long *tcnt = (unsigned long long *)(c.tmpu64c.p + 1);
+    PFG_HIP(hipMemsetAsync(tcnt, 0, 8, c.stream));
+    if (c.ntrail) k_vmap_trail<<<grid_for(c.ntrail, 256, 1024), 256, 0, c.stream>>>(c.trail.p, c.ntrail, mapped, d_trail, tcnt);
+    uint64_t nt = 0;
+    PFG_HIP(hipMemcpyAsync(&nt, tcnt, 8, hipMemcpyDeviceToHost, c.stream));
+    words.resize(nw);
+    vorg.resize((uint64_t)nvars_new + 1);
+    vstate.resize((uint64_t)nvars_new + 1);
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    trail.resize(nt);
+    if (nw) PFG_HIP(hipMemcpyAsync(words.data(), c.mem2.p, nw * 4, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipMemcpyAsync(vorg.data(), d_vorg, 4ull * (nvars_new + 1), hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipMemcpyAsync(vstate.data(), d_vs, (uint64_t)nvars_new + 1, hipMemcpyDeviceToHost, c.stream));
+    if (nt) PFG_HIP(hipMemcpyAsync(trail.data(), d_trail, 4 * nt, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    for (uint64_t w = 0; w < nw; w += 3ull + words[w + 2]) ncls++;
+}
+
 // keep the elected variables of instances with cls 1, then those with cls 2, each
 // group in elected order; returns the two group sizes
 __global__ void k_pv_class(const uint32_t *__restrict__ pvs, uint32_t n, const uint32_t *__restrict__ ibase, uint32_t ninst,

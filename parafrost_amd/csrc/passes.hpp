@@ -26,6 +26,8 @@ void batch_export(Ctx &c, const uint32_t *img, uint64_t c0, uint64_t ncls, uint6
                   uint64_t *d_offs);
 void batch_records(Ctx &c, uint32_t *out, std::vector<uint64_t> &start);
 void set_ifrozen(Ctx &c, const uint32_t *vars, uint32_t n, bool on);
+void vmap_download(Ctx &c, std::vector<uint32_t> &words, uint64_t &ncls, std::vector<uint32_t> &vorg, uint32_t &nvars_new,
+                   std::vector<uint32_t> &trail, std::vector<uint8_t> &vstate);
 void merge_units(Ctx &c, const uint32_t *d_units, uint64_t n);  // ordered dedupe into the trail
 // algorithmic bytes of reading the occurrence lists of elected variables pvs[b, e):
 // 16 per list header + 8 per entry (+ 12 + 4|c| per listed clause when with_clauses);

@@ -756,6 +756,28 @@ int pfg_batch_export(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint64_t *d_
     API_END
 }
 
+int pfg_download_mapped(pfg_ctx *ctx, uint32_t *words, uint64_t *nwords, uint64_t *ncls, uint32_t *vorg,
+                        uint32_t *nvars_new, uint32_t *trail, uint32_t *ntrail, uint8_t *vstate)
+{
+    API_BEGIN
+    if (c.batch) throw Error(PFG_EINVAL, "variable mapping is for single-instance contexts");
+    if (c.unsat) return PFG_EUNSAT;
+    std::vector<uint32_t> w, vo, tr;
+    std::vector<uint8_t> vs;
+    uint64_t nc = 0;
+    uint32_t nv = 0;
+    vmap_download(c, w, nc, vo, nv, tr, vs);
+    if (words && !w.empty()) std::memcpy(words, w.data(), w.size() * 4);
+    if (nwords) *nwords = w.size();
+    if (ncls) *ncls = nc;
+    if (vorg) std::memcpy(vorg, vo.data(), vo.size() * 4);
+    if (nvars_new) *nvars_new = nv;
+    if (trail && !tr.empty()) std::memcpy(trail, tr.data(), tr.size() * 4);
+    if (ntrail) *ntrail = (uint32_t)tr.size();
+    if (vstate) std::memcpy(vstate, vs.data(), vs.size());
+    API_END
+}
+
 int pfg_download_cnf(pfg_ctx *ctx, uint32_t *words, uint64_t *nwords, uint64_t *ncls)
 {
     API_BEGIN

@@ -4,8 +4,9 @@ Runs the reference simplifier, compiled from the reference's own sources by
 oracle/Makefile (target `ref`), on small seeded instances and stores, per case,
 the input clauses and the reference's final state (clause images in clause
 order, unit trail, model stack, variable states) plus per-pass dumps of the
-first rounds. Cases f* run in incremental mode with variables frozen by
-ifreeze() and, for some, units given as unit clauses. Output: tests/golden/reference_vectors.npz (data only).
+first rounds, and the variable-mapped write-back the reference makes after the
+simplifier (ParaFROST::map(true)). Cases f* run in incremental mode with variables
+frozen by ifreeze() and, for some, units given as unit clauses. Output: tests/golden/reference_vectors.npz (data only).
 
 usage: python tests/golden/make_golden.py
 """
@@ -87,6 +88,11 @@ def main():
                                   freeze=freeze)
             if freeze is not None:
                 arrays[f"{name}/freeze"] = freeze
+            # the reference's variable mapping with write-back after the simplifier
+            mp = po.run_reference(path, os.path.join(td, name + ".map.bin"), o, freeze=freeze, mapped=True)
+            if isinstance(mp, po.Mapped):
+                for f in ("lits", "offs", "vorg", "trail", "vstate"):
+                    arrays[f"{name}/map_{f}"] = getattr(mp, f)
             arrays[f"{name}/lits"] = cnf.lits
             arrays[f"{name}/offs"] = cnf.offs
             arrays[f"{name}/meta"] = np.array([cnf.nvars, opts["phases"], opts["mu_pos"], opts["mu_neg"],

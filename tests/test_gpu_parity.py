@@ -181,6 +181,76 @@ def test_frozen_marks_persist_and_clear():
         g.close()
 
 
+def _csr(words):
+    lits, offs, i = [], [0], 0
+    while i < len(words):
+        n = int(words[i + 2])
+        cl = words[i + 3:i + 3 + n]
+        assert int(words[i + 1]) == int(np.bitwise_or.reduce(np.left_shift(1, cl & 31).astype(np.uint32))) if n else True
+        lits.append(cl)
+        offs.append(offs[-1] + n)
+        i += 3 + n
+    return (np.concatenate(lits) if lits else np.zeros(0, np.uint32)), np.array(offs, np.uint64)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_vmap_matches_oracle(seed):
+    """Variable mapping with write-back after sigmify (ParaFROST::map(true)) on the
+    device: renumbered clauses (and their signatures), vorg, mapped units and variable
+    states equal the oracle's; the context itself is left as it was."""
+    import parafrost_amd as pf
+    print(f"vmap seed={seed}", flush=True)
+    cnf, kw = _small(seed + 400)
+    rng = np.random.default_rng(90 + seed)
+    units = np.zeros(0, np.uint32)
+    if seed % 2:
+        vs = rng.choice(np.arange(1, cnf.nvars + 1), size=max(1, cnf.nvars // 60), replace=False)
+        units = (2 * vs + rng.integers(0, 2, len(vs))).astype(np.uint32)
+    o = po.Oracle(cnf, po.default_opts(**kw))
+    if len(units):
+        assert o.enqueue(units)
+    ro = o.sigmify()
+    g = pf.Simplifier(cnf.nvars, pf.default_opts(**kw))
+    try:
+        g.upload(cnf)
+        if len(units):
+            g.enqueue_units(units)
+        try:
+            g.sigmify()
+        except pf.Unsat:
+            assert ro != 0
+            return
+        assert ro == 0
+        before = g.cnf_words()
+        w, vorg, tr, vs = g.mapped()
+        m = o.vmap()
+        lits, offs = _csr(w)
+        assert np.array_equal(lits, m.lits) and np.array_equal(offs, m.offs)
+        assert np.array_equal(vorg, m.vorg) and np.array_equal(vs, m.vstate)
+        assert np.array_equal(np.sort(tr), np.sort(m.trail))
+        assert np.array_equal(g.cnf_words(), before)
+    finally:
+        g.close()
+
+
+def test_vmap_c1():
+    import parafrost_amd as pf
+    cnf = gen.config("c1")
+    o = po.Oracle(cnf, po.default_opts())
+    assert o.sigmify() == 0
+    g = pf.Simplifier(cnf.nvars, pf.default_opts())
+    try:
+        g.upload(cnf)
+        g.sigmify()
+        w, vorg, tr, vs = g.mapped()
+        m = o.vmap()
+        lits, offs = _csr(w)
+        assert np.array_equal(lits, m.lits) and np.array_equal(offs, m.offs) and np.array_equal(vorg, m.vorg)
+        assert m.nvars < cnf.nvars
+    finally:
+        g.close()
+
+
 def test_unsat_by_propagation():
     import parafrost_amd as pf
     lits = np.array([2 * 1 + 1, 2 * 2, 2 * 2 + 1, 2 * 3, 2 * 3 + 1, 2 * 1], np.uint32)  # (~1|2) (~2|3) (~3|1)... closed by units

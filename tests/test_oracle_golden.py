@@ -129,3 +129,18 @@ def test_frozen_variables_change_the_result_and_survive():
         o.sigmify()
         changed += not np.array_equal(o.state().cnf, Z[f"{name}/final_cnf"])
     assert changed >= 4
+
+
+@pytest.mark.parametrize("name", [n for n in NAMES if f"{n}/map_lits" in Z.files])
+def test_oracle_vmap_matches_reference(name):
+    """The variable-mapped write-back after the simplifier (ParaFROST::map(true),
+    src/cpu/vmap.cpp:64-131): mapped clauses in clause order, vorg, mapped trail
+    and variable states."""
+    cnf, opts, unsat = _case(name)
+    o = po.Oracle(cnf, opts)
+    _incremental(name, o)
+    assert o.sigmify() == 0
+    m = o.vmap()
+    for f in ("lits", "offs", "vorg", "trail", "vstate"):
+        assert np.array_equal(getattr(m, f), Z[f"{name}/map_{f}"]), f"{name}: mapped {f}"
+    assert m.nvars + 1 == len(Z[f"{name}/map_vorg"])

@@ -59,3 +59,30 @@ def test_mid_size_frozen(seed):
            gen.scalefree(nv, int(nv * rng.uniform(2.0, 5.0)), 2.5, 0.2, 700 + seed))
     freeze = np.unique(rng.integers(1, nv + 1, int(nv * rng.uniform(0.05, 0.4)))).astype(np.uint32)
     _run(cnf, po.default_opts(phases=int(rng.integers(2, 7))), freeze)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_mid_size_vmap(seed):
+    """variable mapping with write-back after the simplifier, with level-0 units"""
+    rng = np.random.default_rng(750 + seed)
+    nv = int(rng.integers(2000, 15000))
+    cnf = (gen.geometric(nv, int(nv * rng.uniform(2.0, 3.5)), 2, 8, 750 + seed) if seed % 2 else
+           gen.uniform_ksat(nv, int(nv * rng.uniform(2.5, 4.0)), 3, 750 + seed))
+    vs = rng.choice(np.arange(1, nv + 1), 4, replace=False)
+    units = (2 * vs + rng.integers(0, 2, 4)).astype(np.uint32)
+    opts = po.default_opts(phases=int(rng.integers(1, 6)))
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "f.cnf")
+        lits = np.concatenate([cnf.lits, units])
+        offs = np.concatenate([cnf.offs, cnf.offs[-1] + 1 + np.arange(len(units), dtype=np.uint64)])
+        gen.Cnf(cnf.nvars, lits, offs).write_dimacs(p)
+        ref = po.run_reference(p, os.path.join(td, "f.bin"), opts, mapped=True)
+    o = po.Oracle(cnf, opts)
+    assert o.enqueue(units)
+    rc = o.sigmify()
+    if not isinstance(ref, po.Mapped):
+        assert rc != 0
+        return
+    m = o.vmap()
+    for f in ("lits", "offs", "vorg", "trail", "vstate"):
+        assert np.array_equal(getattr(m, f), getattr(ref, f)), f
