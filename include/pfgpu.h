@@ -54,6 +54,8 @@ typedef struct pfg_opts {
     int32_t sub_en, ve_en, ere_en;  /* 1, 1, 1 */
     int32_t lbd_tier1;         /* 2    */
     int32_t stable_sort;       /* 0: lists > 20 ordered as libstdc++ std::sort (reference) */
+    int32_t bce_en;            /* 0    (opt_bce_en, src/cpu/options.cpp:26) */
+    int32_t bce_limit;         /* 3000 (bcemax, src/cpu/options.cpp:35) */
 } pfg_opts;
 
 /* ---- Batches (no counterpart call in the reference, which simplifies one formula per
@@ -91,7 +93,7 @@ int pfg_batch_export(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint64_t *d_
 
 /* per-pass device time accumulated over the context's life, in milliseconds */
 typedef struct pfg_times {
-    double upload, hist, ot, sort_ot, prop, lcve, sub, ve, ere, gc, count, total;
+    double upload, hist, ot, sort_ot, prop, lcve, sub, ve, ere, gc, count, total, bce;
 } pfg_times;
 
 void pfg_default_opts(pfg_opts *o);
@@ -131,6 +133,7 @@ int pfg_lcve(pfg_ctx *ctx, uint32_t *npvs);       /* varReorder + LCVE     (lcve
 int pfg_sort_ot(pfg_ctx *ctx);                    /* sortOT                (segsort.cu:32-41)           */
 int pfg_sub(pfg_ctx *ctx);                        /* subAsync              (kernels.cu:1133-1154)       */
 int pfg_ve(pfg_ctx *ctx);                         /* veAsync + postVE      (kernels.cu:1085-1120)       */
+int pfg_bce(pfg_ctx *ctx);                        /* BCE (elimination.cpp:131-166; bceAsync kernels.cu:1156-1168), if bce_en */
 int pfg_ere(pfg_ctx *ctx);                        /* ereAsync              (kernels.cu:1171-1195)       */
 int pfg_count(pfg_ctx *ctx, int64_t *cls, int64_t *lits); /* parcountAll (kernels.cu:855-925)           */
 int pfg_filter_pvs(pfg_ctx *ctx);                 /* filterPVs / DeviceSelect::If (elimination.cu:37-50) */

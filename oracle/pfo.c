@@ -335,6 +335,7 @@ void pfo_default_opts(pfo_opts *o)
     o->shrink_rate = 2; o->ve_clause_limit = 100; o->ve_lbound_en = 0; o->xor_max_arity = 10;
     o->sub_limit = 3000; o->ere_limit = 3000; o->ere_max_resolvent = 1000;
     o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
+    o->bce_en = 0; o->bce_limit = 3000;
 }
 
 pfo *pfo_create(uint32_t nvars, const pfo_opts *o, const uint32_t *lits, const uint64_t *offs,
@@ -1227,6 +1228,54 @@ void pfo_ve(pfo *s)
 }
 
 /* ------------------------------------------------------------------ */
+/* BCE (src/cpu/elimination.cpp:131-166)                               */
+/* ------------------------------------------------------------------ */
+static int is_tautology(uint32_t x, const uint32_t *c1, const uint32_t *c2)
+{   /* isTautology (src/cpu/simplify.h:119-137) */
+    const int n1 = c_size(c1), n2 = c_size(c2);
+    const uint32_t *a = c1 + 3, *b = c2 + 3;
+    int it1 = 0, it2 = 0;
+    while (it1 < n1 && it2 < n2) {
+        const uint32_t l1 = a[it1], l2 = b[it2], v1 = l1 >> 1, v2 = l2 >> 1;
+        if (v1 == x) it1++;
+        else if (v2 == x) it2++;
+        else if ((l1 ^ l2) == 1u) return 1;
+        else if (v1 < v2) it1++;
+        else if (v2 < v1) it2++;
+        else { it1++; it2++; }
+    }
+    return 0;
+}
+
+/* an original clause of the negative list whose resolvents with every original clause
+ * of the positive list are tautologies is saved (witness: the negative literal) and
+ * deleted; elected variables zeroed by VE are passed over */
+void pfo_bce(pfo *s)
+{
+    if (!s->opts.bce_en) return;
+    for (uint32_t i = 0; i < s->npvs; i++) {
+        const uint32_t v = s->pvs[i];
+        if (!v) continue;
+        const uint32_t p = 2 * v, n = p + 1;
+        const ol_t *poss = &s->ot[p], *negs = &s->ot[n];
+        if (poss->sz > s->opts.bce_limit || negs->sz > s->opts.bce_limit) continue;
+        for (int32_t j = 0; j < negs->sz; j++) {
+            uint32_t *neg = CL(s, negs->d[j]);
+            if (!c_orig(neg)) continue;
+            int all = 1;
+            for (int32_t k = 0; k < poss->sz && all; k++) {
+                const uint32_t *pos = CL(s, poss->d[k]);
+                if (c_orig(pos) && !is_tautology(v, neg, pos)) all = 0;
+            }
+            if (all) {
+                save_clause(s, neg, n);
+                c_markdel(neg);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* ERE (src/cpu/redundancy.h, elimination.cpp:169-197)                 */
 /* ------------------------------------------------------------------ */
 static int merge_ere(uint32_t x, const uint32_t *c1, const uint32_t *c2, const uint32_t *target)
@@ -1375,6 +1424,7 @@ int pfo_sigmify(pfo *s)
         if (s->unsat) return -1;
         pfo_ve(s);
         if (s->unsat) return -1;
+        pfo_bce(s);
         int64_t nc, nl;
         pfo_count_all(s, &nc, &nl);
         pfo_filter_pvs(s);

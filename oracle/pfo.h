@@ -36,6 +36,8 @@ typedef struct pfo_opts {
     int32_t lbd_tier1;         /* opt_lbd_tier1         (options.cpp:135)        = 2     */
     int32_t stable_sort;       /* 0: lists > 20 sorted by libstdc++ std::sort order (reference);
                                   1: stable order (key, creation) for every list */
+    int32_t bce_en;            /* opt_bce_en            (options.cpp:26)         = 0     */
+    int32_t bce_limit;         /* opt_bce_max_occurs    (options.cpp:35)         = 3000  */
 } pfo_opts;
 
 typedef struct pfo pfo;
@@ -69,6 +71,7 @@ int  pfo_stop(pfo *s, int64_t lr);    /* stop       (src/cpu/solve.h:784-786)   
 void pfo_ere(pfo *s);                 /* ERE        (src/cpu/elimination.cpp:169-197)*/
 void pfo_sub(pfo *s);                 /* SUB        (src/cpu/elimination.cpp:108-130)*/
 void pfo_ve(pfo *s);                  /* VE/bve     (src/cpu/bve.cpp:28-151)         */
+void pfo_bce(pfo *s);                 /* BCE        (src/cpu/elimination.cpp:131-166)*/
 void pfo_count_all(pfo *s, int64_t *cls, int64_t *lits); /* countAll (solve.h:724-734)  */
 void pfo_filter_pvs(pfo *s);          /* filterPVs  (src/cpu/solve.h:697-707)        */
 void pfo_shrink(pfo *s);              /* shrinkSimp (src/cpu/simplify.cpp:216-227)   */

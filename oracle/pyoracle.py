@@ -24,7 +24,8 @@ class Opts(ctypes.Structure):
                 ("ve_clause_limit", ctypes.c_int32), ("ve_lbound_en", ctypes.c_int32),
                 ("xor_max_arity", ctypes.c_int32), ("sub_limit", ctypes.c_int32), ("ere_limit", ctypes.c_int32),
                 ("ere_max_resolvent", ctypes.c_int32), ("sub_en", ctypes.c_int32), ("ve_en", ctypes.c_int32),
-                ("ere_en", ctypes.c_int32), ("lbd_tier1", ctypes.c_int32), ("stable_sort", ctypes.c_int32)]
+                ("ere_en", ctypes.c_int32), ("lbd_tier1", ctypes.c_int32), ("stable_sort", ctypes.c_int32),
+                ("bce_en", ctypes.c_int32), ("bce_limit", ctypes.c_int32)]
 
 
 _lib = None
@@ -53,7 +54,7 @@ def lib():
         for f in ("pfo_sigmify", "pfo_prop", "pfo_lcve", "pfo_unsat", "pfo_phase"):
             getattr(L, f).restype = ctypes.c_int
             getattr(L, f).argtypes = [vp]
-        for f in ("pfo_resize_cnf", "pfo_create_ot", "pfo_sort_ot", "pfo_ere", "pfo_sub", "pfo_ve",
+        for f in ("pfo_resize_cnf", "pfo_create_ot", "pfo_sort_ot", "pfo_ere", "pfo_sub", "pfo_ve", "pfo_bce",
                   "pfo_filter_pvs", "pfo_shrink", "pfo_next_phase"):
             getattr(L, f).restype = None
             getattr(L, f).argtypes = [vp]
@@ -267,7 +268,8 @@ def ref_opts_args(o: Opts) -> list[str]:
          f"--vephasemin={o.lits_min}", f"--gcfreq={o.shrink_rate}", f"--veclausemax={o.ve_clause_limit}",
          f"--xormaxarity={o.xor_max_arity}", f"--submax={o.sub_limit}", f"--eremax={o.ere_limit}",
          f"--ereclausemax={o.ere_max_resolvent}", "-velbound" if o.ve_lbound_en else "-no-velbound",
-         "-ere" if o.ere_en else "-no-ere", "--mapperc=1"]
+         "-ere" if o.ere_en else "-no-ere", "-bce" if o.bce_en else "-no-bce", f"--bcemax={o.bce_limit}",
+         "--mapperc=1"]
     return a
 
 

@@ -27,12 +27,13 @@ class Opts(ctypes.Structure):
                 ("ve_clause_limit", ctypes.c_int32), ("ve_lbound_en", ctypes.c_int32),
                 ("xor_max_arity", ctypes.c_int32), ("sub_limit", ctypes.c_int32), ("ere_limit", ctypes.c_int32),
                 ("ere_max_resolvent", ctypes.c_int32), ("sub_en", ctypes.c_int32), ("ve_en", ctypes.c_int32),
-                ("ere_en", ctypes.c_int32), ("lbd_tier1", ctypes.c_int32), ("stable_sort", ctypes.c_int32)]
+                ("ere_en", ctypes.c_int32), ("lbd_tier1", ctypes.c_int32), ("stable_sort", ctypes.c_int32),
+                ("bce_en", ctypes.c_int32), ("bce_limit", ctypes.c_int32)]
 
 
 class Times(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
-                ("upload", "hist", "ot", "sort_ot", "prop", "lcve", "sub", "ve", "ere", "gc", "count", "total")]
+                ("upload", "hist", "ot", "sort_ot", "prop", "lcve", "sub", "ve", "ere", "gc", "count", "total", "bce")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -45,7 +46,7 @@ class KStat(ctypes.Structure):
 
 API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_upload_cnf_device",
        "pfg_enqueue_units", "pfg_sigmify", "pfg_resize_cnf", "pfg_create_ot", "pfg_reduce_ot", "pfg_prop",
-       "pfg_lcve", "pfg_sort_ot", "pfg_sub", "pfg_ve", "pfg_ere", "pfg_count", "pfg_filter_pvs", "pfg_gc",
+       "pfg_lcve", "pfg_sort_ot", "pfg_sub", "pfg_ve", "pfg_bce", "pfg_ere", "pfg_count", "pfg_filter_pvs", "pfg_gc",
        "pfg_stop", "pfg_next_phase", "pfg_hist", "pfg_download_cnf", "pfg_download_trail",
        "pfg_download_resolved", "pfg_download_vstate", "pfg_download_pvs", "pfg_download_ot_list",
        "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
@@ -81,7 +82,7 @@ def lib():
         L.pfg_download_mapped.argtypes = [vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64), vp, ctypes.POINTER(u32), vp,
                                           ctypes.POINTER(u32), vp]
         for f in ("pfg_destroy", "pfg_sigmify", "pfg_resize_cnf", "pfg_create_ot", "pfg_reduce_ot", "pfg_prop",
-                  "pfg_sort_ot", "pfg_sub", "pfg_ve", "pfg_ere", "pfg_filter_pvs", "pfg_gc", "pfg_next_phase",
+                  "pfg_sort_ot", "pfg_sub", "pfg_ve", "pfg_bce", "pfg_ere", "pfg_filter_pvs", "pfg_gc", "pfg_next_phase",
                   "pfg_is_unsat"):
             getattr(L, f).argtypes = [vp]
         L.pfg_lcve.argtypes = [vp, ctypes.POINTER(u32)]
@@ -216,6 +217,7 @@ class Simplifier:
     def sortOT(self): self._check(self.L.pfg_sort_ot(self.h))
     def SUB(self): self._check(self.L.pfg_sub(self.h))
     def VE(self): self._check(self.L.pfg_ve(self.h))
+    def BCE(self): self._check(self.L.pfg_bce(self.h))
     def ERE(self): self._check(self.L.pfg_ere(self.h))
     def filterPVs(self): self._check(self.L.pfg_filter_pvs(self.h))
     def shrinkSimp(self): self._check(self.L.pfg_gc(self.h))

@@ -44,6 +44,7 @@ uint32_t lcve(Ctx &c);                   // returns #elected
 void sub(Ctx &c);
 // ve.hip
 void ve(Ctx &c);
+void bce(Ctx &c);                        // blocked clause elimination (bce_en)
 // ere.hip
 void ere(Ctx &c);
 

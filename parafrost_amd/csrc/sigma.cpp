@@ -56,10 +56,10 @@ PassTimer::~PassTimer()
 
 const char *const kernel_names[K_NUM] = {
     "upload_prep", "hist", "ot_fill", "ot_sort", "prop", "lcve_radix", "lcve_init", "lcve_pass_a", "lcve_pass_b",
-    "sort_pv_lists", "sub_staged", "ve_detect", "ve_apply", "ere_probe", "gc_move", "count"};
+    "sort_pv_lists", "sub_staged", "ve_detect", "ve_apply", "bce_count", "bce_apply", "ere_probe", "gc_move", "count"};
 const char *const kernel_symbols[K_NUM] = {
     "k_upload", nullptr, nullptr, nullptr, nullptr, nullptr, "k_elect_init", "k_elect_pass_a", "k_elect_pass_b",
-    "k_sort_pv_lists", "k_sub_staged", "k_ve_detect", "k_ve_apply", "k_ere_probe", "k_gc_move", "k_count"};
+    "k_sort_pv_lists", "k_sub_staged", "k_ve_detect", "k_ve_apply", nullptr, nullptr, "k_ere_probe", "k_gc_move", "k_count"};
 
 KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
 {
@@ -149,6 +149,7 @@ struct Sigma {
     void ERE() { PassTimer t(c, &c.times.ere); c.hist_live = false; ere(c); }
     void SUB() { c.hist_live = false; if (c.opts.sub_en || c.opts.ve_en) { PassTimer t(c, &c.times.sub); sub(c); } }
     void VE() { c.hist_live = false; if (c.opts.ve_en) { PassTimer t(c, &c.times.ve); ve(c); } }
+    void BCE() { c.hist_live = false; if (c.opts.bce_en) { PassTimer t(c, &c.times.bce); bce(c); } }
     void countAll(int64_t &cls, int64_t &lits, bool with_hist = false)
     {
         PassTimer t(c, &c.times.count);
@@ -178,6 +179,7 @@ struct Sigma {
             if (c.unsat) return PFG_EUNSAT;
             VE();
             if (c.unsat) return PFG_EUNSAT;
+            BCE();
             int64_t nc, nl;
             countAll(nc, nl);
             filterPVs();
@@ -299,6 +301,7 @@ struct Sigma {
                 collect_conflicts(st);
                 VE();
                 collect_conflicts(st);
+                BCE();
             }
             {
                 PassTimer t(c, &c.times.count);
@@ -375,6 +378,7 @@ void pfg_default_opts(pfg_opts *o)
     o->shrink_rate = 2; o->ve_clause_limit = 100; o->ve_lbound_en = 0; o->xor_max_arity = 10;
     o->sub_limit = 3000; o->ere_limit = 3000; o->ere_max_resolvent = 1000;
     o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
+    o->bce_en = 0; o->bce_limit = 3000;
 }
 
 
@@ -540,6 +544,7 @@ int pfg_lcve(pfg_ctx *ctx, uint32_t *npvs)
 int pfg_sort_ot(pfg_ctx *ctx) { API_BEGIN Sigma(c).sortOT(); API_END }
 int pfg_sub(pfg_ctx *ctx) { API_BEGIN Sigma(c).SUB(); API_END }
 int pfg_ve(pfg_ctx *ctx) { API_BEGIN Sigma(c).VE(); API_END }
+int pfg_bce(pfg_ctx *ctx) { API_BEGIN Sigma(c).BCE(); API_END }
 int pfg_ere(pfg_ctx *ctx) { API_BEGIN Sigma(c).ERE(); API_END }
 
 int pfg_count(pfg_ctx *ctx, int64_t *cls, int64_t *lits)

@@ -754,6 +754,20 @@ __global__ void k_chunk_append(const uint64_t *__restrict__ moff, const uint64_t
         if (moff[q + 1] > moff[q]) out[pos[q]] = model_base + moff[q];
 }
 
+// batch: append the record chunks of one pass (per-variable offsets moff into the
+// records that start at c.nres) to the chunk list
+static void append_chunks(Ctx &c, const uint64_t *moff, uint32_t np)
+{
+    ensure(c, c.mchunk, c.nchunk + np + 1, true);
+    ensure(c, c.tmpu32c, (uint64_t)np + 1);
+    ensure(c, c.tmpu64c, (uint64_t)np + 2);
+    k_chunk_flags<<<grid_for(np, 256, 8192), 256, 0, c.stream>>>(moff, np, c.tmpu32c.p);
+    scan_u32_to_u64(c, c.tmpu32c.p, c.tmpu64c.p, np, c.cnt.p + CNT_TOTAL);
+    k_chunk_append<<<grid_for(np, 256, 8192), 256, 0, c.stream>>>(moff, c.tmpu64c.p, np, c.nres, c.mchunk.p + c.nchunk);
+    read_counters(c, CNT_TOTAL + 1);
+    c.nchunk += c.hcnt[CNT_TOTAL];
+}
+
 void ve(Ctx &c)
 {
     if (!c.opts.ve_en || !c.npvs) return;
@@ -797,21 +811,135 @@ void ve(Ctx &c)
         KTimer kt(c, K_VE_APPLY, 4.0 * add_words + 8.0 * add_refs + 4.0 * add_model);
         k_ve_apply<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, O, type, aux, np, roff, woff, uoff, moff);
     }
-    if (c.batch && add_model) {
-        ensure(c, c.mchunk, c.nchunk + np + 1, true);
-        ensure(c, c.tmpu32c, (uint64_t)np + 1);
-        ensure(c, c.tmpu64c, (uint64_t)np + 2);
-        k_chunk_flags<<<grid_for(np, 256, 8192), 256, 0, c.stream>>>(moff, np, c.tmpu32c.p);
-        scan_u32_to_u64(c, c.tmpu32c.p, c.tmpu64c.p, np, c.cnt.p + CNT_TOTAL);
-        k_chunk_append<<<grid_for(np, 256, 8192), 256, 0, c.stream>>>(moff, c.tmpu64c.p, np, c.nres,
-                                                                       c.mchunk.p + c.nchunk);
-        read_counters(c, CNT_TOTAL + 1);
-        c.nchunk += c.hcnt[CNT_TOTAL];
-    }
+    if (c.batch && add_model) append_chunks(c, moff, np);
     c.nrefs += add_refs;
     c.msz += add_words;
     c.nres += add_model;
     if (add_units) merge_units(c, units, add_units);
+}
+
+// ------------------------------------------------------------------ BCE
+// Blocked clause elimination after VE (src/cpu/elimination.cpp:131-166): for every
+// elected variable v still in the list (VE zeroes the ones it eliminated) whose two
+// lists hold at most bce_limit entries, an original clause of the negative list is
+// blocked when its resolvent on v with every original clause of the positive list
+// is a tautology; it is saved to the model stack with witness -v and deleted.
+//
+// The clauses of two elected variables are disjoint (LCVE freezes every variable of
+// an elected variable's clauses, and VE's resolvents hold no elected variable), so
+// the variables are independent and one wave takes one variable, a lane per
+// negative clause. The stack order is the sequential one: elected order, then list
+// order. A count pass sizes each variable's records; after the scan the apply pass
+// decides again (only for variables with records) and writes them.
+
+struct BceArgs {
+    uint32_t *mem;
+    const OL *ot;
+    const uint64_t *occ;
+    uint32_t limit;
+};
+
+// isTautology (src/cpu/simplify.h:119-137): the resolvent of c1 and c2 on x holds a
+// complementary pair (only the common prefix of the two sorted clauses is scanned)
+__device__ __forceinline__ bool resolvent_is_tautology(uint32_t x, const uint32_t *c1, const uint32_t *c2)
+{
+    const int n1 = c_size(c1), n2 = c_size(c2);
+    const uint32_t *a = c1 + 3, *b = c2 + 3;
+    int i = 0, j = 0;
+    while (i < n1 && j < n2) {
+        const uint32_t l1 = a[i], l2 = b[j], v1 = l1 >> 1, v2 = l2 >> 1;
+        if (v1 == x) i++;
+        else if (v2 == x) j++;
+        else if ((l1 ^ l2) == 1u) return true;
+        else if (v1 < v2) i++;
+        else if (v2 < v1) j++;
+        else { i++; j++; }
+    }
+    return false;
+}
+
+template <bool APPLY>
+__global__ void __launch_bounds__(64) k_bce(BceArgs A, const uint32_t *__restrict__ pvs, uint32_t np,
+                                            uint64_t *__restrict__ nmodel, uint32_t *__restrict__ resolved,
+                                            const uint64_t *__restrict__ moff)
+{
+    const int lane = __lane_id();
+    for (uint32_t q = blockIdx.x; q < np; q += gridDim.x) {
+        const uint32_t v = pvs[q];
+        uint64_t words = 0;
+        if (v && (!APPLY || moff[q + 1] > moff[q])) {
+            const uint32_t p = 2 * v, n = p + 1;
+            const OL lp = A.ot[p], ln = A.ot[n];
+            if (lp.sz <= A.limit && ln.sz <= A.limit) {
+                uint32_t *rs = APPLY ? resolved + moff[q] : nullptr;
+                for (uint32_t j0 = 0; j0 < ln.sz; j0 += 64) {
+                    const uint32_t j = j0 + (uint32_t)lane;
+                    uint32_t *neg = nullptr;
+                    uint32_t w = 0;
+                    if (j < ln.sz) {
+                        neg = A.mem + A.occ[ln.off + j];
+                        if (c_orig(neg)) {
+                            bool all = true;
+                            for (uint32_t k = 0; k < lp.sz && all; k++) {
+                                const uint32_t *pos = A.mem + A.occ[lp.off + k];
+                                if (c_orig(pos) && !resolvent_is_tautology(v, neg, pos)) all = false;
+                            }
+                            if (all) w = (uint32_t)c_size(neg) + 1;
+                        }
+                    }
+                    uint32_t incl = w;
+#pragma unroll
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const uint32_t t = __shfl_up(incl, o, 64);
+                        if (lane >= o) incl += t;
+                    }
+                    if (APPLY && w) {   // saveClause(neg, size, -v) (model.cpp:263-280), then markDeleted
+                        uint32_t *r = rs + (incl - w);
+                        const int cs = c_size(neg);
+                        int at = -1;
+                        for (int k = 0; k < cs; k++) {
+                            r[k] = neg[3 + k];
+                            if (neg[3 + k] == n) at = k;
+                        }
+                        if (at > 0) { const uint32_t t = r[at]; r[at] = r[0]; r[0] = t; }
+                        r[cs] = (uint32_t)cs;
+                        c_markdel(neg);
+                    }
+                    const uint32_t tot = __shfl(incl, 63, 64);
+                    if (APPLY) rs += tot;
+                    words += tot;
+                }
+            }
+        }
+        if (!APPLY && lane == 0) nmodel[q] = words;
+    }
+}
+
+void bce(Ctx &c)
+{
+    if (!c.opts.bce_en || !c.npvs) return;
+    const uint32_t np = c.npvs;
+    const BceArgs A{c.mem.p, c.ot.p, c.occ.p, (uint32_t)std::max<int32_t>(c.opts.bce_limit, 0)};
+    ensure(c, c.tmpu64a, (uint64_t)np + 2);
+    ensure(c, c.tmpu64b, (uint64_t)np + 2);
+    uint64_t *nmodel = c.tmpu64a.p, *moff = c.tmpu64b.p;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(np, 65536);
+    {
+        KTimer kt(c, K_BCE_COUNT, pv_bytes(c, 0, np, true));
+        k_bce<false><<<grid, 64, 0, c.stream>>>(A, c.pvs.p, np, nmodel, nullptr, nullptr);
+    }
+    scan_u64(c, nmodel, moff, np, moff + np);
+    uint64_t add_model = 0;
+    PFG_HIP(hipMemcpyAsync(&add_model, moff + np, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    if (!add_model) return;
+    ensure(c, c.resolved, c.nres + add_model + 1, true);
+    {
+        KTimer kt(c, K_BCE_APPLY, 4.0 * add_model);
+        k_bce<true><<<grid, 64, 0, c.stream>>>(A, c.pvs.p, np, nullptr, c.resolved.p + c.nres, moff);
+    }
+    if (c.batch) append_chunks(c, moff, np);
+    c.nres += add_model;
 }
 
 }  // namespace pfg

@@ -6,7 +6,9 @@ the input clauses and the reference's final state (clause images in clause
 order, unit trail, model stack, variable states) plus per-pass dumps of the
 first rounds, and the variable-mapped write-back the reference makes after the
 simplifier (ParaFROST::map(true)). Cases f* run in incremental mode with variables
-frozen by ifreeze() and, for some, units given as unit clauses. Output: tests/golden/reference_vectors.npz (data only).
+frozen by ifreeze() and, for some, units given as unit clauses. Cases b* run with
+blocked clause elimination on, over formulas with planted blocked clauses
+(tests/blocked.py). Output: tests/golden/reference_vectors.npz (data only).
 
 usage: python tests/golden/make_golden.py
 """
@@ -21,9 +23,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import pyoracle as po  # noqa: E402
 from parafrost_amd import gen  # noqa: E402
+from blocked import with_blocked_gadgets  # noqa: E402
 
 PASSES = ("lcve", "sort", "sub", "ve", "ere")
 
@@ -66,6 +70,23 @@ def cases():
             vs = rng.choice(np.arange(1, nv + 1), 3, replace=False)
             units = (2 * vs + rng.integers(0, 2, 3)).astype(np.uint32)
         out.append((f"f{seed}", cnf, opts, freeze, units))
+    # blocked clause elimination on: formulas with planted blocked clauses that VE
+    # leaves in place
+    for seed in range(16):
+        rng = np.random.default_rng(5000 + seed)
+        nv = int(rng.integers(60, 400))
+        kind = seed % 3
+        if kind == 0:
+            cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(2.0, 4.0)), 3, 5000 + seed)
+        elif kind == 1:
+            cnf = gen.geometric(nv, int(nv * rng.uniform(1.5, 3.0)), 2, 6, 5000 + seed)
+        else:
+            cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 3.5)), 2.5, 0.3, 5000 + seed)
+        cnf = with_blocked_gadgets(cnf, int(rng.integers(2, 12)), 5000 + seed)
+        opts = dict(phases=int(rng.integers(1, 6)), mu_pos=int(rng.integers(10, 60)),
+                    mu_neg=int(rng.integers(10, 60)), shrink_rate=int(rng.integers(1, 4)),
+                    bce_en=1, bce_limit=int(rng.integers(100, 3001)))
+        out.append((f"b{seed}", cnf, opts, None, None))
     return out
 
 
@@ -96,7 +117,7 @@ def main():
             arrays[f"{name}/lits"] = cnf.lits
             arrays[f"{name}/offs"] = cnf.offs
             arrays[f"{name}/meta"] = np.array([cnf.nvars, opts["phases"], opts["mu_pos"], opts["mu_neg"],
-                                               opts["shrink_rate"], st.unsat], np.int64)
+                                               opts["shrink_rate"], st.unsat, o.bce_en, o.bce_limit], np.int64)
             for f in ("cnf", "trail", "resolved", "vstate"):
                 arrays[f"{name}/final_{f}"] = getattr(st, f)
             # per-pass states of round 0 and 1

@@ -141,6 +141,8 @@ def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, *
                     raise Mismatch(f"{tag} VE: unsat oracle {L.pfo_unsat(o.h)} gpu {g.is_unsat()}")
                 return rounds, None
             compare(o.state(), state_gpu(g), tag + " VE")
+            L.pfo_bce(o.h); g.BCE()
+            compare(o.state(), state_gpu(g), tag + " BCE")
             import ctypes
             a, b = ctypes.c_int64(), ctypes.c_int64()
             L.pfo_count_all(o.h, ctypes.byref(a), ctypes.byref(b))

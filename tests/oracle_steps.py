@@ -36,6 +36,7 @@ def oracle_passes(o: "po.Oracle", lists_after=("sort",)):
         if L.pfo_unsat(h):
             return
         yield rnd, "ve", o.state()
+        L.pfo_bce(h)
         a, b = ctypes.c_int64(), ctypes.c_int64()
         L.pfo_count_all(h, ctypes.byref(a), ctypes.byref(b))
         L.pfo_filter_pvs(h)

@@ -125,6 +125,16 @@ def test_batch_option_variants():
         _check(cnfs, **kw)
 
 
+def test_batch_bce():
+    """Blocked clause elimination in a batch: its model records join each
+    instance's record chunks in stack order."""
+    from blocked import with_blocked_gadgets
+    rng = np.random.default_rng(4503)
+    cnfs = [with_blocked_gadgets(_instance(rng, k), int(rng.integers(1, 12)), 4503 + k) for k in range(12)]
+    cnfs.insert(5, _full_3cnf())
+    _check(cnfs, bce_en=1, phases=4)
+
+
 def test_batch_relayout_reuses_context():
     """One context laid out again for another set of instances gives the same results."""
     import parafrost_amd as pf

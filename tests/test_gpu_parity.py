@@ -41,6 +41,7 @@ OPTION_VARIANTS = [
     dict(stable_sort=1), dict(ere_en=0), dict(ve_lbound_en=1), dict(xor_max_arity=2), dict(xor_max_arity=20),
     dict(sub_limit=20, ere_limit=20), dict(ve_clause_limit=0), dict(ve_clause_limit=3), dict(ere_max_resolvent=0),
     dict(lits_min=1 << 40), dict(phases=8, mu_pos=10, mu_neg=10), dict(sub_en=0), dict(lcve_min=50),
+    dict(bce_en=1), dict(bce_en=1, bce_limit=100, mu_pos=200, mu_neg=200), dict(bce_en=1, ve_en=0),
 ]
 
 
@@ -52,6 +53,32 @@ def test_lockstep_options(variant):
         cnf, base = _small(seed)
         base.update(kw)
         ls.run_lockstep(cnf, **base)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_lockstep_bce(seed):
+    """Blocked clause elimination on formulas with planted blocked clauses that VE
+    leaves in place; the state after BCE is compared in every round."""
+    from blocked import with_blocked_gadgets
+    print(f"lockstep bce seed={seed}", flush=True)
+    cnf, kw = _small(seed)
+    rng = np.random.default_rng(7500 + seed)
+    cnf = with_blocked_gadgets(cnf, int(rng.integers(2, 20)), 7500 + seed)
+    ls.run_lockstep(cnf, bce_en=1, bce_limit=int(rng.integers(100, 3001)), **kw)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_bce_mid_size_matches_oracle(seed):
+    from blocked import with_blocked_gadgets
+    print(f"bce mid size seed={seed}", flush=True)
+    rng = np.random.default_rng(7600 + seed)
+    nv = int(rng.integers(20000, 80000))
+    if seed % 2:
+        cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(2.5, 4.0)), 3, 7600 + seed)
+    else:
+        cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 3.5)), 2.5, 0.3, 7600 + seed)
+    cnf = with_blocked_gadgets(cnf, nv // 20, 7600 + seed)
+    ls.run_full(cnf, bce_en=1)
 
 
 @pytest.mark.parametrize("seed", range(6))

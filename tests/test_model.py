@@ -30,21 +30,16 @@ def _eval(clauses, nv):
     return ok
 
 
-@pytest.mark.parametrize("seed", range(40))
-def test_equisat_and_model_extension(seed):
-    rng = np.random.default_rng(seed)
-    nv = int(rng.integers(8, 15))
-    if seed % 2:
-        cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(2.5, 4.5)), 3, seed)
-    else:
-        cnf = gen.geometric(nv, int(nv * rng.uniform(1.5, 3.5)), 2, 5, seed)
-    o = po.Oracle(cnf, po.default_opts(phases=int(rng.integers(1, 5)), mu_pos=10, mu_neg=10))
+def _check_semantics(cnf, opts):
+    """returns the oracle's final state (None when it proved UNSAT)"""
+    nv = cnf.nvars
+    o = po.Oracle(cnf, opts)
     rc = o.sigmify()
     F = [list(map(int, c)) for c in cnf.clauses()]
     satF = _eval(F, nv)
     if rc != 0:
         assert not satF.any(), "simplifier proved UNSAT on a satisfiable formula"
-        return
+        return None
     st = o.state()
     surv = [list(c[2]) for c in st.clauses() if not (c[0] & 2)]
     units = [[int(u)] for u in st.trail]
@@ -54,7 +49,7 @@ def test_equisat_and_model_extension(seed):
     for c in surv:
         assert not any((l >> 1) in elim for l in c)
     if not satFp.any():
-        return
+        return st
     models = np.nonzero(satFp)[0][:64]
     bad = 0
     for m in models:
@@ -66,3 +61,36 @@ def test_equisat_and_model_extension(seed):
         if not satF[mm]:
             bad += 1
     assert bad == 0
+    return st
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_equisat_and_model_extension(seed):
+    rng = np.random.default_rng(seed)
+    nv = int(rng.integers(8, 15))
+    if seed % 2:
+        cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(2.5, 4.5)), 3, seed)
+    else:
+        cnf = gen.geometric(nv, int(nv * rng.uniform(1.5, 3.5)), 2, 5, seed)
+    _check_semantics(cnf, po.default_opts(phases=int(rng.integers(1, 5)), mu_pos=10, mu_neg=10))
+
+
+def test_bce_equisat_and_model_extension():
+    """Blocked clause elimination keeps the formula equisatisfiable and its model
+    records (clause saved with witness -v) repair every model of the result; BCE
+    changes the outcome of some of these formulas (planted blocked clauses; on
+    formulas this small VE often removes the gadget variable first)."""
+    from blocked import with_blocked_gadgets
+    changed = 0
+    for seed in range(48):
+        rng = np.random.default_rng(700 + seed)
+        nv = int(rng.integers(8, 12))
+        base = gen.uniform_ksat(nv, int(nv * rng.uniform(1.5, 3.0)), 3, 700 + seed)
+        cnf = with_blocked_gadgets(base, int(rng.integers(1, 4)), 700 + seed)
+        kw = dict(phases=int(rng.integers(1, 4)), mu_pos=10, mu_neg=10)
+        st = _check_semantics(cnf, po.default_opts(bce_en=1, bce_limit=100, **kw))
+        o = po.Oracle(cnf, po.default_opts(**kw))
+        rc = o.sigmify()
+        if st is not None and rc == 0:
+            changed += not np.array_equal(st.resolved, o.state().resolved)
+    assert changed >= 6, changed

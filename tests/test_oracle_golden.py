@@ -31,7 +31,8 @@ Z, NAMES = _load()
 def _case(name):
     meta = Z[f"{name}/meta"]
     cnf = Cnf(int(meta[0]), Z[f"{name}/lits"], Z[f"{name}/offs"])
-    opts = po.default_opts(phases=int(meta[1]), mu_pos=int(meta[2]), mu_neg=int(meta[3]), shrink_rate=int(meta[4]))
+    opts = po.default_opts(phases=int(meta[1]), mu_pos=int(meta[2]), mu_neg=int(meta[3]), shrink_rate=int(meta[4]),
+                           bce_en=int(meta[6]), bce_limit=int(meta[7]))
     return cnf, opts, int(meta[5])
 
 

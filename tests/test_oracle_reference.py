@@ -86,3 +86,17 @@ def test_mid_size_vmap(seed):
     m = o.vmap()
     for f in ("lits", "offs", "vorg", "trail", "vstate"):
         assert np.array_equal(getattr(m, f), getattr(ref, f)), f
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_mid_size_bce(seed):
+    """Blocked clause elimination on, with planted blocked clauses."""
+    from blocked import with_blocked_gadgets
+    rng = np.random.default_rng(900 + seed)
+    nv = int(rng.integers(2000, 12000))
+    if seed % 2:
+        cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(2.5, 4.0)), 3, 900 + seed)
+    else:
+        cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 3.5)), 2.5, 0.3, 900 + seed)
+    cnf = with_blocked_gadgets(cnf, nv // 20, 900 + seed)
+    _run(cnf, po.default_opts(phases=int(rng.integers(2, 6)), bce_en=1, bce_limit=int(rng.integers(100, 3001))))
