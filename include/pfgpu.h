@@ -56,6 +56,8 @@ typedef struct pfg_opts {
     int32_t stable_sort;       /* 0: lists > 20 ordered as libstdc++ std::sort (reference) */
     int32_t bce_en;            /* 0    (opt_bce_en, src/cpu/options.cpp:26) */
     int32_t bce_limit;         /* 3000 (bcemax, src/cpu/options.cpp:35) */
+    int32_t proof_en;          /* 0    (proof, src/cpu/options.cpp:64): binary DRAT lines of the simplifier,
+                                  read with pfg_download_proof; single-instance contexts only */
 } pfg_opts;
 
 /* ---- Batches (no counterpart call in the reference, which simplifies one formula per
@@ -159,6 +161,10 @@ int pfg_download_mapped(pfg_ctx *ctx, uint32_t *words, uint64_t *nwords, uint64_
                         uint32_t *nvars_new, uint32_t *trail, uint32_t *ntrail, uint8_t *vstate);
 int pfg_download_trail(pfg_ctx *ctx, uint32_t *lits, uint32_t *n);
 int pfg_download_resolved(pfg_ctx *ctx, uint32_t *words, uint64_t *n);
+/* The binary DRAT proof lines the simplifier wrote since pfg_create/pfg_reset (PROOF::addUnit,
+ * addResolvent, deleteClause, addEmpty; src/cpu/proof.cpp:92-232 in binary mode, variables in the
+ * context's numbering). bytes NULL: *n = the size. Succeeds after a refutation too. */
+int pfg_download_proof(pfg_ctx *ctx, uint8_t *bytes, uint64_t *n);
 int pfg_download_vstate(pfg_ctx *ctx, uint8_t *vstate);          /* nvars + 1 bytes */
 int pfg_download_pvs(pfg_ctx *ctx, uint32_t *pvs, uint32_t *n);
 /* occurrence list of one literal as clause positions (index in clause order) */

@@ -57,6 +57,7 @@ struct pfo {
     int32_t phase; uint32_t mu_inc; int unsat;
     int64_t nclauses, nliterals;
     uint32_t *outc; int32_t outn, outcap;
+    uint8_t *proof; uint64_t nproof, proofcap; int proof_closed;   /* binary DRAT (proof_en) */
 };
 
 static inline uint32_t *CL(const pfo *s, uint64_t r) { return s->mem + r; }
@@ -335,7 +336,7 @@ void pfo_default_opts(pfo_opts *o)
     o->shrink_rate = 2; o->ve_clause_limit = 100; o->ve_lbound_en = 0; o->xor_max_arity = 10;
     o->sub_limit = 3000; o->ere_limit = 3000; o->ere_max_resolvent = 1000;
     o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
-    o->bce_en = 0; o->bce_limit = 3000;
+    o->bce_en = 0; o->bce_limit = 3000; o->proof_en = 0;
 }
 
 pfo *pfo_create(uint32_t nvars, const pfo_opts *o, const uint32_t *lits, const uint64_t *offs,
@@ -381,17 +382,69 @@ void pfo_destroy(pfo *s)
     for (size_t i = 0; i < nlit; i++) free(s->ot[i].d);
     free(s->ot); free(s->ps); free(s->ns); free(s->value); free(s->vstate); free(s->frozen); free(s->ifrozen);
     free(s->trail); free(s->pvs); free(s->eligible); free(s->scores); free(s->fstack);
-    free(s->mem); free(s->scnf); free(s->resolved); free(s->outc);
+    free(s->mem); free(s->scnf); free(s->resolved); free(s->outc); free(s->proof);
     free(s);
 }
 
+/* ------------------------------------------------------------------ */
+/* binary DRAT proof (src/cpu/proof.cpp:92-232) with the identity      */
+/* variable mapping: a line is 'a' or 'd', each literal 2v+sign as a   */
+/* little-endian base-128 varint (high bit = more), then a zero byte.  */
+/* The simplifier adds units (learnUnit, solve.h:475-479), resolvents, */
+/* strengthened and substituted clauses unless a literal is already    */
+/* true (addResolvent, proof.cpp:208-222), deletes subsumed clauses    */
+/* (removeClause, sclause.cpp:77-84), and ends with the empty clause   */
+/* at a contradiction (learnEmpty, solve.h:250-253), after which the   */
+/* reference stops (killSolver).                                       */
+/* ------------------------------------------------------------------ */
+static void proof_byte(pfo *s, uint8_t b)
+{
+    if (s->nproof == s->proofcap) {
+        s->proofcap = s->proofcap ? 2 * s->proofcap : 4096;
+        s->proof = (uint8_t *)xrealloc(s->proof, (size_t)s->proofcap);
+    }
+    s->proof[s->nproof++] = b;
+}
+
+static void proof_line(pfo *s, uint8_t tag, const uint32_t *lits, int n)
+{
+    if (!s->opts.proof_en || s->proof_closed) return;
+    proof_byte(s, tag);
+    for (int i = 0; i < n; i++) {
+        uint32_t r = lits[i];
+        while (r > 127u) { proof_byte(s, (uint8_t)((r & 127u) | 128u)); r >>= 7; }
+        proof_byte(s, (uint8_t)r);
+    }
+    proof_byte(s, 0);
+}
+
+static void proof_resolvent(pfo *s, const uint32_t *c)
+{   /* PROOF::addResolvent: nothing for a clause with a true literal */
+    for (int i = 0; i < c_size(c); i++)
+        if (s->value[c[3 + i]] > 0) return;
+    proof_line(s, 'a', c + 3, c_size(c));
+}
+
+static void learn_empty(pfo *s)
+{   /* learnEmpty (solve.h:250-253); nothing is logged after it */
+    proof_line(s, 'a', NULL, 0);
+    s->proof_closed = 1;
+    s->unsat = 1;
+}
+
 /* enqueueUnit (src/cpu/solve.h:430-441): marks the variable frozen */
-static void enqueue_unit(pfo *s, uint32_t lit)
+static void assign_unit(pfo *s, uint32_t lit)
 {
     s->vstate[ABSV(lit)] = FROZEN_M; /* learnUnit -> markFrozen (solve.h:475-479, 529-533) */
     s->value[lit] = 1;
     s->value[FLIP(lit)] = 0;
     s->trail[s->ntrail++] = lit;
+}
+
+static void enqueue_unit(pfo *s, uint32_t lit)
+{
+    proof_line(s, 'a', &lit, 1);   /* learnUnit -> addUnit */
+    assign_unit(s, lit);
 }
 
 /* ifreeze / iunfreeze (src/cpu/assume.cpp:45-61); iassume marks its variables the
@@ -404,10 +457,11 @@ int pfo_ifreeze(pfo *s, uint32_t v, int on)
     return 0;
 }
 
+/* root units handed over with the formula: the caller's, not logged by the simplifier */
 int pfo_enqueue(pfo *s, uint32_t lit)
 {
     if (s->value[lit] >= 0) return s->value[lit] ? 0 : -1;
-    enqueue_unit(s, lit);
+    assign_unit(s, lit);
     return 0;
 }
 
@@ -510,11 +564,11 @@ int pfo_prop(pfo *s)
             if (prop_clause(s, f, c)) c_markdel(c);
             else {
                 const int size = c_size(c);
-                if (!size) { s->unsat = 1; return 0; }
+                if (!size) { learn_empty(s); return 0; }
                 if (size == 1) {
                     const uint32_t unit = c[3];
                     if (s->value[unit] < 0) enqueue_unit(s, unit);
-                    else { s->unsat = 1; return 0; }
+                    else { learn_empty(s); return 0; }
                 }
             }
         }
@@ -688,9 +742,12 @@ static void strengthen(pfo *s, uint32_t *c, uint32_t me)
             enqueue_unit(s, unit);
             toblivion_list(s, &s->ot[unit]);
         }
-        else if (!val) s->unsat = 1;
+        else if (!val) learn_empty(s);
     }
-    else if (c_learnt(c)) bump_shrunken(s, c);
+    else {
+        proof_resolvent(s, c);
+        if (c_learnt(c)) bump_shrunken(s, c);
+    }
 }
 
 static void update_ol(pfo *s, ol_t *ol)
@@ -713,7 +770,10 @@ static int subsume_l(pfo *s, ol_t *list, int32_t end, uint32_t *cand)
         if (c_molten(cand) && c_size(sg) > candsz) continue;
         if (c_size(sg) > 1 && sig_sub(sg[1], cand[1]) && sub_cl(sg, cand)) {
             if (c_learnt(sg) && c_orig(cand)) c_setst(sg, ST_ORIGINAL);
-            if (!c_del(cand)) c_markdel(cand);
+            if (!c_del(cand)) {   /* removeClause (sclause.cpp:77-84) */
+                c_markdel(cand);
+                proof_line(s, 'd', cand + 3, c_size(cand));
+            }
             return 1;
         }
     }
@@ -963,6 +1023,7 @@ static int substitute_single(pfo *s, uint32_t p, uint32_t def)
                 if (val < 0) enqueue_unit(s, unit);
                 else if (!val) return 1;
             }
+            else proof_resolvent(s, neg);
         }
     }
     for (int32_t i = 0; i < poss->sz; i++) {
@@ -975,6 +1036,7 @@ static int substitute_single(pfo *s, uint32_t p, uint32_t def)
                 if (val < 0) enqueue_unit(s, unit);
                 else if (!val) return 1;
             }
+            else proof_resolvent(s, pos);
         }
     }
     return 0;
@@ -1145,11 +1207,12 @@ static void new_resolvent(pfo *s)
         const uint32_t unit = s->outc[0];
         const int8_t val = s->value[unit];
         if (val < 0) enqueue_unit(s, unit);
-        else if (!val) s->unsat = 1;
+        else if (!val) learn_empty(s);
     }
     else {
         const uint64_t r = new_clause(s, s->outc, size, ST_ORIGINAL);
         uint32_t *c = CL(s, r);
+        proof_resolvent(s, c);
         calc_sig(c);
         c_markadded(c);
     }
@@ -1190,7 +1253,7 @@ void pfo_ve(pfo *s)
             uint32_t def = find_bn_gate(s, p);
             if (def) {
                 save_bn_gate(s, p, pOrgs, nOrgs);
-                if (substitute_single(s, p, def)) s->unsat = 1;
+                if (substitute_single(s, p, def)) learn_empty(s);
                 v = 0;
             }
             else {
@@ -1320,7 +1383,10 @@ static void forward_equ(pfo *s, uint32_t x, const uint32_t *c1, const uint32_t *
         for (int32_t i = 0; i < minsize; i++) {
             uint32_t *c = CL(s, minlist->d[i]);
             if (len == c_size(c) && (c_learnt(c) || c_st(c) == type) && sig_sub(msig, c[1]) && merge_ere(x, c1, c2, c)) {
-                if (!c_del(c)) c_markdel(c);
+                if (!c_del(c)) {   /* removeClause (sclause.cpp:67-75) */
+                    c_markdel(c);
+                    proof_line(s, 'd', c + 3, c_size(c));
+                }
                 break;
             }
         }
@@ -1444,6 +1510,11 @@ int pfo_sigmify(pfo *s)
 /* state access                                                       */
 /* ------------------------------------------------------------------ */
 int pfo_unsat(const pfo *s) { return s->unsat; }
+uint64_t pfo_dump_proof(const pfo *s, uint8_t *out)
+{
+    if (out && s->nproof) memcpy(out, s->proof, (size_t)s->nproof);
+    return s->nproof;
+}
 int32_t pfo_phase(const pfo *s) { return s->phase; }
 uint32_t pfo_mu_inc(const pfo *s) { return s->mu_inc; }
 uint64_t pfo_num_refs(const pfo *s) { return s->nscnf; }

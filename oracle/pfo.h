@@ -38,6 +38,7 @@ typedef struct pfo_opts {
                                   1: stable order (key, creation) for every list */
     int32_t bce_en;            /* opt_bce_en            (options.cpp:26)         = 0     */
     int32_t bce_limit;         /* opt_bce_max_occurs    (options.cpp:35)         = 3000  */
+    int32_t proof_en;          /* opt_proof_en          (options.cpp:64)         = 0     (binary DRAT) */
 } pfo_opts;
 
 typedef struct pfo pfo;
@@ -87,6 +88,7 @@ uint64_t pfo_vmap(const pfo *s, uint32_t *lits, uint64_t *offs, uint32_t *vorg, 
 
 /* state access */
 int      pfo_unsat(const pfo *s);
+uint64_t pfo_dump_proof(const pfo *s, uint8_t *out);   /* the simplifier's binary DRAT lines */
 int32_t  pfo_phase(const pfo *s);
 uint32_t pfo_mu_inc(const pfo *s);
 uint64_t pfo_num_refs(const pfo *s);

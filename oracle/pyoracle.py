@@ -25,7 +25,7 @@ class Opts(ctypes.Structure):
                 ("xor_max_arity", ctypes.c_int32), ("sub_limit", ctypes.c_int32), ("ere_limit", ctypes.c_int32),
                 ("ere_max_resolvent", ctypes.c_int32), ("sub_en", ctypes.c_int32), ("ve_en", ctypes.c_int32),
                 ("ere_en", ctypes.c_int32), ("lbd_tier1", ctypes.c_int32), ("stable_sort", ctypes.c_int32),
-                ("bce_en", ctypes.c_int32), ("bce_limit", ctypes.c_int32)]
+                ("bce_en", ctypes.c_int32), ("bce_limit", ctypes.c_int32), ("proof_en", ctypes.c_int32)]
 
 
 _lib = None
@@ -66,6 +66,8 @@ def lib():
         L.pfo_mu_inc.argtypes = [vp]
         L.pfo_num_refs.restype = ctypes.c_uint64
         L.pfo_num_refs.argtypes = [vp]
+        L.pfo_dump_proof.restype = ctypes.c_uint64
+        L.pfo_dump_proof.argtypes = [vp, vp]
         L.pfo_dump_cnf.restype = ctypes.c_uint64
         L.pfo_dump_cnf.argtypes = [vp, vp]
         L.pfo_trail.restype = ctypes.c_uint32
@@ -104,6 +106,7 @@ class State:
     pvs: np.ndarray          # uint32
     lists: list = field(default_factory=list)   # per PV literal: np.int64 scnf indices
     unsat: int = 0
+    proof: "np.ndarray | None" = None           # uint8 binary DRAT lines of the simplifier (proof_en)
 
     def clauses(self):
         w, i, out = self.cnf, 0, []
@@ -174,6 +177,14 @@ class Oracle:
         for v in np.asarray(vars, np.uint32).ravel():
             if self.L.pfo_ifreeze(self.h, int(v), 1 if on else 0) != 0:
                 raise ValueError(f"variable {int(v)} out of range")
+
+    def proof(self) -> np.ndarray:
+        """the binary DRAT lines the simplifier has written so far (proof_en)"""
+        n = self.L.pfo_dump_proof(self.h, None)
+        out = np.zeros(n, np.uint8)
+        if n:
+            self.L.pfo_dump_proof(self.h, out.ctypes.data)
+        return out
 
     def vmap(self) -> "Mapped":
         """variable mapping with write-back after sigmify (ParaFROST::map(true))"""
@@ -277,8 +288,14 @@ def run_reference(cnf_path: str, out_path: str, o: Opts, trace_prefix: str | Non
                   timeout: float = 600, freeze=None, mapped: bool = False):
     """freeze: variables frozen as by ifreeze() in incremental mode; mapped: run the
     reference's variable mapping with write-back after the simplifier and return its
-    Mapped dump (None when the formula is refuted)"""
+    Mapped dump (None when the formula is refuted). With o.proof_en the reference
+    writes its binary DRAT proof; the simplifier's part (from the start of the
+    simplifier on) is returned as the state's `proof` bytes."""
     env = dict(os.environ)
+    proof_args = []
+    if o.proof_en:
+        proof_args = ["-proof", f"--proofout={out_path}.drat"]
+        env["PFREF_PROOF_MARK"] = out_path + ".mark"
     if mapped:
         env["PFREF_MAP"] = "1"
     if trace_prefix:
@@ -289,16 +306,23 @@ def run_reference(cnf_path: str, out_path: str, o: Opts, trace_prefix: str | Non
         env["PFREF_FREEZE"] = fz
     if os.path.exists(out_path):
         os.remove(out_path)
-    r = subprocess.run([REF_BIN, cnf_path, out_path] + ref_opts_args(o), env=env,
+    r = subprocess.run([REF_BIN, cnf_path, out_path] + ref_opts_args(o) + proof_args, env=env,
                        capture_output=True, timeout=timeout)
+
+    def with_proof(st):
+        if o.proof_en and st is not None:
+            at = int(np.fromfile(out_path + ".mark", np.uint64)[0])
+            st.proof = np.fromfile(out_path + ".drat", np.uint8)[at:]
+        return st
+
     if not os.path.exists(out_path):
         # the reference ends the process on a proven contradiction (killSolver)
         if b"UNSATISFIABLE" in r.stdout:
-            return State(0, np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint32),
-                         np.zeros(0, np.uint8), np.zeros(0, np.uint32), unsat=1)
+            return with_proof(State(0, np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint32),
+                                    np.zeros(0, np.uint8), np.zeros(0, np.uint32), unsat=1))
         raise RuntimeError(f"reference run failed ({r.returncode}): {r.stdout[-500:]!r} {r.stderr[-500:]!r}")
     if mapped:
         with open(out_path, "rb") as f:
             magic = f.read(4)
         return read_mapped(out_path) if magic == b"PFRM" else None
-    return read_dump(out_path)
+    return with_proof(read_dump(out_path))

@@ -15,6 +15,9 @@
  *   u32 'PFRM', u32 nvars_new, u64 nclauses, per clause (u32 size, lits[]),
  *   u32 nvorg, vorg[] (new variable -> original), u32 ntrail, trail[],
  *   u8 vstate[nvars_new + 1]
+ * env PFREF_PROOF_MARK=<file>: with -proof --proofout=<path>, write the proof's
+ *   length in bytes (u64) when the simplifier starts, so that the simplifier's part
+ *   of the proof can be told from the parser's and shrinkTop's lines
  * env PFREF_FREEZE=<file>: u32 variables (little endian) frozen as by ifreeze()
  *   (src/cpu/assume.cpp:45-52) with the solver in incremental mode, so that
  *   iassumed() (src/cpu/solve.h:864-867) holds for them during LCVE
@@ -30,6 +33,7 @@
 #include <unordered_map>
 #include <cstdio>
 #include <cstdlib>
+#include <sys/stat.h>
 #include <string>
 #include <vector>
 
@@ -155,6 +159,15 @@ int main(int argc, char** argv)
 	   sleep/interrupt hooks and the write-back to the CDCL database. */
 	s->rootify();
 	s->shrinkTop(false);
+	if (const char* mk = getenv("PFREF_PROOF_MARK")) {
+		fflush(NULL);
+		struct stat sb;
+		uint64_t at = (s->opts.proof_en && stat(s->opts.proof_path, &sb) == 0) ? uint64_t(sb.st_size) : 0;
+		FILE* f = fopen(mk, "wb");
+		if (!f) { perror(mk); return 3; }
+		put64(f, at);
+		fclose(f);
+	}
 	if (s->orgs.empty()) { dump(s, out, false, 0); return 0; }
 	s->awaken();
 	if (s->simpstate == AWAKEN_FAIL) { fprintf(stderr, "awaken failed\n"); return 4; }

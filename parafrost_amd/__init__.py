@@ -28,7 +28,7 @@ class Opts(ctypes.Structure):
                 ("xor_max_arity", ctypes.c_int32), ("sub_limit", ctypes.c_int32), ("ere_limit", ctypes.c_int32),
                 ("ere_max_resolvent", ctypes.c_int32), ("sub_en", ctypes.c_int32), ("ve_en", ctypes.c_int32),
                 ("ere_en", ctypes.c_int32), ("lbd_tier1", ctypes.c_int32), ("stable_sort", ctypes.c_int32),
-                ("bce_en", ctypes.c_int32), ("bce_limit", ctypes.c_int32)]
+                ("bce_en", ctypes.c_int32), ("bce_limit", ctypes.c_int32), ("proof_en", ctypes.c_int32)]
 
 
 class Times(ctypes.Structure):
@@ -51,7 +51,8 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_download_resolved", "pfg_download_vstate", "pfg_download_pvs", "pfg_download_ot_list",
        "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
        "pfg_set_accounting", "pfg_set_stage_words", "pfg_create_batch", "pfg_batch_ninst", "pfg_batch_relayout", "pfg_upload_batch",
-       "pfg_sigmify_batch", "pfg_batch_download", "pfg_ifreeze", "pfg_batch_export", "pfg_download_mapped"]
+       "pfg_sigmify_batch", "pfg_batch_download", "pfg_ifreeze", "pfg_batch_export", "pfg_download_mapped",
+       "pfg_download_proof"]
 
 
 def lib_path() -> str:
@@ -79,6 +80,7 @@ def lib():
         L.pfg_upload_cnf_device.argtypes = [vp, vp, vp, u64, u64]
         L.pfg_enqueue_units.argtypes = [vp, vp, u32]
         L.pfg_ifreeze.argtypes = [vp, vp, u32, i32]
+        L.pfg_download_proof.argtypes = [vp, vp, ctypes.POINTER(u64)]
         L.pfg_download_mapped.argtypes = [vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64), vp, ctypes.POINTER(u32), vp,
                                           ctypes.POINTER(u32), vp]
         for f in ("pfg_destroy", "pfg_sigmify", "pfg_resize_cnf", "pfg_create_ot", "pfg_reduce_ot", "pfg_prop",
@@ -263,6 +265,15 @@ class Simplifier:
         self._check(self.L.pfg_download_resolved(self.h, None, ctypes.byref(n)))
         a = np.empty(n.value, np.uint32)
         self._check(self.L.pfg_download_resolved(self.h, a.ctypes.data, ctypes.byref(n)))
+        return a
+
+    def proof(self) -> np.ndarray:
+        """binary DRAT lines written so far (opts proof_en)"""
+        n = ctypes.c_uint64()
+        self._check(self.L.pfg_download_proof(self.h, None, ctypes.byref(n)))
+        a = np.empty(n.value, np.uint8)
+        if n.value:
+            self._check(self.L.pfg_download_proof(self.h, a.ctypes.data, ctypes.byref(n)))
         return a
 
     def vstate(self) -> np.ndarray:

@@ -128,6 +128,14 @@ struct Ctx {
     DBuf<uint32_t> erec;        // LCVE: per-clause candidate-rank records
     DBuf<uint32_t> ufirst;      // per variable: first unit occurrence (all-ones when idle)
     DBuf<uint64_t> cnt;         // small device counters
+    // binary DRAT output (opts.proof_en, single-instance contexts)
+    DBuf<uint8_t> proof;        // the bytes written so far
+    uint64_t nproof = 0;
+    bool proof_closed = false;  // the empty clause has been written
+    DBuf<uint8_t> pev, pord;    // one pass's events (PEv), as logged and in loop order
+    DBuf<uint32_t> pnev, pbytes;
+    DBuf<uint64_t> pcnt, poff, pboff, pfirst;   // counters, offsets; per literal: first unit event
+    uint64_t pcap = 0;
     uint64_t *hcnt = nullptr;   // pinned mirror
     DBuf<uint8_t> scan_tmp;     // scan partial sums
     DBuf<uint8_t> radix_tmp;    // radix sort temporaries

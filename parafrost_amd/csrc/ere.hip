@@ -12,6 +12,7 @@
 #include "device.hpp"
 #include "passes.hpp"
 #include "prims.hpp"
+#include "proof.hpp"
 #include "stage.hpp"
 
 namespace pfg {
@@ -27,6 +28,7 @@ struct EreArgs {
     uint32_t *owner;
     uint64_t *targets;
     unsigned long long *cnt;
+    ProofLog P;   // proof events: the deleted targets (removeClause, redundancy.h:137)
 };
 
 __device__ __forceinline__ bool merge_ere(uint32_t x, const uint32_t *c1, const uint32_t *c2, const uint32_t *target)
@@ -100,7 +102,7 @@ __device__ __forceinline__ bool ere_match(const EreArgs &A, uint32_t x, const ui
 // otherwise claim the target for variable q. Returns false on a claim collision.
 template <bool ST>
 __device__ __forceinline__ bool forward_equ(const EreArgs &A, uint32_t x, const uint32_t *c1, const uint32_t *c2, bool seq,
-                                            uint32_t q)
+                                            uint32_t q, uint32_t &pseq)
 {
     EreRes R;
     if (!ere_resolvent<ST>(A, x, c1, c2, R)) return true;
@@ -109,7 +111,11 @@ __device__ __forceinline__ bool forward_equ(const EreArgs &A, uint32_t x, const 
         const uint64_t r = A.occ[off + i];
         uint32_t *c = A.mem + r;
         if (!ere_match(A, x, c1, c2, R, c)) continue;
-        if (seq) { c_markdel(c); return true; }
+        if (seq) {
+            c_markdel(c);
+            plog(A.P, q, pseq, PE_DEL, r);
+            return true;
+        }
         uint32_t *slot = A.owner + (r >> 1);
         const uint32_t cur = __atomic_load_n(slot, __ATOMIC_RELAXED);
         if (cur == q) continue;   // deleted by this variable already
@@ -117,6 +123,7 @@ __device__ __forceinline__ bool forward_equ(const EreArgs &A, uint32_t x, const 
         const uint32_t old = atomicCAS(slot, NO_OWNER, q);
         if (old != NO_OWNER) return false;
         A.targets[atomicAdd(&A.cnt[CNT_A], 1ull)] = r;
+        plog(A.P, q, pseq, PE_DEL, r);
         return true;
     }
     return true;
@@ -125,7 +132,7 @@ __device__ __forceinline__ bool forward_equ(const EreArgs &A, uint32_t x, const 
 // tab: the variable's staged region (stage.hpp), or null to read the pool
 template <bool ST>
 __device__ __forceinline__ bool ere_var(const EreArgs &A, uint32_t x, bool seq, uint32_t q, const uint32_t *tab,
-                                        const uint32_t *arena)
+                                        const uint32_t *arena, uint32_t &pseq)
 {
     uint32_t dx = 2 * x, fx = dx + 1;
     if (A.ot[dx].sz > A.ot[fx].sz) { dx = fx; fx = 2 * x; }
@@ -139,7 +146,7 @@ __device__ __forceinline__ bool ere_var(const EreArgs &A, uint32_t x, bool seq, 
         for (uint32_t j = 0; j < other.sz; j++) {
             const uint32_t *cj = ST ? arena + tab[ob + j] : A.mem + A.occ[other.off + j];
             if (c_del(cj)) continue;
-            if (!forward_equ<ST>(A, x, ci, cj, seq, q)) return false;
+            if (!forward_equ<ST>(A, x, ci, cj, seq, q, pseq)) return false;
         }
     }
     return true;
@@ -193,8 +200,12 @@ __global__ void k_ere_probe(EreArgs A, const uint32_t *__restrict__ pvs, uint32_
 __global__ void __launch_bounds__(64) k_ere_claim(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs,
                                                  const uint32_t *__restrict__ flag)
 {
-    for (uint64_t q = gtid(); q < npvs; q += gstride())
-        if (flag[q] && !ere_var<false>(A, pvs[q], false, (uint32_t)q, nullptr, nullptr)) atomicOr(&A.cnt[CNT_B], 1ull);
+    for (uint64_t q = gtid(); q < npvs; q += gstride()) {
+        if (!flag[q]) continue;
+        uint32_t pseq = 0;
+        if (!ere_var<false>(A, pvs[q], false, (uint32_t)q, nullptr, nullptr, pseq)) atomicOr(&A.cnt[CNT_B], 1ull);
+        plog_end(A.P, (uint32_t)q, pseq);
+    }
 }
 
 __global__ void k_ere_commit(EreArgs A, uint64_t n, int del)
@@ -209,7 +220,11 @@ __global__ void k_ere_commit(EreArgs A, uint64_t n, int del)
 __global__ void k_ere_seq(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs)
 {
     if (gtid() != 0) return;
-    for (uint32_t q = 0; q < npvs; q++) ere_var<false>(A, pvs[q], true, q, nullptr, nullptr);
+    for (uint32_t q = 0; q < npvs; q++) {
+        uint32_t pseq = 0;
+        ere_var<false>(A, pvs[q], true, q, nullptr, nullptr, pseq);
+        plog_end(A.P, q, pseq);
+    }
 }
 
 void ere(Ctx &c)
@@ -222,10 +237,10 @@ void ere(Ctx &c)
         PFG_HIP(hipMemsetAsync(c.owner.p, 0xFF, c.owner.cap * 4, c.stream));
     }
     ensure(c, c.tmpu64b, c.occ.cap + 1);
-    EreArgs A{c.mem.p, c.ot.p, c.occ.p, c.opts.ere_limit, c.opts.ere_max_resolvent, c.owner.p, c.tmpu64b.p,
-              (unsigned long long *)c.cnt.p};
-    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));
     const uint32_t np = c.npvs;
+    EreArgs A{c.mem.p, c.ot.p, c.occ.p, c.opts.ere_limit, c.opts.ere_max_resolvent, c.owner.p, c.tmpu64b.p,
+              (unsigned long long *)c.cnt.p, proof_begin(c, np, c.nrefs)};
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));
     ensure(c, c.tmpu64a, 2ull * np + 2);
     ensure(c, c.tmpu32c, np + 1);
     uint64_t *npairs = c.tmpu64a.p, *poff = c.tmpu64a.p + np + 1;
@@ -245,7 +260,11 @@ void ere(Ctx &c)
     const uint64_t nt = c.hcnt[CNT_A];
     const bool collision = c.hcnt[CNT_B] != 0;
     if (nt) k_ere_commit<<<grid_for(nt, 256, 4096), 256, 0, c.stream>>>(A, nt, collision ? 0 : 1);
-    if (collision) k_ere_seq<<<1, 64, 0, c.stream>>>(A, c.pvs.p, c.npvs);
+    if (collision) {
+        proof_restart(c, np);   // the replay logs the deletions again, in order
+        k_ere_seq<<<1, 64, 0, c.stream>>>(A, c.pvs.p, c.npvs);
+    }
+    proof_end(c, np, false);
 }
 
 }  // namespace pfg

@@ -47,5 +47,15 @@ void ve(Ctx &c);
 void bce(Ctx &c);                        // blocked clause elimination (bce_en)
 // ere.hip
 void ere(Ctx &c);
+// proof.hip (binary DRAT, opts.proof_en): a pass logs events through the ProofLog
+// of proof_begin (capacity in events; null log when the proof is off), and
+// proof_end orders, decides (decide: units are assigned after the pass) and
+// appends them; proof_units appends the units trail[from, to) of a propagation
+struct ProofLog;
+ProofLog proof_begin(Ctx &c, uint32_t np, uint64_t cap);
+void proof_restart(Ctx &c, uint32_t np);   // drop what was logged since proof_begin
+void proof_end(Ctx &c, uint32_t np, bool decide);
+void proof_units(Ctx &c, uint32_t from, uint32_t to);
+void proof_empty(Ctx &c);                  // the empty clause; nothing is written after it
 
 }  // namespace pfg

@@ -138,7 +138,16 @@ struct Sigma {
     }
     void shrinkSimp() { PassTimer t(c, &c.times.gc); shrink(c); }   // live counts do not change
     void createOT() { PassTimer t(c, &c.times.ot); create_ot(c); }
-    bool propagate() { PassTimer t(c, &c.times.prop); return prop(c); }
+    bool propagate()
+    {
+        PassTimer t(c, &c.times.prop);
+        const uint32_t t0 = c.ntrail;
+        const bool was_unsat = c.unsat;
+        const bool ok = prop(c);
+        proof_units(c, t0, c.ntrail);   // learnUnit of each unit the propagation enqueued
+        if (!ok && !was_unsat) proof_empty(c);
+        return ok;
+    }
     bool LCVE()
     {
         PassTimer t(c, &c.times.lcve);
@@ -378,7 +387,7 @@ void pfg_default_opts(pfg_opts *o)
     o->shrink_rate = 2; o->ve_clause_limit = 100; o->ve_lbound_en = 0; o->xor_max_arity = 10;
     o->sub_limit = 3000; o->ere_limit = 3000; o->ere_max_resolvent = 1000;
     o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
-    o->bce_en = 0; o->bce_limit = 3000;
+    o->bce_en = 0; o->bce_limit = 3000; o->proof_en = 0;
 }
 
 
@@ -440,6 +449,8 @@ int pfg_destroy(pfg_ctx *ctx)
     release(c, c.tmpu8); release(c, c.tmpu64a); release(c, c.tmpu64b); release(c, c.tmpu64c);
     release(c, c.owner); release(c, c.claim); release(c, c.erec); release(c, c.ufirst); release(c, c.cnt);
     release(c, c.scan_tmp); release(c, c.radix_tmp);
+    release(c, c.proof); release(c, c.pev); release(c, c.pord); release(c, c.pnev); release(c, c.pbytes);
+    release(c, c.pcnt); release(c, c.poff); release(c, c.pboff); release(c, c.pfirst);
     release(c, c.ibase); release(c, c.imu); release(c, c.ib0); release(c, c.inel); release(c, c.icnt);
     c.cache_mem.release(); c.pin_res.release(); c.pin_trail.release(); c.pin_refs.release(); c.pin_vstate.release();
     if (c.hcnt) (void)hipHostFree(c.hcnt);
@@ -586,6 +597,7 @@ int pfg_create_batch(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t ni
                      const pfg_opts *opts)
 {
     if (!out || !ninst || !nvars) return PFG_EINVAL;
+    if (opts && opts->proof_en) return PFG_EINVAL;   // the proof is one instance's log
     for (uint32_t i = 0; i < ninst; i++)
         if (!nvars[i]) return PFG_EINVAL;
     std::vector<uint32_t> base;
@@ -814,6 +826,17 @@ int pfg_download_trail(pfg_ctx *ctx, uint32_t *lits, uint32_t *n)
     API_END
 }
 
+int pfg_download_proof(pfg_ctx *ctx, uint8_t *bytes, uint64_t *n)
+{
+    API_BEGIN
+    if (bytes && c.nproof) {
+        PFG_HIP(hipMemcpyAsync(bytes, c.proof.p, c.nproof, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (n) *n = c.nproof;
+    API_END_OK
+}
+
 int pfg_download_resolved(pfg_ctx *ctx, uint32_t *words, uint64_t *n)
 {
     API_BEGIN
@@ -928,6 +951,8 @@ int pfg_reset(pfg_ctx *ctx)
     c.unsat = false;
     c.ot_valid = false;
     c.hist_live = false;
+    c.nproof = 0;
+    c.proof_closed = false;
     API_END
 }
 

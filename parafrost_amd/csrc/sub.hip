@@ -11,6 +11,7 @@
 #include "device.hpp"
 #include "passes.hpp"
 #include "prims.hpp"
+#include "proof.hpp"
 #include "stage.hpp"
 #include <cstdio>
 #include <vector>
@@ -29,6 +30,7 @@ struct SubArgs {
     int lbd_tier1;
     int allow_units;
     int defer_oblivion;   // a unit's clauses are deleted by the caller (k_sub_oblivion)
+    ProofLog P;           // proof events: strengthened clauses, units, subsumed clauses
 };
 
 __device__ __forceinline__ bool sub_cl(const uint32_t *sm, const uint32_t *lr)
@@ -74,8 +76,10 @@ __device__ __forceinline__ void toblivion_list(const SubArgs &A, uint32_t lit)
     o->sz = 0;
 }
 
-// returns the unit it assigned (0 if none)
-__device__ __forceinline__ uint32_t strengthen(const SubArgs &A, uint32_t *c, uint32_t me)
+// returns the unit it assigned (0 if none). ref: the clause's pool reference (c may
+// be a staged copy); q, seq: the proof event position
+__device__ __forceinline__ uint32_t strengthen(const SubArgs &A, uint32_t *c, uint32_t me, uint64_t ref, uint32_t q,
+                                               uint32_t &seq)
 {   // strengthen (src/cpu/elimination.cpp:199-231)
     uint32_t sig = 0;
     int n = 0;
@@ -95,16 +99,27 @@ __device__ __forceinline__ uint32_t strengthen(const SubArgs &A, uint32_t *c, ui
             A.vstate[unit >> 1] = FROZEN_M;
             const unsigned long long pos = atomicAdd(&A.cnt[CNT_NTRAIL], 1ull);
             A.trail[pos] = unit;
+            plog(A.P, q, seq, PE_UNIT, unit);   // learnUnit
             if (!A.defer_oblivion) toblivion_list(A, unit);
             return unit;
         }
-        else if (r < 0) report_conflict(A.cnt, unit >> 1);
+        else if (r < 0) {
+            report_conflict(A.cnt, unit >> 1);
+            plog(A.P, q, seq, PE_CONFLICT, 0);
+        }
     }
-    else if (c_learnt(c)) {   // bumpShrunken (elimination.cpp:27-39)
-        const uint32_t old = c[0] >> 6;
-        if ((int)old > A.lbd_tier1) {
-            const uint32_t nl = min((uint32_t)(c_size(c) - 1), old);
-            if (nl < old) c[0] = (c[0] & 0xFu) | (1u << 4) | (nl << 6);
+    else {
+        if (A.P.ev) {   // addResolvent: not for a clause with a true literal
+            bool sat = false;
+            for (int k = 0; k < n; k++) sat |= A.value[lits[k]] == 1;
+            if (!sat) plog(A.P, q, seq, PE_ADD, ref);
+        }
+        if (c_learnt(c)) {   // bumpShrunken (elimination.cpp:27-39)
+            const uint32_t old = c[0] >> 6;
+            if ((int)old > A.lbd_tier1) {
+                const uint32_t nl = min((uint32_t)(c_size(c) - 1), old);
+                if (nl < old) c[0] = (c[0] & 0xFu) | (1u << 4) | (nl << 6);
+            }
         }
     }
     return 0;
@@ -123,7 +138,8 @@ __device__ __forceinline__ void update_ol(const SubArgs &A, uint32_t lit)
     o->sz = j;
 }
 
-__device__ __forceinline__ void subsume_l(const SubArgs &A, const uint64_t *list, uint32_t end, uint32_t *cand)
+__device__ __forceinline__ void subsume_l(const SubArgs &A, const uint64_t *list, uint32_t end, uint32_t *cand,
+                                          uint64_t ref, uint32_t q, uint32_t &seq)
 {   // subsume (subsume.h:101-117)
     const int candsz = c_size(cand);
     for (uint32_t j = 0; j < end; j++) {
@@ -132,13 +148,17 @@ __device__ __forceinline__ void subsume_l(const SubArgs &A, const uint64_t *list
         if (c_molten(cand) && c_size(sg) > candsz) continue;
         if (c_size(sg) > 1 && sig_sub(sg[1], cand[1]) && sub_cl(sg, cand)) {
             if (c_learnt(sg) && c_orig(cand)) c_setst(sg, ST_ORIGINAL);
-            if (!c_del(cand)) c_markdel(cand);
+            if (!c_del(cand)) {   // removeClause (sclause.cpp:77-84)
+                c_markdel(cand);
+                plog(A.P, q, seq, PE_DEL, ref);
+            }
             return;
         }
     }
 }
 
-__device__ __forceinline__ void selfsubsume_l(const SubArgs &A, uint32_t x, uint32_t fx, uint32_t listlit, uint32_t *cand)
+__device__ __forceinline__ void selfsubsume_l(const SubArgs &A, uint32_t x, uint32_t fx, uint32_t listlit, uint32_t *cand,
+                                              uint64_t ref, uint32_t q, uint32_t &seq)
 {   // selfsubsume (subsume.h:119-138)
     const OL *o = A.ot + listlit;
     const uint64_t *d = A.occ + o->off;
@@ -150,14 +170,14 @@ __device__ __forceinline__ void selfsubsume_l(const SubArgs &A, uint32_t x, uint
         if (subsize > candsz) break;
         if (c_del(sg)) continue;
         if (subsize > 1 && sig_selfsub(sg[1], candsig) && selfsub_cl(x, fx, sg, cand)) {
-            strengthen(A, cand, x);
+            strengthen(A, cand, x, ref, q, seq);
             c_melt(cand);
             return;
         }
     }
 }
 
-__device__ __forceinline__ void self_sub_x(const SubArgs &A, uint32_t p)
+__device__ __forceinline__ void self_sub_x(const SubArgs &A, uint32_t p, uint32_t q, uint32_t &seq)
 {   // self_sub_x (subsume.h:140-176)
     const uint32_t n = p ^ 1u;
     for (int side = 0; side < 2; side++) {
@@ -169,8 +189,8 @@ __device__ __forceinline__ void self_sub_x(const SubArgs &A, uint32_t p)
             uint32_t *cand = A.mem + d[i];
             if (c_size(cand) > HSE_MAX_CL_SIZE) break;
             if (c_del(cand)) continue;
-            selfsubsume_l(A, me, other, other, cand);
-            subsume_l(A, d, i, cand);
+            selfsubsume_l(A, me, other, other, cand, d[i], q, seq);
+            subsume_l(A, d, i, cand, d[i], q, seq);
         }
         update_ol(A, me);
     }
@@ -179,7 +199,8 @@ __device__ __forceinline__ void self_sub_x(const SubArgs &A, uint32_t p)
 // The same pass over a staged variable (stage.hpp, with the original-order table):
 // lists are table ranges, modified clause copies are marked dirty, and at the end
 // the dirty clause images and the compacted lists are written back to the pool.
-__device__ __forceinline__ void subsume_staged(uint32_t *arena, const uint32_t *list, uint32_t end, uint32_t *cand)
+__device__ __forceinline__ void subsume_staged(const SubArgs &A, uint32_t *arena, const uint32_t *list, uint32_t end,
+                                               uint32_t *cand, uint32_t q, uint32_t &seq)
 {   // subsume (subsume.h:101-117)
     const int candsz = c_size(cand);
     for (uint32_t j = 0; j < end; j++) {
@@ -188,14 +209,19 @@ __device__ __forceinline__ void subsume_staged(uint32_t *arena, const uint32_t *
         if (c_molten(cand) && c_size(sg) > candsz) continue;
         if (c_size(sg) > 1 && sig_sub(sg[1], cand[1]) && sub_cl(sg, cand)) {
             if (c_learnt(sg) && c_orig(cand)) { c_setst(sg, ST_ORIGINAL); staged_dirty(sg); }
-            if (!c_del(cand)) { c_markdel(cand); staged_dirty(cand); }
+            if (!c_del(cand)) {   // removeClause (sclause.cpp:77-84)
+                c_markdel(cand);
+                staged_dirty(cand);
+                plog(A.P, q, seq, PE_DEL, staged_ref(cand));
+            }
             return;
         }
     }
 }
 
 __device__ __forceinline__ void selfsubsume_staged(const SubArgs &A, uint32_t x, uint32_t fx, uint32_t *arena,
-                                                   const uint32_t *list, uint32_t n, uint32_t *cand)
+                                                   const uint32_t *list, uint32_t n, uint32_t *cand, uint32_t q,
+                                                   uint32_t &seq)
 {   // selfsubsume (subsume.h:119-138)
     const int candsz = c_size(cand);
     const uint32_t candsig = cand[1];
@@ -205,7 +231,7 @@ __device__ __forceinline__ void selfsubsume_staged(const SubArgs &A, uint32_t x,
         if (subsize > candsz) break;
         if (c_del(sg)) continue;
         if (subsize > 1 && sig_selfsub(sg[1], candsig) && selfsub_cl(x, fx, sg, cand)) {
-            strengthen(A, cand, x);
+            strengthen(A, cand, x, staged_ref(cand), q, seq);
             c_melt(cand);
             staged_dirty(cand);
             return;
@@ -213,7 +239,8 @@ __device__ __forceinline__ void selfsubsume_staged(const SubArgs &A, uint32_t x,
     }
 }
 
-__device__ __forceinline__ void self_sub_x_staged(const SubArgs &A, uint32_t p, uint32_t *tab, uint32_t *arena)
+__device__ __forceinline__ void self_sub_x_staged(const SubArgs &A, uint32_t p, uint32_t *tab, uint32_t *arena, uint32_t q,
+                                                  uint32_t &seq)
 {   // self_sub_x (subsume.h:140-176)
     uint32_t len[2] = {A.ot[p].sz, A.ot[p ^ 1u].sz};
     const uint32_t base[2] = {0u, len[0]}, ne = len[0] + len[1];
@@ -225,8 +252,8 @@ __device__ __forceinline__ void self_sub_x_staged(const SubArgs &A, uint32_t p, 
             uint32_t *cand = arena + mt[i];
             if (c_size(cand) > HSE_MAX_CL_SIZE) break;
             if (c_del(cand)) continue;
-            selfsubsume_staged(A, me, other, arena, tab + base[1 - side], len[1 - side], cand);
-            subsume_staged(arena, mt, i, cand);
+            selfsubsume_staged(A, me, other, arena, tab + base[1 - side], len[1 - side], cand, q, seq);
+            subsume_staged(A, arena, mt, i, cand, q, seq);
         }
         uint32_t j = 0;   // updateOL (subsume.h:27-37)
         for (uint32_t i = 0; i < ms; i++) {
@@ -268,8 +295,10 @@ k_sub_staged(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e
         const uint32_t v = active ? pvs[q] : 0u;
         const bool take = active && A.ot[2 * v].sz <= (uint32_t)A.sub_limit && A.ot[2 * v + 1].sz <= (uint32_t)A.sub_limit;
         stage_rounds(S, arena, arena_words, take, v, STAGE_TAB2, [&](uint32_t *tab) {
-            if (tab) self_sub_x_staged(A, 2 * v, tab, arena);
-            else self_sub_x(A, 2 * v);
+            uint32_t seq = 0;
+            if (tab) self_sub_x_staged(A, 2 * v, tab, arena, (uint32_t)q, seq);
+            else self_sub_x(A, 2 * v, (uint32_t)q, seq);
+            plog_end(A.P, (uint32_t)q, seq);
         });
     }
 }
@@ -306,10 +335,11 @@ __global__ void __launch_bounds__(64) k_sub_wave(SubArgs A, const uint32_t *__re
     __shared__ uint16_t msz[SW_CAP], osz[SW_CAP];
     __shared__ uint32_t msig[SW_CAP], osig[SW_CAP], mbin[SW_CAP], obin[SW_CAP];
     __shared__ uint32_t clit[SW_CL];
-    __shared__ uint32_t s_unit;
+    __shared__ uint32_t s_unit, s_seq;
     const uint32_t lane = threadIdx.x;
     const uint32_t v = pvs[q], p = 2 * v;
     if (A.ot[p].sz > (uint32_t)A.sub_limit || A.ot[p + 1].sz > (uint32_t)A.sub_limit) return;
+    if (lane == 0) s_seq = 0;   // proof events: lane 0 strengthens, one lane deletes, barriers between
     for (int side = 0; side < 2; side++) {
         const uint32_t me = side ? p + 1 : p, other = me ^ 1u;
         const OL om = A.ot[me], oo = A.ot[other];
@@ -371,7 +401,9 @@ __global__ void __launch_bounds__(64) k_sub_wave(SubArgs A, const uint32_t *__re
                 }
                 if (found) {
                     if (lane == 0) {
-                        s_unit = strengthen(A, cand, me);
+                        uint32_t seq = s_seq;
+                        s_unit = strengthen(A, cand, me, d[i], q, seq);
+                        s_seq = seq;
                         c_melt(cand);
                         msz[i] = (uint16_t)c_size(cand);
                         msig[i] = cand[1];
@@ -427,7 +459,12 @@ __global__ void __launch_bounds__(64) k_sub_wave(SubArgs A, const uint32_t *__re
                         if (lane == (uint32_t)(__ffsll((long long)fm) - 1)) {
                             uint32_t *sg = A.mem + d[b + 64u * (uint32_t)first + lane];
                             if (c_learnt(sg) && c_orig(cand)) c_setst(sg, ST_ORIGINAL);
-                            if (!c_del(cand)) c_markdel(cand);
+                            if (!c_del(cand)) {   // removeClause (sclause.cpp:77-84)
+                                c_markdel(cand);
+                                uint32_t seq = s_seq;
+                                plog(A.P, q, seq, PE_DEL, d[i]);
+                                s_seq = seq;
+                            }
                             msz[i] = (uint16_t)(ci | SW_DEL);
                         }
                         break;
@@ -439,6 +476,7 @@ __global__ void __launch_bounds__(64) k_sub_wave(SubArgs A, const uint32_t *__re
         if (lane == 0) update_ol(A, me);
         __syncthreads();
     }
+    if (lane == 0) plog_end(A.P, q, s_seq);
 }
 
 // the rest of toblivion(ot[unit]) for the units assigned since CNT_SUBFROM: every
@@ -463,7 +501,11 @@ __global__ void k_sub_range(SubArgs A, const uint32_t *__restrict__ pvs, uint32_
     for (uint64_t q = gtid() + b; q < e; q += gstride()) {
         const uint32_t v = pvs[q];
         const uint32_t p = 2 * v;
-        if (A.ot[p].sz <= (uint32_t)A.sub_limit && A.ot[p + 1].sz <= (uint32_t)A.sub_limit) self_sub_x(A, p);
+        if (A.ot[p].sz <= (uint32_t)A.sub_limit && A.ot[p + 1].sz <= (uint32_t)A.sub_limit) {
+            uint32_t seq = 0;
+            self_sub_x(A, p, (uint32_t)q, seq);
+            plog_end(A.P, (uint32_t)q, seq);
+        }
     }
 }
 
@@ -529,9 +571,9 @@ __global__ void k_iota(uint32_t *out, uint32_t n)
 void sub(Ctx &c)
 {
     if (!c.npvs || !(c.opts.sub_en || c.opts.ve_en)) return;
-    SubArgs A{c.mem.p, c.ot.p, c.occ.p, c.value.p, c.vstate.p, c.trail.p, (unsigned long long *)c.cnt.p,
-              c.opts.sub_limit, c.opts.lbd_tier1, 0, 0};
     const uint32_t np = c.npvs;
+    SubArgs A{c.mem.p, c.ot.p, c.occ.p, c.value.p, c.vstate.p, c.trail.p, (unsigned long long *)c.cnt.p,
+              c.opts.sub_limit, c.opts.lbd_tier1, 0, 0, proof_begin(c, np, 2 * c.nrefs + 2ull * np)};
     ensure(c, c.tmpu32a, np);
     ensure(c, c.tmpu32b, np);
     ensure(c, c.scores, np);
@@ -575,6 +617,7 @@ void sub(Ctx &c)
     c.ntrail = (uint32_t)c.hcnt[CNT_NTRAIL];
     if (c.hcnt[CNT_FLAGS] & FLAG_INTERNAL) throw Error(PFG_EINTERNAL, "SUB produced a unit outside a unit-capable variable");
     if (c.hcnt[CNT_FLAGS] & FLAG_UNSAT) c.unsat = true;
+    proof_end(c, np, false);
 }
 
 }  // namespace pfg

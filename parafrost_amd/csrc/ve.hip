@@ -16,6 +16,7 @@
 #include "device.hpp"
 #include "passes.hpp"
 #include "prims.hpp"
+#include "proof.hpp"
 #include "stage.hpp"
 
 namespace pfg {
@@ -618,6 +619,7 @@ struct VeOut {
     uint8_t *vstate;
     uint32_t *pvs;
     uint64_t ref_base, data_base, model_base;
+    ProofLog P;   // proof events: resolvents, substituted clauses and units, decided after the pass
 };
 
 __device__ __forceinline__ void save_side(const VeArgs &A, uint32_t *&rs, uint32_t p, bool pbig)
@@ -678,6 +680,7 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
         const uint32_t v = O.pvs[q], p = 2 * v, n = p + 1;
         uint32_t *rs = O.resolved + O.model_base + moff[q];
         uint32_t *us = O.units + uoff[q];
+        uint32_t seq = 0;
         if (t == T_BN) {
             const uint32_t def = aux[q], def_f = def ^ 1u;
             save_side(A, rs, p, (tw >> 8) & 1u);
@@ -688,7 +691,12 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
                 for (uint32_t i = 0; i < o.sz; i++) {
                     uint32_t *c = A.mem + A.occ[o.off + i];
                     if (c_learnt(c) || c_molten(c) || has_lit(c, kill)) c_markdel(c);
-                    else if (c_orig(c)) substitute_one(lit, c, into, us);
+                    else if (c_orig(c)) {
+                        const uint32_t *u0 = us;
+                        substitute_one(lit, c, into, us);
+                        if (us != u0) plog(O.P, (uint32_t)q, seq, PE_CAND_UNIT, us[-1]);
+                        else plog(O.P, (uint32_t)q, seq, PE_CAND_ADD, A.occ[o.off + i]);
+                    }
                 }
             }
         }
@@ -718,6 +726,7 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
                             uint32_t one[2];
                             merge_out(v, ci, cj, one, len);
                             *us++ = one[0];
+                            plog(O.P, (uint32_t)q, seq, PE_CAND_UNIT, one[0]);
                             continue;
                         }
                         uint32_t *dst = A.mem + w;
@@ -727,6 +736,7 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
                         dst[0] = ST_ORIGINAL | 8u;   // markAdded
                         dst[1] = sig;
                         dst[2] = (uint32_t)len;
+                        plog(O.P, (uint32_t)q, seq, PE_CAND_ADD, w);
                         O.refs[r++] = w;
                         w += 3 + (uint64_t)len;
                     }
@@ -737,6 +747,7 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
         }
         O.vstate[v] = MELTED_M;
         O.pvs[q] = 0;
+        plog_end(O.P, (uint32_t)q, seq);
     }
 }
 
@@ -806,7 +817,8 @@ void ve(Ctx &c)
     ensure(c, c.units, add_units + 1);
     A.mem = c.mem.p;
     uint32_t *units = c.units.p;
-    VeOut O{c.resolved.p, units, c.refs.p, c.vstate.p, c.pvs.p, c.nrefs, c.msz, c.nres};
+    VeOut O{c.resolved.p, units, c.refs.p, c.vstate.p, c.pvs.p, c.nrefs, c.msz, c.nres,
+            proof_begin(c, np, c.nrefs + add_refs + add_units + np)};
     {
         KTimer kt(c, K_VE_APPLY, 4.0 * add_words + 8.0 * add_refs + 4.0 * add_model);
         k_ve_apply<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, O, type, aux, np, roff, woff, uoff, moff);
@@ -815,6 +827,7 @@ void ve(Ctx &c)
     c.nrefs += add_refs;
     c.msz += add_words;
     c.nres += add_model;
+    proof_end(c, np, true);   // before the units are assigned: decisions read the values before the pass
     if (add_units) merge_units(c, units, add_units);
 }
 

@@ -172,7 +172,8 @@ class Server:
         self.cv = threading.Condition()
         self.closed = False
         self.finished = []
-        self.phase_s = {"intake": 0.0, "assemble": 0.0, "upload": 0.0, "simplify": 0.0, "results": 0.0}
+        self.phase_s = {"intake": 0.0, "assemble": 0.0, "upload": 0.0, "simplify": 0.0, "results": 0.0,
+                        "download": 0.0, "export": 0.0}
         self.batches = 0
         self.error = None   # the first exception of a worker: stops the server, raised by run()
 
@@ -287,11 +288,16 @@ class Server:
                 st = ctx.sigmify_batch() if len(work) else []
                 td = time.perf_counter()
                 new = []
+                t_dl = t_ex = 0.0
                 for i, (r, res, _) in enumerate(work):
+                    t0 = time.perf_counter()
                     w, t, rec, vs = ctx.instance(i)
+                    t1 = time.perf_counter()
+                    t_dl += t1 - t0
                     r.status = int(st[i])
                     if r.status == 0:
                         lits, offs = ctx.export(i, self.dev)
+                        t_ex += time.perf_counter() - t1
                         new.append((r, Resident(lits, offs, None, None, t, r.t1)))
                     else:
                         new.append((r, Resident(None, None, None, None, np.zeros(0, np.uint32), r.t1, refuted=True)))
@@ -313,7 +319,8 @@ class Server:
                     self._spill()
                     self.finished.extend(batch)
                     self.batches += 1
-                    for k, dt in (("assemble", tb - ta), ("upload", tc - tb), ("simplify", td - tc), ("results", te - td)):
+                    for k, dt in (("assemble", tb - ta), ("upload", tc - tb), ("simplify", td - tc), ("results", te - td),
+                                  ("download", t_dl), ("export", t_ex)):
                         self.phase_s[k] += dt
                     self.cv.notify_all()
         finally:

@@ -8,7 +8,8 @@ first rounds, and the variable-mapped write-back the reference makes after the
 simplifier (ParaFROST::map(true)). Cases f* run in incremental mode with variables
 frozen by ifreeze() and, for some, units given as unit clauses. Cases b* run with
 blocked clause elimination on, over formulas with planted blocked clauses
-(tests/blocked.py). Output: tests/golden/reference_vectors.npz (data only).
+(tests/blocked.py). Cases without units also hold the reference's binary DRAT proof
+lines of the simplifier. Output: tests/golden/reference_vectors.npz (data only).
 
 usage: python tests/golden/make_golden.py
 """
@@ -114,6 +115,12 @@ def main():
             if isinstance(mp, po.Mapped):
                 for f in ("lits", "offs", "vorg", "trail", "vstate"):
                     arrays[f"{name}/map_{f}"] = getattr(mp, f)
+            if units is None:
+                # the reference's binary DRAT lines of the simplifier (-proof)
+                op = po.default_opts(**opts)
+                op.proof_en = 1
+                pr = po.run_reference(path, os.path.join(td, name + ".proof.bin"), op, freeze=freeze)
+                arrays[f"{name}/proof"] = pr.proof
             arrays[f"{name}/lits"] = cnf.lits
             arrays[f"{name}/offs"] = cnf.offs
             arrays[f"{name}/meta"] = np.array([cnf.nvars, opts["phases"], opts["mu_pos"], opts["mu_neg"],

@@ -19,6 +19,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import pyoracle as po  # noqa: E402
 import parafrost_amd as pf  # noqa: E402
+import drat  # noqa: E402
 
 
 class Mismatch(AssertionError):
@@ -69,10 +70,21 @@ def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, *
     """Run both engines pass by pass; raise Mismatch at the first difference.
     setup(g), if given, configures the GPU simplifier before the upload; freeze lists
     variables frozen by ifreeze (incremental mode). Returns the number of rounds and
-    the final state."""
+    the final state. With proof_en the binary DRAT lines written so far are compared
+    after every pass too (drat.normalized: runs of unit lines as sets)."""
     oo, go = _opts_pair(**kw)
     o = po.Oracle(cnf, oo)
     g = pf.Simplifier(cnf.nvars, go)
+
+    def proof(where):
+        if not oo.proof_en:
+            return
+        a, b = drat.normalized(drat.parse(o.proof())), drat.normalized(drat.parse(g.proof()))
+        if a != b:
+            k = next((i for i in range(min(len(a), len(b))) if a[i] != b[i]), min(len(a), len(b)))
+            raise Mismatch(f"{where}: proof differs at line {k} of {len(a)}/{len(b)}: "
+                           f"oracle {a[k] if k < len(a) else None} gpu {b[k] if k < len(b) else None}")
+
     if setup is not None:
         setup(g)
     g.upload(cnf)
@@ -102,6 +114,7 @@ def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, *
                 okg = 0
             if okp != okg:
                 raise Mismatch(f"{tag} prop: conflict oracle {not okp} gpu {not okg}")
+            proof(tag + " prop")
             if not okp:
                 return rounds, None
             compare(o.state(), state_gpu(g), tag + " prop")
@@ -120,6 +133,7 @@ def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, *
                     raise Mismatch(f"{tag} stop differs")
                 L.pfo_ere(o.h); g.ERE()
                 compare(o.state(), state_gpu(g), tag + " ERE")
+                proof(tag + " ERE")
                 break
             L.pfo_sub(o.h)
             try:
@@ -129,8 +143,10 @@ def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, *
             if L.pfo_unsat(o.h) or g.is_unsat():
                 if bool(L.pfo_unsat(o.h)) != g.is_unsat():
                     raise Mismatch(f"{tag} SUB: unsat oracle {L.pfo_unsat(o.h)} gpu {g.is_unsat()}")
+                proof(tag + " SUB")
                 return rounds, None
             compare(o.state(), state_gpu(g), tag + " SUB")
+            proof(tag + " SUB")
             L.pfo_ve(o.h)
             try:
                 g.VE()
@@ -139,8 +155,10 @@ def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, *
             if L.pfo_unsat(o.h) or g.is_unsat():
                 if bool(L.pfo_unsat(o.h)) != g.is_unsat():
                     raise Mismatch(f"{tag} VE: unsat oracle {L.pfo_unsat(o.h)} gpu {g.is_unsat()}")
+                proof(tag + " VE")
                 return rounds, None
             compare(o.state(), state_gpu(g), tag + " VE")
+            proof(tag + " VE")
             L.pfo_bce(o.h); g.BCE()
             compare(o.state(), state_gpu(g), tag + " BCE")
             import ctypes
@@ -155,11 +173,18 @@ def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, *
             litsbefore = nl
             L.pfo_next_phase(o.h); g.next_phase()
             rounds += 1
-        L.pfo_prop(o.h)
-        g.prop()
+        okp = L.pfo_prop(o.h)
+        try:
+            g.prop()
+        except pf.Unsat:
+            if okp:
+                raise Mismatch("final prop: conflict on the gpu only")
+            proof("final prop")
+            return rounds, None
         L.pfo_shrink(o.h); g.shrinkSimp()
         fo, fg = o.state(), state_gpu(g)
         compare(fo, fg, "final")
+        proof("final")
         return rounds, fg
     finally:
         g.close()
@@ -182,6 +207,10 @@ def run_full(cnf, **kw):
             raise Mismatch(f"unsat oracle {ro} gpu {rg}")
         if ro == 0:
             compare(o.state(), state_gpu(g), "sigmify")
+        if oo.proof_en:
+            a, b = drat.normalized(drat.parse(o.proof())), drat.normalized(drat.parse(g.proof()))
+            if a != b:
+                raise Mismatch(f"proof differs ({len(a)} / {len(b)} lines)")
         return o.state(), g.times()
     finally:
         g.close()

@@ -41,3 +41,17 @@ def test_default_opts_match_reference_defaults():
     assert (o.phases, o.mu_pos, o.mu_neg, o.lcve_min, o.lits_min, o.shrink_rate) == (5, 32, 32, 1, 500, 2)
     assert (o.ve_clause_limit, o.xor_max_arity, o.sub_limit, o.ere_limit, o.ere_max_resolvent) == (100, 10, 3000, 3000, 1000)
     assert (o.sub_en, o.ve_en, o.ere_en, o.ve_lbound_en) == (1, 1, 1, 0)
+    assert (o.bce_en, o.bce_limit, o.proof_en) == (0, 3000, 0)   # options.cpp:26, 35, 64
+
+
+def test_batch_context_rejects_proof():
+    """The proof is one instance's log: a batch context with proof_en is refused
+    before any device work."""
+    import ctypes
+    import parafrost_amd as pf
+    L = pf.lib()
+    o = pf.default_opts(proof_en=1)
+    h = ctypes.c_void_p()
+    nv = (ctypes.c_uint32 * 2)(10, 20)
+    assert L.pfg_create_batch(ctypes.byref(h), 0, 0, 2, nv, ctypes.byref(o)) == pf.PFG_EINVAL
+    assert not h.value

@@ -81,6 +81,50 @@ def test_bce_mid_size_matches_oracle(seed):
     ls.run_full(cnf, bce_en=1)
 
 
+@pytest.mark.parametrize("seed", range(16))
+def test_lockstep_proof(seed):
+    """Binary DRAT lines after every pass equal the oracle's (units of one batched
+    propagation as a set); covers refuted instances, BCE and the ERE round."""
+    from blocked import with_blocked_gadgets
+    print(f"lockstep proof seed={seed}", flush=True)
+    cnf, kw = _small(seed)
+    if seed % 4 == 3:
+        rng = np.random.default_rng(7700 + seed)
+        nv = max(64, int(rng.integers(64, 400)))
+        cnf = gen.scalefree(nv, int(nv * rng.uniform(4.0, 8.0)), 2.5, 0.1, 7700 + seed)   # often refuted
+    if seed % 4 == 1:
+        cnf = with_blocked_gadgets(cnf, 8, 7700 + seed)
+        kw["bce_en"] = 1
+    ls.run_lockstep(cnf, proof_en=1, **kw)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_proof_mid_size_matches_oracle(seed):
+    print(f"proof mid size seed={seed}", flush=True)
+    rng = np.random.default_rng(7800 + seed)
+    nv = int(rng.integers(20000, 80000))
+    if seed % 2:
+        cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(2.5, 4.4)), 3, 7800 + seed)
+    else:
+        cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 5.0)), 2.5, 0.2, 7800 + seed)
+    ls.run_full(cnf, proof_en=1, phases=int(rng.integers(2, 6)))
+
+
+def test_proof_off_writes_nothing():
+    import parafrost_amd as pf
+    cnf, kw = _small(3)
+    g = pf.Simplifier(cnf.nvars, pf.default_opts(**kw))
+    try:
+        g.upload(cnf)
+        try:
+            g.sigmify()
+        except pf.Unsat:
+            pass
+        assert g.proof().size == 0
+    finally:
+        g.close()
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_lockstep_long_lists(seed):
     """Mid-size scale-free instances with high temperatures: elected variables with

@@ -145,3 +145,38 @@ def test_oracle_vmap_matches_reference(name):
     for f in ("lits", "offs", "vorg", "trail", "vstate"):
         assert np.array_equal(getattr(m, f), Z[f"{name}/map_{f}"]), f"{name}: mapped {f}"
     assert m.nvars + 1 == len(Z[f"{name}/map_vorg"])
+
+
+@pytest.mark.parametrize("name", [n for n in NAMES if f"{n}/proof" in Z.files])
+def test_oracle_proof_matches_reference(name):
+    """The binary DRAT lines of the simplifier, byte for byte (the reference run with
+    -proof; its parser's lines are not part of them)."""
+    cnf, opts, unsat = _case(name)
+    opts.proof_en = 1
+    o = po.Oracle(cnf, opts)
+    if f"{name}/freeze" in Z.files:
+        o.ifreeze(Z[f"{name}/freeze"])
+    o.sigmify()
+    assert np.array_equal(o.proof(), Z[f"{name}/proof"])
+
+
+def test_oracle_proofs_pass_rup_check():
+    """Independent of the reference: every line the oracle adds is implied by unit
+    propagation over the formula and the lines before it (resolvents, strengthened
+    and substituted clauses and units all are), and a refutation ends with the
+    empty clause."""
+    import drat
+    checked = refuted = 0
+    for name in [n for n in NAMES if f"{n}/proof" in Z.files][:40]:
+        cnf, opts, unsat = _case(name)
+        opts.proof_en = 1
+        o = po.Oracle(cnf, opts)
+        if f"{name}/freeze" in Z.files:
+            o.ifreeze(Z[f"{name}/freeze"])
+        rc = o.sigmify()
+        lines = drat.parse(o.proof())
+        checked += drat.check_rup([list(map(int, c)) for c in cnf.clauses()], lines)
+        if rc != 0:
+            refuted += 1
+            assert lines[-1] == ("a", ())
+    assert checked > 1000 and refuted >= 1

@@ -100,3 +100,28 @@ def test_mid_size_bce(seed):
         cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 3.5)), 2.5, 0.3, 900 + seed)
     cnf = with_blocked_gadgets(cnf, nv // 20, 900 + seed)
     _run(cnf, po.default_opts(phases=int(rng.integers(2, 6)), bce_en=1, bce_limit=int(rng.integers(100, 3001))))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_mid_size_proof(seed):
+    """Binary DRAT output (-proof), byte for byte; half with BCE on, some refuted."""
+    rng = np.random.default_rng(950 + seed)
+    nv = int(rng.integers(2000, 12000))
+    kind = seed % 4
+    if kind == 0:
+        cnf = gen.uniform_ksat(nv, int(nv * rng.uniform(3.0, 4.4)), 3, 950 + seed)
+    elif kind == 1:
+        cnf = gen.geometric(nv, int(nv * rng.uniform(1.5, 3.5)), 2, 8, 950 + seed)
+    elif kind == 2:
+        cnf = gen.scalefree(nv, int(nv * rng.uniform(2.0, 4.0)), 2.5, 0.3, 950 + seed)
+    else:
+        cnf = gen.scalefree(nv, int(nv * rng.uniform(4.0, 8.0)), 2.5, 0.1, 950 + seed)
+    opts = po.default_opts(phases=int(rng.integers(2, 6)), bce_en=seed % 2, proof_en=1)
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "f.cnf")
+        cnf.write_dimacs(p)
+        ref = po.run_reference(p, os.path.join(td, "f.bin"), opts)
+    o = po.Oracle(cnf, opts)
+    o.sigmify()
+    assert len(ref.proof) > 0
+    assert np.array_equal(o.proof(), ref.proof)
