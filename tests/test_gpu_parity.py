@@ -194,7 +194,8 @@ def test_lockstep_root_units(seed):
     rng = np.random.default_rng(seed)
     vs = rng.choice(np.arange(1, cnf.nvars + 1), size=max(1, cnf.nvars // 25), replace=False)
     units = (2 * vs + rng.integers(0, 2, len(vs))).astype(np.uint32)
-    ls.run_lockstep(cnf, units=units, **kw)
+    # the proof too: the caller's units are not logged, the ones propagation finds are
+    ls.run_lockstep(cnf, units=units, proof_en=1, **kw)
 
 
 @pytest.mark.parametrize("seed", range(8))
@@ -209,7 +210,7 @@ def test_lockstep_frozen(seed):
     if seed % 2:
         vs = rng.choice(np.arange(1, cnf.nvars + 1), size=max(1, cnf.nvars // 40), replace=False)
         units = (2 * vs + rng.integers(0, 2, len(vs))).astype(np.uint32)
-    ls.run_lockstep(cnf, units=units, freeze=freeze, **kw)
+    ls.run_lockstep(cnf, units=units, freeze=freeze, proof_en=seed % 2, **kw)
 
 
 def test_frozen_marks_persist_and_clear():
