@@ -299,7 +299,8 @@ def run_c5(args):
     seed 5), replayed as a Poisson stream at each offered load. Each request simplifies
     its instance's resident simplified clauses plus its new frames (units enqueued,
     interface variables frozen). Two slots of continuous batching, 2.5e8 literals per
-    batch. Latency = results on the host - arrival. The plain path runs the same
+    batch. Latency = results on the host - arrival (both paths download results into
+    host memory they reuse from call to call). The plain path runs the same
     request chain one request at a time in a single-instance context with the state
     on the host; both count the literals of the formulas they simplify (equal)."""
     import parafrost_amd as pf  # noqa: F401  (loads torch and the library first)
@@ -322,7 +323,7 @@ def run_c5(args):
               f"{st['literals_per_s'] / 1e6:.1f} M literals/s ({st['literals']} literals)", file=sys.stderr, flush=True)
     served_lits = sweep[-1]["literals"]
     # plain: the same chain, one request at a time, state on the host
-    p = serve.PlainChain(max(r.nvars for r in reqs))
+    p = serve.PlainChain(max(r.nvars for r in reqs), reuse_host=True)
     p.run(serve.make_requests(1, seed=98, gates=(2000, 4000), max_frames=2)[0])   # warm-up
     p.state.clear()
     t1 = time.perf_counter()

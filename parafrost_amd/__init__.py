@@ -130,6 +130,36 @@ def default_opts(**kw) -> Opts:
     return o
 
 
+class HostPool:
+    """Grow-only host arrays reused from call to call, so that in a steady state downloads
+    land in memory that is already mapped. Arrays taken since the last reset() do not overlap and stay
+    valid until the next reset(); growing starts a new array (the old one lives on in
+    the arrays taken from it)."""
+
+    def __init__(self):
+        self.bufs, self.cur = {}, {}
+
+    def reset(self):
+        self.cur = {k: 0 for k in self.cur}
+
+    def take(self, name: str, n: int, dtype) -> np.ndarray:
+        a, c = self.bufs.get(name), self.cur.get(name, 0)
+        if a is None or a.dtype != np.dtype(dtype) or c + n > a.size:
+            a = np.empty(max(n, 2 * (a.size if a is not None else 0), 1 << 16), dtype)
+            self.bufs[name], c = a, 0
+        self.cur[name] = c + n
+        return a[c:c + n]
+
+
+def _new(pool, name, n, dtype):
+    if pool is None:
+        return np.zeros(n, dtype) if np.dtype(dtype) == np.uint8 else np.empty(n, dtype)
+    v = pool.take(name, n, dtype)
+    if np.dtype(dtype) == np.uint8:
+        v.fill(0)
+    return v
+
+
 class SimplifierError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
@@ -246,24 +276,25 @@ class Simplifier:
         return h
 
     # ---- output
-    def cnf_words(self) -> np.ndarray:
+    def cnf_words(self, pool=None) -> np.ndarray:
+        """pool: a HostPool the array is taken from (reused host memory), else a new array"""
         nw, nc = ctypes.c_uint64(), ctypes.c_uint64()
         self._check(self.L.pfg_download_cnf(self.h, None, ctypes.byref(nw), ctypes.byref(nc)))
-        w = np.empty(nw.value, np.uint32)
+        w = _new(pool, "words", nw.value, np.uint32)
         self._check(self.L.pfg_download_cnf(self.h, w.ctypes.data, ctypes.byref(nw), ctypes.byref(nc)))
         return w
 
-    def trail(self) -> np.ndarray:
+    def trail(self, pool=None) -> np.ndarray:
         n = ctypes.c_uint32()
         self._check(self.L.pfg_download_trail(self.h, None, ctypes.byref(n)))
-        a = np.empty(n.value, np.uint32)
+        a = _new(pool, "trail", n.value, np.uint32)
         self._check(self.L.pfg_download_trail(self.h, a.ctypes.data, ctypes.byref(n)))
         return a
 
-    def resolved(self) -> np.ndarray:
+    def resolved(self, pool=None) -> np.ndarray:
         n = ctypes.c_uint64()
         self._check(self.L.pfg_download_resolved(self.h, None, ctypes.byref(n)))
-        a = np.empty(n.value, np.uint32)
+        a = _new(pool, "records", n.value, np.uint32)
         self._check(self.L.pfg_download_resolved(self.h, a.ctypes.data, ctypes.byref(n)))
         return a
 
@@ -417,16 +448,17 @@ class BatchSimplifier(Simplifier):
                                             ctypes.byref(nc)))
         return lits[:nl.value], offs
 
-    def instance(self, i: int):
-        """(clause words, trail, model records, variable states) of instance i in its own numbering."""
+    def instance(self, i: int, pool=None):
+        """(clause words, trail, model records, variable states) of instance i in its own numbering;
+        pool: a HostPool the arrays are taken from (reused host memory), else new arrays."""
         nw, nc, nr = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         nt = ctypes.c_uint32()
         self._check(self.L.pfg_batch_download(self.h, i, None, ctypes.byref(nw), ctypes.byref(nc), None, ctypes.byref(nr),
                                               None, None, ctypes.byref(nt)))
-        w = np.zeros(nw.value, np.uint32)
-        r = np.zeros(nr.value, np.uint32)
-        t = np.zeros(nt.value, np.uint32)
-        vs = np.zeros(int(self.inst_nvars[i]) + 1, np.uint8)
+        w = _new(pool, "words", nw.value, np.uint32)
+        r = _new(pool, "records", nr.value, np.uint32)
+        t = _new(pool, "trail", nt.value, np.uint32)
+        vs = _new(pool, "vstate", int(self.inst_nvars[i]) + 1, np.uint8)
         self._check(self.L.pfg_batch_download(self.h, i, w.ctypes.data, ctypes.byref(nw), ctypes.byref(nc), r.ctypes.data,
                                               ctypes.byref(nr), vs.ctypes.data, t.ctypes.data, ctypes.byref(nt)))
         return w, t, r, vs

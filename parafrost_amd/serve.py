@@ -31,7 +31,9 @@ pass; requests that arrive meanwhile wait for the next free slot, so the work of
 two batches overlaps on the device. The resident store has an HBM budget: the
 least recently used idle instances beyond it are moved to host memory and
 brought back by their next request. Latency of a request = results on the host
-- arrival.
+- arrival. Results (clauses, units, model records, variable states) land in host
+memory each slot reuses from batch to batch, valid until the slot's next batch;
+keep_results=True gives every request arrays of its own instead.
 """
 from __future__ import annotations
 
@@ -42,7 +44,7 @@ from collections import deque
 
 import numpy as np
 
-from . import BatchSimplifier, Simplifier, Unsat, default_opts
+from . import BatchSimplifier, HostPool, Simplifier, Unsat, default_opts
 from . import gen
 
 
@@ -241,6 +243,8 @@ class Server:
     def _serve(self, cap_vars: int):
         torch = self.torch
         ctx = self.ctx_cls([cap_vars], self.opts)
+        # results not kept: downloaded into this slot's reused host memory, valid until its next batch
+        pool = None if self.keep else HostPool()
         if self.dev.type == "cuda":
             torch.cuda.set_device(self.dev)
         try:
@@ -289,16 +293,19 @@ class Server:
                 td = time.perf_counter()
                 new = []
                 t_dl = t_ex = 0.0
+                if pool is not None:
+                    pool.reset()
                 for i, (r, res, _) in enumerate(work):
                     t0 = time.perf_counter()
-                    w, t, rec, vs = ctx.instance(i)
+                    w, t, rec, vs = ctx.instance(i, pool)
                     t1 = time.perf_counter()
                     t_dl += t1 - t0
                     r.status = int(st[i])
                     if r.status == 0:
                         lits, offs = ctx.export(i, self.dev)
                         t_ex += time.perf_counter() - t1
-                        new.append((r, Resident(lits, offs, None, None, t, r.t1)))
+                        # the units stay with the instance: its own copy, not the slot's reused memory
+                        new.append((r, Resident(lits, offs, None, None, t.copy(), r.t1)))
                     else:
                         new.append((r, Resident(None, None, None, None, np.zeros(0, np.uint32), r.t1, refuted=True)))
                     if self.keep:
@@ -379,8 +386,11 @@ class PlainChain:
     its requests (clauses written back from the downloaded database), the whole
     formula uploaded from host memory for every request."""
 
-    def __init__(self, cap_vars: int, opts=None):
+    def __init__(self, cap_vars: int, opts=None, reuse_host: bool = False):
+        """reuse_host: results are downloaded into reused host memory and stay valid
+        until the next run() (as a served batch's results until the slot's next batch)"""
         self.g = Simplifier(cap_vars, opts or default_opts())
+        self.pool = HostPool() if reuse_host else None
         self.state = {}   # key -> (Cnf of the simplified clauses, units, refuted)
 
     def close(self):
@@ -411,7 +421,9 @@ class PlainChain:
         except Unsat:
             self.state[r.key] = (None, None, True)
             return 4, None, int(cnf.nlits)
-        w, t, rec = g.cnf_words(), g.trail(), g.resolved()
+        if self.pool is not None:
+            self.pool.reset()
+        w, t, rec = g.cnf_words(self.pool), g.trail(self.pool), g.resolved(self.pool)
         vs = g.vstate()[:r.nvars + 1]
-        self.state[r.key] = (gen.images_to_csr(w, r.nvars), t, False)
+        self.state[r.key] = (gen.images_to_csr(w, r.nvars), t.copy(), False)
         return 0, (w, t, rec, vs), int(cnf.nlits)

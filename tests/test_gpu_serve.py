@@ -96,3 +96,21 @@ def test_serving_store_spills_to_host():
     stats = srv.run(reqs, rate=2000.0)
     assert stats["spills"] > 0
     _check_against_plain(reqs)
+
+
+def test_serving_with_reused_host_memory():
+    """Results not kept: every slot downloads into host memory it reuses from batch to
+    batch (so does the plain chain). The instances' state must not live there: the
+    statuses and formula sizes of a long extension chain equal the plain chain's."""
+    from parafrost_amd import serve
+    reqs = serve.make_requests(24, seed=34, gates=(800, 3000), max_frames=8, extend_frac=0.8)
+    serve.Server(slots=2, batch_lits=3e4).run(reqs, rate=2000.0)
+    p = serve.PlainChain(max(r.nvars for r in reqs), reuse_host=True)
+    try:
+        for r in reqs:
+            st, _, nl = p.run(r)
+            assert st == r.status, f"request {r.rid}: status plain {st}, served {r.status}"
+            if st == 0:
+                assert nl == r.nlits, f"request {r.rid}: formula size plain {nl}, served {r.nlits}"
+    finally:
+        p.close()

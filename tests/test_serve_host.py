@@ -123,7 +123,7 @@ class _HostContext:
             raise RuntimeError("device failure")
         return np.zeros(len(self.nv), np.int32)
 
-    def instance(self, i):
+    def instance(self, i, pool=None):
         return (np.zeros(3, np.uint32), np.array([2 * i + 2], np.uint32), np.zeros(0, np.uint32),
                 np.zeros(int(self.nv[i]) + 1, np.uint8))
 
@@ -161,3 +161,18 @@ def test_request_loop_reports_worker_failure():
             srv.run(_reqs(), rate=5000.0)
     finally:
         _HostContext.fail = False
+
+
+def test_host_pool_slices_stay_valid_until_reset():
+    from parafrost_amd import HostPool
+    p = HostPool()
+    a = p.take("w", 10, np.uint32)
+    a[:] = 7
+    b = p.take("w", 1 << 17, np.uint32)   # grows: a new array, a keeps its memory
+    b[:] = 9
+    assert (a == 7).all() and (b == 9).all() and not np.shares_memory(a, b)
+    c = p.take("w", 5, np.uint32)
+    assert not np.shares_memory(b, c)
+    p.reset()
+    d = p.take("w", 5, np.uint32)   # after reset the newest array is reused from its start
+    assert np.shares_memory(c, d)
