@@ -18,6 +18,8 @@
  * env PFREF_PROOF_MARK=<file>: with -proof --proofout=<path>, write the proof's
  *   length in bytes (u64) when the simplifier starts, so that the simplifier's part
  *   of the proof can be told from the parser's and shrinkTop's lines
+ * env PFREF_TIME=1: print "PFREF_TIME <seconds>" to stderr, the wall time of the
+ *   simplifier (rootify .. shrinkSimp; not the parse, not the dump)
  * env PFREF_FREEZE=<file>: u32 variables (little endian) frozen as by ifreeze()
  *   (src/cpu/assume.cpp:45-52) with the solver in incremental mode, so that
  *   iassumed() (src/cpu/solve.h:864-867) holds for them during LCVE
@@ -33,6 +35,7 @@
 #include <unordered_map>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <sys/stat.h>
 #include <string>
 #include <vector>
@@ -157,6 +160,7 @@ int main(int argc, char** argv)
 	};
 	/* ParaFROST::sigmifying(), src/cpu/simplify.cpp:150-214, without the timers,
 	   sleep/interrupt hooks and the write-back to the CDCL database. */
+	const auto t_start = std::chrono::steady_clock::now();
 	s->rootify();
 	s->shrinkTop(false);
 	if (const char* mk = getenv("PFREF_PROOF_MARK")) {
@@ -197,6 +201,9 @@ int main(int argc, char** argv)
 	if (!s->prop()) { dump(s, out, false, 1); return 0; }
 	s->countFinal();
 	s->shrinkSimp();
+	if (getenv("PFREF_TIME"))
+		fprintf(stderr, "PFREF_TIME %.6f\n",
+			std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count());
 	if (getenv("PFREF_MAP")) {
 		s->map(true);
 		dump_mapped(s, out);

@@ -125,3 +125,21 @@ def test_mid_size_proof(seed):
     o.sigmify()
     assert len(ref.proof) > 0
     assert np.array_equal(o.proof(), ref.proof)
+
+
+def test_reference_timer_for_the_cpu_baseline():
+    """PFREF_TIME makes the reference build report the simplifier's own wall time (the
+    CPU baseline of bench.py); the state it dumps is unchanged."""
+    import subprocess
+    cnf = gen.config("c1")
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "f.cnf")
+        cnf.write_dimacs(p)
+        env = dict(os.environ, PFREF_TIME="1")
+        r = subprocess.run([po.REF_BIN, p, os.path.join(td, "t.bin")], env=env, capture_output=True, text=True,
+                           timeout=300)
+        t = [float(x.split()[1]) for x in r.stderr.splitlines() if x.startswith("PFREF_TIME ")]
+        assert r.returncode == 0 and len(t) == 1 and 0 < t[0] < 60
+        ref = po.run_reference(p, os.path.join(td, "f.bin"), po.default_opts())
+        timed = po.read_dump(os.path.join(td, "t.bin"))
+        assert np.array_equal(ref.cnf, timed.cnf) and np.array_equal(ref.resolved, timed.resolved)
