@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--c5-requests", type=int, default=64, help="c5: requests in the stream")
     ap.add_argument("--c5-rates", default="50,200,500,2000", help="c5: offered loads swept (requests / s)")
+    ap.add_argument("--c5-slots", type=int, default=3, help="c5: batches in flight (one context each)")
+    ap.add_argument("--c5-batch-lits", type=float, default=2.5e8, help="c5: literal budget of one batch")
     ap.add_argument("--batch-lits", type=float, default=1.0e9,
                     help="c4: literal budget of one batch (instances are admitted in order until it is reached)")
     ap.add_argument("--c4-instances", type=int, default=4096, help="c4: number of instances")
@@ -331,7 +333,7 @@ def run_c5(args):
     1-64 frames; half of the requests extend an instance served before by 1-8 frames;
     seed 5), replayed as a Poisson stream at each offered load. Each request simplifies
     its instance's resident simplified clauses plus its new frames (units enqueued,
-    interface variables frozen). Two slots of continuous batching, 2.5e8 literals per
+    interface variables frozen). Three slots of continuous batching (--c5-slots), 2.5e8 literals per
     batch. Latency = results on the host - arrival (both paths download results into
     host memory they reuse from call to call). The plain path runs the same
     request chain one request at a time in a single-instance context with the state
@@ -350,7 +352,7 @@ def run_c5(args):
         for r in reqs:
             r.done = r.arrival = 0.0
             r.status, r.result, r.nlits = -1, None, 0
-        st = serve.Server(slots=2, batch_lits=2.5e8).run(reqs, rate=rate)
+        st = serve.Server(slots=args.c5_slots, batch_lits=args.c5_batch_lits).run(reqs, rate=rate)
         sweep.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()})
         print(f"# c5 rate {rate}: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms "
               f"{st['literals_per_s'] / 1e6:.1f} M literals/s ({st['literals']} literals)", file=sys.stderr, flush=True)
@@ -371,8 +373,8 @@ def run_c5(args):
         "value": best["literals_per_s"], "unit": "literals/s", "n_gpus": 1, "steps": len(rates), "warmup": 1,
         "ms_per_step": best["span_s"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic",
-        "config": {"path": "served (incremental, HBM-resident instances, continuous batching, 2 slots, "
-                           "2.5e8 literals per batch)",
+        "config": {"path": f"served (incremental, HBM-resident instances, continuous batching, {args.c5_slots} slots, "
+                           f"{args.c5_batch_lits:.3g} literals per batch)",
                    "workload": f"{len(reqs)} incremental BMC-like requests: random AIGs 10k-100k ANDs, 30% inverted "
                                "edges, 64 latches, Tseitin, 1-64 frames, half extend an instance served before; seed 5",
                    "requests": len(reqs), "literals": served_lits, "generation_s": round(gen_s, 1)},

@@ -28,7 +28,7 @@ Worker slots each own a batch context (its own HIP stream). A free slot admits
 the oldest waiting requests whose instance is idle until the batch's literal
 budget is reached, lays its context out for them and simplifies them in one
 pass; requests that arrive meanwhile wait for the next free slot, so the work of
-two batches overlaps on the device. The resident store has an HBM budget: the
+the slots' batches overlaps on the device. The resident store has an HBM budget: the
 least recently used idle instances beyond it are moved to host memory and
 brought back by their next request. Latency of a request = results on the host
 - arrival. Results (clauses, units, model records, variable states) land in host
