@@ -1162,20 +1162,6 @@ struct KeyLess {
     }
 };
 
-template <typename LT>
-__device__ void ins_sort_sorth(uint64_t *d, int n, LT lt)
-{   // insertionSort (src/cpu/sort.h:41-54)
-    if (n == 2 && lt(d[1], d[0])) { uint64_t t = d[1]; d[1] = d[0]; d[0] = t; }
-    else if (n > 2) {
-        for (int i = 1; i < n; i++) {
-            const uint64_t t = d[i];
-            int j;
-            for (j = i; j > 0 && lt(t, d[j - 1]); j--) d[j] = d[j - 1];
-            d[j] = t;
-        }
-    }
-}
-
 // libstdc++ std::sort (introsort) order, restated for the device
 template <typename LT>
 __device__ void is_adjust_heap(uint64_t *f, int64_t hole, int64_t len, uint64_t value, LT lt)
@@ -1304,9 +1290,28 @@ __device__ void stable_sort_dev(uint64_t *d, uint64_t *tmp, int64_t n, LT lt)
     }
 }
 
-__global__ void k_sort_pv_lists(const uint32_t *__restrict__ pvs, uint32_t npvs, const OL *__restrict__ ot, uint64_t *occ,
-                                const uint32_t *__restrict__ mem, int stable, uint64_t *tmp)
+// A list of up to SORT_SHORT entries is sorted in LDS: each entry's CNF_CMP_KEY is
+// read from its clause once, as two words compared in order (size, first literal |
+// last literal, signature), so the insertion sort compares registers instead of
+// gathering clause headers per comparison; the comparisons and their outcomes are
+// those of KeyLess, so the order is the same.
+constexpr int SORT_SHORT = 20;
+
+__device__ __forceinline__ void short_key(const uint32_t *mem, uint64_t r, uint64_t &hi, uint64_t &lo)
 {
+    const uint32_t *c = mem + r;
+    const uint32_t sz = c[2];
+    const uint32_t first = sz ? c[3] : 0u, last = sz ? c[3 + sz - 1] : 0u;
+    hi = ((uint64_t)sz << 32) | first;
+    lo = ((uint64_t)last << 32) | c[1];
+}
+
+__global__ void __launch_bounds__(64) k_sort_pv_lists(const uint32_t *__restrict__ pvs, uint32_t npvs,
+                                                     const OL *__restrict__ ot, uint64_t *occ,
+                                                     const uint32_t *__restrict__ mem, int stable, uint64_t *tmp)
+{
+    __shared__ uint64_t s_hi[SORT_SHORT][64], s_lo[SORT_SHORT][64], s_ref[SORT_SHORT][64];
+    const int lane = threadIdx.x;
     const KeyLess lt{mem};
     for (uint64_t q = gtid(); q < 2ull * npvs; q += gstride()) {
         const uint32_t lit = 2 * pvs[q >> 1] + (uint32_t)(q & 1);
@@ -1314,7 +1319,34 @@ __global__ void k_sort_pv_lists(const uint32_t *__restrict__ pvs, uint32_t npvs,
         uint64_t *d = occ + o.off;
         const int64_t n = o.sz;
         if (n <= 1) continue;
-        if (n <= 20) ins_sort_sorth(d, (int)n, lt);
+        if (n <= SORT_SHORT) {   // insertionSort (src/cpu/sort.h:41-54) on (key, reference)
+            for (int i = 0; i < (int)n; i++) {
+                const uint64_t r = d[i];
+                short_key(mem, r, s_hi[i][lane], s_lo[i][lane]);
+                s_ref[i][lane] = r;
+            }
+            auto less = [&](uint64_t ah, uint64_t al, int j) {
+                const uint64_t bh = s_hi[j][lane];
+                return ah < bh || (ah == bh && al < s_lo[j][lane]);
+            };
+            if (n == 2) {
+                if (less(s_hi[1][lane], s_lo[1][lane], 0)) { d[0] = s_ref[1][lane]; d[1] = s_ref[0][lane]; }
+                continue;
+            }
+            for (int i = 1; i < (int)n; i++) {
+                const uint64_t th = s_hi[i][lane], tl = s_lo[i][lane], tr = s_ref[i][lane];
+                int j;
+                for (j = i; j > 0 && less(th, tl, j - 1); j--) {
+                    s_hi[j][lane] = s_hi[j - 1][lane];
+                    s_lo[j][lane] = s_lo[j - 1][lane];
+                    s_ref[j][lane] = s_ref[j - 1][lane];
+                }
+                s_hi[j][lane] = th;
+                s_lo[j][lane] = tl;
+                s_ref[j][lane] = tr;
+            }
+            for (int i = 0; i < (int)n; i++) d[i] = s_ref[i][lane];
+        }
         else if (stable) stable_sort_dev(d, tmp + o.off, n, lt);
         else introsort_dev(d, n, lt);
     }
