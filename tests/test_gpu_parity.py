@@ -335,6 +335,16 @@ def test_unsat_by_propagation():
     g.close()
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_lockstep_adversarial(seed):
+    print(f"lockstep adversarial seed={seed}", flush=True)
+    from adversarial import adversarial
+    cnf = adversarial(seed)
+    rng = np.random.default_rng(seed)
+    ls.run_lockstep(cnf, phases=int(rng.integers(1, 6)), mu_pos=int(rng.integers(10, 80)),
+                    mu_neg=int(rng.integers(10, 80)), bce_en=seed % 2, proof_en=(seed // 2) % 2)
+
+
 def test_empty_and_tiny():
     import parafrost_amd as pf
     g = pf.Simplifier(5)

@@ -143,3 +143,27 @@ def test_reference_timer_for_the_cpu_baseline():
         ref = po.run_reference(p, os.path.join(td, "f.bin"), po.default_opts())
         timed = po.read_dump(os.path.join(td, "t.bin"))
         assert np.array_equal(ref.cnf, timed.cnf) and np.array_equal(ref.resolved, timed.resolved)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_adversarial(seed):
+    """Duplicate clauses, clauses past SUB's size break, a hub variable, pure literals,
+    implication chains; half with BCE on, half with the proof compared too."""
+    from adversarial import adversarial
+    rng = np.random.default_rng(seed)
+    opts = po.default_opts(phases=int(rng.integers(1, 6)), mu_pos=int(rng.integers(10, 80)),
+                           mu_neg=int(rng.integers(10, 80)), bce_en=seed % 2, proof_en=(seed // 2) % 2)
+    cnf = adversarial(seed)
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "f.cnf")
+        cnf.write_dimacs(p)
+        ref = po.run_reference(p, os.path.join(td, "f.bin"), opts)
+    o = po.Oracle(cnf, opts)
+    rc = o.sigmify()
+    st = o.state()
+    assert (rc != 0) == bool(ref.unsat)
+    if not ref.unsat:
+        for f in ("cnf", "trail", "resolved", "vstate"):
+            assert np.array_equal(getattr(st, f), getattr(ref, f)), f
+    if opts.proof_en:
+        assert np.array_equal(o.proof(), ref.proof)
