@@ -411,3 +411,48 @@ def test_c3_full_matches_oracle_and_invariants():
         assert k >= 1
         j -= k + 1
     assert j == -1
+
+
+def test_hbm_budget_refuses_cleanly():
+    """A context whose HBM budget is too small for the formula reports PFG_ENOMEM
+    (the reference's CNFALLOC_FAIL: the solver goes on without simplifying). The
+    budget counts what the context holds, including buffers grown before the refusal:
+    the same context then either simplifies a smaller formula exactly or refuses it
+    the same way, and a fresh context with that budget simplifies it exactly."""
+    import parafrost_amd as pf
+    big = gen.geometric(200_000, 840_000, 2, 8, seed=2)   # needs well over 100 MB
+    s5, _ = _small(5)
+    small = gen.Cnf(big.nvars, s5.lits, s5.offs)           # a few hundred of the variables used
+    budget = 48 << 20
+    o = po.Oracle(small, po.default_opts())
+    ro = o.sigmify()
+
+    def run_small(g):
+        g.upload(small)
+        try:
+            g.sigmify()
+            rg = 0
+        except pf.Unsat:
+            rg = -1
+        assert (ro != 0) == (rg != 0)
+        if ro == 0:
+            ls.compare(o.state(), ls.state_gpu(g), "small formula")
+
+    g = pf.Simplifier(big.nvars, pf.default_opts(), hbm_budget=budget)
+    try:
+        with pytest.raises(pf.SimplifierError) as e:
+            g.upload(big)
+            g.sigmify()
+        assert e.value.code == pf.PFG_ENOMEM and "budget" in str(e.value)
+        g.reset_for_upload()
+        try:
+            run_small(g)
+        except pf.SimplifierError as e2:
+            assert e2.code == pf.PFG_ENOMEM, str(e2)
+    finally:
+        g.close()
+    g = pf.Simplifier(big.nvars, pf.default_opts(), hbm_budget=budget)
+    try:
+        run_small(g)
+    finally:
+        g.close()
