@@ -11,9 +11,12 @@ the whole round loop runs on the device. Inputs are resident in HBM before the
 timed region. Metric: input literals / second.
 
 Prints one JSON line (rank 0). Extra objects:
-  roofline     -- HBM roofline of the dominant kernel (see DESIGN.md for bytes)
-  cpu_baseline -- the oracle (C restatement of the reference CPU simplifier),
-                  single thread, on a bounded sample of the same workload.
+  roofline     -- HBM roofline of the dominant kernel: algorithmic bytes per launch
+                  (computed on the device by one extra, untimed step) over the
+                  kernel's average HIP-event duration, against 8 TB/s
+  cpu_baseline -- the reference CPU simplifier built from its sources (or, where
+                  that build is absent, the oracle, its C restatement), single
+                  thread, on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -413,8 +416,8 @@ def cpu_baseline_instances(B):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
-                                   "(see DESIGN.md), --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU (sharding across GPUs is designed, not built); "
+                                   "--gpus must be 1"}))
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
