@@ -135,6 +135,16 @@ def test_batch_bce():
     _check(cnfs, bce_en=1, phases=4)
 
 
+def test_batch_adversarial():
+    """Duplicate clauses, clauses past the subsumption size break, a hub variable, pure
+    literals and implication chains, batched with ordinary and refuted instances."""
+    from adversarial import adversarial
+    rng = np.random.default_rng(4504)
+    cnfs = [adversarial(s) for s in range(4)] + [_instance(rng, k) for k in range(6)] + [_full_3cnf()]
+    _check(cnfs, phases=4)
+    _check(cnfs[:4], bce_en=1, phases=3, mu_pos=60, mu_neg=60)
+
+
 def test_batch_relayout_reuses_context():
     """One context laid out again for another set of instances gives the same results."""
     import parafrost_amd as pf
