@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--c4-instances", type=int, default=4096, help="c4: number of instances")
     ap.add_argument("--plain-sample", type=int, default=64, help="c4: instances timed on the plain path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--bce", action="store_true", help="c1-c3: blocked clause elimination on (off by default, as in the reference)")
+    ap.add_argument("--proof", action="store_true", help="c1-c3: binary DRAT proof output on (reports its size)")
     ap.add_argument("--check", action="store_true", help="compare the final state with the oracle")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE counters)")
@@ -445,7 +447,7 @@ def main():
     d_offs = torch.from_numpy(cnf.offs.view(np.int64)).to(dev)
     torch.cuda.synchronize()
 
-    s = pf.Simplifier(cnf.nvars, pf.default_opts())
+    s = pf.Simplifier(cnf.nvars, pf.default_opts(bce_en=int(args.bce), proof_en=int(args.proof)))
 
     outcome = {"refuted": False}
 
@@ -503,7 +505,8 @@ def main():
                                 "c2": "mixed-width CNF 2M vars / 8.4M clauses, widths 2-8 geometric, seed 2",
                                 "c3": "scale-free CNF 5M vars / 20M clauses, 20% Tseitin gates, seed 3"}[args.config],
                    "input_literals": L, "variables": cnf.nvars, "clauses": cnf.ncls,
-                   "passes": "OT+prop+LCVE+sortOT+SUB+BVE(gates)+ERE+GC, 5 phases",
+                   "passes": "OT+prop+LCVE+sortOT+SUB+BVE(gates)" + ("+BCE" if args.bce else "") + "+ERE+GC, 5 phases"
+                             + (", DRAT proof output" if args.proof else ""),
                    "clauses_out": stats["clauses"], "literals_out": stats["literals"],
                    "eliminated": stats["eliminated"], "pass_ms_per_step": {k: round(v / args.steps, 3) for k, v in times.items()},
                    "outcome": ("refuted: the formula is unsatisfiable; a step is the rounds up to the refutation"
@@ -511,6 +514,8 @@ def main():
         "roofline": roof,
         "kernels": kernels,
     }
+    if args.proof:
+        out["config"]["proof_bytes_per_step"] = int(s.proof().size)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config)
     if args.check:
