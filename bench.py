@@ -52,7 +52,6 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--bce", action="store_true", help="c1-c3: blocked clause elimination on (off by default, as in the reference)")
     ap.add_argument("--proof", action="store_true", help="c1-c3: binary DRAT proof output on (reports its size)")
-    ap.add_argument("--check", action="store_true", help="compare the final state with the oracle")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--summarize-trace", metavar="KERNEL_STATS_CSV",
@@ -518,15 +517,6 @@ def main():
         out["config"]["proof_bytes_per_step"] = int(s.proof().size)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config)
-    if args.check:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as po
-        o = po.Oracle(cnf, po.default_opts())
-        o.sigmify()
-        so = o.state()
-        w = s.cnf_words()
-        out["check"] = bool(np.array_equal(so.cnf, w) and np.array_equal(so.resolved, s.resolved())
-                            and np.array_equal(so.vstate, s.vstate()))
     print(json.dumps(out))
 
 

@@ -385,6 +385,14 @@ def test_c2_full_matches_oracle():
 
 
 @pytest.mark.slow
+def test_c2_full_bce_and_proof_match_oracle():
+    """The benchmark workload with both optional passes on: BCE (nothing is blocked
+    after VE on c2) and the DRAT proof (280,056 lines), state and proof bit for bit."""
+    print("c2 full, BCE and proof", flush=True)
+    ls.run_full(gen.config("c2"), bce_en=1, proof_en=1)
+
+
+@pytest.mark.slow
 def test_c3_full_matches_oracle_and_invariants():
     """Scale-free 5M vars / 20M clauses: oracle parity plus size-independent checks:
     eliminated variables occur in no surviving clause, literals sorted and unique
