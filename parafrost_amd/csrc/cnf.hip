@@ -262,16 +262,47 @@ __global__ void k_live_sizes(const uint32_t *__restrict__ mem, const uint64_t *_
     }
 }
 
-__global__ void k_ot_expand(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
-                            const uint64_t *__restrict__ pos, uint64_t *__restrict__ keys)
+// One wave per 64 consecutive clauses: their keys form one contiguous run (positions
+// are the scan of the live sizes), written in order with one key per lane and step,
+// so stores and literal loads are contiguous across the wave; a lane finds its clause
+// by a binary search over the wave's prefix of sizes (the last clause whose prefix is
+// not past the key, which skips deleted and empty clauses).
+__global__ void __launch_bounds__(256) k_ot_expand(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs,
+                                                  uint64_t n, const uint64_t *__restrict__ pos, uint64_t *__restrict__ keys)
 {
-    for (uint64_t i = gtid(); i < n; i += gstride()) {
-        const uint64_t r = refs[i];
-        const uint32_t *c = mem + r;
-        if (c_del(c)) continue;
-        const int sz = c_size(c);
-        uint64_t *k = keys + pos[i];
-        for (int t = 0; t < sz; t++) k[t] = ((uint64_t)c[3 + t] << 32) | r;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    for (uint64_t base = wave * 64; base < n; base += nwaves * 64) {
+        const uint64_t i = base + lane;
+        uint64_t r = 0;
+        uint32_t sz = 0;
+        if (i < n) {
+            r = refs[i];
+            const uint32_t *c = mem + r;
+            sz = c_del(c) ? 0u : (uint32_t)c_size(c);
+        }
+        uint32_t incl = sz;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if ((int)lane >= o) incl += t;
+        }
+        const uint32_t pre = incl - sz, total = __shfl(incl, 63, 64);
+        const uint64_t out0 = pos[base];
+        const uint32_t rlo = (uint32_t)r, rhi = (uint32_t)(r >> 32);
+        for (uint32_t ob = 0; ob < total; ob += 64) {
+            const uint32_t o = ob + lane;
+            uint32_t m = 0;
+#pragma unroll
+            for (uint32_t step = 32; step > 0; step >>= 1) {
+                const uint32_t cand = m + step;   // at most 63
+                if (__shfl(pre, (int)cand, 64) <= o) m = cand;
+            }
+            const uint32_t pm = __shfl(pre, (int)m, 64);
+            const uint64_t rm = ((uint64_t)(uint32_t)__shfl(rhi, (int)m, 64) << 32) | (uint32_t)__shfl(rlo, (int)m, 64);
+            if (o < total) keys[out0 + o] = ((uint64_t)mem[rm + 3 + (o - pm)] << 32) | rm;
+        }
     }
 }
 
