@@ -992,18 +992,48 @@ __global__ void k_gc_sizes(const uint32_t *__restrict__ mem, const uint64_t *__r
     }
 }
 
-__global__ void k_gc_move(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
-                          const uint64_t *__restrict__ woff, const uint64_t *__restrict__ idx,
-                          const uint32_t *__restrict__ live, uint32_t *__restrict__ mem2, uint64_t *__restrict__ refs2)
+// One wave per 64 consecutive clauses: their live images form one contiguous run of
+// the new pool (woff is the scan of the live image sizes), copied one word per lane and
+// step, so loads and stores are contiguous across the wave; a lane finds the clause of
+// its word by a binary search over the wave's prefix of image sizes.
+__global__ void __launch_bounds__(256) k_gc_move(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs,
+                                                uint64_t n, const uint64_t *__restrict__ woff,
+                                                const uint64_t *__restrict__ idx, const uint32_t *__restrict__ live,
+                                                uint32_t *__restrict__ mem2, uint64_t *__restrict__ refs2)
 {
-    // one thread per clause: neighbouring threads copy neighbouring clauses
-    for (uint64_t i = gtid(); i < n; i += gstride()) {
-        if (!live[i]) continue;
-        const uint32_t *c = mem + refs[i];
-        const uint64_t words = 3ull + (uint64_t)c_size(c);
-        uint32_t *d = mem2 + woff[i];
-        for (uint64_t k = 0; k < words; k++) d[k] = c[k];
-        refs2[idx[i]] = woff[i];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    for (uint64_t base = wave * 64; base < n; base += nwaves * 64) {
+        const uint64_t i = base + lane;
+        uint64_t r = 0;
+        uint32_t words = 0;
+        if (i < n && live[i]) {
+            r = refs[i];
+            words = 3u + (uint32_t)c_size(mem + r);
+            refs2[idx[i]] = woff[i];
+        }
+        uint32_t incl = words;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if ((int)lane >= o) incl += t;
+        }
+        const uint32_t pre = incl - words, total = __shfl(incl, 63, 64);
+        const uint64_t out0 = woff[base];
+        const uint32_t rlo = (uint32_t)r, rhi = (uint32_t)(r >> 32);
+        for (uint32_t ob = 0; ob < total; ob += 64) {
+            const uint32_t o = ob + lane;
+            uint32_t m = 0;
+#pragma unroll
+            for (uint32_t step = 32; step > 0; step >>= 1) {
+                const uint32_t cand = m + step;   // at most 63
+                if (__shfl(pre, (int)cand, 64) <= o) m = cand;
+            }
+            const uint32_t pm = __shfl(pre, (int)m, 64);
+            const uint64_t rm = ((uint64_t)(uint32_t)__shfl(rhi, (int)m, 64) << 32) | (uint32_t)__shfl(rlo, (int)m, 64);
+            if (o < total) mem2[out0 + o] = mem[rm + (o - pm)];
+        }
     }
 }
 
