@@ -1381,7 +1381,29 @@ __global__ void __launch_bounds__(64) k_sort_pv_lists(const uint32_t *__restrict
         const int64_t n = o.sz;
         if (n <= 1) continue;
         if (n <= SORT_SHORT) {   // insertionSort (src/cpu/sort.h:41-54) on (key, reference)
-            for (int i = 0; i < (int)n; i++) {
+            int i = 0;
+            for (; i + 4 <= (int)n; i += 4) {   // four entries' loads in flight together
+                const uint64_t r0 = d[i], r1 = d[i + 1], r2 = d[i + 2], r3 = d[i + 3];
+                const uint32_t *c0 = mem + r0, *c1 = mem + r1, *c2 = mem + r2, *c3 = mem + r3;
+                const uint32_t z0 = c0[2], z1 = c1[2], z2 = c2[2], z3 = c3[2];
+                const uint32_t f0 = z0 ? c0[3] : 0u, f1 = z1 ? c1[3] : 0u, f2 = z2 ? c2[3] : 0u, f3 = z3 ? c3[3] : 0u;
+                const uint32_t g0 = c0[1], g1 = c1[1], g2 = c2[1], g3 = c3[1];
+                const uint32_t l0 = z0 ? c0[2 + z0] : 0u, l1 = z1 ? c1[2 + z1] : 0u, l2 = z2 ? c2[2 + z2] : 0u,
+                               l3 = z3 ? c3[2 + z3] : 0u;
+                s_hi[i][lane] = ((uint64_t)z0 << 32) | f0;
+                s_hi[i + 1][lane] = ((uint64_t)z1 << 32) | f1;
+                s_hi[i + 2][lane] = ((uint64_t)z2 << 32) | f2;
+                s_hi[i + 3][lane] = ((uint64_t)z3 << 32) | f3;
+                s_lo[i][lane] = ((uint64_t)l0 << 32) | g0;
+                s_lo[i + 1][lane] = ((uint64_t)l1 << 32) | g1;
+                s_lo[i + 2][lane] = ((uint64_t)l2 << 32) | g2;
+                s_lo[i + 3][lane] = ((uint64_t)l3 << 32) | g3;
+                s_ref[i][lane] = r0;
+                s_ref[i + 1][lane] = r1;
+                s_ref[i + 2][lane] = r2;
+                s_ref[i + 3][lane] = r3;
+            }
+            for (; i < (int)n; i++) {
                 const uint64_t r = d[i];
                 short_key(mem, r, s_hi[i][lane], s_lo[i][lane]);
                 s_ref[i][lane] = r;
