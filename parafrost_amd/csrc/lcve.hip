@@ -146,13 +146,23 @@ __global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t 
             c = A.mem + refs[i];
             if (!c_del(c)) {
                 sz = c_size(c);
-                for (int t = 0; t < sz; t++) {
-                    const uint32_t x = A.crank[c[3 + t] >> 1];
-                    if (x == NO_CAND) continue;
+                auto take = [&](uint32_t x) {
+                    if (x == NO_CAND) return;
                     if (k < (uint32_t)ELECT_CACHE) s_cr[k][tid] = x;
                     k++;
                     if ((x & 1u) && (x >> 1) < m) m = x >> 1;
+                };
+                int t = 0;
+                for (; t + 4 <= sz; t += 4) {   // four rank gathers in flight, taken in order
+                    const uint32_t a0 = c[3 + t], a1 = c[4 + t], a2 = c[5 + t], a3 = c[6 + t];
+                    const uint32_t x0 = A.crank[a0 >> 1], x1 = A.crank[a1 >> 1], x2 = A.crank[a2 >> 1],
+                                   x3 = A.crank[a3 >> 1];
+                    take(x0);
+                    take(x1);
+                    take(x2);
+                    take(x3);
                 }
+                for (; t < sz; t++) take(A.crank[c[3 + t] >> 1]);
             }
         }
         const bool keep = k >= 2u;
