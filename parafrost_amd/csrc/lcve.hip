@@ -217,9 +217,8 @@ __global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32
             constexpr uint32_t KR = 8;   // undecided candidates of the first KR entries are kept in registers
             uint32_t und[KR];
             uint32_t nund = 0;
-            for (uint32_t t = 1; t <= k; t++) {
-                const uint32_t x = __builtin_nontemporal_load(rr + t), j = x >> 1;
-                const uint8_t st = A.status[j];
+            auto take = [&](uint32_t t, uint32_t x, uint8_t st) {
+                const uint32_t j = x >> 1;
                 const bool freezer = x & 1u;
                 if (st == S_IN && freezer) in_freezer = true;
                 if (st == S_UNDEC) {
@@ -228,6 +227,21 @@ __global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32
                     if (t <= KR) und[nund++] = j;
                 }
                 if (st == S_UNDEC || (st == S_IN && freezer)) alive++;
+            };
+            uint32_t t = 1;
+            for (; t + 3 <= k; t += 4) {   // four codes and their status gathers in flight, taken in order
+                const uint32_t x0 = __builtin_nontemporal_load(rr + t), x1 = __builtin_nontemporal_load(rr + t + 1),
+                               x2 = __builtin_nontemporal_load(rr + t + 2), x3 = __builtin_nontemporal_load(rr + t + 3);
+                const uint8_t s0 = A.status[x0 >> 1], s1 = A.status[x1 >> 1], s2 = A.status[x2 >> 1],
+                              s3 = A.status[x3 >> 1];
+                take(t, x0, s0);
+                take(t + 1, x1, s1);
+                take(t + 2, x2, s2);
+                take(t + 3, x3, s3);
+            }
+            for (; t <= k; t++) {
+                const uint32_t x = __builtin_nontemporal_load(rr + t);
+                take(t, x, A.status[x >> 1]);
             }
             if (undec) {
                 // a candidate seen undecided above that another record has marked out since
