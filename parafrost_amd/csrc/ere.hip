@@ -190,9 +190,25 @@ __global__ void k_ere_probe(EreArgs A, const uint32_t *__restrict__ pvs, uint32_
         EreRes R;
         if (!ere_resolvent<false>(A, x, ci, cj, R)) continue;
         const uint64_t off = A.ot[R.best].off;
-        for (uint32_t k = 0; k < R.minsize; k++) {
-            if (ere_match(A, x, ci, cj, R, A.mem + A.occ[off + k])) { flag[q] = 1u; break; }
+        // the probe asks whether any clause matches, so four candidates' headers are
+        // loaded together and checked before the merge
+        bool hit = false;
+        uint32_t k = 0;
+        for (; k + 4 <= R.minsize && !hit; k += 4) {
+            const uint32_t *c0 = A.mem + A.occ[off + k], *c1 = A.mem + A.occ[off + k + 1],
+                           *c2 = A.mem + A.occ[off + k + 2], *c3 = A.mem + A.occ[off + k + 3];
+            const uint32_t *cc[4] = {c0, c1, c2, c3};
+            const uint32_t h[4] = {c0[0], c1[0], c2[0], c3[0]}, z[4] = {c0[2], c1[2], c2[2], c3[2]},
+                           g[4] = {c0[1], c1[1], c2[1], c3[1]};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const bool head = (int)z[u] == R.len && ((h[u] & ST_LEARNT) || (h[u] & 3u) == R.type) &&
+                                  sig_sub(R.msig, g[u]);
+                if (!hit && head && merge_ere(x, ci, cj, cc[u])) hit = true;
+            }
         }
+        for (; k < R.minsize && !hit; k++) hit = ere_match(A, x, ci, cj, R, A.mem + A.occ[off + k]);
+        if (hit) flag[q] = 1u;
     }
 }
 
