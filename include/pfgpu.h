@@ -33,7 +33,8 @@ enum {
     PFG_ENOMEM = 2,     /* HBM budget exceeded (reference CNFALLOC/OTALLOC_FAIL) */
     PFG_EDEVICE = 3,    /* HIP runtime error                                  */
     PFG_EUNSAT = 4,     /* formula proven unsatisfiable (reference learnEmpty)   */
-    PFG_EINTERNAL = 5   /* an internal invariant failed                      */
+    PFG_EINTERNAL = 5,  /* an internal invariant failed                      */
+    PFG_EINTR = 6       /* stopped by pfg_interrupt (reference interrupted()) */
 };
 
 typedef struct pfg_ctx pfg_ctx;
@@ -171,6 +172,32 @@ int pfg_download_pvs(pfg_ctx *ctx, uint32_t *pvs, uint32_t *n);
 int pfg_download_ot_list(pfg_ctx *ctx, uint32_t lit, int64_t *idx, uint32_t *n);
 
 int pfg_times_get(pfg_ctx *ctx, pfg_times *t);
+
+/* ParaFROST's interrupted() (src/gpu/solve.h:181; polled before every round,
+ * src/gpu/simplify.cu:182, and before SUB/VE/BCE/ERE, src/gpu/elimination.cu:28,55,65,76,
+ * where the reference then ends the solver): asks the running or next pfg_sigmify /
+ * pfg_sigmify_batch on the context to stop at its next poll. That call leaves the round
+ * loop, propagates pending units, compacts the clause database as at a normal end and
+ * returns PFG_EINTR (per instance: PFG_EINTR for instances that were still running);
+ * the context then holds a valid, partly simplified formula. The request is consumed
+ * by the call it stops. Safe to call from another host thread. */
+int pfg_interrupt(pfg_ctx *ctx);
+
+/* The solver's variable mapping (vorg, src/cpu/solve.h:76 / src/gpu/simplify.cu:141-146
+ * with initDevVorg, src/gpu/kernels.cu:739): vorg[v] = the original variable of context
+ * variable v, n = nvars + 1 entries (vorg[0] unused). After a map(true) the solver's
+ * variables are renumbered, and model records (saveWitness/saveLiteral,
+ * src/cpu/model.cpp:263-280; ORIGINIZELIT, src/gpu/primitives.cuh:36-46) and proof lines
+ * (proof.cpp:118-135) are written in original numbering through it. vorg NULL restores
+ * the identity. Single-instance contexts; pfg_reset clears it. */
+int pfg_set_vorg(pfg_ctx *ctx, const uint32_t *vorg, uint32_t n);
+
+/* How often the less common code paths ran since pfg_create / pfg_reset_stats (no
+ * counterpart in the reference): [0] extra ERE claiming rounds (run when a target was
+ * claimed by two elected variables), [1] unit-capable SUB variables run alone on one
+ * wave, [2] of those, the ones whose lists exceed the wave's LDS arrays (sub_limit
+ * above 3000). *n = the number of counters; at most max are written. */
+int pfg_path_counts(pfg_ctx *ctx, uint64_t *counts, int max, int *n);
 
 /* per-kernel device time (HIP events around each launch) and algorithmic HBM bytes.
  * kernel = the one HIP kernel the timer brackets, or NULL when it spans several launches.

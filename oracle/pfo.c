@@ -58,6 +58,7 @@ struct pfo {
     int64_t nclauses, nliterals;
     uint32_t *outc; int32_t outn, outcap;
     uint8_t *proof; uint64_t nproof, proofcap; int proof_closed;   /* binary DRAT (proof_en) */
+    uint32_t *vorg;            /* variable -> original variable (NULL: identity), solve.h:76 */
 };
 
 static inline uint32_t *CL(const pfo *s, uint64_t r) { return s->mem + r; }
@@ -345,6 +346,12 @@ pfo *pfo_create(uint32_t nvars, const pfo_opts *o, const uint32_t *lits, const u
     pfo *s = (pfo *)calloc(1, sizeof(pfo));
     s->nvars = nvars;
     if (o) s->opts = *o; else pfo_default_opts(&s->opts);
+    /* the option checks of the reference (src/cpu/options.cpp:305-310; ve_en stands
+       for ve_en || ve_plus_en) */
+    const int any = s->opts.ve_en || s->opts.sub_en || s->opts.bce_en;
+    if (!s->opts.phases && any) s->opts.phases = 1;
+    if (s->opts.phases && !any) s->opts.phases = 0;
+    if (s->opts.phases > 1 && !s->opts.ve_en) s->opts.phases = 1;
     const size_t nlit = 2 * ((size_t)nvars + 1);
     s->ot = (ol_t *)calloc(nlit, sizeof(ol_t));
     s->ps = (uint32_t *)calloc((size_t)nvars + 1, sizeof(uint32_t));
@@ -382,14 +389,15 @@ void pfo_destroy(pfo *s)
     for (size_t i = 0; i < nlit; i++) free(s->ot[i].d);
     free(s->ot); free(s->ps); free(s->ns); free(s->value); free(s->vstate); free(s->frozen); free(s->ifrozen);
     free(s->trail); free(s->pvs); free(s->eligible); free(s->scores); free(s->fstack);
-    free(s->mem); free(s->scnf); free(s->resolved); free(s->outc); free(s->proof);
+    free(s->mem); free(s->scnf); free(s->resolved); free(s->outc); free(s->proof); free(s->vorg);
     free(s);
 }
 
 /* ------------------------------------------------------------------ */
-/* binary DRAT proof (src/cpu/proof.cpp:92-232) with the identity      */
-/* variable mapping: a line is 'a' or 'd', each literal 2v+sign as a   */
-/* little-endian base-128 varint (high bit = more), then a zero byte.  */
+/* binary DRAT proof (src/cpu/proof.cpp:92-232): a line is 'a' or 'd', */
+/* each literal 2v+sign in original numbering (V2DEC(vorg[v], sign),   */
+/* proof.cpp:118-135) as a little-endian base-128 varint (high bit =   */
+/* more), then a zero byte.                                            */
 /* The simplifier adds units (learnUnit, solve.h:475-479), resolvents, */
 /* strengthened and substituted clauses unless a literal is already    */
 /* true (addResolvent, proof.cpp:208-222), deletes subsumed clauses    */
@@ -406,12 +414,30 @@ static void proof_byte(pfo *s, uint8_t b)
     s->proof[s->nproof++] = b;
 }
 
+/* V2DEC(vorg[ABS(lit)], SIGN(lit)) (model.cpp:263-280, proof.cpp:123) */
+static inline uint32_t org_lit(const pfo *s, uint32_t lit)
+{
+    return s->vorg ? ((s->vorg[ABSV(lit)] << 1) | (lit & 1u)) : lit;
+}
+
+/* the variable mapping of the caller (identity when never set): model records and
+ * proof lines are written in original numbering through it */
+int pfo_set_vorg(pfo *s, const uint32_t *vorg, uint32_t n)
+{
+    if (n != s->nvars + 1) return -1;
+    for (uint32_t v = 1; v <= s->nvars; v++)
+        if (!vorg[v]) return -1;
+    s->vorg = (uint32_t *)xrealloc(s->vorg, sizeof(uint32_t) * n);
+    memcpy(s->vorg, vorg, sizeof(uint32_t) * n);
+    return 0;
+}
+
 static void proof_line(pfo *s, uint8_t tag, const uint32_t *lits, int n)
 {
     if (!s->opts.proof_en || s->proof_closed) return;
     proof_byte(s, tag);
     for (int i = 0; i < n; i++) {
-        uint32_t r = lits[i];
+        uint32_t r = org_lit(s, lits[i]);
         while (r > 127u) { proof_byte(s, (uint8_t)((r & 127u) | 128u)); r >>= 7; }
         proof_byte(s, (uint8_t)r);
     }
@@ -466,9 +492,10 @@ int pfo_enqueue(pfo *s, uint32_t lit)
 }
 
 /* ------------------------------------------------------------------ */
-/* model stack (src/cpu/model.cpp:263-307), identity variable mapping  */
+/* model stack (src/cpu/model.cpp:263-307): literals in original       */
+/* numbering through vorg (saveWitness / saveLiteral)                  */
 /* ------------------------------------------------------------------ */
-static void save_witness(pfo *s, uint32_t w) { res_push(s, w); res_push(s, 1); }
+static void save_witness(pfo *s, uint32_t w) { res_push(s, org_lit(s, w)); res_push(s, 1); }
 
 static void save_clause(pfo *s, const uint32_t *c, uint32_t witlit)
 {
@@ -477,7 +504,7 @@ static void save_clause(pfo *s, const uint32_t *c, uint32_t witlit)
     int pos = -1;
     for (int i = 0; i < size; i++) {
         const uint32_t lit = c[3 + i];
-        res_push(s, lit);
+        res_push(s, org_lit(s, lit));
         if (lit == witlit) pos = i;
     }
     if (pos > 0) {

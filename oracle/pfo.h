@@ -56,6 +56,12 @@ void pfo_destroy(pfo *s);
  * the incremental ifreeze/iunfreeze (src/cpu/assume.cpp:45-61); 0 ok, -1 bad variable */
 int pfo_ifreeze(pfo *s, uint32_t v, int on);
 
+/* the caller's variable mapping vorg[nvars + 1] (vorg[v] = original variable of v;
+ * ParaFROST's vorg, src/cpu/solve.h:76, vmap.cpp:103): model records and proof lines
+ * are written in original numbering (model.cpp:263-280, proof.cpp:118-135). 0 ok,
+ * -1 bad size or a zero entry. Without it the mapping is the identity. */
+int pfo_set_vorg(pfo *s, const uint32_t *vorg, uint32_t n);
+
 /* assign units before simplification (they are propagated by the first prop()). */
 int pfo_enqueue(pfo *s, uint32_t lit);
 

@@ -48,6 +48,7 @@ def lib():
         L.pfo_enqueue.argtypes = [vp, ctypes.c_uint32]
         L.pfo_enqueue.restype = ctypes.c_int
         L.pfo_ifreeze.argtypes = [vp, ctypes.c_uint32, ctypes.c_int]
+        L.pfo_set_vorg.argtypes = [vp, u32p, ctypes.c_uint32]
         L.pfo_ifreeze.restype = ctypes.c_int
         L.pfo_vmap.restype = ctypes.c_uint64
         L.pfo_vmap.argtypes = [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint32), vp, ctypes.POINTER(ctypes.c_uint32), vp]
@@ -172,6 +173,13 @@ class Oracle:
     def sigmify(self) -> int:
         return self.L.pfo_sigmify(self.h)
 
+    def set_vorg(self, vorg):
+        """the caller's variable mapping (vorg[v] = original variable of v): model
+        records and proof lines are written in original numbering"""
+        v = np.ascontiguousarray(vorg, np.uint32)
+        if self.L.pfo_set_vorg(self.h, v, len(v)) != 0:
+            raise ValueError("vorg needs nvars + 1 entries, all nonzero past entry 0")
+
     def ifreeze(self, vars, on: bool = True):
         """ifreeze / iunfreeze (src/cpu/assume.cpp:45-61): LCVE passes over these variables."""
         for v in np.asarray(vars, np.uint32).ravel():
@@ -280,18 +288,27 @@ def ref_opts_args(o: Opts) -> list[str]:
          f"--xormaxarity={o.xor_max_arity}", f"--submax={o.sub_limit}", f"--eremax={o.ere_limit}",
          f"--ereclausemax={o.ere_max_resolvent}", "-velbound" if o.ve_lbound_en else "-no-velbound",
          "-ere" if o.ere_en else "-no-ere", "-bce" if o.bce_en else "-no-bce", f"--bcemax={o.bce_limit}",
+         # ve_en stands for the reference's ve_en || ve_plus_en (options.cpp:283)
+         "-ve" if o.ve_en else "-no-ve", "-ve+" if o.ve_en else "-no-ve+", "-sub" if o.sub_en else "-no-sub",
          "--mapperc=1"]
     return a
 
 
 def run_reference(cnf_path: str, out_path: str, o: Opts, trace_prefix: str | None = None,
-                  timeout: float = 600, freeze=None, mapped: bool = False):
+                  timeout: float = 600, freeze=None, mapped: bool = False, second: bool = False):
     """freeze: variables frozen as by ifreeze() in incremental mode; mapped: run the
     reference's variable mapping with write-back after the simplifier and return its
     Mapped dump (None when the formula is refuted). With o.proof_en the reference
     writes its binary DRAT proof; the simplifier's part (from the start of the
-    simplifier on) is returned as the state's `proof` bytes."""
+    simplifier on) is returned as the state's `proof` bytes. second (with mapped):
+    then run the simplifier a second time on the mapped formula (a second sigmify()
+    call, phases lowered as sigmify() lowers them) and return (Mapped, State of the
+    second call, model-stack size before it), or None when the first call refutes;
+    the State's proof is the second call's part."""
     env = dict(os.environ)
+    if second:
+        env["PFREF_SECOND"] = "1"
+        mapped = True
     proof_args = []
     if o.proof_en:
         proof_args = ["-proof", f"--proofout={out_path}.drat"]
@@ -324,5 +341,18 @@ def run_reference(cnf_path: str, out_path: str, o: Opts, trace_prefix: str | Non
     if mapped:
         with open(out_path, "rb") as f:
             magic = f.read(4)
-        return read_mapped(out_path) if magic == b"PFRM" else None
+        if magic != b"PFRM":
+            return None
+        m = read_mapped(out_path)
+        if not second:
+            return m
+        if r.returncode != 0 or not os.path.exists(out_path + ".2"):
+            raise RuntimeError(f"reference second call failed ({r.returncode}): {r.stderr[-500:]!r}")
+        st = read_dump(out_path + ".2")
+        with open(out_path + ".2", "rb") as f:
+            nres_first = int(np.frombuffer(f.read()[-8:], np.uint64)[0])
+        if o.proof_en:
+            at = int(np.fromfile(out_path + ".mark.2", np.uint64)[0])
+            st.proof = np.fromfile(out_path + ".drat", np.uint8)[at:]
+        return m, st, nres_first
     return with_proof(read_dump(out_path))

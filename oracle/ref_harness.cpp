@@ -15,6 +15,13 @@
  *   u32 'PFRM', u32 nvars_new, u64 nclauses, per clause (u32 size, lits[]),
  *   u32 nvorg, vorg[] (new variable -> original), u32 ntrail, trail[],
  *   u8 vstate[nvars_new + 1]
+ * env PFREF_SECOND=1 (with PFREF_MAP): after the mapped dump, finish the write-back
+ *   as sigmifying() does (rebuildWT, retrail; simplify.cpp:206-208), lower the phases
+ *   as sigmify() does after a call (simplify.cpp:125-128) and run the simplifier a
+ *   second time on the mapped formula; its state goes to <out.bin>.2 in the dump
+ *   format below, followed by u64 = the model-stack size before the second call
+ *   (its records are the stack's suffix, written through the mapped vorg), and the
+ *   proof length at its start goes to PFREF_PROOF_MARK's file with ".2" appended
  * env PFREF_PROOF_MARK=<file>: with -proof --proofout=<path>, write the proof's
  *   length in bytes (u64) when the simplifier starts, so that the simplifier's part
  *   of the proof can be told from the parser's and shrinkTop's lines
@@ -158,55 +165,76 @@ int main(int argc, char** argv)
 	auto tdump = [&](const char* pass) {
 		if (trace) dump(s, std::string(trace) + "." + std::to_string(round) + "." + pass + ".bin", true, 0);
 	};
+	auto mark_proof = [&](const std::string& suffix) {
+		if (const char* mk = getenv("PFREF_PROOF_MARK")) {
+			fflush(NULL);
+			struct stat sb;
+			uint64_t at = (s->opts.proof_en && stat(s->opts.proof_path, &sb) == 0) ? uint64_t(sb.st_size) : 0;
+			FILE* f = fopen((std::string(mk) + suffix).c_str(), "wb");
+			if (!f) { perror(mk); exit(3); }
+			put64(f, at);
+			fclose(f);
+		}
+	};
 	/* ParaFROST::sigmifying(), src/cpu/simplify.cpp:150-214, without the timers,
-	   sleep/interrupt hooks and the write-back to the CDCL database. */
+	   sleep/interrupt hooks and the write-back to the CDCL database.
+	   Returns 0 simplified, 1 refuted (dumped), 2 nothing to simplify (dumped). */
+	auto sigmifying = [&](const std::string& dst, const std::string& mark_suffix) -> int {
+		s->rootify();
+		s->shrinkTop(false);
+		mark_proof(mark_suffix);
+		if (s->orgs.empty()) { dump(s, dst, false, 0); return 2; }
+		s->awaken();
+		if (s->simpstate == AWAKEN_FAIL) { fprintf(stderr, "awaken failed\n"); exit(4); }
+		int64 litsbefore = inf.nLiterals, diff = INT64_MAX;
+		while (litsbefore) {
+			s->resizeCNF();
+			s->createOT();
+			if (!s->prop()) { dump(s, dst, false, 1); return 1; }
+			tdump("prop");
+			if (!s->LCVE()) break;
+			tdump("lcve");
+			s->sortOT();
+			tdump("sort");
+			if (s->stop(diff)) { s->ERE(); tdump("ere"); break; }
+			s->SUB();
+			tdump("sub");
+			s->VE();
+			tdump("ve");
+			s->BCE();
+			s->countAll(), s->filterPVs();
+			inf.nClauses = inf.n_cls_after, inf.nLiterals = inf.n_lits_after;
+			diff = litsbefore - inf.nLiterals, litsbefore = inf.nLiterals;
+			s->phase++, s->mu_inc++;
+			s->mu_inc += s->phase == s->opts.phases;
+			round++;
+		}
+		if (!s->prop()) { dump(s, dst, false, 1); return 1; }
+		s->countFinal();
+		s->shrinkSimp();
+		return 0;
+	};
 	const auto t_start = std::chrono::steady_clock::now();
-	s->rootify();
-	s->shrinkTop(false);
-	if (const char* mk = getenv("PFREF_PROOF_MARK")) {
-		fflush(NULL);
-		struct stat sb;
-		uint64_t at = (s->opts.proof_en && stat(s->opts.proof_path, &sb) == 0) ? uint64_t(sb.st_size) : 0;
-		FILE* f = fopen(mk, "wb");
-		if (!f) { perror(mk); return 3; }
-		put64(f, at);
-		fclose(f);
-	}
-	if (s->orgs.empty()) { dump(s, out, false, 0); return 0; }
-	s->awaken();
-	if (s->simpstate == AWAKEN_FAIL) { fprintf(stderr, "awaken failed\n"); return 4; }
-	int64 litsbefore = inf.nLiterals, diff = INT64_MAX;
-	while (litsbefore) {
-		s->resizeCNF();
-		s->createOT();
-		if (!s->prop()) { dump(s, out, false, 1); return 0; }
-		tdump("prop");
-		if (!s->LCVE()) break;
-		tdump("lcve");
-		s->sortOT();
-		tdump("sort");
-		if (s->stop(diff)) { s->ERE(); tdump("ere"); break; }
-		s->SUB();
-		tdump("sub");
-		s->VE();
-		tdump("ve");
-		s->BCE();
-		s->countAll(), s->filterPVs();
-		inf.nClauses = inf.n_cls_after, inf.nLiterals = inf.n_lits_after;
-		diff = litsbefore - inf.nLiterals, litsbefore = inf.nLiterals;
-		s->phase++, s->mu_inc++;
-		s->mu_inc += s->phase == s->opts.phases;
-		round++;
-	}
-	if (!s->prop()) { dump(s, out, false, 1); return 0; }
-	s->countFinal();
-	s->shrinkSimp();
+	if (sigmifying(out, "")) return 0;
 	if (getenv("PFREF_TIME"))
 		fprintf(stderr, "PFREF_TIME %.6f\n",
 			std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count());
 	if (getenv("PFREF_MAP")) {
 		s->map(true);
 		dump_mapped(s, out);
+		if (!getenv("PFREF_SECOND")) return 0;
+		/* the rest of sigmifying's write-back, then a second sigmify() call */
+		s->rebuildWT(s->opts.sigma_priorbins);
+		if (s->retrail()) { fprintf(stderr, "refuted after the mapping\n"); return 5; }
+		if (s->opts.phases > 2) s->opts.phases--;
+		const uint64_t nres_first = s->model.resolved.size();
+		round = 0;
+		const std::string out2 = out + ".2";
+		if (sigmifying(out2, ".2") == 0) dump(s, out2, false, 0);
+		FILE* f = fopen(out2.c_str(), "ab");
+		if (!f) { perror(out2.c_str()); return 3; }
+		put64(f, nres_first);
+		fclose(f);
 		return 0;
 	}
 	dump(s, out, false, 0);

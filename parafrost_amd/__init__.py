@@ -17,8 +17,9 @@ from . import gen  # noqa: F401  (workload generators)
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-PFG_OK, PFG_EINVAL, PFG_ENOMEM, PFG_EDEVICE, PFG_EUNSAT, PFG_EINTERNAL = range(6)
-_ERRNAMES = {1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EUNSAT", 5: "EINTERNAL"}
+PFG_OK, PFG_EINVAL, PFG_ENOMEM, PFG_EDEVICE, PFG_EUNSAT, PFG_EINTERNAL, PFG_EINTR = range(7)
+_ERRNAMES = {1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EUNSAT", 5: "EINTERNAL", 6: "EINTR"}
+PATH_NAMES = ("ere_extra_rounds", "sub_wave_vars", "sub_wave_big_vars")
 
 
 class Opts(ctypes.Structure):
@@ -52,7 +53,7 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
        "pfg_set_accounting", "pfg_set_stage_words", "pfg_create_batch", "pfg_batch_ninst", "pfg_batch_relayout", "pfg_upload_batch",
        "pfg_sigmify_batch", "pfg_batch_download", "pfg_ifreeze", "pfg_batch_export", "pfg_download_mapped",
-       "pfg_download_proof"]
+       "pfg_download_proof", "pfg_interrupt", "pfg_set_vorg", "pfg_path_counts"]
 
 
 def lib_path() -> str:
@@ -111,6 +112,9 @@ def lib():
                                          ctypes.POINTER(u64), vp, vp, ctypes.POINTER(u32)]
         L.pfg_batch_export.argtypes = [vp, u32, vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.pfg_reset.argtypes = [vp]
+        L.pfg_interrupt.argtypes = [vp]
+        L.pfg_set_vorg.argtypes = [vp, vp, u32]
+        L.pfg_path_counts.argtypes = [vp, vp, i32, ctypes.POINTER(i32)]
         L.pfg_last_error.argtypes = [vp]
         L.pfg_last_error.restype = ctypes.c_char_p
         for f in API:
@@ -170,6 +174,10 @@ class Unsat(SimplifierError):
     pass
 
 
+class Interrupted(SimplifierError):
+    """the call was stopped by interrupt(); the context holds a valid, partly simplified formula"""
+
+
 class Simplifier:
     """One device-resident CNF instance (a pfg_ctx)."""
 
@@ -188,6 +196,8 @@ class Simplifier:
         msg = msg.decode() if msg else ""
         if rc == PFG_EUNSAT:
             raise Unsat(rc, "formula is unsatisfiable")
+        if rc == PFG_EINTR:
+            raise Interrupted(rc, "interrupted")
         raise SimplifierError(rc, msg)
 
     def close(self):
@@ -232,6 +242,26 @@ class Simplifier:
         self._check(self.L.pfg_download_mapped(self.h, w.ctypes.data, ctypes.byref(nw), ctypes.byref(nc), vo.ctypes.data,
                                                ctypes.byref(nv), tr.ctypes.data, ctypes.byref(nt), vs.ctypes.data))
         return w[:nw.value], vo, tr[:nt.value], vs
+
+    def set_vorg(self, vorg):
+        """the solver's variable mapping (vorg[v] = original variable of v, nvars + 1
+        entries; None = identity): model records and proof lines come out in original numbering"""
+        if vorg is None:
+            self._check(self.L.pfg_set_vorg(self.h, None, 0))
+            return
+        a = np.ascontiguousarray(vorg, np.uint32)
+        self._check(self.L.pfg_set_vorg(self.h, a.ctypes.data, len(a)))
+
+    def interrupt(self):
+        """ask the running or next sigmify to stop at its next poll (thread-safe)"""
+        self._check(self.L.pfg_interrupt(self.h))
+
+    def path_counts(self) -> dict:
+        """how often the less common code paths ran (names in PATH_NAMES)"""
+        a = np.zeros(16, np.uint64)
+        n = ctypes.c_int()
+        self._check(self.L.pfg_path_counts(self.h, a.ctypes.data, 16, ctypes.byref(n)))
+        return {PATH_NAMES[i] if i < len(PATH_NAMES) else f"path{i}": int(a[i]) for i in range(min(n.value, 16))}
 
     def ifreeze(self, vars, on: bool = True):
         """ifreeze / iunfreeze (src/cpu/assume.cpp:45-61): LCVE never elects these variables."""

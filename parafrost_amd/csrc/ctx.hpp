@@ -6,6 +6,7 @@
 #include <stdexcept>
 #include <vector>
 #include <array>
+#include <atomic>
 #include "../../include/pfgpu.h"
 
 namespace pfg {
@@ -76,6 +77,15 @@ enum KId {
 extern const char *const kernel_names[K_NUM];
 extern const char *const kernel_symbols[K_NUM];
 
+// how often the less common code paths ran (pfg_path_counts): the tests check
+// that their cases reach them
+enum PathId {
+    PATH_ERE_ROUNDS,     // extra ERE claiming rounds (a target claimed by two elected variables)
+    PATH_SUB_WAVE,       // unit-capable variables run alone on one wave
+    PATH_SUB_WAVE_BIG,   // ... of them with lists longer than the LDS arrays
+    PATH_NUM
+};
+
 struct KStat {
     double ms = 0, bytes = 0;
     uint64_t launches = 0;
@@ -115,6 +125,8 @@ struct Ctx {
     uint64_t nres = 0;
     DBuf<uint64_t> mchunk;      // batch: first word of each eliminated variable's records, in stack order
     uint64_t nchunk = 0;
+    DBuf<uint32_t> vorg;        // the caller's variable -> original variable map (pfg_set_vorg)
+    bool has_vorg = false;      // false: identity
     DBuf<uint32_t> pvs;         // elected variables
     DBuf<uint32_t> units;       // units produced by one BVE pass, in elected order
     uint32_t npvs = 0;
@@ -169,12 +181,14 @@ struct Ctx {
     bool sorted_lists = false;
 
     pfg_times times{};
+    uint64_t path[PATH_NUM] = {};
     KStat kstat[K_NUM];
     std::vector<std::array<hipEvent_t, 2>> kev;     // event pairs of pending kernel timers
     std::vector<std::pair<int, double>> kpend;      // (kernel, algorithmic bytes) per pending pair
     hipEvent_t ev[8][2] = {};   // per nesting depth of pass timers
     int ev_depth = 0;
     std::string last_error;
+    std::atomic<int> interrupt{0};   // set by pfg_interrupt, polled between passes and rounds
 };
 
 // allocation helpers (budget-checked)

@@ -16,6 +16,12 @@ constexpr int HSE_MAX_CL_SIZE = 1000;  // subsume.h:25
 __device__ __forceinline__ uint32_t absv(uint32_t l) { return l >> 1; }
 __device__ __forceinline__ uint32_t flip(uint32_t l) { return l ^ 1u; }
 __device__ __forceinline__ uint32_t maphash(uint32_t l) { return 1u << (l & 31u); }
+// V2DEC(vorg[ABS(lit)], SIGN(lit)): a literal in the caller's original numbering
+// (model.cpp:263-280, proof.cpp:123); vorg null = identity
+__device__ __forceinline__ uint32_t org_lit(const uint32_t *vorg, uint32_t l)
+{
+    return vorg ? ((vorg[l >> 1] << 1) | (l & 1u)) : l;
+}
 
 __device__ __forceinline__ uint32_t c_st(const uint32_t *c) { return c[0] & 3u; }
 __device__ __forceinline__ bool c_orig(const uint32_t *c) { return (c[0] & 3u) == 0; }

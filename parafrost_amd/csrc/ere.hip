@@ -2,13 +2,21 @@
 // (ParaFROST src/cpu/redundancy.h:26-143, elimination.cpp:169-197).
 //
 // For every non-tautological resolvent of a variable's clauses the reference
-// deletes the first clause on the shortest occurrence list (among the literals
-// met before one antecedent runs out) that equals it. Targets never contain an
-// elected variable, so two variables interact only when they pick the same
-// clause. The parallel pass claims each target for its variable; if no clause
-// is claimed twice, every variable saw exactly what the sequential loop shows
-// it, and the claimed clauses are deleted. Otherwise the pass is redone in
-// elected order by a single thread.
+// deletes the first live clause on the shortest occurrence list (among the
+// literals met before one antecedent runs out) that equals it; a deleted clause
+// never matches (its status is neither learnt nor the resolvent's type). Targets
+// never contain an elected variable, so two variables interact only through the
+// clauses they delete: a variable that meets a clause an earlier variable deleted
+// goes on to the next equal clause.
+//
+// Rounds of optimistic claims, committed as a prefix in elected order: every
+// variable still to run walks its loop against the deletions committed so far plus
+// its own, and claims each target with atomicMin on its elected position. The
+// first variable of the round cannot be affected by the others, and by induction
+// neither can any variable before q_bad, the first one whose target went to an
+// earlier variable. Claims of the variables before q_bad are committed (their
+// clauses deleted, their proof lines written); the variables from q_bad on run
+// again in the next round. Without a shared target the pass is one round.
 #include "device.hpp"
 #include "passes.hpp"
 #include "prims.hpp"
@@ -25,8 +33,11 @@ struct EreArgs {
     const uint64_t *occ;
     int ere_limit;
     int maxsize;
-    uint32_t *owner;
-    uint64_t *targets;
+    uint32_t *owner;      // per (ref >> 1): the smallest elected position claiming it this round
+    uint64_t *targets;    // claims of the round: target reference,
+    uint32_t *tq;         // ... its claimant's elected position,
+    uint32_t *tseq;       // ... and the claim's place in that variable's loop
+    uint64_t tcap;
     unsigned long long *cnt;
     ProofLog P;   // proof events: the deleted targets (removeClause, redundancy.h:137)
 };
@@ -98,46 +109,42 @@ __device__ __forceinline__ bool ere_match(const EreArgs &A, uint32_t x, const ui
     return R.len == c_size(c) && (c_learnt(c) || c_st(c) == R.type) && sig_sub(R.msig, c[1]) && merge_ere(x, c1, c2, c);
 }
 
-// forward_equ (redundancy.h:87-143). seq: delete in place (sequential replay);
-// otherwise claim the target for variable q. Returns false on a claim collision.
+// forward_equ (redundancy.h:87-143): claim the first matching live target for the
+// variable at elected position q. A clause this variable claimed before is deleted by
+// then in the sequential loop, so the search goes past it.
 template <bool ST>
-__device__ __forceinline__ bool forward_equ(const EreArgs &A, uint32_t x, const uint32_t *c1, const uint32_t *c2, bool seq,
+__device__ __forceinline__ void forward_equ(const EreArgs &A, uint32_t x, const uint32_t *c1, const uint32_t *c2,
                                             uint32_t q, uint32_t &pseq)
 {
     EreRes R;
-    if (!ere_resolvent<ST>(A, x, c1, c2, R)) return true;
+    if (!ere_resolvent<ST>(A, x, c1, c2, R)) return;
     const uint64_t off = A.ot[R.best].off;
     for (uint32_t i = 0; i < R.minsize; i++) {
         const uint64_t r = A.occ[off + i];
-        uint32_t *c = A.mem + r;
+        const uint32_t *c = A.mem + r;
         if (!ere_match(A, x, c1, c2, R, c)) continue;
-        if (seq) {
-            c_markdel(c);
-            plog(A.P, q, pseq, PE_DEL, r);
-            return true;
+        const uint32_t old = atomicMin(A.owner + (r >> 1), q);
+        if (old == q) continue;
+        const unsigned long long k = atomicAdd(&A.cnt[CNT_A], 1ull);
+        if (k < A.tcap) {
+            A.targets[k] = r;
+            A.tq[k] = q;
+            A.tseq[k] = pseq;
         }
-        uint32_t *slot = A.owner + (r >> 1);
-        const uint32_t cur = __atomic_load_n(slot, __ATOMIC_RELAXED);
-        if (cur == q) continue;   // deleted by this variable already
-        if (cur != NO_OWNER) return false;
-        const uint32_t old = atomicCAS(slot, NO_OWNER, q);
-        if (old != NO_OWNER) return false;
-        A.targets[atomicAdd(&A.cnt[CNT_A], 1ull)] = r;
-        plog(A.P, q, pseq, PE_DEL, r);
-        return true;
+        pseq++;
+        return;
     }
-    return true;
 }
 
 // tab: the variable's staged region (stage.hpp), or null to read the pool
 template <bool ST>
-__device__ __forceinline__ bool ere_var(const EreArgs &A, uint32_t x, bool seq, uint32_t q, const uint32_t *tab,
+__device__ __forceinline__ void ere_var(const EreArgs &A, uint32_t x, uint32_t q, const uint32_t *tab,
                                         const uint32_t *arena, uint32_t &pseq)
 {
     uint32_t dx = 2 * x, fx = dx + 1;
     if (A.ot[dx].sz > A.ot[fx].sz) { dx = fx; fx = 2 * x; }
     const OL me = A.ot[dx], other = A.ot[fx];
-    if (!(me.sz <= (uint32_t)A.ere_limit && other.sz <= (uint32_t)A.ere_limit)) return true;
+    if (!(me.sz <= (uint32_t)A.ere_limit && other.sz <= (uint32_t)A.ere_limit)) return;
     const uint32_t ps = A.ot[2 * x].sz;
     const uint32_t mb = (dx & 1u) ? ps : 0u, ob = (fx & 1u) ? ps : 0u;
     for (uint32_t i = 0; i < me.sz; i++) {
@@ -146,17 +153,17 @@ __device__ __forceinline__ bool ere_var(const EreArgs &A, uint32_t x, bool seq, 
         for (uint32_t j = 0; j < other.sz; j++) {
             const uint32_t *cj = ST ? arena + tab[ob + j] : A.mem + A.occ[other.off + j];
             if (c_del(cj)) continue;
-            if (!forward_equ<ST>(A, x, ci, cj, seq, q, pseq)) return false;
+            forward_equ<ST>(A, x, ci, cj, q, pseq);
         }
     }
-    return true;
 }
 
-// ERE runs in two steps. The probe looks at every (variable, clause of the shorter
-// list, clause of the other list) pair independently, one thread each, and flags
-// the variable when the pair's shortest list holds a clause equal to the resolvent
+// Before the rounds, a probe looks at every (variable, clause of the shorter list,
+// clause of the other list) pair independently, one thread each, and flags the
+// variable when the pair's shortest list holds a live clause equal to the resolvent
 // at all. A variable without such a pair deletes nothing, so only flagged
-// variables run the claiming pass below, which is the reference's loop.
+// variables run the claiming rounds; the number of matching pairs bounds the claims
+// of a round.
 
 // pairs of variable q (0 when a list exceeds ere_limit)
 __global__ void k_ere_pairs(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs, uint64_t *__restrict__ npairs)
@@ -208,38 +215,45 @@ __global__ void k_ere_probe(EreArgs A, const uint32_t *__restrict__ pvs, uint32_
             }
         }
         for (; k < R.minsize && !hit; k++) hit = ere_match(A, x, ci, cj, R, A.mem + A.occ[off + k]);
-        if (hit) flag[q] = 1u;
+        if (hit) {
+            flag[q] = 1u;
+            atomicAdd(&A.cnt[CNT_C], 1ull);
+        }
     }
 }
 
-// the claiming pass, thread per flagged variable
-__global__ void __launch_bounds__(64) k_ere_claim(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs,
-                                                 const uint32_t *__restrict__ flag)
+// one claiming round: thread per flagged variable from elected position qfrom on
+__global__ void __launch_bounds__(64) k_ere_claim(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t qfrom,
+                                                 uint32_t npvs, const uint32_t *__restrict__ flag)
 {
-    for (uint64_t q = gtid(); q < npvs; q += gstride()) {
+    for (uint64_t q = qfrom + gtid(); q < npvs; q += gstride()) {
         if (!flag[q]) continue;
         uint32_t pseq = 0;
-        if (!ere_var<false>(A, pvs[q], false, (uint32_t)q, nullptr, nullptr, pseq)) atomicOr(&A.cnt[CNT_B], 1ull);
+        ere_var<false>(A, pvs[q], (uint32_t)q, nullptr, nullptr, pseq);
         plog_end(A.P, (uint32_t)q, pseq);
     }
 }
 
-__global__ void k_ere_commit(EreArgs A, uint64_t n, int del)
+// q_bad (CNT_D): the first elected position with a target that went to an earlier one
+__global__ void k_ere_lost(EreArgs A, uint64_t n)
 {
-    for (uint64_t i = gtid(); i < n; i += gstride()) {
-        const uint64_t r = A.targets[i];
-        if (del) c_markdel(A.mem + r);
-        A.owner[r >> 1] = NO_OWNER;
-    }
+    for (uint64_t k = gtid(); k < n; k += gstride())
+        if (A.owner[A.targets[k] >> 1] != A.tq[k]) atomicMin(&A.cnt[CNT_D], (unsigned long long)A.tq[k]);
 }
 
-__global__ void k_ere_seq(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs)
+// claims of positions before q_bad are final: their targets are deleted and their
+// proof lines written; every claim is released
+__global__ void k_ere_commit(EreArgs A, uint64_t n)
 {
-    if (gtid() != 0) return;
-    for (uint32_t q = 0; q < npvs; q++) {
-        uint32_t pseq = 0;
-        ere_var<false>(A, pvs[q], true, q, nullptr, nullptr, pseq);
-        plog_end(A.P, q, pseq);
+    const uint64_t qbad = A.cnt[CNT_D];
+    for (uint64_t k = gtid(); k < n; k += gstride()) {
+        const uint64_t r = A.targets[k];
+        const uint32_t q = A.tq[k];
+        if (q < qbad) {
+            c_markdel(A.mem + r);
+            plog_at(A.P, q, A.tseq[k], PE_DEL, r);
+        }
+        A.owner[r >> 1] = NO_OWNER;
     }
 }
 
@@ -252,16 +266,15 @@ void ere(Ctx &c)
         ensure(c, c.owner, need);
         PFG_HIP(hipMemsetAsync(c.owner.p, 0xFF, c.owner.cap * 4, c.stream));
     }
-    ensure(c, c.tmpu64b, c.occ.cap + 1);
     const uint32_t np = c.npvs;
-    EreArgs A{c.mem.p, c.ot.p, c.occ.p, c.opts.ere_limit, c.opts.ere_max_resolvent, c.owner.p, c.tmpu64b.p,
-              (unsigned long long *)c.cnt.p, proof_begin(c, np, c.nrefs)};
-    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));
+    EreArgs A{c.mem.p, c.ot.p, c.occ.p, c.opts.ere_limit, c.opts.ere_max_resolvent, c.owner.p, nullptr, nullptr,
+              nullptr, 0, (unsigned long long *)c.cnt.p, proof_begin(c, np, c.nrefs)};
     ensure(c, c.tmpu64a, 2ull * np + 2);
     ensure(c, c.tmpu32c, np + 1);
     uint64_t *npairs = c.tmpu64a.p, *poff = c.tmpu64a.p + np + 1;
     uint32_t *flag = c.tmpu32c.p;
     PFG_HIP(hipMemsetAsync(flag, 0, 4ull * np, c.stream));
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_C, 0, 8, c.stream));
     k_ere_pairs<<<grid_for(np, 256, 4096), 256, 0, c.stream>>>(A, c.pvs.p, np, npairs);
     scan_u64(c, npairs, poff, np, poff + np);
     uint64_t total = 0;
@@ -271,14 +284,33 @@ void ere(Ctx &c)
         KTimer kt(c, K_ERE, pv_bytes(c, 0, np, true));
         k_ere_probe<<<grid_for(total, 256, 65535), 256, 0, c.stream>>>(A, c.pvs.p, np, poff, total, flag);
     }
-    k_ere_claim<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, np, flag);
-    read_counters(c, 3);
-    const uint64_t nt = c.hcnt[CNT_A];
-    const bool collision = c.hcnt[CNT_B] != 0;
-    if (nt) k_ere_commit<<<grid_for(nt, 256, 4096), 256, 0, c.stream>>>(A, nt, collision ? 0 : 1);
-    if (collision) {
-        proof_restart(c, np);   // the replay logs the deletions again, in order
-        k_ere_seq<<<1, 64, 0, c.stream>>>(A, c.pvs.p, c.npvs);
+    read_counters(c, CNT_C + 1);
+    const uint64_t hits = c.hcnt[CNT_C];   // matching pairs: at most one claim each per round
+    if (hits) {
+        ensure(c, c.tmpu64b, hits + 1);
+        ensure(c, c.tmpu32d, 2 * hits + 2);
+        A.targets = c.tmpu64b.p;
+        A.tq = c.tmpu32d.p;
+        A.tseq = c.tmpu32d.p + hits + 1;
+        A.tcap = hits;
+        uint32_t qfrom = 0;
+        for (;;) {
+            const uint64_t init[4] = {0, 0, 0, ~0ull};   // CNT_A claims, CNT_B, CNT_C, CNT_D q_bad
+            PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_A, init, sizeof(init), hipMemcpyHostToDevice, c.stream));
+            k_ere_claim<<<grid_for(np - qfrom, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, qfrom, np, flag);
+            read_counters(c, CNT_A + 1);
+            const uint64_t nt = c.hcnt[CNT_A];
+            if (nt > hits) throw Error(PFG_EINTERNAL, "ERE claimed more targets than matching pairs");
+            if (!nt) break;
+            k_ere_lost<<<grid_for(nt, 256, 4096), 256, 0, c.stream>>>(A, nt);
+            k_ere_commit<<<grid_for(nt, 256, 4096), 256, 0, c.stream>>>(A, nt);
+            read_counters(c, CNT_D + 1);
+            const uint64_t qbad = c.hcnt[CNT_D];
+            if (qbad == ~0ull) break;
+            if (qbad <= qfrom) throw Error(PFG_EINTERNAL, "ERE claim round made no progress");
+            c.path[PATH_ERE_ROUNDS]++;
+            qfrom = (uint32_t)qbad;
+        }
     }
     proof_end(c, np, false);
 }

@@ -11,8 +11,9 @@
 // an earlier unit. Everything after the first contradiction is dropped, and the
 // empty clause ends the proof (the reference stops there: killSolver).
 //
-// Encoding (proof.cpp:118-135, V2DEC with the identity variable map): 'a' or 'd',
-// each literal 2v+sign as a little-endian base-128 varint (bit 7 = more), 0.
+// Encoding (proof.cpp:118-135): 'a' or 'd', each literal V2DEC(vorg[v], sign) in the
+// caller's original numbering (identity when no mapping was given) as a
+// little-endian base-128 varint (bit 7 = more), 0.
 #include "device.hpp"
 #include "passes.hpp"
 #include "prims.hpp"
@@ -122,7 +123,8 @@ __device__ __forceinline__ bool pf_line(const PEv &e, uint64_t i, const uint32_t
 
 __global__ void k_pf_bytes(const PEv *__restrict__ ord, uint64_t n, const uint32_t *__restrict__ mem,
                            const int8_t *__restrict__ value, const uint64_t *__restrict__ first,
-                           const unsigned long long *cpos, uint32_t *__restrict__ bytes)
+                           const unsigned long long *cpos, const uint32_t *__restrict__ vorg,
+                           uint32_t *__restrict__ bytes)
 {
     const uint64_t cp = *cpos;
     for (uint64_t i = gtid(); i < n; i += gstride()) {
@@ -131,7 +133,7 @@ __global__ void k_pf_bytes(const PEv *__restrict__ ord, uint64_t n, const uint32
         uint8_t tag;
         if (pf_line(ord[i], i, mem, value, first, cp, lits, m, unit, tag)) {
             b = 2;
-            for (uint32_t k = 0; k < m; k++) b += varint_len(lits[k]);
+            for (uint32_t k = 0; k < m; k++) b += varint_len(org_lit(vorg, lits[k]));
         }
         bytes[i] = b;
     }
@@ -139,7 +141,8 @@ __global__ void k_pf_bytes(const PEv *__restrict__ ord, uint64_t n, const uint32
 
 __global__ void k_pf_write(const PEv *__restrict__ ord, uint64_t n, const uint32_t *__restrict__ mem,
                            const int8_t *__restrict__ value, const uint64_t *__restrict__ first,
-                           const unsigned long long *cpos, const uint64_t *__restrict__ boff, uint8_t *__restrict__ out)
+                           const unsigned long long *cpos, const uint32_t *__restrict__ vorg,
+                           const uint64_t *__restrict__ boff, uint8_t *__restrict__ out)
 {
     const uint64_t cp = *cpos;
     for (uint64_t i = gtid(); i < n; i += gstride()) {
@@ -149,7 +152,7 @@ __global__ void k_pf_write(const PEv *__restrict__ ord, uint64_t n, const uint32
         if (!pf_line(ord[i], i, mem, value, first, cp, lits, m, unit, tag)) continue;
         uint8_t *o = out + boff[i];
         *o++ = tag;
-        for (uint32_t k = 0; k < m; k++) o = varint_put(o, lits[k]);
+        for (uint32_t k = 0; k < m; k++) o = varint_put(o, org_lit(vorg, lits[k]));
         *o = 0;
     }
 }
@@ -210,8 +213,10 @@ static void encode(Ctx &c, uint64_t n, bool decide)
                                                                 (unsigned long long *)c.pcnt.p + 1);
     ensure(c, c.pbytes, n + 1);
     ensure(c, c.pboff, n + 2);
+    const uint32_t *vorg = c.has_vorg ? c.vorg.p : nullptr;
     k_pf_bytes<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(ord, n, c.mem.p, c.value.p, c.pfirst.p,
-                                                             (const unsigned long long *)c.pcnt.p + 1, c.pbytes.p);
+                                                             (const unsigned long long *)c.pcnt.p + 1, vorg,
+                                                             c.pbytes.p);
     scan_u32_to_u64(c, c.pbytes.p, c.pboff.p, n, c.pcnt.p + 2);
     uint64_t h[2];
     PFG_HIP(hipMemcpyAsync(h, c.pcnt.p + 1, 16, hipMemcpyDeviceToHost, c.stream));
@@ -220,8 +225,8 @@ static void encode(Ctx &c, uint64_t n, bool decide)
     if (total) {
         ensure(c, c.proof, c.nproof + total + 2, true);
         k_pf_write<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(ord, n, c.mem.p, c.value.p, c.pfirst.p,
-                                                                 (const unsigned long long *)c.pcnt.p + 1, c.pboff.p,
-                                                                 c.proof.p + c.nproof);
+                                                                 (const unsigned long long *)c.pcnt.p + 1, vorg,
+                                                                 c.pboff.p, c.proof.p + c.nproof);
         c.nproof += total;
     }
     if (decide) k_pf_reset<<<grid_for(n, 256, 8192), 256, 0, c.stream>>>(ord, n, c.pfirst.p);

@@ -54,6 +54,22 @@ __device__ __forceinline__ void plog(const ProofLog &L, uint32_t q, uint32_t &se
     seq++;
 }
 
+// an event whose place in its variable's loop is known (seq), logged later
+__device__ __forceinline__ void plog_at(const ProofLog &L, uint32_t q, uint32_t seq, uint32_t tag, uint64_t payload)
+{
+    if (!L.ev) return;
+    const unsigned long long i = atomicAdd(L.n, 1ull);
+    if (i < L.cap) {
+        PEv e;
+        e.q = q;
+        e.seq = seq;
+        e.tag = tag;
+        e.pad = 0;
+        e.payload = payload;
+        L.ev[i] = e;
+    }
+}
+
 __device__ __forceinline__ void plog_end(const ProofLog &L, uint32_t q, uint32_t seq)
 {
     if (L.ev) L.nev[q] = seq;

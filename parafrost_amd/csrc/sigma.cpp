@@ -172,22 +172,37 @@ struct Sigma {
         c.mu_inc += (c.phase == c.opts.phases);
     }
 
+    // interrupted() (src/gpu/solve.h:181): pfg_interrupt asked this call to stop. The
+    // reference polls it before every round (simplify.cu:182) and before SUB, VE, BCE
+    // and ERE (elimination.cu:28,55,65,76); here the call then leaves the round loop
+    // and writes back what the finished passes produced.
+    bool interrupted() { return c.interrupt.exchange(0) != 0; }
+
     int sigmifying()
     {
         c.phase = 0;
         c.mu_inc = 0;
         int64_t litsbefore = c.nliterals, diff = INT64_MAX;
+        bool intr = false;
         while (litsbefore) {
+            if ((intr = interrupted())) break;
             resizeCNF();
             createOT();
             if (!propagate()) return PFG_EUNSAT;
             if (!LCVE()) break;
             sortOT();
-            if (stop(diff)) { ERE(); break; }
+            if (stop(diff)) {
+                if ((intr = interrupted())) break;
+                ERE();
+                break;
+            }
+            if ((intr = interrupted())) break;
             SUB();
             if (c.unsat) return PFG_EUNSAT;
+            if ((intr = interrupted())) break;
             VE();
             if (c.unsat) return PFG_EUNSAT;
+            if ((intr = interrupted())) break;
             BCE();
             int64_t nc, nl;
             countAll(nc, nl);
@@ -207,7 +222,7 @@ struct Sigma {
         c.nclauses = nc;
         c.nliterals = nl;
         shrinkSimp();
-        return PFG_OK;
+        return intr ? PFG_EINTR : PFG_OK;
     }
 
     // ---- batches: the same round structure, decided per instance
@@ -257,7 +272,9 @@ struct Sigma {
         std::vector<int64_t> before = nl, diff(ni, INT64_MAX);
         std::vector<uint64_t> nel(ni);
         int round = 0;
+        bool intr = false;
         for (;;) {
+            if ((intr = interrupted())) break;
             bool any = false;
             for (uint32_t i = 0; i < ni; i++) {
                 if (st[i] == B_RUN && before[i] == 0) st[i] = B_DONE;
@@ -336,9 +353,9 @@ struct Sigma {
         c.nclauses = tc;
         c.nliterals = tl;
         shrinkSimp();
-        for (uint32_t i = 0; i < ni; i++) status[i] = st[i] == B_REFUTED ? PFG_EUNSAT : PFG_OK;
+        for (uint32_t i = 0; i < ni; i++) status[i] = st[i] == B_REFUTED ? PFG_EUNSAT : st[i] == B_RUN && intr ? PFG_EINTR : PFG_OK;
         c.batch_gen++;
-        return PFG_OK;
+        return intr ? PFG_EINTR : PFG_OK;
     }
 };
 
@@ -391,6 +408,16 @@ void pfg_default_opts(pfg_opts *o)
 }
 
 
+// the option checks of the reference simplifier (src/cpu/options.cpp:305-310; ve_en
+// stands for its ve_en || ve_plus_en)
+static void normalize_opts(pfg_opts &o)
+{
+    const bool any = o.ve_en || o.sub_en || o.bce_en;
+    if (!o.phases && any) o.phases = 1;
+    if (o.phases && !any) o.phases = 0;
+    if (o.phases > 1 && !o.ve_en) o.phases = 1;
+}
+
 int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, const pfg_opts *opts)
 {
     if (!out || !nvars) return PFG_EINVAL;
@@ -405,6 +432,7 @@ int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, c
     c.nvars_cap = nvars;
     c.nlits2 = 2ull * ((uint64_t)nvars + 1);
     if (opts) c.opts = *opts; else pfg_default_opts(&c.opts);
+    normalize_opts(c.opts);
     try {
         PFG_HIP(hipSetDevice(device));
         PFG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -674,13 +702,14 @@ int pfg_sigmify_batch(pfg_ctx *ctx, int32_t *status)
     if (!ctx || !status) return PFG_EINVAL;
     API_BEGIN
     double t0 = 0;
+    int rc;
     {
         PassTimer t(c, &t0);
         Sigma s(c);
-        const int rc = s.sigmifying_batch(status);
-        if (rc != PFG_OK) return rc;
+        rc = s.sigmifying_batch(status);
     }
     c.times.total += t0;
+    if (rc != PFG_OK) return rc;
     API_END
 }
 
@@ -929,6 +958,7 @@ int pfg_reset_stats(pfg_ctx *ctx)
     resolve_ktimers(c);
     for (auto &k : c.kstat) k = KStat{};
     c.times = pfg_times{};
+    for (auto &p : c.path) p = 0;
     API_END_OK
 }
 
@@ -953,6 +983,7 @@ int pfg_reset(pfg_ctx *ctx)
     c.hist_live = false;
     c.nproof = 0;
     c.proof_closed = false;
+    c.has_vorg = false;
     API_END
 }
 
@@ -964,6 +995,39 @@ int pfg_times_get(pfg_ctx *ctx, pfg_times *t)
 }
 
 int pfg_is_unsat(pfg_ctx *ctx) { return ctx && ctx->c.unsat ? 1 : 0; }
+
+int pfg_interrupt(pfg_ctx *ctx)
+{
+    if (!ctx) return PFG_EINVAL;
+    ctx->c.interrupt.store(1);
+    return PFG_OK;
+}
+
+int pfg_set_vorg(pfg_ctx *ctx, const uint32_t *vorg, uint32_t n)
+{
+    if (!ctx) return PFG_EINVAL;
+    API_BEGIN
+    if (c.batch) throw Error(PFG_EINVAL, "a variable mapping applies to single-instance contexts");
+    if (!vorg) { c.has_vorg = false; return c.unsat ? PFG_EUNSAT : PFG_OK; }
+    if ((uint64_t)n != (uint64_t)c.nvars + 1) throw Error(PFG_EINVAL, "vorg needs nvars + 1 entries");
+    for (uint32_t v = 1; v < n; v++)
+        if (vorg[v] == 0 || vorg[v] >= (1u << 31)) throw Error(PFG_EINVAL, "vorg entry out of range");
+    ensure(c, c.vorg, n);
+    PFG_HIP(hipMemcpyAsync(c.vorg.p, vorg, 4ull * n, hipMemcpyHostToDevice, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    c.has_vorg = true;
+    API_END_OK
+}
+
+int pfg_path_counts(pfg_ctx *ctx, uint64_t *counts, int max, int *n)
+{
+    if (!ctx || (max > 0 && !counts)) return PFG_EINVAL;
+    const Ctx &c = ctx->c;
+    const int k = std::min<int>(max, PATH_NUM);
+    for (int i = 0; i < k; i++) counts[i] = c.path[i];
+    if (n) *n = PATH_NUM;
+    return PFG_OK;
+}
 
 const char *pfg_last_error(pfg_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
 

@@ -330,12 +330,42 @@ __device__ __forceinline__ bool sw_has(const uint32_t *lits, uint32_t n, uint32_
     return lo < n && lits[lo] == x;
 }
 
+// per-entry arrays of the two lists of one side: in LDS for lists of up to SW_CAP
+// entries, in a device buffer of 2 * 10 bytes per entry for longer ones
+struct SwLists {
+    uint16_t *msz, *osz;
+    uint32_t *msig, *osig, *mbin, *obin;
+};
+
+__device__ __forceinline__ void sub_wave_body(const SubArgs &A, const uint32_t *__restrict__ pvs, uint32_t q,
+                                              const SwLists &L, uint32_t *clit, uint32_t &s_unit, uint32_t &s_seq);
+
 __global__ void __launch_bounds__(64) k_sub_wave(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t q)
 {
     __shared__ uint16_t msz[SW_CAP], osz[SW_CAP];
     __shared__ uint32_t msig[SW_CAP], osig[SW_CAP], mbin[SW_CAP], obin[SW_CAP];
     __shared__ uint32_t clit[SW_CL];
     __shared__ uint32_t s_unit, s_seq;
+    sub_wave_body(A, pvs, q, SwLists{msz, osz, msig, osig, mbin, obin}, clit, s_unit, s_seq);
+}
+
+// the same for lists longer than SW_CAP (sub_limit above its default): the per-entry
+// arrays live in `big` (cap entries per array)
+__global__ void __launch_bounds__(64) k_sub_wave_big(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t q,
+                                                     uint32_t *big, uint32_t cap)
+{
+    __shared__ uint32_t clit[SW_CL];
+    __shared__ uint32_t s_unit, s_seq;
+    uint16_t *sz16 = (uint16_t *)big;
+    uint32_t *w = big + cap;   // cap u16 pairs = cap words
+    sub_wave_body(A, pvs, q, SwLists{sz16, sz16 + cap, w, w + cap, w + 2ull * cap, w + 3ull * cap}, clit, s_unit, s_seq);
+}
+
+__device__ __forceinline__ void sub_wave_body(const SubArgs &A, const uint32_t *__restrict__ pvs, uint32_t q,
+                                              const SwLists &L, uint32_t *clit, uint32_t &s_unit, uint32_t &s_seq)
+{
+    uint16_t *msz = L.msz, *osz = L.osz;
+    uint32_t *msig = L.msig, *osig = L.osig, *mbin = L.mbin, *obin = L.obin;
     const uint32_t lane = threadIdx.x;
     const uint32_t v = pvs[q], p = 2 * v;
     if (A.ot[p].sz > (uint32_t)A.sub_limit || A.ot[p + 1].sz > (uint32_t)A.sub_limit) return;
@@ -496,19 +526,6 @@ __global__ void k_sub_oblivion_sz(SubArgs A)
     for (uint64_t t = from + threadIdx.x; t < to; t += blockDim.x) A.ot[A.trail[t]].sz = 0;
 }
 
-__global__ void k_sub_range(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e)
-{
-    for (uint64_t q = gtid() + b; q < e; q += gstride()) {
-        const uint32_t v = pvs[q];
-        const uint32_t p = 2 * v;
-        if (A.ot[p].sz <= (uint32_t)A.sub_limit && A.ot[p + 1].sz <= (uint32_t)A.sub_limit) {
-            uint32_t seq = 0;
-            self_sub_x(A, p, (uint32_t)q, seq);
-            plog_end(A.P, (uint32_t)q, seq);
-        }
-    }
-}
-
 // next live binary at or after position i of a list (its other literal in *a); the
 // binaries of an elected variable's sorted list form its prefix
 __device__ __forceinline__ uint32_t next_binary(const uint64_t *list, uint32_t sz, uint32_t i, const uint32_t *mem,
@@ -608,7 +625,18 @@ void sub(Ctx &c)
             k_sub_oblivion_sz<<<1, 64, 0, c.stream>>>(A);
             A.defer_oblivion = 0;
         }
-        else k_sub_range<<<1, 1, 0, c.stream>>>(A, c.pvs.p, u, u + 1);
+        else {   // lists longer than the LDS arrays: the same wave with its arrays in HBM
+            const uint32_t cap = (uint32_t)c.opts.sub_limit + 1;
+            ensure(c, c.tmpu32d, 5ull * cap);
+            PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_SUBFROM, c.cnt.p + CNT_NTRAIL, 8, hipMemcpyDeviceToDevice, c.stream));
+            A.defer_oblivion = 1;
+            k_sub_wave_big<<<1, 64, 0, c.stream>>>(A, c.pvs.p, u, c.tmpu32d.p, cap);
+            k_sub_oblivion<<<256, 256, 0, c.stream>>>(A);
+            k_sub_oblivion_sz<<<1, 64, 0, c.stream>>>(A);
+            A.defer_oblivion = 0;
+            c.path[PATH_SUB_WAVE_BIG]++;
+        }
+        c.path[PATH_SUB_WAVE]++;
         b = u + 1;
     }
     A.allow_units = 0;

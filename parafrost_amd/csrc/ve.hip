@@ -620,10 +620,12 @@ struct VeOut {
     uint32_t *pvs;
     uint64_t ref_base, data_base, model_base;
     ProofLog P;   // proof events: resolvents, substituted clauses and units, decided after the pass
+    const uint32_t *vorg;   // the caller's variable mapping for the model records (null: identity)
 };
 
-__device__ __forceinline__ void save_side(const VeArgs &A, uint32_t *&rs, uint32_t p, bool pbig)
-{   // saveClause/saveWitness for the smaller side (simplify.h:299-331, model.cpp:263-307)
+__device__ __forceinline__ void save_side(const VeArgs &A, const uint32_t *vorg, uint32_t *&rs, uint32_t p, bool pbig)
+{   // saveClause/saveWitness for the smaller side (simplify.h:299-331, model.cpp:263-307),
+    // literals written in original numbering
     const uint32_t n = p ^ 1u;
     const uint32_t side = pbig ? n : p;
     const uint32_t sz = A.ot[side].sz;
@@ -632,12 +634,12 @@ __device__ __forceinline__ void save_side(const VeArgs &A, uint32_t *&rs, uint32
         if (!c_orig(c)) continue;
         const int cs = c_size(c);
         int pos = -1;
-        for (int k = 0; k < cs; k++) { rs[k] = c[3 + k]; if (c[3 + k] == side) pos = k; }
+        for (int k = 0; k < cs; k++) { rs[k] = org_lit(vorg, c[3 + k]); if (c[3 + k] == side) pos = k; }
         if (pos > 0) { const uint32_t t = rs[pos]; rs[pos] = rs[0]; rs[0] = t; }
         rs[cs] = (uint32_t)cs;
         rs += cs + 1;
     }
-    rs[0] = pbig ? p : n;   // witness
+    rs[0] = org_lit(vorg, pbig ? p : n);   // witness
     rs[1] = 1;
     rs += 2;
 }
@@ -683,7 +685,7 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
         uint32_t seq = 0;
         if (t == T_BN) {
             const uint32_t def = aux[q], def_f = def ^ 1u;
-            save_side(A, rs, p, (tw >> 8) & 1u);
+            save_side(A, O.vorg, rs, p, (tw >> 8) & 1u);
             // negatives, then positives
             for (int s = 0; s < 2; s++) {
                 const uint32_t lit = s ? p : n, kill = s ? def_f : def, into = s ? def : def_f;
@@ -742,7 +744,7 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
                     }
                 }
             }
-            save_side(A, rs, p, pbig);
+            save_side(A, O.vorg, rs, p, pbig);
             delete_lists(A, p);
         }
         O.vstate[v] = MELTED_M;
@@ -818,7 +820,7 @@ void ve(Ctx &c)
     A.mem = c.mem.p;
     uint32_t *units = c.units.p;
     VeOut O{c.resolved.p, units, c.refs.p, c.vstate.p, c.pvs.p, c.nrefs, c.msz, c.nres,
-            proof_begin(c, np, c.nrefs + add_refs + add_units + np)};
+            proof_begin(c, np, c.nrefs + add_refs + add_units + np), c.has_vorg ? c.vorg.p : nullptr};
     {
         KTimer kt(c, K_VE_APPLY, 4.0 * add_words + 8.0 * add_refs + 4.0 * add_model);
         k_ve_apply<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, O, type, aux, np, roff, woff, uoff, moff);
@@ -850,6 +852,7 @@ struct BceArgs {
     const OL *ot;
     const uint64_t *occ;
     uint32_t limit;
+    const uint32_t *vorg;   // model records in original numbering (null: identity)
 };
 
 // isTautology (src/cpu/simplify.h:119-137): the resolvent of c1 and c2 on x holds a
@@ -911,7 +914,7 @@ __global__ void __launch_bounds__(64) k_bce(BceArgs A, const uint32_t *__restric
                         const int cs = c_size(neg);
                         int at = -1;
                         for (int k = 0; k < cs; k++) {
-                            r[k] = neg[3 + k];
+                            r[k] = org_lit(A.vorg, neg[3 + k]);
                             if (neg[3 + k] == n) at = k;
                         }
                         if (at > 0) { const uint32_t t = r[at]; r[at] = r[0]; r[0] = t; }
@@ -932,7 +935,8 @@ void bce(Ctx &c)
 {
     if (!c.opts.bce_en || !c.npvs) return;
     const uint32_t np = c.npvs;
-    const BceArgs A{c.mem.p, c.ot.p, c.occ.p, (uint32_t)std::max<int32_t>(c.opts.bce_limit, 0)};
+    const BceArgs A{c.mem.p, c.ot.p, c.occ.p, (uint32_t)std::max<int32_t>(c.opts.bce_limit, 0),
+                    c.has_vorg ? c.vorg.p : nullptr};
     ensure(c, c.tmpu64a, (uint64_t)np + 2);
     ensure(c, c.tmpu64b, (uint64_t)np + 2);
     uint64_t *nmodel = c.tmpu64a.p, *moff = c.tmpu64b.p;

@@ -41,3 +41,72 @@ def adversarial(seed):
     lits = np.array([l for c in clean for l in c], np.uint32)
     offs = np.concatenate([[0], np.cumsum([len(c) for c in clean])]).astype(np.uint64)
     return gen.Cnf(nv, lits, offs)
+
+
+def ere_shared_targets(copies: int = 4):
+    """Formulas where eager redundancy elimination meets a target more than once
+    (run with phases=0, so that the first round is the ERE round). Per copy, with
+    fresh variables:
+      x: (x a b), (~x c)      -> resolvent (a b c)
+      y: (y a c), (~y b)      -> resolvent (a b c)
+      T = (a b c) twice (a duplicate)
+      z: (z d), (z d e), (~z e) -> resolvent (d e) from both pairs
+      D = (d e) twice
+    and clauses with ~a, ~b, ~c, ~d, ~e over filler variables, so that no literal is
+    pure and x, y, z come first in the elimination order (smallest scores). In the
+    reference's sequential loop x deletes the first T and y meets it deleted and stops;
+    z's first pair deletes the first D and its second pair meets it deleted and stops:
+    both duplicates stay."""
+    cls = []
+    nv = 0
+
+    def fresh(k):
+        nonlocal nv
+        out = list(range(nv + 1, nv + 1 + k))
+        nv += k
+        return out
+
+    P = lambda v: 2 * v
+    N = lambda v: 2 * v + 1
+    for _ in range(copies):
+        x, y, z, a, b, c, d, e = fresh(8)
+        cls += [[P(x), P(a), P(b)], [N(x), P(c)], [P(y), P(a), P(c)], [N(y), P(b)],
+                [P(a), P(b), P(c)], [P(a), P(b), P(c)],
+                [P(z), P(d)], [P(z), P(d), P(e)], [N(z), P(e)], [P(d), P(e)], [P(d), P(e)]]
+        for v in (a, b, c, d, e):
+            f, g = fresh(2)
+            cls += [[N(v), P(f), P(g)], [N(f), N(g), P(v)], [P(f), N(g)], [N(f), P(g)]]
+    clean = [sorted(c) for c in cls]
+    lits = np.array([l for c in clean for l in c], np.uint32)
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in clean])]).astype(np.uint64)
+    return gen.Cnf(nv, lits, offs)
+
+
+def sub_long_unit_capable(nlong: int = 3500, seed: int = 0):
+    """One variable x with more than nlong clauses on each side (longer than the
+    subsumption wave's LDS arrays) and a complementary binary pair (x a), (~x a), so
+    that strengthening makes a unit: run with sub_limit above nlong and every other
+    variable frozen (ifreeze), so that LCVE elects x. Its long lists hold duplicates and
+    subsumed clauses for the wave to find."""
+    rng = np.random.default_rng(9900 + seed)
+    nv = 4 * nlong + 10
+    x, a = 1, 2
+    cls = [[2 * x, 2 * a], [2 * x + 1, 2 * a]]
+    for s in (0, 1):
+        for i in range(nlong):
+            o = rng.choice(np.arange(3, nv + 1), int(rng.integers(1, 4)), replace=False)
+            cls.append([2 * x + s] + [int(2 * v + rng.integers(0, 2)) for v in o])
+            if i % 50 == 0:   # a duplicate and a superset of the clause before
+                cls.append(list(cls[-1]))
+                cls.append(cls[-1] + [2 * int(rng.integers(3, nv + 1)) + 1])
+    clean = []
+    for c in cls:
+        seen, out = set(), []
+        for l in sorted(c):
+            if (l >> 1) not in seen:
+                seen.add(l >> 1)
+                out.append(l)
+        clean.append(out)
+    lits = np.array([l for c in clean for l in c], np.uint32)
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in clean])]).astype(np.uint64)
+    return gen.Cnf(nv, lits, offs), np.arange(2, nv + 1, dtype=np.uint32)

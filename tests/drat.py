@@ -53,6 +53,43 @@ def normalized(lines) -> list:
     return out
 
 
+def fast_normalized(b) -> list:
+    """normalized() for large proofs, on line bytes instead of parsed literals: a line
+    ends at its zero byte (every varint byte of a literal >= 2 is nonzero); a unit line
+    is a tag, one varint and the zero. Runs of unit lines are sorted as byte strings
+    (the comparison is of sets, so any fixed order serves)."""
+    a = np.asarray(b, np.uint8)
+    if not len(a):
+        return []
+    ends = np.flatnonzero(a == 0)
+    starts = np.concatenate([[0], ends[:-1] + 1])
+    # varint ends inside a line: bytes below 128 other than the tag and the zero
+    term = (a < 128).astype(np.int64)
+    cs = np.concatenate([[0], np.cumsum(term)])
+    nlits = cs[ends] - cs[starts + 1]          # terminal bytes in (tag, zero)
+    raw = a.tobytes()
+    lines = [raw[s:e + 1] for s, e in zip(starts.tolist(), ends.tolist())]
+    unit = (a[starts] == ord("a")) & (nlits == 1)
+    empty = (a[starts] == ord("a")) & (ends == starts + 1)
+    n = len(lines)
+    if empty[-1]:
+        k = n - 1
+        while k > 0 and unit[k - 1]:
+            k -= 1
+        lines = lines[:k] + [lines[-1]]
+        unit = np.concatenate([unit[:k], [False]])
+    out, run = [], []
+    for ln, u in zip(lines, unit.tolist()):
+        if u:
+            run.append(ln)
+            continue
+        out.extend(sorted(run))
+        run = []
+        out.append(ln)
+    out.extend(sorted(run))
+    return out
+
+
 def _propagate(clauses, assign):
     """unit propagation to a fixpoint; True on a conflict"""
     changed = True

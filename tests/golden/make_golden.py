@@ -9,7 +9,10 @@ simplifier (ParaFROST::map(true)). Cases f* run in incremental mode with variabl
 frozen by ifreeze() and, for some, units given as unit clauses. Cases b* run with
 blocked clause elimination on, over formulas with planted blocked clauses
 (tests/blocked.py). Cases without units also hold the reference's binary DRAT proof
-lines of the simplifier. Output: tests/golden/reference_vectors.npz (data only).
+lines of the simplifier. Cases g* and b* that the first call leaves satisfiable also
+hold a second simplifier call on the mapped formula (second_*: the reference's second
+sigmify() after map(true), whose model records and proof lines are written in original
+numbering through the mapped vorg). Output: tests/golden/reference_vectors.npz (data only).
 
 usage: python tests/golden/make_golden.py
 """
@@ -115,6 +118,20 @@ def main():
             if isinstance(mp, po.Mapped):
                 for f in ("lits", "offs", "vorg", "trail", "vstate"):
                     arrays[f"{name}/map_{f}"] = getattr(mp, f)
+            if isinstance(mp, po.Mapped) and freeze is None:
+                # a second sigmify() on the mapped formula, proof on (records and proof
+                # lines through the mapped vorg)
+                o2 = po.default_opts(**opts)
+                o2.proof_en = 1
+                r2 = po.run_reference(path, os.path.join(td, name + ".second.bin"), o2, second=True)
+                if r2 is not None:
+                    m2, s2, nfirst = r2
+                    assert np.array_equal(m2.lits, mp.lits) and np.array_equal(m2.vorg, mp.vorg)
+                    for f in ("cnf", "trail", "vstate", "proof"):
+                        arrays[f"{name}/second_{f}"] = getattr(s2, f)
+                    arrays[f"{name}/second_resolved"] = s2.resolved[nfirst:]
+                    arrays[f"{name}/second_meta"] = np.array([o2.phases - 1 if o2.phases > 2 else o2.phases, s2.unsat],
+                                                              np.int64)
             if units is None:
                 # the reference's binary DRAT lines of the simplifier (-proof)
                 op = po.default_opts(**opts)

@@ -40,6 +40,10 @@ def state_gpu(g: "pf.Simplifier", lists: bool = False) -> po.State:
 
 
 def compare(o: po.State, g: po.State, where: str, lists: bool = False):
+    _compare(o, g, where, lists)
+
+
+def _compare(o: po.State, g: po.State, where: str, lists: bool = False):
     def fail(what, a, b):
         msg = f"{where}: {what} differs"
         if len(a) != len(b):
@@ -66,15 +70,28 @@ def compare(o: po.State, g: po.State, where: str, lists: bool = False):
                 fail(f"occurrence list of literal {2 * int(o.pvs[k // 2]) + (k & 1)}", a, b)
 
 
-def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, **kw):
+def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, vorg=None, inspect=None, keep=None,
+                 **kw):
     """Run both engines pass by pass; raise Mismatch at the first difference.
     setup(g), if given, configures the GPU simplifier before the upload; freeze lists
-    variables frozen by ifreeze (incremental mode). Returns the number of rounds and
-    the final state. With proof_en the binary DRAT lines written so far are compared
-    after every pass too (drat.normalized: runs of unit lines as sets)."""
+    variables frozen by ifreeze (incremental mode); vorg is the caller's variable
+    mapping (model records and proof lines in original numbering); inspect(g) is
+    called on the GPU simplifier after the last comparison; keep (a dict), if given,
+    receives the last GPU state compared ("gpu") and where ("where"). Returns the number of
+    rounds and the final state. With proof_en the binary DRAT lines written so far are
+    compared after every pass too (drat.normalized: runs of unit lines as sets)."""
     oo, go = _opts_pair(**kw)
     o = po.Oracle(cnf, oo)
     g = pf.Simplifier(cnf.nvars, go)
+
+    def compare(a, gs, where, lists=False):
+        _compare(a, gs, where, lists)
+        if keep is not None:
+            keep["gpu"], keep["where"] = gs, where
+
+    if vorg is not None:
+        o.set_vorg(vorg)
+        g.set_vorg(vorg)
 
     def proof(where):
         if not oo.proof_en:
@@ -185,13 +202,16 @@ def run_lockstep(cnf, lists: bool = True, units=None, setup=None, freeze=None, *
         fo, fg = o.state(), state_gpu(g)
         compare(fo, fg, "final")
         proof("final")
+        if inspect is not None:
+            inspect(g)
         return rounds, fg
     finally:
         g.close()
 
 
-def run_full(cnf, **kw):
-    """Whole sigmify on both sides, final states compared."""
+def run_full(cnf, proof_cmp=None, **kw):
+    """Whole sigmify on both sides, final states compared; with proof_en the proofs
+    too, through proof_cmp(bytes) (default drat.normalized(drat.parse(bytes)))."""
     oo, go = _opts_pair(**kw)
     o = po.Oracle(cnf, oo)
     ro = o.sigmify()
@@ -208,7 +228,8 @@ def run_full(cnf, **kw):
         if ro == 0:
             compare(o.state(), state_gpu(g), "sigmify")
         if oo.proof_en:
-            a, b = drat.normalized(drat.parse(o.proof())), drat.normalized(drat.parse(g.proof()))
+            norm = proof_cmp or (lambda b: drat.normalized(drat.parse(b)))
+            a, b = norm(o.proof()), norm(g.proof())
             if a != b:
                 raise Mismatch(f"proof differs ({len(a)} / {len(b)} lines)")
         return o.state(), g.times()

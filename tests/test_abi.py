@@ -55,3 +55,15 @@ def test_batch_context_rejects_proof():
     nv = (ctypes.c_uint32 * 2)(10, 20)
     assert L.pfg_create_batch(ctypes.byref(h), 0, 0, 2, nv, ctypes.byref(o)) == pf.PFG_EINVAL
     assert not h.value
+
+
+def test_new_entry_points_reject_a_null_context():
+    """pfg_interrupt, pfg_set_vorg and pfg_path_counts check their context before any
+    device work."""
+    import parafrost_amd as pf
+    L = pf.lib()
+    assert L.pfg_interrupt(None) == pf.PFG_EINVAL
+    assert L.pfg_set_vorg(None, None, 0) == pf.PFG_EINVAL
+    n = ctypes.c_int()
+    assert L.pfg_path_counts(None, None, 0, ctypes.byref(n)) == pf.PFG_EINVAL
+    assert pf.PFG_EINTR == 6 and pf.PATH_NAMES[0] == "ere_extra_rounds"

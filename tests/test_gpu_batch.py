@@ -210,3 +210,20 @@ def test_batch_export_matches_download():
                     h.close()
     finally:
         g.close()
+
+
+def test_c4_workload_instances_match_oracle():
+    """24 instances of the batching workload itself (the c4 generator's own specs and
+    seeds: random 3-SAT at ratio 4.0-4.3 and 4-SAT at 9.5-10.0, variables log-uniform in
+    [1,000, 200,000]), taken evenly over the size order -- from 1,003 to 199,129
+    variables, three above 100,000, 4-SAT and 3-SAT mixed, 22.6M literals -- in one
+    batch, every instance against the oracle alone."""
+    from parafrost_amd import gen as G
+    nv, ncls, ks, seeds = G.c4_specs(4096)
+    order = np.argsort(nv, kind="stable")
+    pick = np.sort(order[np.linspace(0, len(order) - 1, 24).round().astype(np.int64)])
+    assert (nv[pick] > 100_000).sum() >= 2 and set(ks[pick].tolist()) == {3, 4}
+    b = G.batch_ksat(nv[pick], ncls[pick], ks[pick], seeds[pick])
+    cnfs = [b.instance(i) for i in range(b.ninst)]
+    print(f"c4 instances: {b.ninst}, {b.nlits} literals", flush=True)
+    _check(cnfs)

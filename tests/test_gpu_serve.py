@@ -55,10 +55,19 @@ def test_serving_equals_plain_chain():
     assert ext and all(r.nlits < gen.aig_bmc(r.seed, r.ngates, 0, r.t1)[0].nlits for r in ext)
 
 
-def test_serving_matches_oracle_chain():
+@pytest.mark.parametrize("case", ["small", "c5_gates"])
+def test_serving_matches_oracle_chain(case):
+    """Every request against the oracle's own request chain. c5_gates: requests in the
+    serving workload's size range (10k-100k AND gates), one of them an extension."""
     from parafrost_amd import serve
-    reqs = serve.make_requests(10, seed=32, gates=(300, 900), max_frames=6, extend_frac=0.6)
-    srv = serve.Server(slots=2, batch_lits=4e3, keep_results=True)
+    if case == "small":
+        reqs = serve.make_requests(10, seed=32, gates=(300, 900), max_frames=6, extend_frac=0.6)
+        srv = serve.Server(slots=2, batch_lits=4e3, keep_results=True)
+    else:
+        reqs = serve.make_requests(4, seed=36, gates=(10_000, 100_000), max_frames=5, extend_frac=0.6)
+        assert any(r.t0 > 0 for r in reqs) and min(r.ngates for r in reqs) >= 10_000
+        srv = serve.Server(slots=2, batch_lits=2e6, keep_results=True)
+    print(f"serving vs oracle chain: {case}, gates {[r.ngates for r in reqs]}", flush=True)
     srv.run(reqs, rate=1000.0)
     state = {}
     for r in reqs:

@@ -180,3 +180,55 @@ def test_oracle_proofs_pass_rup_check():
             refuted += 1
             assert lines[-1] == ("a", ())
     assert checked > 1000 and refuted >= 1
+
+
+def second_call_case(name):
+    """(mapped formula, options, vorg, units) of a case's second simplifier call: the
+    reference's second sigmify() on the formula map(true) left after the first call"""
+    cnf, opts, unsat = _case(name)
+    meta = Z[f"{name}/second_meta"]
+    m = Cnf(len(Z[f"{name}/map_vorg"]) - 1, Z[f"{name}/map_lits"], Z[f"{name}/map_offs"])
+    opts.phases = int(meta[0])
+    opts.proof_en = 1
+    return m, opts, Z[f"{name}/map_vorg"], Z[f"{name}/map_trail"], int(meta[1])
+
+
+SECOND = [n for n in NAMES if f"{n}/second_meta" in Z.files]
+
+
+@pytest.mark.parametrize("name", SECOND)
+def test_oracle_second_call_with_vorg_matches_reference(name):
+    """A second simplifier call after the variable mapping: the reference renumbers its
+    variables (map(true)) and writes model records and proof lines through vorg in the
+    original numbering (model.cpp:263-280, proof.cpp:118-135). The oracle, given the
+    mapped formula, its vorg and the kept unit, reproduces the second call's state,
+    its model records (the stack's suffix) and its proof lines."""
+    m, opts, vorg, units, unsat = second_call_case(name)
+    o = po.Oracle(m, opts)
+    o.set_vorg(vorg)
+    assert o.enqueue(units)
+    rc = o.sigmify()
+    assert (rc != 0) == bool(unsat)
+    st = o.state()
+    assert np.array_equal(st.cnf, Z[f"{name}/second_cnf"])
+    assert np.array_equal(np.sort(st.trail), np.sort(Z[f"{name}/second_trail"]))
+    assert np.array_equal(st.vstate, Z[f"{name}/second_vstate"])
+    assert np.array_equal(st.resolved, Z[f"{name}/second_resolved"])
+    assert np.array_equal(o.proof(), Z[f"{name}/second_proof"])
+
+
+def test_second_call_vorg_is_not_trivial():
+    """The second-call cases exercise a renumbering: vorg differs from the identity,
+    records and proof lines of some cases hold renumbered literals, and without the
+    mapping the oracle's records differ from the reference's."""
+    moved = differs = 0
+    for name in SECOND:
+        m, opts, vorg, units, unsat = second_call_case(name)
+        moved += int((vorg[1:] != np.arange(1, len(vorg))).any())
+        if not len(Z[f"{name}/second_resolved"]):
+            continue
+        o = po.Oracle(m, opts)
+        assert o.enqueue(units)
+        o.sigmify()
+        differs += not np.array_equal(o.state().resolved, Z[f"{name}/second_resolved"])
+    assert moved >= len(SECOND) // 2 and differs >= 10

@@ -167,3 +167,40 @@ def test_adversarial(seed):
             assert np.array_equal(getattr(st, f), getattr(ref, f)), f
     if opts.proof_en:
         assert np.array_equal(o.proof(), ref.proof)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_mid_size_second_call_vorg(seed):
+    """A second simplifier call after map(true): the mapped formula with its vorg and
+    kept unit; model records (the stack's suffix) and proof lines in original numbering."""
+    rng = np.random.default_rng(780 + seed)
+    nv = int(rng.integers(2000, 12000))
+    cnf = (gen.geometric(nv, int(nv * rng.uniform(1.5, 3.5)), 2, 8, 780 + seed) if seed % 2 else
+           gen.uniform_ksat(nv, int(nv * rng.uniform(2.5, 4.2)), 3, 780 + seed))
+    opts = po.default_opts(phases=int(rng.integers(1, 6)), bce_en=seed % 3 == 2, proof_en=1)
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "f.cnf")
+        if seed % 2 == 0:   # level-0 units: the mapping keeps one assigned variable
+            vs = rng.choice(np.arange(1, nv + 1), 3, replace=False)
+            units = (2 * vs + rng.integers(0, 2, 3)).astype(np.uint32)
+            lits = np.concatenate([cnf.lits, units])
+            offs = np.concatenate([cnf.offs, cnf.offs[-1] + 1 + np.arange(3, dtype=np.uint64)])
+            gen.Cnf(cnf.nvars, lits, offs).write_dimacs(p)
+        else:
+            cnf.write_dimacs(p)
+        r = po.run_reference(p, os.path.join(td, "f.bin"), opts, second=True)
+    if r is None:
+        return
+    m, ref, nfirst = r
+    o2 = po.default_opts(phases=opts.phases - 1 if opts.phases > 2 else opts.phases, bce_en=opts.bce_en, proof_en=1)
+    o = po.Oracle(gen.Cnf(m.nvars, m.lits, m.offs), o2)
+    o.set_vorg(m.vorg)
+    assert o.enqueue(m.trail)
+    rc = o.sigmify()
+    assert (rc != 0) == bool(ref.unsat)
+    st = o.state()
+    assert np.array_equal(st.cnf, ref.cnf)
+    assert np.array_equal(np.sort(st.trail), np.sort(ref.trail))
+    assert np.array_equal(st.vstate, ref.vstate)
+    assert np.array_equal(st.resolved, ref.resolved[nfirst:])
+    assert np.array_equal(o.proof(), ref.proof)
