@@ -1,22 +1,31 @@
 """Benchmark of the MI355X SIGmA simplifier.
 
-Workload (one GPU): the single large instance -- random mixed-width CNF with
-2,000,000 variables and 8,400,000 clauses, widths 2..8 (2 + geometric(1/2)),
-seed 2 -- simplified by the full multi-round inprocessing loop (LCVE, SUB, BVE
-with gates, ERE, GC; reference defaults, 5 phases).
+Default workload (the headline metric, one GPU): batching -- 4,096 independent
+random 3-SAT / 4-SAT instances (variables log-uniform in [1,000, 200,000], ratios
+U[4.0, 4.3] / U[9.5, 10.0], seed 4, 4.06e9 literals), simplified in batches of
+instances that share one context's kernel launches, against the plain path (one
+context per instance); then serving -- a Poisson stream of incremental BMC-like
+requests (random AIGs of 10k-100k AND gates, 1-64 frames, seed 5) with continuous
+batching over HBM-resident instances, p50/p99 latency per offered load, against the
+same request chain served one at a time.
 
-A step = one complete simplification of that instance: the clause image is
-rebuilt in HBM from the resident input literals (prep: sort + signature), then
-the whole round loop runs on the device. Inputs are resident in HBM before the
-timed region. Metric: input literals / second.
+A step = one pass of the batched path over every instance (inputs resident in HBM;
+each batch's clause image is rebuilt from them, then the whole round loop runs on the
+device). value = input literals / second of the batched path.
 
-Prints one JSON line (rank 0). Extra objects:
-  roofline     -- HBM roofline of the dominant kernel: algorithmic bytes per launch
-                  (computed on the device by one extra, untimed step) over the
-                  kernel's average HIP-event duration, against 8 TB/s
-  cpu_baseline -- the reference CPU simplifier built from its sources (or, where
-                  that build is absent, the oracle, its C restatement), single
-                  thread, on a bounded sample of the same workload.
+Prints one JSON line. Extra objects:
+  plain, vs_plain -- the plain per-instance path on a spread sample of the instances
+  served          -- the serving workload: literals / s, p50 / p99 latency per offered
+                     load, and its plain chain
+  writeback_ms    -- downloading every instance's results (clauses, model records,
+                     variable states, units) to host memory once, outside the timed steps
+  roofline        -- HBM roofline of the dominant kernel: algorithmic bytes per launch
+                     (computed on the device by one extra, untimed step) over the
+                     kernel's average HIP-event duration, against 8 TB/s
+  cpu_baseline    -- the reference CPU simplifier built from its sources (or, where
+                     that build is absent, the oracle, its C restatement), single
+                     thread, on a bounded sample of the same instances.
+Other workloads: --config c1 | c2 | c3 (one instance), c4 or c5 alone.
 """
 from __future__ import annotations
 
@@ -40,7 +49,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="headline", choices=["headline", "c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--c5-requests", type=int, default=64, help="c5: requests in the stream")
     ap.add_argument("--c5-rates", default="50,200,500,2000", help="c5: offered loads swept (requests / s)")
     ap.add_argument("--c5-slots", type=int, default=3, help="c5: batches in flight (one context each)")
@@ -50,6 +59,10 @@ def parse():
     ap.add_argument("--c4-instances", type=int, default=4096, help="c4: number of instances")
     ap.add_argument("--plain-sample", type=int, default=64, help="c4: instances timed on the plain path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="c2-c4: measure candidate tunings on the workload first and record the best "
+                         "(parafrost_amd/tuning/<config>.json); by default the recorded tuning is applied")
+    ap.add_argument("--no-tuning", action="store_true", help="run with the library's default tuning")
     ap.add_argument("--bce", action="store_true", help="c1-c3: blocked clause elimination on (off by default, as in the reference)")
     ap.add_argument("--proof", action="store_true", help="c1-c3: binary DRAT proof output on (reports its size)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -157,10 +170,11 @@ def cpu_baseline_reference(sample, scale: str):
             t_total += t[0]
             lits += sample.nlits
             reps += 1
-    return {"value": lits / t_total, "unit": "literals/s", "cores": 1, "kind": "reference",
-            "sample": f"{reps} x full simplification by the reference CPU simplifier (src/cpu, built from its "
-                      f"sources; parse excluded) of the same generator at {scale} ({sample.nvars} vars, "
-                      f"{sample.ncls} clauses, {sample.nlits} literals), single thread, {t_total:.1f} s"}
+    return dict({"value": lits / t_total, "unit": "literals/s", "cores": 1, "kind": "reference",
+                 "sample": f"{reps} x full simplification by the reference CPU simplifier (src/cpu, built from its "
+                           f"sources; parse excluded) of the same generator at {scale} ({sample.nvars} vars, "
+                           f"{sample.ncls} clauses, {sample.nlits} literals), single thread, {t_total:.1f} s"},
+                **host_info())
 
 
 def cpu_baseline(cfg: str):
@@ -225,7 +239,7 @@ def run_c4(args):
     instances are admitted in order into batches of at most --batch-lits literals; each
     batch is one context whose kernel launches serve all of its instances. A step = every
     batch simplified once (inputs resident in HBM). The plain path (one context per
-    instance) is timed on an evenly spread sample of instances."""
+    instance) is timed on an evenly spread sample of instances. Returns the JSON record."""
     import torch
     import parafrost_amd as pf
     from parafrost_amd import gen
@@ -233,8 +247,6 @@ def run_c4(args):
     B = gen.c4(ninst=args.c4_instances)
     gen_s = time.perf_counter() - t0
     dev = torch.device("cuda:0")
-    d_lits = torch.from_numpy(B.lits.view(np.int32)).to(dev)
-    torch.cuda.synchronize()
     # batches by literal budget, in instance order
     lit_first = B.offs[B.cls_first.astype(np.int64)]
     groups, a = [], 0
@@ -255,20 +267,21 @@ def run_c4(args):
     torch.cuda.synchronize()
     cap = max(int(gen.batch_bases(nv)[-1]) for nv, *_ in inputs)
     # one context sized for the largest group, laid out again for every group
-    probe = np.zeros(1, np.uint32)
-    probe[0] = cap
-    s = pf.BatchSimplifier(probe, pf.default_opts())
+    s = pf.BatchSimplifier(np.array([cap], np.uint32), pf.default_opts())
 
-    def step(account=False):
+    def step(account=False, download=None):
         st = []
         s.set_accounting(account)
         for (nv, lits, offs, ncls, nl) in inputs:
             s.relayout(nv)
             s.upload_device(lits.data_ptr(), offs.data_ptr(), ncls, nl)
             st.append(s.sigmify_batch())
+            if download is not None:
+                download(len(nv))
         s.set_accounting(False)
         return st
 
+    tuning = setup_tuning(args, s, step, "c4", "batched c4 workload")
     for _ in range(args.warmup):
         step()
     s.reset_times()
@@ -283,7 +296,25 @@ def run_c4(args):
     step(account=True)
     ka = s.kernel_times()
     roof, kernels = roofline(kt, ka)
+    if roof:
+        roof["traffic"] = traffic_for("c4", roof["kernel"])
+        if roof["traffic"] is not None:
+            roof["traffic_source"] = "profiles/traffic_c4.txt (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE)"
     unsat = int(sum(int((x != 0).sum()) for x in st))
+    # write-back: every instance's results to host memory (reused arrays), outside the timed steps
+    pool = pf.HostPool()
+    wb = {"s": 0.0, "words": 0}
+
+    def download(n):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for i in range(n):
+            pool.reset()
+            w, t, r, vs = s.instance(i, pool)
+            wb["words"] += len(w) + len(t) + len(r) + len(vs) // 4
+        wb["s"] += time.perf_counter() - t1
+
+    step(download=download)
     s.close()
     del inputs
     # plain path on a spread sample
@@ -324,11 +355,15 @@ def run_c4(args):
                   "sample": f"{len(pick)} instances spread over the batch, {plain_l} literals, one context each, "
                             "second of two runs timed"},
         "vs_plain": value / plain,
+        "tuning": tuning,
+        "writeback_ms": round(wb["s"] * 1e3, 1),
+        "writeback": f"every instance's clauses, model records, variable states and units downloaded into reused "
+                     f"host memory after its batch ({wb['words']} words), once, outside the timed steps",
         "roofline": roof, "kernels": kernels,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_instances(B)
-    print(json.dumps(out))
+    return out
 
 
 def run_c5(args):
@@ -388,30 +423,103 @@ def run_c5(args):
                           "context, results downloaded, clauses written back on the host"},
         "vs_plain": best["literals_per_s"] / plain,
     }
-    print(json.dumps(out))
+    return out
+
+
+def setup_tuning(args, s, step, name, workload):
+    """--autotune: measure and record; otherwise apply the recorded tuning (if any)."""
+    import torch
+    from parafrost_amd import tune
+    if args.no_tuning:
+        return {"source": "library defaults", **s.tuning()}
+    if args.autotune:
+        rec = tune.tune(s, step, torch.cuda.synchronize, name, reps=1, workload=workload,
+                        log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+        return {"source": f"measured now ({len(rec['trials'])} candidates)", **s.tuning()}
+    rec = tune.apply(s, name)
+    src = f"parafrost_amd/tuning/{name}.json" if rec else "library defaults (no record for this device)"
+    return {"source": src, **s.tuning()}
+
+
+def host_info():
+    """the GPU box host's online cores and CPU model"""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"host_cores_online": os.cpu_count(), "cpu_model": model}
 
 
 def cpu_baseline_instances(B):
-    """The oracle on instances of the batch (spread over sizes) until about 10 s."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as po
-    order = np.argsort(B.inst_nvars)
-    pick = order[np.linspace(0, len(order) // 2, 40).round().astype(np.int64)]
+    """The reference CPU simplifier (oracle/_ref/pfref, default options, its own timer
+    around the simplifier) on instances of the batch spread over all sizes, in a seeded
+    random order, until about 15 s of simplifier time; the oracle (its C restatement)
+    where that build is absent."""
+    order = np.argsort(B.inst_nvars, kind="stable")
+    pick = order[np.linspace(0, len(order) - 1, 200).round().astype(np.int64)]
+    pick = pick[np.random.default_rng(4).permutation(len(pick))]   # spread over the sizes until the time is up
     t_total, lits, n = 0.0, 0, 0
-    for i in pick:
-        c = B.instance(int(i))
-        o = po.Oracle(c, po.default_opts())
-        t0 = time.perf_counter()
-        o.sigmify()
-        t_total += time.perf_counter() - t0
-        lits += c.nlits
-        n += 1
-        del o
-        if t_total > 10.0:
-            break
-    return {"value": lits / t_total, "unit": "literals/s", "cores": 1, "kind": "port",
-            "sample": f"{n} instances of the batch (smaller half, spread), full sigmify each, single thread, "
-                      f"{lits} literals in {t_total:.1f} s"}
+    kind = "reference" if os.path.exists(REF_BIN) else "port"
+    if kind == "reference":
+        import subprocess
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            path, outp = os.path.join(td, "i.cnf"), os.path.join(td, "o.bin")
+            for i in pick:
+                c = B.instance(int(i))
+                c.write_dimacs(path)
+                r = subprocess.run([REF_BIN, path, outp], env=dict(os.environ, PFREF_TIME="1"), capture_output=True,
+                                   text=True, timeout=600)
+                t = [float(x.split()[1]) for x in r.stderr.splitlines() if x.startswith("PFREF_TIME ")]
+                if r.returncode != 0:
+                    kind = "port"
+                    break
+                if not t:   # refuted: the simplifier ends the process before its timer prints
+                    continue
+                t_total += t[0]
+                lits += c.nlits
+                n += 1
+                if t_total > 15.0:
+                    break
+    if kind == "port":
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        t_total, lits, n = 0.0, 0, 0
+        for i in pick:
+            c = B.instance(int(i))
+            o = po.Oracle(c, po.default_opts())
+            t0 = time.perf_counter()
+            o.sigmify()
+            t_total += time.perf_counter() - t0
+            lits += c.nlits
+            n += 1
+            del o
+            if t_total > 15.0:
+                break
+    what = ("the reference CPU simplifier (src/cpu, built from its sources; parse excluded)" if kind == "reference"
+            else "the oracle (single-thread C restatement of the reference CPU simplifier)")
+    return dict({"value": lits / t_total, "unit": "literals/s", "cores": 1, "kind": kind,
+                 "sample": f"{n} instances of the batch (spread over the sizes, seeded random order), full "
+                           f"simplification each by {what}, single thread, {lits} literals in {t_total:.1f} s"},
+                **host_info())
+
+
+def run_headline(args):
+    """The metric's workloads: the batched path (c4) as the line, the served path (c5)
+    as its `served` object."""
+    out = run_c4(args)
+    print(f"# c4 batched {out['value'] / 1e6:.1f} M literals/s, plain {out['plain']['value'] / 1e6:.1f}, "
+          f"x{out['vs_plain']:.2f}", file=sys.stderr, flush=True)
+    c5 = run_c5(args)
+    out["served"] = {"workload": c5["config"]["workload"], "path": c5["config"]["path"],
+                     "value": c5["value"], "unit": "literals/s", "p50_ms": c5["p50_ms"], "p99_ms": c5["p99_ms"],
+                     "sweep": c5["served"], "plain": c5["plain"], "vs_plain": c5["vs_plain"]}
+    out["config"]["served_workload"] = c5["config"]["workload"]
+    return out
 
 
 def main():
@@ -429,11 +537,14 @@ def main():
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
         return
+    if args.config == "headline":
+        print(json.dumps(run_headline(args)))
+        return
     if args.config == "c4":
-        run_c4(args)
+        print(json.dumps(run_c4(args)))
         return
     if args.config == "c5":
-        run_c5(args)
+        print(json.dumps(run_c5(args)))
         return
     import torch
     import parafrost_amd as pf
@@ -458,6 +569,7 @@ def main():
         except pf.Unsat:   # c3: the simplifier proves the formula unsatisfiable mid-way
             outcome["refuted"] = True
 
+    tuning = setup_tuning(args, s, step, args.config, f"single instance {args.config}")
     for _ in range(args.warmup):
         step()
     s.reset_times()
@@ -510,6 +622,7 @@ def main():
                    "eliminated": stats["eliminated"], "pass_ms_per_step": {k: round(v / args.steps, 3) for k, v in times.items()},
                    "outcome": ("refuted: the formula is unsatisfiable; a step is the rounds up to the refutation"
                                if outcome["refuted"] else "simplified")},
+        "tuning": tuning,
         "roofline": roof,
         "kernels": kernels,
     }

@@ -94,6 +94,21 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
  * pfg_upload_cnf_device / pfg_upload_batch together with new clauses. */
 int pfg_batch_export(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint64_t *d_offs, uint64_t *nlits, uint64_t *ncls);
 
+/* Launch and algorithm choices of the kernels (no counterpart in the reference, whose
+ * OPTIMIZEBLOCKS* macros halve block sizes by a fixed rule, src/gpu/grid.cuh:54-85).
+ * Results never depend on them; parafrost_amd/tune.py measures candidates on a
+ * workload and records the best. */
+typedef struct pfg_tuning {
+    int32_t ot_build;      /* occurrence table: 0 LSD radix sort of (literal, reference) keys,
+                              1 counting (histogram, atomic fill, every list sorted by reference) */
+    uint32_t clause_grid;  /* 256-thread blocks of the clause-streaming passes (LCVE, histogram, fill),
+                              rounded up to a multiple of 8 (one window per XCD) */
+    uint32_t sub_group;    /* lanes per elected variable in subsumption: 8, 16, 32 or 64 */
+    uint32_t stage_words;  /* LDS words per block of the staged kernels (pfg_set_stage_words) */
+} pfg_tuning;
+int pfg_set_tuning(pfg_ctx *ctx, const pfg_tuning *t);
+int pfg_get_tuning(pfg_ctx *ctx, pfg_tuning *t);
+
 /* per-pass device time accumulated over the context's life, in milliseconds */
 typedef struct pfg_times {
     double upload, hist, ot, sort_ot, prop, lcve, sub, ve, ere, gc, count, total, bce;

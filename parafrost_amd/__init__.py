@@ -40,6 +40,15 @@ class Times(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    """launch and algorithm choices (pfg_tuning); results never depend on them"""
+    _fields_ = [("ot_build", ctypes.c_int32), ("clause_grid", ctypes.c_uint32), ("sub_group", ctypes.c_uint32),
+                ("stage_words", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
 class KStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char_p), ("kernel", ctypes.c_char_p), ("ms", ctypes.c_double),
                 ("launches", ctypes.c_uint64), ("bytes", ctypes.c_double)]
@@ -53,7 +62,7 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
        "pfg_set_accounting", "pfg_set_stage_words", "pfg_create_batch", "pfg_batch_ninst", "pfg_batch_relayout", "pfg_upload_batch",
        "pfg_sigmify_batch", "pfg_batch_download", "pfg_ifreeze", "pfg_batch_export", "pfg_download_mapped",
-       "pfg_download_proof", "pfg_interrupt", "pfg_set_vorg", "pfg_path_counts"]
+       "pfg_download_proof", "pfg_interrupt", "pfg_set_vorg", "pfg_path_counts", "pfg_set_tuning", "pfg_get_tuning"]
 
 
 def lib_path() -> str:
@@ -113,6 +122,8 @@ def lib():
         L.pfg_batch_export.argtypes = [vp, u32, vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.pfg_reset.argtypes = [vp]
         L.pfg_interrupt.argtypes = [vp]
+        L.pfg_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+        L.pfg_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
         L.pfg_set_vorg.argtypes = [vp, vp, u32]
         L.pfg_path_counts.argtypes = [vp, vp, i32, ctypes.POINTER(i32)]
         L.pfg_last_error.argtypes = [vp]
@@ -251,6 +262,21 @@ class Simplifier:
             return
         a = np.ascontiguousarray(vorg, np.uint32)
         self._check(self.L.pfg_set_vorg(self.h, a.ctypes.data, len(a)))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._check(self.L.pfg_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        """change some of the launch and algorithm choices (see Tuning)"""
+        t = Tuning()
+        self._check(self.L.pfg_get_tuning(self.h, ctypes.byref(t)))
+        for k, v in kw.items():
+            if not hasattr(t, k):
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        self._check(self.L.pfg_set_tuning(self.h, ctypes.byref(t)))
 
     def interrupt(self):
         """ask the running or next sigmify to stop at its next poll (thread-safe)"""

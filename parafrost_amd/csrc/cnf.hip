@@ -64,10 +64,13 @@ void upload(Ctx &c, const uint32_t *d_lits, const uint64_t *d_offs, uint64_t ncl
 
 // ------------------------------------------------------------------ histogram
 // thread per clause over live (non-deleted) clauses (histSimp, histogram.cu:34-51)
+// clause-streaming passes with per-literal atomics walk the clauses in XCD windows
+// (xcd_block): in a batch, each XCD's window then touches one stretch of instances,
+// whose counters and lists stay in its L2
 __global__ void k_hist(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
                        uint32_t *__restrict__ hist)
 {
-    for (uint64_t i = gtid(); i < n; i += gstride()) {
+    for (uint64_t i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += gstride()) {
         const uint32_t *c = mem + refs[i];
         if (c_del(c)) continue;
         const int sz = c_size(c);
@@ -81,7 +84,7 @@ void hist(Ctx &c)
     PFG_HIP(hipMemsetAsync(c.hist.p, 0, c.nlits2 * 4, c.stream));
     if (c.nrefs) {
         KTimer kt(c, K_HIST, 20.0 * c.nrefs + 4.0 * c.nliterals + 4.0 * c.nlits2);
-        k_hist<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.hist.p);
+        k_hist<<<xcd_grid(c.nrefs, 256, c.tune.clause_grid), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.hist.p);
     }
 }
 
@@ -101,7 +104,7 @@ __global__ void k_ot_init(OL *__restrict__ ot, const uint64_t *__restrict__ segs
 __global__ void k_ot_fill(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
                           OL *__restrict__ ot, uint64_t *__restrict__ occ)
 {
-    for (uint64_t i = gtid(); i < n; i += gstride()) {
+    for (uint64_t i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += gstride()) {
         const uint64_t r = refs[i];
         const uint32_t *c = mem + r;
         if (c_del(c)) continue;
@@ -367,13 +370,17 @@ static void create_ot_radix(Ctx &c)
     c.hist_live = true;
 }
 
+// Two ways to the same table (every list in clause order, histogram of live literals):
+// an LSD radix sort of (literal, reference) keys, or counting -- the histogram, list
+// starts by a scan, an atomic fill and a sort of every list by reference (the fill
+// order is not the clause order). The tuning picks one; references beyond 32 bits
+// always take the counting way.
 void create_ot(Ctx &c)
 {
-    if (c.msz < (1ull << 32) && c.msz + 3 * c.nrefs < (1ull << 32)) {
+    if (c.tune.ot_build == 0 && c.msz < (1ull << 32) && c.msz + 3 * c.nrefs < (1ull << 32)) {
         create_ot_radix(c);
         return;
     }
-    // references beyond 32 bits: histogram, atomic fill, per-list ordering
     if (!c.hist_live) hist(c);
     ensure(c, c.ot, c.nlits2);
     ensure(c, c.tmpu64a, c.nlits2);
@@ -384,7 +391,8 @@ void create_ot(Ctx &c)
     k_ot_init<<<grid_for(c.nlits2, 256, 8192), 256, 0, c.stream>>>(c.ot.p, c.tmpu64a.p, c.hist.p, c.nlits2);
     if (c.nrefs) {
         KTimer kt(c, K_OT_FILL, 20.0 * c.nrefs + 12.0 * total + 16.0 * c.nlits2);
-        k_ot_fill<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.ot.p, c.occ.p);
+        k_ot_fill<<<xcd_grid(c.nrefs, 256, c.tune.clause_grid), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs, c.ot.p,
+                                                                                    c.occ.p);
     }
     {
         KTimer kt(c, K_OT_SORT, 16.0 * total);
@@ -556,7 +564,7 @@ void count_inst(Ctx &c, std::vector<int64_t> &cls, std::vector<int64_t> &lits)
 {
     PFG_HIP(hipMemsetAsync(c.icnt.p, 0, 16ull * c.ninst, c.stream));
     if (c.nrefs) {
-        KTimer kt(c, K_COUNT, 20.0 * c.nrefs + 4.0 * c.nliterals);
+        KTimer kt(c, K_COUNT_INST, 20.0 * c.nrefs + 4.0 * c.nliterals);
         k_count_inst<<<grid_for((c.nrefs + CI_ITEMS - 1) / CI_ITEMS, CI_BLOCK, 8192), CI_BLOCK, 0, c.stream>>>(
             c.mem.p, c.refs.p, c.nrefs, c.ibase.p, c.ninst, (unsigned long long *)c.icnt.p);
     }

@@ -72,7 +72,8 @@ constexpr uint64_t CONF_CAP = 1ull << 20;
 // rest a short sequence of launches (kernel_symbols entry null)
 enum KId {
     K_UPLOAD, K_HIST, K_OT_FILL, K_OT_SORT, K_PROP, K_LCVE_RADIX, K_LCVE_INIT, K_LCVE_PASS_A, K_LCVE_PASS_B,
-    K_SORT_PV, K_SUB, K_VE_DETECT, K_VE_APPLY, K_BCE_COUNT, K_BCE_APPLY, K_ERE, K_GC, K_COUNT, K_NUM
+    K_SORT_PV, K_SUB, K_VE_DETECT, K_VE_APPLY, K_BCE_COUNT, K_BCE_APPLY, K_ERE, K_GC, K_COUNT, K_SUB_REST,
+    K_COUNT_INST, K_NUM
 };
 extern const char *const kernel_names[K_NUM];
 extern const char *const kernel_symbols[K_NUM];
@@ -84,6 +85,13 @@ enum PathId {
     PATH_SUB_WAVE,       // unit-capable variables run alone on one wave
     PATH_SUB_WAVE_BIG,   // ... of them with lists longer than the LDS arrays
     PATH_NUM
+};
+
+// launch and algorithm choices (pfg_tuning); results never depend on them
+struct Tuning {
+    int32_t ot_build = 0;          // 0 radix, 1 counting
+    uint32_t clause_grid = 2048;   // blocks of 256 threads for the XCD-windowed clause passes
+    uint32_t sub_group = 16;       // lanes per variable in the first subsumption configuration
 };
 
 struct KStat {
@@ -182,6 +190,7 @@ struct Ctx {
 
     pfg_times times{};
     uint64_t path[PATH_NUM] = {};
+    Tuning tune;
     KStat kstat[K_NUM];
     std::vector<std::array<hipEvent_t, 2>> kev;     // event pairs of pending kernel timers
     std::vector<std::pair<int, double>> kpend;      // (kernel, algorithmic bytes) per pending pair

@@ -46,6 +46,27 @@ __device__ __forceinline__ bool sig_selfsub(uint32_t A, uint32_t B)
 __device__ __forceinline__ uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
 
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round robin. For a
+// grid-stride loop over items in memory order, xcd_block() renumbers the blocks so
+// that every XCD takes a contiguous share of each stride step: XCD x gets logical
+// blocks [x, x + 1) * grid / 8. With a grid of resident size the XCDs' windows move
+// through the items together, so that data indexed by nearby items (one instance's
+// variables in a batch) stays in the XCD's L2. The grid must be a multiple of 8
+// (xcd_grid).
+constexpr unsigned NUM_XCD = 8;
+__device__ __forceinline__ uint64_t xcd_block()
+{
+    return (uint64_t)(blockIdx.x % NUM_XCD) * (gridDim.x / NUM_XCD) + blockIdx.x / NUM_XCD;
+}
+
+inline unsigned xcd_grid(uint64_t n, unsigned block, unsigned maxblocks)
+{
+    uint64_t g = (n + block - 1) / block;
+    if (g > maxblocks) g = maxblocks;
+    g = (g + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    return (unsigned)(g ? g : NUM_XCD);
+}
+
 inline unsigned grid_for(uint64_t n, unsigned block, unsigned maxblocks = 8192)
 {
     uint64_t g = (n + block - 1) / block;

@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t 
 {
     __shared__ uint32_t s_cr[ELECT_CACHE][256];
     const unsigned tid = threadIdx.x;
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
+    for (uint64_t b = xcd_block() * blockDim.x; b < n; b += gstride()) {
         const uint64_t i = b + tid;
         const uint32_t *c = nullptr;
         int sz = 0;
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32
                                                       uint64_t *__restrict__ next, unsigned long long *nnext)
 {
     const uint64_t n = *nactive;
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
+    for (uint64_t b = xcd_block() * blockDim.x; b < n; b += gstride()) {
         const uint64_t i = b + threadIdx.x;
         bool keep = false;
         uint64_t r = 0;
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(1024) k_elect_pass_b(ElectArgs A, const uint32
                                                       uint32_t *__restrict__ next, unsigned long long *nnext)
 {
     const uint64_t n = *nwork;
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += gstride()) {
+    for (uint64_t b = xcd_block() * blockDim.x; b < n; b += gstride()) {
         const uint64_t q = b + threadIdx.x;
         bool keep = false;
         uint32_t j = 0;
@@ -419,7 +419,7 @@ uint32_t lcve(Ctx &c)
     if (c.nrefs) {
         KTimer kt(c, K_LCVE_INIT, 20.0 * c.nrefs + 8.0 * c.nliterals);
         PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_D, 0, 8, c.stream));
-        k_elect_init<<<grid_for(c.nrefs, 256, 16384), 256, 0, c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act,
+        k_elect_init<<<xcd_grid(c.nrefs, 256, c.tune.clause_grid), 256, 0, c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act,
                                                                           ra + 1, (unsigned long long *)c.cnt.p);
     }
     uint64_t hw[2] = {0, 0};
@@ -437,12 +437,14 @@ uint32_t lcve(Ctx &c)
         for (int k = 0; k < batch && r < MAXR - 1; k++, r++) {
             if (r > 0) {
                 KTimer kt(c, K_LCVE_PASS_A, elect_bytes(c, act, ra + r, nact));
-                k_elect_pass_a<<<grid_for(nact, 1024, 8192), 1024, 0, c.stream>>>(A, c.erec.p, act, ra + r, act2, ra + r + 1);
+                k_elect_pass_a<<<xcd_grid(nact, 1024, std::max(8u, c.tune.clause_grid / 4)), 1024, 0, c.stream>>>(A, c.erec.p, act, ra + r, act2,
+                                                                                           ra + r + 1);
                 std::swap(act, act2);
             }
             {
                 KTimer kt(c, K_LCVE_PASS_B, 0.0);
-                k_elect_pass_b<<<grid_for(nwork, 1024, 8192), 1024, 0, c.stream>>>(A, work, rw + r, work2, rw + r + 1);
+                k_elect_pass_b<<<xcd_grid(nwork, 1024, std::max(8u, c.tune.clause_grid / 4)), 1024, 0, c.stream>>>(A, work, rw + r, work2,
+                                                                                            rw + r + 1);
             }
             std::swap(work, work2);
         }

@@ -279,11 +279,218 @@ __device__ __forceinline__ void self_sub_x_staged(const SubArgs &A, uint32_t p, 
     }
 }
 
+// ---- a lane group per elected variable (the parallel ranges: no unit can arise)
+// The 64 lanes of a wave form 64 / G groups; a group takes one elected variable at a
+// time. The variable's two lists and the images of their clauses are copied into the
+// group's LDS region cooperatively (each lane takes list entries: reference, size,
+// image), then self_sub_x (subsume.h:140-176) runs with its candidate scans split
+// across the group, G candidates per step: a clause is strengthened if some
+// strengthening clause lies before the first longer one, and deleted by the first
+// earlier clause that subsumes it (ballots pick what the sequential loop picks).
+// Changes are made in LDS and written through to the pool by the lane that makes
+// them; the compacted lists go back at the end of each side. Groups of one wave
+// exchange nothing, and a group's lanes communicate through LDS in program order
+// (one wave), so no workgroup barrier is used. Variables whose lists or images do
+// not fit keep rest[q] = 1 for the next, larger configuration.
+__device__ __forceinline__ void wave_order() { __builtin_amdgcn_wave_barrier(); }
+
+template <int G>
+__device__ __forceinline__ uint32_t group_incl_scan(uint32_t x)
+{
+    const uint32_t gl = __lane_id() % G;
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        const uint32_t t = __shfl_up(x, o, G);
+        if (gl >= (uint32_t)o) x += t;
+    }
+    return x;
+}
+
+// the staged image at c (LDS) written back to its pool copy
+__device__ __forceinline__ void sv_write_image(const SubArgs &A, const uint32_t *c)
+{
+    uint32_t *g = A.mem + staged_ref(c);
+    const int n = c_size(c);
+    for (int t = 0; t < 3 + n; t++) g[t] = c[t];
+}
+
+// first: the first configuration (every variable of the range); otherwise only the
+// variables with rest[q] set
+template <int G, uint32_t ENT, uint32_t WORDS>
+__global__ void __launch_bounds__(64) k_sub_grp(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e,
+                                                uint32_t words_cap, uint32_t *__restrict__ rest, int first)
+{
+    constexpr uint32_t NG = 64 / G;
+    __shared__ uint32_t img_all[NG][WORDS];
+    __shared__ uint16_t pos_all[NG][2][ENT];
+    __shared__ uint32_t seq_all[NG];
+    const uint32_t lane = __lane_id(), g = lane / G, gl = lane % G;
+    uint32_t *img = img_all[g];
+    uint16_t(*pos)[ENT] = pos_all[g];
+    uint32_t &s_seq = seq_all[g];
+    const uint32_t cap = min(words_cap, WORDS);
+    const uint32_t leader = g * G;
+    auto gballot = [&](bool x) -> uint64_t {
+        const uint64_t m = __ballot(x);
+        return G == 64 ? m : (m >> leader) & ((1ull << G) - 1ull);
+    };
+    const uint64_t n = e - b;
+    for (uint64_t it = (uint64_t)blockIdx.x * NG + g; it < n; it += (uint64_t)gridDim.x * NG) {
+        const uint32_t q = (uint32_t)(e - 1 - it);   // elected variables come in increasing score order: heaviest first
+        if (!first && !rest[q]) continue;
+        const uint32_t v = pvs[q], p = 2 * v;
+        const OL lp = A.ot[p], ln = A.ot[p + 1];
+        if (lp.sz > (uint32_t)A.sub_limit || ln.sz > (uint32_t)A.sub_limit) {   // not processed (subsume.h:145)
+            if (gl == 0) rest[q] = 0;
+            continue;
+        }
+        bool fits = lp.sz <= ENT && ln.sz <= ENT;
+        const uint32_t ne = lp.sz + ln.sz;
+        uint32_t total = 0;
+        for (uint32_t k0 = 0; fits && k0 < ne; k0 += G) {
+            const uint32_t k = k0 + gl;
+            uint64_t r = 0;
+            uint32_t w = 0;
+            if (k < ne) {
+                r = k < lp.sz ? A.occ[lp.off + k] : A.occ[ln.off + (k - lp.sz)];
+                w = 5u + min(A.mem[r + 2], cap);   // a clause larger than the cap does not fit anyway
+            }
+            const uint32_t incl = group_incl_scan<G>(w);
+            const uint32_t at = total + incl - w;
+            if (k < ne && at + w <= cap) {
+                const uint32_t *gsrc = A.mem + r;
+                img[at] = (uint32_t)r;
+                img[at + 1] = (uint32_t)(r >> 32);
+                for (uint32_t t = 0; t + 2 < w; t++) img[at + 2 + t] = gsrc[t];
+                if (k < lp.sz) pos[0][k] = (uint16_t)(at + 2);
+                else pos[1][k - lp.sz] = (uint16_t)(at + 2);
+            }
+            total += __shfl(incl, G - 1, G);
+            fits = total <= cap;
+        }
+        if (gl == 0) rest[q] = fits ? 0u : 1u;
+        if (!fits) continue;
+        if (gl == 0) s_seq = 0;
+        wave_order();
+        uint32_t len[2] = {lp.sz, ln.sz};
+        for (int side = 0; side < 2; side++) {
+            const uint32_t me = p ^ (uint32_t)side, other = me ^ 1u;
+            uint16_t *mp = pos[side];
+            const uint16_t *op = pos[1 - side];
+            const uint32_t ms = len[side], os = len[1 - side];
+            for (uint32_t i = 0; i < ms; i++) {
+                uint32_t *cand = img + mp[i];
+                const uint32_t csz = cand[2];
+                if (csz > (uint32_t)HSE_MAX_CL_SIZE) break;
+                if (c_del(cand)) continue;
+                {   // selfsubsume (subsume.h:119-138): any strengthening clause before the first longer one
+                    const uint32_t candsig = cand[1];
+                    bool found = false;
+                    for (uint32_t jb = 0; jb < os; jb += G) {
+                        const uint32_t j = jb + gl;
+                        bool brk = false, hit = false;
+                        if (j < os) {
+                            const uint32_t *sg = img + op[j];
+                            const uint32_t subsize = sg[2];
+                            if (subsize > csz) brk = true;
+                            else if (!c_del(sg) && subsize > 1 && sig_selfsub(sg[1], candsig))
+                                hit = selfsub_cl(me, other, sg, cand);
+                        }
+                        const uint64_t mb = gballot(brk), mh = gballot(hit);
+                        const uint64_t before = mb ? ((1ull << (__ffsll((long long)mb) - 1)) - 1ull) : ~0ull;
+                        if (mh & before) { found = true; break; }
+                        if (mb) break;
+                    }
+                    if (found) {
+                        if (gl == 0) {
+                            uint32_t seq = s_seq;
+                            strengthen(A, cand, me, staged_ref(cand), q, seq);
+                            s_seq = seq;
+                            c_melt(cand);
+                            sv_write_image(A, cand);
+                        }
+                        wave_order();
+                    }
+                }
+                {   // subsume (subsume.h:101-117): the first earlier clause that subsumes it
+                    const uint32_t csz2 = cand[2], candsig = cand[1];
+                    const bool molten = c_molten(cand);
+                    for (uint32_t jb = 0; jb < i; jb += G) {
+                        const uint32_t j = jb + gl;
+                        bool hit = false;
+                        if (j < i) {
+                            const uint32_t *sg = img + mp[j];
+                            const uint32_t sz = sg[2];
+                            if (!c_del(sg) && !(molten && sz > csz2) && sz > 1 && sig_sub(sg[1], candsig))
+                                hit = sub_cl(sg, cand);
+                        }
+                        const uint64_t mh = gballot(hit);
+                        if (mh) {
+                            if (gl == (uint32_t)(__ffsll((long long)mh) - 1)) {
+                                uint32_t *sg = img + mp[j];
+                                if (c_learnt(sg) && c_orig(cand)) {
+                                    c_setst(sg, ST_ORIGINAL);
+                                    A.mem[staged_ref(sg)] = sg[0];
+                                }
+                                if (!c_del(cand)) {   // removeClause (sclause.cpp:77-84)
+                                    c_markdel(cand);
+                                    A.mem[staged_ref(cand)] = cand[0];
+                                    uint32_t seq = s_seq;
+                                    plog(A.P, q, seq, PE_DEL, staged_ref(cand));
+                                    s_seq = seq;
+                                }
+                            }
+                            break;
+                        }
+                    }
+                    wave_order();
+                }
+            }
+            // updateOL (subsume.h:27-37): molten clauses are frozen and leave the list,
+            // deleted ones leave it, the rest keep their order. Each step reads its G
+            // entries before it writes positions below them.
+            uint32_t kept = 0;
+            for (uint32_t jb = 0; jb < ms; jb += G) {
+                const uint32_t j = jb + gl;
+                uint16_t at = 0;
+                bool keep = false;
+                if (j < ms) {
+                    at = mp[j];
+                    uint32_t *c = img + at;
+                    if (c_molten(c)) {
+                        c_freeze(c);
+                        A.mem[staged_ref(c)] = c[0];
+                    }
+                    else keep = !c_del(c);
+                }
+                const uint64_t mk = gballot(keep);
+                const uint32_t rank = (uint32_t)__popcll(mk & ((1ull << gl) - 1ull));
+                wave_order();
+                if (keep) mp[kept + rank] = at;
+                kept += (uint32_t)__popcll(mk);
+                wave_order();
+            }
+            if (kept != ms) {   // nothing left the list: its order is unchanged
+                OL *o = A.ot + me;
+                uint64_t *d = A.occ + o->off;
+                for (uint32_t j = gl; j < kept; j += G) d[j] = staged_ref(img + mp[j]);
+                if (gl == 0) o->sz = kept;
+            }
+            len[side] = kept;
+            wave_order();
+        }
+        if (gl == 0) plog_end(A.P, q, s_seq);
+        wave_order();
+    }
+}
+
 constexpr uint32_t SUB_ARENA_WORDS = 8192;   // LDS words per 64-lane block (32 KB)
 
-// parallel range (no unit can arise): one 64-lane block per 64 elected variables
+// parallel range (no unit can arise): one 64-lane block per 64 elected variables, a lane
+// per variable; only the variables k_sub_grp left (rest[q] != 0)
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_sub_staged(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e, uint32_t arena_words)
+k_sub_staged(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e, uint32_t arena_words,
+             const uint32_t *__restrict__ rest)
 {
     __shared__ uint32_t arena[SUB_ARENA_WORDS];
     const StageSrc S{A.mem, A.ot, A.occ};
@@ -293,7 +500,8 @@ k_sub_staged(SubArgs A, const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e
         const uint64_t q = b + (nblk - 1 - it) * 64 + threadIdx.x;
         const bool active = q < e;
         const uint32_t v = active ? pvs[q] : 0u;
-        const bool take = active && A.ot[2 * v].sz <= (uint32_t)A.sub_limit && A.ot[2 * v + 1].sz <= (uint32_t)A.sub_limit;
+        const bool take = active && rest[q] && A.ot[2 * v].sz <= (uint32_t)A.sub_limit &&
+                          A.ot[2 * v + 1].sz <= (uint32_t)A.sub_limit;
         stage_rounds(S, arena, arena_words, take, v, STAGE_TAB2, [&](uint32_t *tab) {
             uint32_t seq = 0;
             if (tab) self_sub_x_staged(A, 2 * v, tab, arena, (uint32_t)q, seq);
@@ -605,11 +813,27 @@ void sub(Ctx &c)
         PFG_HIP(hipStreamSynchronize(c.stream));
     }
     uint32_t b = 0;
+    ensure(c, c.tmpu32c, np);
+    uint32_t *rest = c.tmpu32c.p;
     auto run_range = [&](uint32_t e) {
         if (e > b) {
-            KTimer kt(c, K_SUB, pv_bytes(c, b, e, true));
+            const uint32_t cap = c.stage_words;
+            {   // G lanes per variable (tuning) with 4 G list entries and 32 G staged words per group
+                KTimer kt(c, K_SUB, pv_bytes(c, b, e, true));
+                const uint32_t G = c.tune.sub_group;
+                const uint32_t grid = (uint32_t)std::min<uint64_t>((e - b + 64 / G - 1) / (64 / G), 65536);
+                if (G == 8) k_sub_grp<8, 32, 256><<<grid, 64, 0, c.stream>>>(A, c.pvs.p, b, e, cap, rest, 1);
+                else if (G == 32) k_sub_grp<32, 128, 1024><<<grid, 64, 0, c.stream>>>(A, c.pvs.p, b, e, cap, rest, 1);
+                else if (G == 64) k_sub_grp<64, 256, 2048><<<grid, 64, 0, c.stream>>>(A, c.pvs.p, b, e, cap, rest, 1);
+                else k_sub_grp<16, 64, 512><<<grid, 64, 0, c.stream>>>(A, c.pvs.p, b, e, cap, rest, 1);
+            }
+            KTimer kt(c, K_SUB_REST, 0.0);
+            // a wave per variable for lists of up to 256 entries and 2048 words, then a lane
+            // per variable (larger regions, or from the pool) for what is left
+            k_sub_grp<64, 256, 2048><<<(uint32_t)std::min<uint64_t>(e - b, 16384), 64, 0, c.stream>>>(A, c.pvs.p, b, e,
+                                                                                                       cap, rest, 0);
             k_sub_staged<<<grid_for(e - b, 64, 16384), 64, 0, c.stream>>>(A, c.pvs.p, b, e,
-                                                                           std::min(SUB_ARENA_WORDS, c.stage_words));
+                                                                           std::min(SUB_ARENA_WORDS, cap), rest);
         }
     };
     for (uint64_t k = 0; k < nu; k++) {

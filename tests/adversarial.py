@@ -94,7 +94,7 @@ def sub_long_unit_capable(nlong: int = 3500, seed: int = 0):
     cls = [[2 * x, 2 * a], [2 * x + 1, 2 * a]]
     for s in (0, 1):
         for i in range(nlong):
-            o = rng.choice(np.arange(3, nv + 1), int(rng.integers(1, 4)), replace=False)
+            o = rng.choice(np.arange(3, nv + 1), int(rng.integers(2, 4)), replace=False)   # no other binaries
             cls.append([2 * x + s] + [int(2 * v + rng.integers(0, 2)) for v in o])
             if i % 50 == 0:   # a duplicate and a superset of the clause before
                 cls.append(list(cls[-1]))

@@ -67,3 +67,11 @@ def test_new_entry_points_reject_a_null_context():
     n = ctypes.c_int()
     assert L.pfg_path_counts(None, None, 0, ctypes.byref(n)) == pf.PFG_EINVAL
     assert pf.PFG_EINTR == 6 and pf.PATH_NAMES[0] == "ere_extra_rounds"
+
+
+def test_tuning_entry_points_reject_a_null_context():
+    import parafrost_amd as pf
+    L = pf.lib()
+    t = pf.Tuning()
+    assert L.pfg_set_tuning(None, ctypes.byref(t)) == pf.PFG_EINVAL
+    assert L.pfg_get_tuning(None, ctypes.byref(t)) == pf.PFG_EINVAL

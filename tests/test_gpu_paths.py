@@ -40,7 +40,7 @@ def test_ere_shared_targets_match_the_sequential_loop():
         st = po.Oracle(cnf, po.default_opts(phases=0, ve_en=0, sub_en=0))
         st.sigmify()
         live = [c for c in st.state().clauses() if not c[0] & 2]
-        assert len(live) == len(cnf.clauses()) - 4 * copies   # both copies of T and of D are gone
+        assert len(live) == len(list(cnf.clauses())) - 4 * copies   # both copies of T and of D are gone
 
 
 @pytest.mark.parametrize("seed", range(4))
