@@ -93,6 +93,15 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
  * sigmify on the context; NULL buffers query the sizes. The clauses can be given back to
  * pfg_upload_cnf_device / pfg_upload_batch together with new clauses. */
 int pfg_batch_export(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint64_t *d_offs, uint64_t *nlits, uint64_t *ncls);
+/* No counterpart in the reference: pfg_batch_export for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops). Same arguments and
+ * the same output (clause order, literals, offsets), but it allocates nothing: it walks
+ * the clause pool directly and keeps its per-chunk counts in a fixed device array of
+ * 128 KB per GPU, held until the process ends. Limit: a pool of at most 2^33 clauses
+ * (PFG_EINVAL beyond). Calls that overlap on one GPU share that array and race; run
+ * them one after another. */
+int pfg_batch_export_noalloc(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint64_t *d_offs, uint64_t *nlits,
+                             uint64_t *ncls);
 
 /* Launch and algorithm choices of the kernels (no counterpart in the reference, whose
  * OPTIMIZEBLOCKS* macros halve block sizes by a fixed rule, src/gpu/grid.cuh:54-85).

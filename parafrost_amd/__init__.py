@@ -62,7 +62,8 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_times_get", "pfg_is_unsat", "pfg_last_error", "pfg_kernel_stats", "pfg_reset_stats", "pfg_reset",
        "pfg_set_accounting", "pfg_set_stage_words", "pfg_create_batch", "pfg_batch_ninst", "pfg_batch_relayout", "pfg_upload_batch",
        "pfg_sigmify_batch", "pfg_batch_download", "pfg_ifreeze", "pfg_batch_export", "pfg_download_mapped",
-       "pfg_download_proof", "pfg_interrupt", "pfg_set_vorg", "pfg_path_counts", "pfg_set_tuning", "pfg_get_tuning"]
+       "pfg_download_proof", "pfg_interrupt", "pfg_set_vorg", "pfg_path_counts", "pfg_set_tuning", "pfg_get_tuning",
+       "pfg_batch_export_noalloc"]
 
 
 def lib_path() -> str:
@@ -120,6 +121,7 @@ def lib():
         L.pfg_batch_download.argtypes = [vp, u32, vp, ctypes.POINTER(u64), ctypes.POINTER(u64), vp,
                                          ctypes.POINTER(u64), vp, vp, ctypes.POINTER(u32)]
         L.pfg_batch_export.argtypes = [vp, u32, vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.pfg_batch_export_noalloc.argtypes = [vp, u32, vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.pfg_reset.argtypes = [vp]
         L.pfg_interrupt.argtypes = [vp]
         L.pfg_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]

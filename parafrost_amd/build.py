@@ -16,7 +16,8 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PFG_ARCH", "gfx950")
 
-HIP_SOURCES = ["prims.hip", "cnf.hip", "lcve.hip", "sub.hip", "ve.hip", "ere.hip", "proof.hip", "sigma.cpp"]
+HIP_SOURCES = ["prims.hip", "cnf.hip", "lcve.hip", "sub.hip", "ve.hip", "ere.hip", "proof.hip", "noalloc.hip",
+               "sigma.cpp"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-parameter",
              "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

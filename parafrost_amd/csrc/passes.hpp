@@ -58,4 +58,7 @@ void proof_end(Ctx &c, uint32_t np, bool decide);
 void proof_units(Ctx &c, uint32_t from, uint32_t to);
 void proof_empty(Ctx &c);                  // the empty clause; nothing is written after it
 
+// pfg_batch_export without allocation (noalloc.hip)
+void batch_export_noalloc(Ctx &c, uint32_t inst, uint32_t *d_lits, uint64_t *d_offs, uint64_t *nlits, uint64_t *ncls);
+
 }  // namespace pfg

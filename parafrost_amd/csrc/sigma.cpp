@@ -805,6 +805,14 @@ int pfg_batch_export(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint64_t *d_
     API_END
 }
 
+int pfg_batch_export_noalloc(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint64_t *d_offs, uint64_t *nlits,
+                             uint64_t *ncls)
+{
+    API_BEGIN
+    batch_export_noalloc(c, inst, d_lits, d_offs, nlits, ncls);
+    API_END
+}
+
 int pfg_download_mapped(pfg_ctx *ctx, uint32_t *words, uint64_t *nwords, uint64_t *ncls, uint32_t *vorg,
                         uint32_t *nvars_new, uint32_t *trail, uint32_t *ntrail, uint8_t *vstate)
 {
