@@ -114,6 +114,8 @@ typedef struct pfg_tuning {
                               rounded up to a multiple of 8 (one window per XCD) */
     uint32_t sub_group;    /* lanes per elected variable in subsumption: 8, 16, 32 or 64 */
     uint32_t stage_words;  /* LDS words per block of the staged kernels (pfg_set_stage_words) */
+    int32_t lcve_graph;    /* 1: the LCVE election rounds are captured once per call as a HIP graph
+                              (four rounds, round number read on the device) and replayed */
 } pfg_tuning;
 int pfg_set_tuning(pfg_ctx *ctx, const pfg_tuning *t);
 int pfg_get_tuning(pfg_ctx *ctx, pfg_tuning *t);
@@ -220,7 +222,8 @@ int pfg_set_vorg(pfg_ctx *ctx, const uint32_t *vorg, uint32_t n);
  * counterpart in the reference): [0] extra ERE claiming rounds (run when a target was
  * claimed by two elected variables), [1] unit-capable SUB variables run alone on one
  * wave, [2] of those, the ones whose lists exceed the wave's LDS arrays (sub_limit
- * above 3000). *n = the number of counters; at most max are written. */
+ * above 3000), [3] LCVE calls whose election rounds ran from a captured graph.
+ * *n = the number of counters; at most max are written. */
 int pfg_path_counts(pfg_ctx *ctx, uint64_t *counts, int max, int *n);
 
 /* per-kernel device time (HIP events around each launch) and algorithmic HBM bytes.

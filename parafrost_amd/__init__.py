@@ -19,7 +19,7 @@ _LIB = None
 
 PFG_OK, PFG_EINVAL, PFG_ENOMEM, PFG_EDEVICE, PFG_EUNSAT, PFG_EINTERNAL, PFG_EINTR = range(7)
 _ERRNAMES = {1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EUNSAT", 5: "EINTERNAL", 6: "EINTR"}
-PATH_NAMES = ("ere_extra_rounds", "sub_wave_vars", "sub_wave_big_vars")
+PATH_NAMES = ("ere_extra_rounds", "sub_wave_vars", "sub_wave_big_vars", "lcve_graph_calls")
 
 
 class Opts(ctypes.Structure):
@@ -43,7 +43,7 @@ class Times(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """launch and algorithm choices (pfg_tuning); results never depend on them"""
     _fields_ = [("ot_build", ctypes.c_int32), ("clause_grid", ctypes.c_uint32), ("sub_group", ctypes.c_uint32),
-                ("stage_words", ctypes.c_uint32)]
+                ("stage_words", ctypes.c_uint32), ("lcve_graph", ctypes.c_int32)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

@@ -64,7 +64,7 @@ struct DBuf {
 // the variables involved follow it (at most CONF_CAP), so that a batch can tell
 // which instances were refuted.
 enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_SUBFROM = 5, CNT_NTRAIL = 8, CNT_FLAGS = 9, CNT_ACCT = 10,
-       CNT_CONF = 16 };
+       CNT_ROUND = 11, CNT_CONF = 16 };
 enum { FLAG_UNSAT = 1, FLAG_INTERNAL = 2 };
 constexpr uint64_t CONF_CAP = 1ull << 20;
 
@@ -73,7 +73,7 @@ constexpr uint64_t CONF_CAP = 1ull << 20;
 enum KId {
     K_UPLOAD, K_HIST, K_OT_FILL, K_OT_SORT, K_PROP, K_LCVE_RADIX, K_LCVE_INIT, K_LCVE_PASS_A, K_LCVE_PASS_B,
     K_SORT_PV, K_SUB, K_VE_DETECT, K_VE_APPLY, K_BCE_COUNT, K_BCE_APPLY, K_ERE, K_GC, K_COUNT, K_SUB_REST,
-    K_COUNT_INST, K_NUM
+    K_COUNT_INST, K_LCVE_GRAPH, K_NUM
 };
 extern const char *const kernel_names[K_NUM];
 extern const char *const kernel_symbols[K_NUM];
@@ -84,6 +84,7 @@ enum PathId {
     PATH_ERE_ROUNDS,     // extra ERE claiming rounds (a target claimed by two elected variables)
     PATH_SUB_WAVE,       // unit-capable variables run alone on one wave
     PATH_SUB_WAVE_BIG,   // ... of them with lists longer than the LDS arrays
+    PATH_LCVE_GRAPH,     // LCVE calls whose election rounds ran as a captured graph
     PATH_NUM
 };
 
@@ -92,6 +93,7 @@ struct Tuning {
     int32_t ot_build = 0;          // 0 radix, 1 counting
     uint32_t clause_grid = 2048;   // blocks of 256 threads for the XCD-windowed clause passes
     uint32_t sub_group = 16;       // lanes per variable in the first subsumption configuration
+    int32_t lcve_graph = 0;        // LCVE election rounds replayed from a captured graph
 };
 
 struct KStat {

@@ -175,9 +175,13 @@ __global__ void k_ere_pairs(EreArgs A, const uint32_t *__restrict__ pvs, uint32_
     }
 }
 
+// acct (accounting only): adds the bytes of the resolvent checks -- per candidate
+// examined its list entry (8) and header (12), per candidate whose size, status and
+// signature pass the literals compared (4 per literal of the resolvent)
 __global__ void k_ere_probe(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs, const uint64_t *__restrict__ poff,
-                            uint64_t total, uint32_t *__restrict__ flag)
+                            uint64_t total, uint32_t *__restrict__ flag, unsigned long long *acct)
 {
+    unsigned long long bytes = 0;
     for (uint64_t g = gtid(); g < total; g += gstride()) {
         uint32_t lo = 0, hi = npvs;   // last q with poff[q] <= g
         while (hi - lo > 1) {
@@ -211,14 +215,23 @@ __global__ void k_ere_probe(EreArgs A, const uint32_t *__restrict__ pvs, uint32_
             for (int u = 0; u < 4; u++) {
                 const bool head = (int)z[u] == R.len && ((h[u] & ST_LEARNT) || (h[u] & 3u) == R.type) &&
                                   sig_sub(R.msig, g[u]);
+                if (acct) bytes += 20ull + (!hit && head ? 4ull * (uint64_t)R.len : 0ull);
                 if (!hit && head && merge_ere(x, ci, cj, cc[u])) hit = true;
             }
         }
-        for (; k < R.minsize && !hit; k++) hit = ere_match(A, x, ci, cj, R, A.mem + A.occ[off + k]);
+        for (; k < R.minsize && !hit; k++) {
+            const uint32_t *t = A.mem + A.occ[off + k];
+            hit = ere_match(A, x, ci, cj, R, t);
+            if (acct) bytes += 20ull + (hit ? 4ull * (uint64_t)R.len : 0ull);
+        }
         if (hit) {
             flag[q] = 1u;
             atomicAdd(&A.cnt[CNT_C], 1ull);
         }
+    }
+    if (acct) {
+        bytes = block_sum(bytes);
+        if (threadIdx.x == 0 && bytes) atomicAdd(acct, bytes);
     }
 }
 
@@ -281,8 +294,18 @@ void ere(Ctx &c)
     PFG_HIP(hipMemcpyAsync(&total, poff + np, 8, hipMemcpyDeviceToHost, c.stream));
     PFG_HIP(hipStreamSynchronize(c.stream));
     if (total) {
-        KTimer kt(c, K_ERE, pv_bytes(c, 0, np, true));
-        k_ere_probe<<<grid_for(total, 256, 65535), 256, 0, c.stream>>>(A, c.pvs.p, np, poff, total, flag);
+        double check_bytes = 0;
+        if (c.account) {   // an untimed run of the probe adds up the bytes of the resolvent checks
+            PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_ACCT, 0, 8, c.stream));
+            k_ere_probe<<<grid_for(total, 256, 65535), 256, 0, c.stream>>>(A, c.pvs.p, np, poff, total, flag,
+                                                                        (unsigned long long *)c.cnt.p + CNT_ACCT);
+            read_counters(c, CNT_ACCT + 1);
+            check_bytes = (double)c.hcnt[CNT_ACCT];
+            PFG_HIP(hipMemsetAsync(flag, 0, 4ull * np, c.stream));
+            PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_C, 0, 8, c.stream));
+        }
+        KTimer kt(c, K_ERE, pv_bytes(c, 0, np, true) + check_bytes);
+        k_ere_probe<<<grid_for(total, 256, 65535), 256, 0, c.stream>>>(A, c.pvs.p, np, poff, total, flag, nullptr);
     }
     read_counters(c, CNT_C + 1);
     const uint64_t hits = c.hcnt[CNT_C];   // matching pairs: at most one claim each per round

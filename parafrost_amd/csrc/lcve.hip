@@ -195,10 +195,9 @@ __global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t 
 
 // pass A over the active clause records (count read on the device: rounds are
 // launched back to back and the host looks at the counts every few rounds)
-__global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32_t *__restrict__ rec,
-                                                      const uint64_t *__restrict__ active,
-                                                      const unsigned long long *__restrict__ nactive,
-                                                      uint64_t *__restrict__ next, unsigned long long *nnext)
+__device__ __forceinline__ void elect_pass_a(const ElectArgs &A, const uint32_t *__restrict__ rec,
+                                             const uint64_t *__restrict__ active, const unsigned long long *nactive,
+                                             uint64_t *__restrict__ next, unsigned long long *nnext)
 {
     const uint64_t n = *nactive;
     for (uint64_t b = xcd_block() * blockDim.x; b < n; b += gstride()) {
@@ -265,10 +264,18 @@ __global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32
     }
 }
 
+__global__ void __launch_bounds__(1024) k_elect_pass_a(ElectArgs A, const uint32_t *__restrict__ rec,
+                                                      const uint64_t *__restrict__ active,
+                                                      const unsigned long long *__restrict__ nactive,
+                                                      uint64_t *__restrict__ next, unsigned long long *nnext)
+{
+    elect_pass_a(A, rec, active, nactive, next, nnext);
+}
+
 // pass B over the undecided candidates
-__global__ void __launch_bounds__(1024) k_elect_pass_b(ElectArgs A, const uint32_t *__restrict__ work,
-                                                      const unsigned long long *__restrict__ nwork,
-                                                      uint32_t *__restrict__ next, unsigned long long *nnext)
+__device__ __forceinline__ void elect_pass_b(const ElectArgs &A, const uint32_t *__restrict__ work,
+                                             const unsigned long long *nwork, uint32_t *__restrict__ next,
+                                             unsigned long long *nnext)
 {
     const uint64_t n = *nwork;
     for (uint64_t b = xcd_block() * blockDim.x; b < n; b += gstride()) {
@@ -284,6 +291,45 @@ __global__ void __launch_bounds__(1024) k_elect_pass_b(ElectArgs A, const uint32
         }
         block_append(keep, j, next, nnext);
     }
+}
+
+__global__ void __launch_bounds__(1024) k_elect_pass_b(ElectArgs A, const uint32_t *__restrict__ work,
+                                                      const unsigned long long *__restrict__ nwork,
+                                                      uint32_t *__restrict__ next, unsigned long long *nnext)
+{
+    elect_pass_b(A, work, nwork, next, nnext);
+}
+
+// The election rounds as a captured graph (tuning lcve_graph): the same passes, with
+// the round number r read from the device (*round), so one captured sequence of rounds
+// replays until no candidate is undecided. Round r's pass A reads the active records
+// of buffer (r odd ? act0 : act1) and writes the other; pass B reads the work list of
+// (r even ? work0 : work1). Rounds past MAXR - 1 do nothing (the host reports it).
+struct ElectRounds {
+    const uint32_t *rec;
+    uint64_t *act0, *act1;
+    uint32_t *work0, *work1;
+    unsigned long long *rw, *ra, *round;
+    uint32_t maxr;
+};
+
+__global__ void __launch_bounds__(1024) k_elect_pass_a_g(ElectArgs A, ElectRounds R)
+{
+    const uint32_t r = (uint32_t)*R.round;
+    if (r == 0 || r + 1 >= R.maxr) return;
+    elect_pass_a(A, R.rec, (r & 1u) ? R.act0 : R.act1, R.ra + r, (r & 1u) ? R.act1 : R.act0, R.ra + r + 1);
+}
+
+__global__ void __launch_bounds__(1024) k_elect_pass_b_g(ElectArgs A, ElectRounds R)
+{
+    const uint32_t r = (uint32_t)*R.round;
+    if (r + 1 >= R.maxr) return;
+    elect_pass_b(A, (r & 1u) ? R.work1 : R.work0, R.rw + r, (r & 1u) ? R.work0 : R.work1, R.rw + r + 1);
+}
+
+__global__ void k_elect_round_next(ElectRounds R)
+{
+    if (threadIdx.x == 0 && *R.round + 1 < R.maxr) *R.round += 1;
 }
 
 __global__ void k_dynbreak(const uint32_t *__restrict__ elig, uint32_t nvars, const uint8_t *__restrict__ kind,
@@ -432,6 +478,59 @@ uint32_t lcve(Ctx &c)
     // round 0's pass A ran inside k_elect_init; rounds are launched in batches and
     // the count of undecided candidates is checked after each batch
     int r = 0;
+    if (c.tune.lcve_graph && nwork && !c.account) {
+        // round 0 (pass B only) on the stream, then captured batches of GRAPH_ROUNDS rounds
+        {
+            KTimer kt(c, K_LCVE_PASS_B, 0.0);
+            k_elect_pass_b<<<xcd_grid(nwork, 1024, std::max(8u, c.tune.clause_grid / 4)), 1024, 0, c.stream>>>(
+                A, work, rw, work2, rw + 1);
+        }
+        r = 1;
+        constexpr int GRAPH_ROUNDS = 4;
+        const unsigned long long one = 1;
+        unsigned long long *round = (unsigned long long *)c.cnt.p + CNT_ROUND;
+        PFG_HIP(hipMemcpyAsync(round, &one, 8, hipMemcpyHostToDevice, c.stream));
+        const ElectRounds R{c.erec.p, act, act2, work, work2, rw, ra, round, (uint32_t)MAXR};
+        const unsigned ga = xcd_grid(nact, 1024, std::max(8u, c.tune.clause_grid / 4));
+        const unsigned gb = xcd_grid(nwork, 1024, std::max(8u, c.tune.clause_grid / 4));
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        PFG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < GRAPH_ROUNDS; k++) {
+            k_elect_pass_a_g<<<ga, 1024, 0, c.stream>>>(A, R);
+            k_elect_pass_b_g<<<gb, 1024, 0, c.stream>>>(A, R);
+            k_elect_round_next<<<1, 64, 0, c.stream>>>(R);
+        }
+        PFG_HIP(hipStreamEndCapture(c.stream, &graph));
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            throw Error(PFG_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        }
+        try {
+            while (nwork) {
+                {
+                    KTimer kt(c, K_LCVE_GRAPH, 0.0);
+                    PFG_HIP(hipGraphLaunch(exec, c.stream));
+                }
+                r += GRAPH_ROUNDS;
+                if (r >= MAXR - 1) throw Error(PFG_EINTERNAL, "LCVE election did not converge");
+                PFG_HIP(hipMemcpyAsync(hw, rw + r, 8, hipMemcpyDeviceToHost, c.stream));
+                PFG_HIP(hipMemcpyAsync(hw + 1, ra + r, 8, hipMemcpyDeviceToHost, c.stream));
+                PFG_HIP(hipStreamSynchronize(c.stream));
+                nwork = hw[0];
+                nact = hw[1];
+            }
+        }
+        catch (...) {
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            throw;
+        }
+        PFG_HIP(hipGraphExecDestroy(exec));
+        PFG_HIP(hipGraphDestroy(graph));
+        c.path[PATH_LCVE_GRAPH]++;
+    }
     while (nwork) {
         const int batch = r == 0 ? 2 : 4;
         for (int k = 0; k < batch && r < MAXR - 1; k++, r++) {

@@ -58,11 +58,11 @@ PassTimer::~PassTimer()
 const char *const kernel_names[K_NUM] = {
     "upload_prep", "hist", "ot_fill", "ot_sort", "prop", "lcve_radix", "lcve_init", "lcve_pass_a", "lcve_pass_b",
     "sort_pv_lists", "sub", "ve_detect", "ve_apply", "bce_count", "bce_apply", "ere_probe", "gc_move", "count",
-    "sub_rest", "count_inst"};
+    "sub_rest", "count_inst", "lcve_graph"};
 const char *const kernel_symbols[K_NUM] = {
     "k_upload", nullptr, nullptr, nullptr, nullptr, nullptr, "k_elect_init", "k_elect_pass_a", "k_elect_pass_b",
     "k_sort_pv_lists", "k_sub_grp", "k_ve_detect", "k_ve_apply", nullptr, nullptr, "k_ere_probe", "k_gc_move", "k_count",
-    nullptr, "k_count_inst"};
+    nullptr, "k_count_inst", nullptr};
 
 KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
 {
@@ -1011,7 +1011,8 @@ int pfg_set_tuning(pfg_ctx *ctx, const pfg_tuning *t)
 {
     if (!ctx || !t) return PFG_EINVAL;
     Ctx &c = ctx->c;
-    if (t->ot_build < 0 || t->ot_build > 1 || !t->clause_grid || t->clause_grid > (1u << 20) ||
+    if (t->ot_build < 0 || t->ot_build > 1 || t->lcve_graph < 0 || t->lcve_graph > 1 || !t->clause_grid ||
+        t->clause_grid > (1u << 20) ||
         !(t->sub_group == 8 || t->sub_group == 16 || t->sub_group == 32 || t->sub_group == 64)) {
         c.last_error = "tuning value out of range";
         return PFG_EINVAL;
@@ -1019,6 +1020,7 @@ int pfg_set_tuning(pfg_ctx *ctx, const pfg_tuning *t)
     c.tune.ot_build = t->ot_build;
     c.tune.clause_grid = (t->clause_grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     c.tune.sub_group = t->sub_group;
+    c.tune.lcve_graph = t->lcve_graph;
     c.stage_words = t->stage_words;
     return PFG_OK;
 }
@@ -1031,6 +1033,7 @@ int pfg_get_tuning(pfg_ctx *ctx, pfg_tuning *t)
     t->clause_grid = c.tune.clause_grid;
     t->sub_group = c.tune.sub_group;
     t->stage_words = c.stage_words;
+    t->lcve_graph = c.tune.lcve_graph;
     return PFG_OK;
 }
 

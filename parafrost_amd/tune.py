@@ -2,8 +2,9 @@
 
 A tuning never changes a result, only the time: the occurrence-table build (radix
 sort of (literal, reference) keys, or counting with an atomic fill and a sort of each
-list), the block count of the clause-streaming passes (their XCD windows), and the
-lane group per elected variable in subsumption. tune() measures a workload's step
+list), the block count of the clause-streaming passes (their XCD windows), the lane
+group per elected variable in subsumption, and whether the LCVE election rounds are
+launched one by one or replayed from a captured HIP graph. tune() measures a workload's step
 under candidate tunings -- coordinate descent over the knobs in SPACE, every candidate
 timed over a few steps after a warm-up -- and records the best, with every
 measurement, in tuning/<name>.json next to this module; apply() reads that record back
@@ -24,6 +25,7 @@ SPACE = {
     "ot_build": [0, 1],
     "clause_grid": [1024, 2048, 4096, 8192],
     "sub_group": [8, 16, 32, 64],
+    "lcve_graph": [0, 1],
 }
 
 
@@ -95,6 +97,7 @@ def apply(simplifier, name: str) -> dict | None:
     rec = load(name)
     if rec is None or rec.get("device") not in (_device_name(), "unknown"):
         return None
-    knobs = {k: v for k, v in rec["tuning"].items() if k in ("ot_build", "clause_grid", "sub_group", "stage_words")}
+    knobs = {k: v for k, v in rec["tuning"].items()
+             if k in ("ot_build", "clause_grid", "sub_group", "stage_words", "lcve_graph")}
     simplifier.set_tuning(**knobs)
     return rec

@@ -10,13 +10,15 @@ from parafrost_amd import gen
 pytestmark = pytest.mark.gpu
 
 TUNINGS = [dict(ot_build=1), dict(ot_build=1, clause_grid=8), dict(sub_group=8), dict(sub_group=32),
-           dict(sub_group=64), dict(ot_build=1, sub_group=8, clause_grid=8192)]
+           dict(sub_group=64), dict(ot_build=1, sub_group=8, clause_grid=8192), dict(lcve_graph=1),
+           dict(lcve_graph=1, clause_grid=8)]
 
 
 @pytest.mark.parametrize("k", range(len(TUNINGS)))
 def test_tunings_lockstep(k):
     t = TUNINGS[k]
     print(f"tuning {t}", flush=True)
+    graph_calls = 0
     for seed in range(3):
         rng = np.random.default_rng(9300 + seed)
         nv = int(rng.integers(200, 3000))
@@ -26,7 +28,12 @@ def test_tunings_lockstep(k):
             cnf = gen.geometric(nv, int(nv * 3.0), 2, 8, 9300 + seed)
         else:   # long lists: staged-size limits of the lane groups, std::sort-ordered lists
             cnf = gen.scalefree(max(nv, 2000), int(max(nv, 2000) * 4.0), 2.5, 0.2, 9300 + seed)
-        ls.run_lockstep(cnf, setup=lambda g: g.set_tuning(**t), phases=4, mu_pos=64, mu_neg=64)
+        seen = {}
+        ls.run_lockstep(cnf, setup=lambda g: g.set_tuning(**t), phases=4, mu_pos=64, mu_neg=64,
+                        inspect=lambda g: seen.update(g.path_counts()))
+        graph_calls += seen.get("lcve_graph_calls", 0)   # (a refuted formula ends before the last comparison)
+    if t.get("lcve_graph"):
+        assert graph_calls >= 1   # the election rounds ran from the captured graph
 
 
 def test_tuning_batch_counting_build():
@@ -38,7 +45,7 @@ def test_tuning_batch_counting_build():
             for i, n in enumerate(rng.integers(100, 3000, 12))]
     g = pf.BatchSimplifier([c.nvars for c in cnfs], pf.default_opts())
     try:
-        g.set_tuning(ot_build=1, sub_group=8, clause_grid=64)
+        g.set_tuning(ot_build=1, sub_group=8, clause_grid=64, lcve_graph=1)
         g.upload_instances(cnfs)
         st = g.sigmify_batch()
         for i, c in enumerate(cnfs):
