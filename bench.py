@@ -525,7 +525,7 @@ def run_headline(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU (sharding across GPUs is designed, not built); "
+        print(json.dumps({"error": "bench.py runs on one GPU (sharding across GPUs is designed, not built); "
                                    "--gpus must be 1"}))
         sys.exit(2)
     if args.copy_calibration:

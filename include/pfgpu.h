@@ -59,6 +59,10 @@ typedef struct pfg_opts {
     int32_t bce_limit;         /* 3000 (bcemax, src/cpu/options.cpp:35) */
     int32_t proof_en;          /* 0    (proof, src/cpu/options.cpp:64): binary DRAT lines of the simplifier,
                                   read with pfg_download_proof; single-instance contexts only */
+    int32_t ve_fun_en;         /* 0    function-table gates in BVE (the GPU reference's vefun,
+                                  src/gpu/options.cpp:31, function.cuh:263-311; src/cpu has none: the core
+                                  variables are numbered by first appearance, which no result
+                                  depends on); off by default, as in the reference */
 } pfg_opts;
 
 /* ---- Batches (no counterpart call in the reference, which simplifies one formula per

@@ -337,7 +337,7 @@ void pfo_default_opts(pfo_opts *o)
     o->shrink_rate = 2; o->ve_clause_limit = 100; o->ve_lbound_en = 0; o->xor_max_arity = 10;
     o->sub_limit = 3000; o->ere_limit = 3000; o->ere_max_resolvent = 1000;
     o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
-    o->bce_en = 0; o->bce_limit = 3000; o->proof_en = 0;
+    o->bce_en = 0; o->bce_limit = 3000; o->proof_en = 0; o->ve_fun_en = 0;
 }
 
 pfo *pfo_create(uint32_t nvars, const pfo_opts *o, const uint32_t *lits, const uint64_t *offs,
@@ -907,7 +907,8 @@ static int count_subst(pfo *s, uint32_t x, int clsbefore, ol_t *me, ol_t *other,
         for (int32_t j = 0; j < other->sz; j++) {
             const uint32_t *cj = CL(s, other->d[j]);
             if (!c_orig(cj)) continue;
-            if (molten_rule && a == c_molten(cj)) continue;
+            if (molten_rule == 1 && a == c_molten(cj)) continue;
+            if (molten_rule == 2 && a && c_molten(cj)) continue;
             int rsize;
             if ((rsize = merge_len(x, ci, cj)) > 1) {
                 if (++(*nAdded) > clsbefore || (rlimit && rsize > rlimit)) return 1;
@@ -1245,6 +1246,8 @@ static void new_resolvent(pfo *s)
     }
 }
 
+/* rule: 0 every pair, 1 gate clauses with non-gate ones only (xsubstitute), 2 not two
+   non-core clauses (function-table gates) */
 static void x_resolve(pfo *s, uint32_t x, int substitute)
 {   /* xsubstitute / xresolve (src/cpu/bve.cpp:153-190) */
     uint32_t dx = 2 * x, fx = dx + 1;
@@ -1257,12 +1260,126 @@ static void x_resolve(pfo *s, uint32_t x, int substitute)
         for (int32_t j = 0; j < other->sz; j++) {
             const uint64_t rj = other->d[j];
             if (!c_orig(CL(s, rj))) continue;
-            if (substitute && a == c_molten(CL(s, rj))) continue;
+            if (substitute == 1 && a == c_molten(CL(s, rj))) continue;
+            if (substitute == 2 && a && c_molten(CL(s, rj))) continue;
             /* the pool may move when a resolvent is appended: re-read pointers */
             if (merge_out(s, x, CL(s, ri), CL(s, rj))) new_resolvent(s);
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Function-table gates (option ve_fun_en; the GPU reference's         */
+/* find_fun_gate, src/gpu/function.cuh:263-311, "inspired by the       */
+/* function table reasoning in Lingeling"). src/cpu has no counterpart, */
+/* and the reference numbers the variables by a global map; here the   */
+/* distinct variables other than x of x's original clauses are         */
+/* numbered in order of first appearance over the positive list, then  */
+/* the negative list, at most FUN_VARS of them (no decision depends on */
+/* the numbering: tables are only tested for falsity). A clause        */
+/* without x's literal is a function of them, kept as a truth table of */
+/* 2^FUN_VARS bits: 64 words, word w bit a is the assignment 64 w + a, */
+/* variable k is bit k of it. Parity pinned by brute force (equisat,   */
+/* model extension, the skipped resolvents) in tests/test_model.py.    */
+/* ------------------------------------------------------------------ */
+#define FUN_VARS 12
+#define FUN_WORDS 64
+static const uint64_t fun_magic[6] = {0xaaaaaaaaaaaaaaaaULL, 0xccccccccccccccccULL, 0xf0f0f0f0f0f0f0f0ULL,
+                                      0xff00ff00ff00ff00ULL, 0xffff0000ffff0000ULL, 0xffffffff00000000ULL};
+
+/* the word w of the table of variable k's literal (neg: its negation) */
+static uint64_t fun_lit_word(int k, int neg, int w)
+{
+    uint64_t v = k < 6 ? fun_magic[k] : (((uint64_t)w >> (k - 6)) & 1u) ? ~0ull : 0ull;
+    return neg ? ~v : v;
+}
+
+static int fun_index(const uint32_t *cv, int nc, uint32_t v)
+{
+    for (int k = 0; k < nc; k++) if (cv[k] == v) return k;
+    return -1;
+}
+
+/* f &= every original clause of l[0, tail), without `lit`, as a function */
+static void fun_and_list(pfo *s, const ol_t *l, int32_t tail, uint32_t lit, const uint32_t *cv, int nc, uint64_t *f)
+{
+    for (int32_t i = 0; i < tail; i++) {
+        const uint32_t *c = CL(s, l->d[i]);
+        if (!c_orig(c)) continue;
+        uint64_t cls[FUN_WORDS] = {0};
+        for (int k = 0; k < c_size(c); k++) {
+            const uint32_t o = c[3 + k];
+            if (o == lit) continue;
+            const int m = fun_index(cv, nc, ABSV(o));
+            for (int w = 0; w < FUN_WORDS; w++) cls[w] |= fun_lit_word(m, (int)(o & 1u), w);
+        }
+        for (int w = 0; w < FUN_WORDS; w++) f[w] &= cls[w];
+    }
+}
+
+static int fun_false(const uint64_t *f)
+{
+    for (int w = 0; w < FUN_WORDS; w++) if (f[w]) return 0;
+    return 1;
+}
+
+/* 1: x is defined by its clauses as a function of at most FUN_VARS variables (the
+   positive and the negative side cannot both hold), with the resolvents under the
+   bound; the clauses outside the minimized core are molten and *nAdded counts the
+   resolvents of pairs that are not both outside it. On a count past the bound the
+   clauses are frozen again and *nAdded keeps the count. */
+static int find_fun_gate(pfo *s, uint32_t p, int nOrgCls, int *nAdded)
+{
+    const uint32_t n = p ^ 1u, x = ABSV(p);
+    ol_t *poss = &s->ot[p], *negs = &s->ot[n];
+    uint32_t cv[FUN_VARS];
+    int nc = 0;
+    for (int side = 0; side < 2; side++) {
+        const ol_t *l = side ? negs : poss;
+        for (int32_t i = 0; i < l->sz; i++) {
+            const uint32_t *c = CL(s, l->d[i]);
+            if (!c_orig(c)) continue;
+            for (int k = 0; k < c_size(c); k++) {
+                const uint32_t v = ABSV(c[3 + k]);
+                if (v == x || fun_index(cv, nc, v) >= 0) continue;
+                if (nc == FUN_VARS) return 0;
+                cv[nc++] = v;
+            }
+        }
+    }
+    uint64_t pos[FUN_WORDS], neg[FUN_WORDS], fun[FUN_WORDS];
+    for (int w = 0; w < FUN_WORDS; w++) pos[w] = neg[w] = ~0ull;
+    fun_and_list(s, poss, poss->sz, p, cv, nc, pos);
+    fun_and_list(s, negs, negs->sz, n, cv, nc, neg);
+    for (int w = 0; w < FUN_WORDS; w++) if (pos[w] & neg[w]) return 0;   /* collapsefun */
+    /* core minimization (function.cuh:288-299): a positive clause that the earlier
+       positive clauses and the negative side already contradict without it leaves the
+       core, as does a negative clause that the earlier negative clauses contradict */
+    int core = 0;
+    for (int32_t i = poss->sz - 1; i >= 0; i--) {
+        memcpy(fun, neg, sizeof(fun));
+        fun_and_list(s, poss, i, p, cv, nc, fun);
+        uint32_t *c = CL(s, poss->d[i]);
+        if (fun_false(fun) && c_orig(c)) { c_melt(c); core = 1; }
+    }
+    for (int32_t i = negs->sz - 1; i >= 0; i--) {
+        for (int w = 0; w < FUN_WORDS; w++) fun[w] = ~0ull;
+        fun_and_list(s, negs, i, n, cv, nc, fun);
+        uint32_t *c = CL(s, negs->d[i]);
+        if (fun_false(fun) && c_orig(c)) { c_melt(c); core = 1; }
+    }
+    *nAdded = 0;
+    if (count_subst(s, x, nOrgCls, poss, negs, nAdded, 2)) {
+        if (core) {   /* freezeClauses (elimination.cuh:106-120) */
+            for (int32_t i = 0; i < poss->sz; i++) c_freeze(CL(s, poss->d[i]));
+            for (int32_t i = 0; i < negs->sz; i++) c_freeze(CL(s, negs->d[i]));
+        }
+        return 0;
+    }
+    return 1;
+}
+
+#define FUNCTION 4
 
 void pfo_ve(pfo *s)
 {   /* bve (src/cpu/bve.cpp:28-151) */
@@ -1297,9 +1414,11 @@ void pfo_ve(pfo *s)
                     else if (find_xor_gate(s, p, nOrgCls, &nAdded)) type = SUBSTITUTION;
                     else if (!nAdded && find_xor_gate(s, n, nOrgCls, &nAdded)) type = SUBSTITUTION;
                 }
+                /* function-table gates after the other gates (kernels.cu:543-545) */
+                if (!type && s->opts.ve_fun_en && nOrgCls > 2 && find_fun_gate(s, p, nOrgCls, &nAdded)) type = FUNCTION;
                 if (!type && !nAdded && !count_subst(s, v, nOrgCls, poss, negs, &nAdded, 0)) type = RESOLUTION;
-                if (type & SUBSTITUTION) {
-                    if (nAdded) x_resolve(s, v, 1);
+                if (type & (SUBSTITUTION | FUNCTION)) {
+                    if (nAdded) x_resolve(s, v, type == FUNCTION ? 2 : 1);
                     toblivion_var(s, p, pOrgs, nOrgs);
                     v = 0;
                 }

@@ -39,6 +39,9 @@ typedef struct pfo_opts {
     int32_t bce_en;            /* opt_bce_en            (options.cpp:26)         = 0     */
     int32_t bce_limit;         /* opt_bce_max_occurs    (options.cpp:35)         = 3000  */
     int32_t proof_en;          /* opt_proof_en          (options.cpp:64)         = 0     (binary DRAT) */
+    int32_t ve_fun_en;         /* GPU reference's vefun (src/gpu/options.cpp:31): function-table gates
+                                  (no src/cpu counterpart: the variable numbering is chosen
+                                  here; no result depends on it) = 0 */
 } pfo_opts;
 
 typedef struct pfo pfo;

@@ -58,11 +58,11 @@ PassTimer::~PassTimer()
 const char *const kernel_names[K_NUM] = {
     "upload_prep", "hist", "ot_fill", "ot_sort", "prop", "lcve_radix", "lcve_init", "lcve_pass_a", "lcve_pass_b",
     "sort_pv_lists", "sub", "ve_detect", "ve_apply", "bce_count", "bce_apply", "ere_probe", "gc_move", "count",
-    "sub_rest", "count_inst", "lcve_graph"};
+    "sub_rest", "count_inst", "lcve_graph", "ve_fun"};
 const char *const kernel_symbols[K_NUM] = {
     "k_upload", nullptr, nullptr, nullptr, nullptr, nullptr, "k_elect_init", "k_elect_pass_a", "k_elect_pass_b",
     "k_sort_pv_lists", "k_sub_grp", "k_ve_detect", "k_ve_apply", nullptr, nullptr, "k_ere_probe", "k_gc_move", "k_count",
-    nullptr, "k_count_inst", nullptr};
+    nullptr, "k_count_inst", nullptr, "k_ve_fun"};
 
 KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
 {
@@ -407,7 +407,7 @@ void pfg_default_opts(pfg_opts *o)
     o->shrink_rate = 2; o->ve_clause_limit = 100; o->ve_lbound_en = 0; o->xor_max_arity = 10;
     o->sub_limit = 3000; o->ere_limit = 3000; o->ere_max_resolvent = 1000;
     o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
-    o->bce_en = 0; o->bce_limit = 3000; o->proof_en = 0;
+    o->bce_en = 0; o->bce_limit = 3000; o->proof_en = 0; o->ve_fun_en = 0;
 }
 
 

@@ -21,7 +21,8 @@
 
 namespace pfg {
 
-enum { T_NONE = 0, T_PURE = 1, T_BN = 2, T_SUBST = 3, T_RES = 4 };
+// T_FUNPEND: no gate of the other kinds; the function-table gate (k_ve_fun) decides
+enum { T_NONE = 0, T_PURE = 1, T_BN = 2, T_SUBST = 3, T_RES = 4, T_CORE = 5, T_FUNPEND = 6 };
 
 struct VeArgs {
     uint32_t *mem;
@@ -74,9 +75,19 @@ __device__ __forceinline__ int count_orgs(const VeArgs &A, uint32_t lit)
     return n;
 }
 
-// countSubstituted / countResolvents (src/cpu/simplify.h:227-287); on success the
+// which resolvent pairs are made: every pair (plain resolution), pairs of one gate and
+// one non-gate clause (substitution), pairs not both outside a function table's core
+enum { R_ALL = 0, R_SUBST = 1, R_CORE = 2 };
+
+__device__ __forceinline__ bool skip_pair(int rule, bool ci_molten, bool cj_molten)
+{
+    return rule == R_SUBST ? ci_molten == cj_molten : rule == R_CORE ? ci_molten && cj_molten : false;
+}
+
+// countSubstituted / countResolvents (src/cpu/simplify.h:227-287), and for R_CORE the
+// GPU reference's countCoreSubstituted (src/gpu/function.cuh:191-259); on success the
 // totals needed for placement are returned through words/units.
-__device__ __forceinline__ bool count_subst(const VeArgs &A, uint32_t x, int clsbefore, uint32_t me, uint32_t other, bool molten_rule,
+__device__ __forceinline__ bool count_subst(const VeArgs &A, uint32_t x, int clsbefore, uint32_t me, uint32_t other, int rule,
                             int &nAdded, uint64_t &words, uint32_t &units)
 {
     int nAddedLits = 0;
@@ -89,7 +100,7 @@ __device__ __forceinline__ bool count_subst(const VeArgs &A, uint32_t x, int cls
         for (uint32_t j = 0; j < os; j++) {
             const uint32_t *cj = cl_at(A, other, j);
             if (!c_orig(cj)) continue;
-            if (molten_rule && a == c_molten(cj)) continue;
+            if (skip_pair(rule, a, c_molten(cj))) continue;
             const int rsize = merge_len(x, ci, cj);
             if (rsize > 1) {
                 if (++nAdded > clsbefore || (A.rlimit && rsize > A.rlimit)) return true;
@@ -198,7 +209,7 @@ __device__ __forceinline__ bool find_ao_gate(const VeArgs &A, uint32_t dx, int n
             if (!eq) continue;
             melt(A, c);
             nAdded = 0;
-            if (count_subst(A, x, nOrgCls, dx, f_dx, true, nAdded, words, units)) { freeze(A, c); break; }
+            if (count_subst(A, x, nOrgCls, dx, f_dx, R_SUBST, nAdded, words, units)) { freeze(A, c); break; }
             return true;
         }
     }
@@ -286,7 +297,7 @@ __device__ __forceinline__ bool find_ite_gate(const VeArgs &A, uint32_t dx, int 
             if (!d2) continue;
             melt(A, ci); melt(A, cj); melt(A, d1); melt(A, d2);
             nAdded = 0;
-            if (count_subst(A, dx >> 1, nOrgCls, dx, fx, true, nAdded, words, units)) {
+            if (count_subst(A, dx >> 1, nOrgCls, dx, fx, R_SUBST, nAdded, words, units)) {
                 freeze(A, ci); freeze(A, cj); freeze(A, d1); freeze(A, d2);
                 return false;
             }
@@ -487,7 +498,7 @@ __device__ __forceinline__ bool find_xor_gate(const VeArgs &A, uint32_t dx, int 
         else {
             melt(A, ci);
             nAdded = 0;
-            if (count_subst(A, v, nOrgCls, dx, fx, true, nAdded, words, units)) { freeze_arities(A, dx, fx); break; }
+            if (count_subst(A, v, nOrgCls, dx, fx, R_SUBST, nAdded, words, units)) { freeze_arities(A, dx, fx); break; }
             return true;
         }
     }
@@ -531,8 +542,10 @@ struct VeDetectOut {
     uint64_t *nwords, *nmodel;
 };
 
-// detection of one elected variable (pvs[q] = v); A carries the lane's stage
-__device__ __forceinline__ void ve_detect_var(const VeArgs &A, const VeDetectOut &O, uint64_t q, uint32_t v)
+// detection of one elected variable (pvs[q] = v); A carries the lane's stage. With
+// function-table gates on (fun_en), a variable that no other gate defines is left
+// pending for k_ve_fun, its aux word saying whether a gate search counted resolvents.
+__device__ __forceinline__ void ve_detect_var(const VeArgs &A, const VeDetectOut &O, uint64_t q, uint32_t v, bool fun_en)
 {
         const uint32_t p = 2 * v, n = p + 1;
         const int pOrgs = count_orgs(A, p), nOrgs = count_orgs(A, n);
@@ -568,8 +581,12 @@ __device__ __forceinline__ void ve_detect_var(const VeArgs &A, const VeDetectOut
                     else if (!nAdded && find_xor_gate(A, n, nOrgCls, nAdded, words, units)) gate = true;
                 }
                 if (gate) t = T_SUBST;
-                else if (!nAdded && !count_subst(A, v, nOrgCls, p, n, false, nAdded, words, units)) t = T_RES;
-                if (t != T_NONE) {
+                else if (fun_en && nOrgCls > 2) {
+                    t = T_FUNPEND;
+                    a = nAdded != 0;
+                }
+                else if (!nAdded && !count_subst(A, v, nOrgCls, p, n, R_ALL, nAdded, words, units)) t = T_RES;
+                if (t == T_SUBST || t == T_RES) {
                     nr = (uint32_t)nAdded;
                     nw = words;
                     nu = units;
@@ -592,7 +609,7 @@ __device__ __forceinline__ void ve_detect_var(const VeArgs &A, const VeDetectOut
 // detected from the pool instead. Detection of one variable touches only its own
 // clauses, so the rounds do not change results.
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_ve_detect(VeArgs G, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t npvs, uint32_t arena_words)
+k_ve_detect(VeArgs G, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t npvs, uint32_t arena_words, int fun_en)
 {
     __shared__ uint32_t arena[VE_ARENA_WORDS];
     const StageSrc S{G.mem, G.ot, G.occ};
@@ -605,10 +622,162 @@ k_ve_detect(VeArgs G, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t 
         stage_rounds(S, arena, arena_words, active, v, 0, [&](const uint32_t *tab) {
             if (tab) {
                 const VeArgs A{G.mem, G.ot, G.occ, G.rlimit, G.lbound, G.xor_max_arity, v, G.ot[2 * v].sz, tab, arena};
-                ve_detect_var(A, O, q, v);
+                ve_detect_var(A, O, q, v, fun_en != 0);
             }
-            else ve_detect_var(G, O, q, v);   // larger than the arena: from the pool
+            else ve_detect_var(G, O, q, v, fun_en != 0);   // larger than the arena: from the pool
         });
+    }
+}
+
+// ------------------------------------------------------------------ function tables
+// Function-table gates (option ve_fun_en; the GPU reference's find_fun_gate,
+// src/gpu/function.cuh:263-311), for the variables that detection left pending. The
+// variables of x's original clauses other than x, at most FUN_VARS of them, are
+// numbered in order of first appearance; a clause without x's literal is then a
+// Boolean function of them, a truth table of 2^FUN_VARS bits = FUN_WORDS words. One
+// wave takes one variable and lane w holds word w of every table, so building a
+// clause's table costs each lane one OR per literal.
+//   collapse: x is defined when (positive side) AND (negative side) is false;
+//   core: a positive clause whose earlier positive clauses already contradict the
+//     negative side, and a negative clause whose earlier negative clauses are false
+//     by themselves, leave the core (molten), as the reference's backward loops do
+//     (their decisions do not depend on one another, so one forward pass makes them);
+//   resolvents: every pair but those of two clauses outside the core, under the
+//     clause bound; past it the clauses are frozen again and plain resolution is
+//     not tried.
+// Otherwise, plain resolution decides as it does without the option.
+constexpr int FUN_VARS = 12, FUN_WORDS = 64;
+
+// word w of the table of core variable k's literal (neg: of its negation)
+__device__ __forceinline__ uint64_t fun_lit_word(int k, uint32_t neg, uint32_t w)
+{
+    uint64_t m;
+    switch (k) {
+        case 0: m = 0xaaaaaaaaaaaaaaaaull; break;
+        case 1: m = 0xccccccccccccccccull; break;
+        case 2: m = 0xf0f0f0f0f0f0f0f0ull; break;
+        case 3: m = 0xff00ff00ff00ff00ull; break;
+        case 4: m = 0xffff0000ffff0000ull; break;
+        case 5: m = 0xffffffff00000000ull; break;
+        default: m = ((w >> (k - 6)) & 1u) ? ~0ull : 0ull;
+    }
+    return neg ? ~m : m;
+}
+
+// word w of the table of clause c without the literal `lit`
+__device__ __forceinline__ uint64_t fun_clause_word(const uint32_t *c, uint32_t lit, const uint32_t *cv, int nc, uint32_t w)
+{
+    uint64_t f = 0;
+    const int sz = c_size(c);
+    for (int k = 0; k < sz; k++) {
+        const uint32_t o = c[3 + k];
+        if (o == lit) continue;
+        int m = 0;
+        while (m < nc && cv[m] != (o >> 1)) m++;
+        f |= fun_lit_word(m, o & 1u, w);
+    }
+    return f;
+}
+
+__device__ __forceinline__ uint64_t fun_and_list(const VeArgs &A, uint32_t lit, const uint32_t *cv, int nc, uint32_t w)
+{
+    uint64_t f = ~0ull;
+    const uint32_t sz = A.ot[lit].sz;
+    for (uint32_t i = 0; i < sz; i++) {
+        const uint32_t *c = cl_at(A, lit, i);
+        if (c_orig(c)) f &= fun_clause_word(c, lit, cv, nc, w);
+    }
+    return f;
+}
+
+// one wave per group of 64 elected variables; its pending variables one after another
+__global__ void __launch_bounds__(64) k_ve_fun(VeArgs A, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t npvs)
+{
+    __shared__ uint32_t cv[FUN_VARS];
+    __shared__ int nc_s;
+    const uint32_t lane = threadIdx.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * 64; base < npvs; base += (uint64_t)gridDim.x * 64) {
+        const bool pend = base + lane < npvs && (O.type[base + lane] & 0xFFu) == T_FUNPEND;
+        uint64_t todo = __ballot(pend);
+        while (todo) {
+            const uint64_t q = base + (uint64_t)__builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint32_t v = pvs[q], p = 2 * v, n = p + 1;
+            const bool counted = O.aux[q] != 0;
+            __syncthreads();   // the previous variable's readers of cv are done
+            if (lane == 0) {
+                int nc = 0;
+                for (uint32_t s = 0; s < 2 && nc >= 0; s++) {
+                    const uint32_t lit = p | s, sz = A.ot[lit].sz;
+                    for (uint32_t i = 0; i < sz && nc >= 0; i++) {
+                        const uint32_t *c = cl_at(A, lit, i);
+                        if (!c_orig(c)) continue;
+                        const int cs = c_size(c);
+                        for (int k = 0; k < cs; k++) {
+                            const uint32_t u = c[3 + k] >> 1;
+                            if (u == v) continue;
+                            int m = 0;
+                            while (m < nc && cv[m] != u) m++;
+                            if (m < nc) continue;
+                            if (nc == FUN_VARS) { nc = -1; break; }
+                            cv[nc++] = u;
+                        }
+                    }
+                }
+                nc_s = nc;
+            }
+            __syncthreads();
+            const int nc = nc_s;
+            bool collapsed = false, core = false;
+            if (nc >= 0) {
+                const uint64_t neg = fun_and_list(A, n, cv, nc, lane);
+                const uint64_t pos = fun_and_list(A, p, cv, nc, lane);
+                collapsed = __ballot((pos & neg) != 0) == 0;
+                if (collapsed) {
+                    for (uint32_t s = 0; s < 2; s++) {
+                        const uint32_t lit = p | s, sz = A.ot[lit].sz;
+                        uint64_t pre = ~0ull;   // the earlier original clauses of the list
+                        for (uint32_t i = 0; i < sz; i++) {
+                            uint32_t *c = A.mem + A.occ[A.ot[lit].off + i];
+                            if (!c_orig(c)) continue;
+                            const uint64_t f = s ? pre : (neg & pre);
+                            if (__ballot(f != 0) == 0) {
+                                core = true;
+                                if (lane == 0) melt(A, c);
+                            }
+                            pre &= fun_clause_word(c, lit, cv, nc, lane);
+                        }
+                    }
+                }
+            }
+            if (lane == 0) {
+                const int pOrgs = count_orgs(A, p), nOrgs = count_orgs(A, n), nOrgCls = pOrgs + nOrgs;
+                uint32_t t = T_NONE;
+                int nAdded = counted ? 1 : 0;
+                uint64_t words = 0;
+                uint32_t units = 0;
+                if (collapsed) {
+                    nAdded = 0;
+                    if (!count_subst(A, v, nOrgCls, p, n, R_CORE, nAdded, words, units)) t = T_CORE;
+                    else if (core) {   // freezeClauses (src/gpu/elimination.cuh:106-120)
+                        for (uint32_t s = 0; s < 2; s++) {
+                            const uint32_t lit = p | s, sz = A.ot[lit].sz;
+                            for (uint32_t i = 0; i < sz; i++) {
+                                uint32_t *c = A.mem + A.occ[A.ot[lit].off + i];
+                                if (c_orig(c)) freeze(A, c);
+                            }
+                        }
+                    }
+                }
+                if (t == T_NONE && !nAdded && !count_subst(A, v, nOrgCls, p, n, R_ALL, nAdded, words, units)) t = T_RES;
+                O.type[q] = t;
+                O.aux[q] = (uint32_t)(pOrgs > nOrgs);   // model side
+                O.nres[q] = t != T_NONE ? (uint32_t)nAdded : 0u;
+                O.nwords[q] = t != T_NONE ? words : 0ull;
+                O.nunits[q] = t != T_NONE ? units : 0u;
+                O.nmodel[q] = t != T_NONE ? model_words(A, p, pOrgs, nOrgs) : 0ull;
+            }
+        }
     }
 }
 
@@ -704,13 +873,13 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
         }
         else {
             const bool pbig = aux[q] != 0;
-            if ((t == T_SUBST || t == T_RES) && roff[q + 1] > roff[q]) {   // xsubstitute/xresolve only if nAddedCls
+            if ((t == T_SUBST || t == T_RES || t == T_CORE) && roff[q + 1] > roff[q]) {   // xsubstitute/xresolve only if nAddedCls
                 uint32_t dx = p, fx = n;
                 if (A.ot[dx].sz > A.ot[fx].sz) { dx = n; fx = p; }
                 const OL om = A.ot[dx], oo = A.ot[fx];
                 uint64_t w = O.data_base + woff[q];
                 uint64_t r = O.ref_base + roff[q];
-                const bool subst = t == T_SUBST;
+                const int rule = t == T_SUBST ? R_SUBST : t == T_CORE ? R_CORE : R_ALL;
                 for (uint32_t i = 0; i < om.sz; i++) {
                     const uint32_t *ci = A.mem + A.occ[om.off + i];
                     if (!c_orig(ci)) continue;
@@ -718,7 +887,7 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
                     for (uint32_t j = 0; j < oo.sz; j++) {
                         const uint32_t *cj = A.mem + A.occ[oo.off + j];
                         if (!c_orig(cj)) continue;
-                        if (subst && a == c_molten(cj)) continue;
+                        if (skip_pair(rule, a, c_molten(cj))) continue;
                         // the length is known first so that nothing is written past this
                         // variable's region (merge_out writes before it sees a tautology)
                         const int rl = merge_len(v, ci, cj);
@@ -799,7 +968,12 @@ void ve(Ctx &c)
         const VeDetectOut DO{type, aux, nres, nunits, nwords, nmodel};
         KTimer kt(c, K_VE_DETECT, pv_bytes(c, 0, np, true));
         k_ve_detect<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np,
-                                                                  std::min(VE_ARENA_WORDS, c.stage_words));
+                                                                  std::min(VE_ARENA_WORDS, c.stage_words), c.opts.ve_fun_en);
+    }
+    if (c.opts.ve_fun_en) {
+        const VeDetectOut DO{type, aux, nres, nunits, nwords, nmodel};
+        KTimer kt(c, K_VE_FUN, pv_bytes(c, 0, np, true));
+        k_ve_fun<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np);
     }
     // exclusive scans with the totals stored one past the end
     scan_u32_to_u64(c, nres, roff, np, roff + np);

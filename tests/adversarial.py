@@ -110,3 +110,78 @@ def sub_long_unit_capable(nlong: int = 3500, seed: int = 0):
     lits = np.array([l for c in clean for l in c], np.uint32)
     offs = np.concatenate([[0], np.cumsum([len(c) for c in clean])]).astype(np.uint64)
     return gen.Cnf(nv, lits, offs), np.arange(2, nv + 1, dtype=np.uint32)
+
+
+def function_gates(seed: int, nv_free: int = 8):
+    """Small formulas for function-table gates (ve_fun_en), at most 20 variables for brute
+    force, with the variables to freeze so that only the gadget variables x are elected.
+
+    Definition gadget: x is a random Boolean function of 3 free variables, one clause per
+    input assignment (inputs equal to the assignment imply x, or ~x), plus clauses of x
+    longer than the definition's. The function table collapses, but the minimization
+    (a clause leaves the core when the clauses before it already contradict) only melts
+    positive clauses here, so the resolvents equal plain resolution's.
+
+    Contradiction gadget: ~x joined to every clause of the pigeonhole formula of 3
+    pigeons in 2 holes (6 variables, minimally unsatisfiable, no clause pair that
+    self-subsumes, so subsumption leaves it as it is), then 2 longer negative clauses of x
+    after it and one positive clause. The negative prefix before the extra clauses is
+    contradictory, so they leave the core; the positive clause leaves it too (the negative
+    side alone is false), and the 2 resolvents between them, never tautologies, are the
+    ones a gate skips and plain resolution adds."""
+    rng = np.random.default_rng(9600 + seed)
+    cls = []
+    nv = nv_free
+    # definition gadget
+    ins = rng.choice(np.arange(1, nv_free + 1), size=3, replace=False)
+    nv += 1
+    x = nv
+    table = rng.integers(0, 2, 8)
+    for a in range(8):
+        # inputs equal to a  ->  x == table[a]:  (some input differs) | (x or ~x)
+        c = [2 * int(v) + ((a >> i) & 1) for i, v in enumerate(ins)]
+        c.append(2 * x + (0 if table[a] else 1))
+        cls.append(sorted(c))
+    half = nv_free // 2
+    for sign in (0, 1):
+        for _ in range(int(rng.integers(1, 3))):
+            vs = np.arange(1, half + 1) + sign * half
+            cls.append(sorted([2 * x + sign] + [int(2 * v + rng.integers(0, 2)) for v in vs]))
+    # contradiction gadget: pigeon i in hole j is variable p0 + 2 i + j
+    p0 = nv + 1
+    nv += 6
+    nv += 1
+    x = nv
+    php = [[2 * (p0 + 2 * i), 2 * (p0 + 2 * i + 1)] for i in range(3)]
+    for j in range(2):
+        for i in range(3):
+            for k in range(i + 1, 3):
+                php.append([2 * (p0 + 2 * i + j) + 1, 2 * (p0 + 2 * k + j) + 1])
+    for c in php:
+        cls.append(sorted(c + [2 * x + 1]))
+    a, b = rng.choice(np.arange(1, half + 1), size=2, replace=False)
+    cls.append(sorted([2 * x, 2 * int(a), 2 * int(b)]))
+    for _ in range(2):
+        vs = rng.choice(np.arange(half + 1, nv_free + 1), size=3, replace=False)
+        cls.append(sorted([2 * x + 1] + [int(2 * v + rng.integers(0, 2)) for v in vs]))
+    for _ in range(int(rng.integers(nv_free, 2 * nv_free))):
+        vs = rng.choice(np.arange(1, nv_free + 1), size=3, replace=False)
+        cls.append(sorted(int(2 * v + rng.integers(0, 2)) for v in vs))
+    lits = np.array([l for c in cls for l in c], np.uint32)
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in cls])]).astype(np.uint64)
+    freeze = np.array([v for v in range(1, nv + 1) if v not in (nv_free + 1, nv)], np.uint32)
+    return gen.Cnf(nv, lits, offs), freeze
+
+
+def function_gates_tiled(copies: int, seed: int):
+    """copies of function_gates(seed + i) over disjoint variables, with their frozen
+    variables: many pending function-table gates in one elimination pass"""
+    lits, offs, freeze, nv = [], [np.zeros(1, np.uint64)], [], 0
+    for i in range(copies):
+        cnf, fr = function_gates(seed * 100003 + i)
+        lits.append(cnf.lits + 2 * nv)
+        offs.append(cnf.offs[1:] + offs[-1][-1])
+        freeze.append(fr + nv)
+        nv += cnf.nvars
+    return (gen.Cnf(nv, np.concatenate(lits).astype(np.uint32), np.concatenate(offs)),
+            np.concatenate(freeze).astype(np.uint32))

@@ -44,6 +44,7 @@ OPTION_VARIANTS = [
     dict(sub_limit=20, ere_limit=20), dict(ve_clause_limit=0), dict(ve_clause_limit=3), dict(ere_max_resolvent=0),
     dict(lits_min=1 << 40), dict(phases=8, mu_pos=10, mu_neg=10), dict(sub_en=0), dict(lcve_min=50),
     dict(bce_en=1), dict(bce_en=1, bce_limit=100, mu_pos=200, mu_neg=200), dict(bce_en=1, ve_en=0),
+    dict(ve_fun_en=1), dict(ve_fun_en=1, ve_lbound_en=1, ve_clause_limit=3),
 ]
 
 

@@ -8,7 +8,11 @@ the oracle through the C ABI:
   (sub_limit above 3000);
 * the caller's variable mapping (vorg): a second simplifier call on the formula the
   reference's map(true) left, with model records and proof lines in original numbering;
-* pfg_interrupt: a call stopped before its first round, and one stopped while it runs.
+* pfg_interrupt: a call stopped before its first round, and one stopped while it runs;
+* function-table gates (ve_fun_en): variables defined as functions of 3 others and
+  variables whose negative clauses hold a contradictory prefix, where the gate leaves
+  out the resolvents between clauses outside the core (tests/adversarial.py:
+  function_gates), alone, tiled by the hundred, and on general formulas.
 """
 import threading
 import time
@@ -19,7 +23,7 @@ import pytest
 import lockstep as ls
 import pyoracle as po
 import parafrost_amd as pf
-from adversarial import adversarial, ere_shared_targets, sub_long_unit_capable
+from adversarial import adversarial, ere_shared_targets, function_gates, function_gates_tiled, sub_long_unit_capable
 from parafrost_amd import gen
 from test_oracle_golden import SECOND, second_call_case
 
@@ -178,3 +182,27 @@ def test_interrupt_batch():
                 assert np.array_equal(s.cnf, w) and np.array_equal(s.resolved, r) and np.array_equal(s.vstate, vs)
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_function_gates_lockstep(seed):
+    print(f"function gates seed={seed}", flush=True)
+    cnf, freeze = function_gates(seed)
+    ls.run_lockstep(cnf, freeze=freeze, ve_fun_en=1, phases=1 + seed % 3, mu_pos=64, mu_neg=64, proof_en=seed % 2)
+
+
+@pytest.mark.parametrize("copies,seed", [(60, 0), (700, 1)])
+def test_function_gates_tiled_lockstep(copies, seed):
+    """Many pending variables per group of 64 elected ones, BCE and the proof on for one."""
+    print(f"function gates tiled copies={copies} seed={seed}", flush=True)
+    cnf, freeze = function_gates_tiled(copies, seed)
+    ls.run_lockstep(cnf, freeze=freeze, ve_fun_en=1, phases=2, mu_pos=64, mu_neg=64, proof_en=seed, bce_en=seed)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_function_gates_on_general_formulas(seed):
+    """Formulas of the generators and the adversarial ones: most variables have more than
+    12 neighbours or no collapsing table, and fall back to plain resolution."""
+    print(f"function gates general seed={seed}", flush=True)
+    cnf = adversarial(seed) if seed % 2 else gen.scalefree(3000, 9000, 2.5, 0.3, 70 + seed)
+    ls.run_lockstep(cnf, ve_fun_en=1, phases=3, mu_pos=40, mu_neg=40, proof_en=1)

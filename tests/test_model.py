@@ -30,10 +30,12 @@ def _eval(clauses, nv):
     return ok
 
 
-def _check_semantics(cnf, opts):
+def _check_semantics(cnf, opts, freeze=None):
     """returns the oracle's final state (None when it proved UNSAT)"""
     nv = cnf.nvars
     o = po.Oracle(cnf, opts)
+    if freeze is not None:
+        o.ifreeze(freeze)
     rc = o.sigmify()
     F = [list(map(int, c)) for c in cnf.clauses()]
     satF = _eval(F, nv)
@@ -94,3 +96,35 @@ def test_bce_equisat_and_model_extension():
         if st is not None and rc == 0:
             changed += not np.array_equal(st.resolved, o.state().resolved)
     assert changed >= 6, changed
+
+
+def test_function_gates_equisat_and_model_extension():
+    """Function-table gates (ve_fun_en, the GPU reference's vefun; src/cpu has none, so
+    only mathematics pins them): on formulas with a variable defined as a random function
+    of 3 others and a variable whose negative clauses hold a contradictory prefix, the
+    result is equisatisfiable, every model of it extends to a model of the input, and
+    the gate leaves out resolvents between clauses outside the core only: those of the
+    contradiction gadget, never tautologies, in every formula."""
+    from adversarial import function_gates
+    for seed in range(40):
+        cnf, freeze = function_gates(seed)
+        assert cnf.nvars <= 20
+        kw = dict(phases=int(1 + seed % 3), mu_pos=64, mu_neg=64)
+        st = _check_semantics(cnf, po.default_opts(ve_fun_en=1, **kw), freeze)
+        o = po.Oracle(cnf, po.default_opts(**kw))
+        o.ifreeze(freeze)
+        assert o.sigmify() == 0 and st is not None
+        plain = o.state()
+        got = {tuple(c[2]) for c in st.clauses() if not (c[0] & 2)}
+        ref = {tuple(c[2]) for c in plain.clauses() if not (c[0] & 2)}
+        assert got < ref, seed
+        # the skipped resolvents: the positive clause of the contradiction gadget with
+        # each of its 2 extra negative clauses
+        x = cnf.nvars
+        xs = [list(map(int, c)) for c in cnf.clauses() if any(l >> 1 == x for l in c)]
+        pos = [c for c in xs if 2 * x in c]
+        neg_long = [c for c in xs if 2 * x + 1 in c and len(c) == 4]
+        assert len(pos) == 1 and len(neg_long) == 2
+        want = {tuple(sorted(set(pos[0] + n) - {2 * x, 2 * x + 1})) for n in neg_long}
+        # (both, unless a later pass removed one, e.g. as a duplicate)
+        assert ref - got <= want, seed
