@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--batch-lits", type=float, default=1.0e9,
                     help="c4: literal budget of one batch (instances are admitted in order until it is reached)")
     ap.add_argument("--c4-instances", type=int, default=4096, help="c4: number of instances")
-    ap.add_argument("--plain-sample", type=int, default=64, help="c4: instances timed on the plain path")
+    ap.add_argument("--plain-sample", type=int, default=64,
+                    help="c4: instances timed on the plain path (0: none, for profiler runs of the batched path)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="c2-c4: measure candidate tunings on the workload first and record the best "
@@ -317,8 +318,10 @@ def run_c4(args):
     step(download=download)
     s.close()
     del inputs
-    # plain path on a spread sample
+    # plain path on a spread sample (none with --plain-sample 0: profiler runs of the batched path)
     pick = np.unique(np.linspace(0, B.ninst - 1, min(args.plain_sample, B.ninst)).round().astype(np.int64))
+    if args.plain_sample <= 0:
+        pick = pick[:0]
     plain_t, plain_l = 0.0, 0
     for i in pick:
         c = B.instance(int(i))
@@ -339,7 +342,7 @@ def run_c4(args):
                 plain_t += time.perf_counter() - t1
                 plain_l += c.nlits
         s.close()
-    plain = plain_l / plain_t
+    plain = plain_l / plain_t if plain_t else None
     value = B.nlits / dt
     out = {
         "metric": "simplified literals/sec on 1 MI355X (batched/served vs plain); p50/p99 latency",
@@ -354,7 +357,7 @@ def run_c4(args):
         "plain": {"value": plain, "unit": "literals/s",
                   "sample": f"{len(pick)} instances spread over the batch, {plain_l} literals, one context each, "
                             "second of two runs timed"},
-        "vs_plain": value / plain,
+        "vs_plain": value / plain if plain else None,
         "tuning": tuning,
         "writeback_ms": round(wb["s"] * 1e3, 1),
         "writeback": f"every instance's clauses, model records, variable states and units downloaded into reused "
@@ -437,7 +440,8 @@ def setup_tuning(args, s, step, name, workload):
                         log=lambda m: print("# " + m, file=sys.stderr, flush=True))
         return {"source": f"measured now ({len(rec['trials'])} candidates)", **s.tuning()}
     rec = tune.apply(s, name)
-    src = f"parafrost_amd/tuning/{name}.json" if rec else "library defaults (no record for this device)"
+    src = (f"parafrost_amd/tuning/{name}.json" if rec else
+           f"library defaults (no record of {name} for {tune._device_arch()}: {tune._device_name()})")
     return {"source": src, **s.tuning()}
 
 
@@ -512,8 +516,8 @@ def run_headline(args):
     """The metric's workloads: the batched path (c4) as the line, the served path (c5)
     as its `served` object."""
     out = run_c4(args)
-    print(f"# c4 batched {out['value'] / 1e6:.1f} M literals/s, plain {out['plain']['value'] / 1e6:.1f}, "
-          f"x{out['vs_plain']:.2f}", file=sys.stderr, flush=True)
+    print(f"# c4 batched {out['value'] / 1e6:.1f} M literals/s, plain {(out['plain']['value'] or 0) / 1e6:.1f}, "
+          f"x{out['vs_plain'] or 0:.2f}", file=sys.stderr, flush=True)
     c5 = run_c5(args)
     out["served"] = {"workload": c5["config"]["workload"], "path": c5["config"]["path"],
                      "value": c5["value"], "unit": "literals/s", "p50_ms": c5["p50_ms"], "p99_ms": c5["p99_ms"],

@@ -120,6 +120,10 @@ typedef struct pfg_tuning {
     uint32_t stage_words;  /* LDS words per block of the staged kernels (pfg_set_stage_words) */
     int32_t lcve_graph;    /* 1: the LCVE election rounds are captured once per call as a HIP graph
                               (four rounds, round number read on the device) and replayed */
+    uint32_t ve_arena;     /* LDS words of a 64-lane BVE detection block: 8192 or 4096 (twice the
+                              blocks per CU; variables that do not fit together wait a round) */
+    uint32_t init_grid;    /* 256-thread blocks of LCVE's record-building pass (0: clause_grid): fewer
+                              blocks keep each XCD's window of the rank table in its L2 */
 } pfg_tuning;
 int pfg_set_tuning(pfg_ctx *ctx, const pfg_tuning *t);
 int pfg_get_tuning(pfg_ctx *ctx, pfg_tuning *t);

@@ -44,7 +44,8 @@ class Times(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """launch and algorithm choices (pfg_tuning); results never depend on them"""
     _fields_ = [("ot_build", ctypes.c_int32), ("clause_grid", ctypes.c_uint32), ("sub_group", ctypes.c_uint32),
-                ("stage_words", ctypes.c_uint32), ("lcve_graph", ctypes.c_int32)]
+                ("stage_words", ctypes.c_uint32), ("lcve_graph", ctypes.c_int32), ("ve_arena", ctypes.c_uint32),
+                ("init_grid", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

@@ -517,46 +517,66 @@ bool prop(Ctx &c)
 
 // ------------------------------------------------------------------ batches
 // live clauses and literals per instance (instance of a clause = instance of its
-// first literal's variable; empty clauses, left only in refuted instances, are not counted). A block takes a tile of consecutive clauses; counts of
-// clauses of the tile's first instance are summed over the block and added once,
-// the others (a tile crossing an instance boundary) are added per clause.
+// first literal's variable; empty clauses, left only in refuted instances, are not
+// counted). A lane keeps the variable range of the instance it counts for: clauses of
+// one instance are mostly contiguous in the pool, so the binary search over the
+// instance bases runs once per run of clauses. A lane's sums go out when its instance
+// changes inside a tile, and after every tile as one atomic pair per wave when all the
+// wave's counting lanes share the instance (the usual case).
 constexpr int CI_BLOCK = 256, CI_ITEMS = 16;
+
+__device__ __forceinline__ void ci_flush_wave(unsigned long long *icnt, uint32_t cur, unsigned long long &nc,
+                                              unsigned long long &nl)
+{
+    const uint64_t act = __ballot(nc != 0);
+    if (!act) return;
+    const uint32_t i0 = (uint32_t)__shfl((int)cur, __ffsll((long long)act) - 1, 64);
+    if (__ballot(nc != 0 && cur != i0) == 0) {
+        unsigned long long a = nc, b = nl;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_down(a, o, 64);
+            b += __shfl_down(b, o, 64);
+        }
+        if (__lane_id() == 0) {
+            atomicAdd(&icnt[2 * i0], a);
+            atomicAdd(&icnt[2 * i0 + 1], b);
+        }
+    }
+    else if (nc) {
+        atomicAdd(&icnt[2 * cur], nc);
+        atomicAdd(&icnt[2 * cur + 1], nl);
+    }
+    nc = nl = 0;
+}
+
 __global__ void __launch_bounds__(CI_BLOCK) k_count_inst(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs,
                                                         uint64_t n, const uint32_t *__restrict__ ibase, uint32_t ninst,
                                                         unsigned long long *__restrict__ icnt)
 {
-    __shared__ uint32_t s_first;
+    uint32_t cur = 0, lo = 1, hi = 0;   // instance counted for; its variables are (lo, hi]
+    unsigned long long nc = 0, nl = 0;
     for (uint64_t t0 = (uint64_t)blockIdx.x * CI_BLOCK * CI_ITEMS; t0 < n; t0 += (uint64_t)gridDim.x * CI_BLOCK * CI_ITEMS) {
-        if (threadIdx.x == 0) {
-            uint32_t f = 0xFFFFFFFFu;
-            for (uint64_t i = t0; i < n && i < t0 + CI_BLOCK * CI_ITEMS; i++) {   // first live clause of the tile
-                const uint32_t *cl = mem + refs[i];
-                if (!c_del(cl) && c_size(cl)) { f = find_inst(ibase, ninst, cl[3] >> 1); break; }
-            }
-            s_first = f;
-        }
-        __syncthreads();
-        const uint32_t first = s_first;
-        unsigned long long nc = 0, nl = 0;
         for (int k = 0; k < CI_ITEMS; k++) {
             const uint64_t i = t0 + (uint64_t)k * CI_BLOCK + threadIdx.x;
             if (i >= n) break;
             const uint32_t *cl = mem + refs[i];
             if (c_del(cl) || !c_size(cl)) continue;   // an empty clause (refuted instance) names no instance
-            const uint32_t inst = find_inst(ibase, ninst, cl[3] >> 1);
-            if (inst == first) { nc++; nl += (unsigned)c_size(cl); }
-            else {
-                atomicAdd(&icnt[2 * inst], 1ull);
-                atomicAdd(&icnt[2 * inst + 1], (unsigned long long)c_size(cl));
+            const uint32_t v = cl[3] >> 1;
+            if (v <= lo || v > hi) {
+                if (nc) {
+                    atomicAdd(&icnt[2 * cur], nc);
+                    atomicAdd(&icnt[2 * cur + 1], nl);
+                    nc = nl = 0;
+                }
+                cur = find_inst(ibase, ninst, v);
+                lo = ibase[cur];
+                hi = cur + 1 < ninst ? ibase[cur + 1] : 0xFFFFFFFFu;
             }
+            nc++;
+            nl += (unsigned)c_size(cl);
         }
-        nc = block_sum(nc);
-        nl = block_sum(nl);
-        if (threadIdx.x == 0 && nc) {
-            atomicAdd(&icnt[2 * first], nc);
-            atomicAdd(&icnt[2 * first + 1], nl);
-        }
-        __syncthreads();
+        ci_flush_wave(icnt, cur, nc, nl);
     }
 }
 
@@ -1232,8 +1252,8 @@ struct KeyLess {
 };
 
 // libstdc++ std::sort (introsort) order, restated for the device
-template <typename LT>
-__device__ void is_adjust_heap(uint64_t *f, int64_t hole, int64_t len, uint64_t value, LT lt)
+template <typename T, typename LT>
+__device__ void is_adjust_heap(T *f, int64_t hole, int64_t len, T value, LT lt)
 {
     const int64_t top = hole;
     int64_t child = hole;
@@ -1257,8 +1277,8 @@ __device__ void is_adjust_heap(uint64_t *f, int64_t hole, int64_t len, uint64_t 
     f[hole] = value;
 }
 
-template <typename LT>
-__device__ void is_heapsort(uint64_t *f, int64_t len, LT lt)
+template <typename T, typename LT>
+__device__ void is_heapsort(T *f, int64_t len, LT lt)
 {
     if (len >= 2) {
         for (int64_t parent = (len - 2) / 2;; parent--) {
@@ -1268,52 +1288,52 @@ __device__ void is_heapsort(uint64_t *f, int64_t len, LT lt)
     }
     while (len > 1) {
         len--;
-        const uint64_t v = f[len];
+        const T v = f[len];
         f[len] = f[0];
         is_adjust_heap(f, 0, len, v, lt);
     }
 }
 
-template <typename LT>
-__device__ void is_linear_insert(uint64_t *last, LT lt)
+template <typename T, typename LT>
+__device__ void is_linear_insert(T *last, LT lt)
 {
-    const uint64_t v = *last;
-    uint64_t *next = last - 1;
+    const T v = *last;
+    T *next = last - 1;
     while (lt(v, *next)) { *last = *next; last = next; --next; }
     *last = v;
 }
 
-template <typename LT>
-__device__ void is_insertion(uint64_t *first, uint64_t *last, LT lt)
+template <typename T, typename LT>
+__device__ void is_insertion(T *first, T *last, LT lt)
 {
     if (first == last) return;
-    for (uint64_t *i = first + 1; i != last; ++i) {
+    for (T *i = first + 1; i != last; ++i) {
         if (lt(*i, *first)) {
-            const uint64_t v = *i;
-            for (uint64_t *p = i; p != first; --p) *p = *(p - 1);
+            const T v = *i;
+            for (T *p = i; p != first; --p) *p = *(p - 1);
             *first = v;
         }
         else is_linear_insert(i, lt);
     }
 }
 
-template <typename LT>
-__device__ void introsort_dev(uint64_t *d, int64_t n, LT lt)
+template <typename T, typename LT>
+__device__ void introsort_dev(T *d, int64_t n, LT lt)
 {
     if (n < 2) return;
-    struct Frame { int64_t f, l, depth; };
+    struct Frame { uint32_t f, l, depth; };   // lists hold fewer than 2^32 entries
     Frame st[72];
     int sp = 0;
     int64_t lg = 63 - __clzll((long long)n);
-    st[sp++] = {0, n, 2 * lg};
+    st[sp++] = {0u, (uint32_t)n, (uint32_t)(2 * lg)};
     while (sp) {
         Frame fr = st[--sp];
         int64_t f = fr.f, l = fr.l, depth = fr.depth;
         while (l - f > 16) {
             if (depth == 0) { is_heapsort(d + f, l - f, lt); break; }
             --depth;
-            uint64_t *first = d + f, *last = d + l;
-            uint64_t *a = first + 1, *b = first + (l - f) / 2, *cc = last - 1, *pick;
+            T *first = d + f, *last = d + l;
+            T *a = first + 1, *b = first + (l - f) / 2, *cc = last - 1, *pick;
             if (lt(*a, *b)) {
                 if (lt(*b, *cc)) pick = b;
                 else if (lt(*a, *cc)) pick = cc;
@@ -1322,24 +1342,24 @@ __device__ void introsort_dev(uint64_t *d, int64_t n, LT lt)
             else if (lt(*a, *cc)) pick = a;
             else if (lt(*b, *cc)) pick = cc;
             else pick = b;
-            { const uint64_t t = *first; *first = *pick; *pick = t; }
-            uint64_t *lo = first + 1, *hi = last;
+            { const T t = *first; *first = *pick; *pick = t; }
+            T *lo = first + 1, *hi = last;
             for (;;) {
                 while (lt(*lo, *first)) ++lo;
                 --hi;
                 while (lt(*first, *hi)) --hi;
                 if (!(lo < hi)) break;
-                const uint64_t t = *lo; *lo = *hi; *hi = t;
+                const T t = *lo; *lo = *hi; *hi = t;
                 ++lo;
             }
             const int64_t cut = lo - d;
-            st[sp++] = {cut, l, depth};   // the right part is sorted independently
+            st[sp++] = {(uint32_t)cut, (uint32_t)l, (uint32_t)depth};   // the right part is sorted independently
             l = cut;
         }
     }
     if (n > 16) {
         is_insertion(d, d + 16, lt);
-        for (uint64_t *i = d + 16; i != d + n; ++i) is_linear_insert(i, lt);
+        for (T *i = d + 16; i != d + n; ++i) is_linear_insert(i, lt);
     }
     else is_insertion(d, d + n, lt);
 }
@@ -1375,9 +1395,25 @@ __device__ __forceinline__ void short_key(const uint32_t *mem, uint64_t r, uint6
     lo = ((uint64_t)last << 32) | c[1];
 }
 
+// A longer list is sorted by the introsort over entries that carry their key (the two
+// words of short_key) next to the reference, in a region of a scratch buffer taken by
+// an atomic: the introsort makes the comparisons KeyLess would make, with the same
+// outcomes, so it moves the entries exactly as it would move the references, while a
+// comparison reads the entries instead of four scattered clause words each.
+struct SortEnt {
+    uint64_t hi, lo, ref;
+};
+struct EntLess {
+    __device__ bool operator()(const SortEnt &a, const SortEnt &b) const
+    {
+        return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
+    }
+};
+
 __global__ void __launch_bounds__(64) k_sort_pv_lists(const uint32_t *__restrict__ pvs, uint32_t npvs,
                                                      const OL *__restrict__ ot, uint64_t *occ,
-                                                     const uint32_t *__restrict__ mem, int stable, uint64_t *tmp)
+                                                     const uint32_t *__restrict__ mem, int stable, uint64_t *tmp,
+                                                     SortEnt *ents, unsigned long long *nents)
 {
     __shared__ uint64_t s_hi[SORT_SHORT][64], s_lo[SORT_SHORT][64], s_ref[SORT_SHORT][64];
     const int lane = threadIdx.x;
@@ -1439,8 +1475,31 @@ __global__ void __launch_bounds__(64) k_sort_pv_lists(const uint32_t *__restrict
             for (int i = 0; i < (int)n; i++) d[i] = s_ref[i][lane];
         }
         else if (stable) stable_sort_dev(d, tmp + o.off, n, lt);
-        else introsort_dev(d, n, lt);
+        else {
+            SortEnt *e = ents + atomicAdd(nents, (unsigned long long)n);
+            for (int64_t i = 0; i < n; i++) {
+                const uint64_t r = d[i];
+                uint64_t hi, lo;
+                short_key(mem, r, hi, lo);
+                e[i] = SortEnt{hi, lo, r};
+            }
+            introsort_dev(e, n, EntLess{});
+            for (int64_t i = 0; i < n; i++) d[i] = e[i].ref;
+        }
     }
+}
+
+// entries of the lists longer than SORT_SHORT (the introsort's scratch entries)
+__global__ void k_long_entries(const uint32_t *__restrict__ pvs, uint32_t npvs, const OL *__restrict__ ot,
+                               unsigned long long *out)
+{
+    unsigned long long acc = 0;
+    for (uint64_t q = gtid(); q < 2ull * npvs; q += gstride()) {
+        const uint32_t sz = ot[2 * pvs[q >> 1] + (uint32_t)(q & 1)].sz;
+        if (sz > (uint32_t)SORT_SHORT) acc += sz;
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0 && acc) atomicAdd(out, acc);
 }
 
 __global__ void k_pv_bytes(const uint32_t *__restrict__ pvs, uint32_t b, uint32_t e, const OL *__restrict__ ot,
@@ -1476,10 +1535,23 @@ void sort_ot(Ctx &c)
 {
     if (!c.npvs) return;
     uint64_t *tmp = nullptr;
+    SortEnt *ents = nullptr;
     if (c.opts.stable_sort) { ensure(c, c.tmpu64b, c.occ.cap); tmp = c.tmpu64b.p; }
+    else {
+        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));
+        k_long_entries<<<grid_for(2ull * c.npvs, 256, 2048), 256, 0, c.stream>>>(c.pvs.p, c.npvs, c.ot.p,
+                                                                                (unsigned long long *)c.cnt.p + CNT_A);
+        read_counters(c, CNT_A + 1);
+        const uint64_t ne = c.hcnt[CNT_A];
+        if (ne) {
+            ensure(c, c.tmpu64b, 3 * ne);   // SortEnt = 3 words
+            ents = (SortEnt *)c.tmpu64b.p;
+        }
+    }
     KTimer kt(c, K_SORT_PV, 2.0 * pv_bytes(c, 0, c.npvs, false));   // lists read and written once
     k_sort_pv_lists<<<grid_for(2ull * c.npvs, 64, 16384), 64, 0, c.stream>>>(c.pvs.p, c.npvs, c.ot.p, c.occ.p, c.mem.p,
-                                                                              c.opts.stable_sort, tmp);
+                                                                              c.opts.stable_sort, tmp, ents,
+                                                                              (unsigned long long *)c.cnt.p + CNT_B);
 }
 
 }  // namespace pfg

@@ -94,6 +94,8 @@ struct Tuning {
     uint32_t clause_grid = 2048;   // blocks of 256 threads for the XCD-windowed clause passes
     uint32_t sub_group = 16;       // lanes per variable in the first subsumption configuration
     int32_t lcve_graph = 0;        // LCVE election rounds replayed from a captured graph
+    uint32_t ve_arena = 8192;      // LDS words of a BVE detection block (4096: twice the blocks per CU)
+    uint32_t init_grid = 0;        // blocks of the LCVE record-building pass (0: clause_grid)
 };
 
 struct KStat {

@@ -465,7 +465,8 @@ uint32_t lcve(Ctx &c)
     if (c.nrefs) {
         KTimer kt(c, K_LCVE_INIT, 20.0 * c.nrefs + 8.0 * c.nliterals);
         PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_D, 0, 8, c.stream));
-        k_elect_init<<<xcd_grid(c.nrefs, 256, c.tune.clause_grid), 256, 0, c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act,
+        k_elect_init<<<xcd_grid(c.nrefs, 256, c.tune.init_grid ? c.tune.init_grid : c.tune.clause_grid), 256, 0,
+                       c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act,
                                                                           ra + 1, (unsigned long long *)c.cnt.p);
     }
     uint64_t hw[2] = {0, 0};

@@ -1012,7 +1012,7 @@ int pfg_set_tuning(pfg_ctx *ctx, const pfg_tuning *t)
     if (!ctx || !t) return PFG_EINVAL;
     Ctx &c = ctx->c;
     if (t->ot_build < 0 || t->ot_build > 1 || t->lcve_graph < 0 || t->lcve_graph > 1 || !t->clause_grid ||
-        t->clause_grid > (1u << 20) ||
+        t->clause_grid > (1u << 20) || t->init_grid > (1u << 20) || !(t->ve_arena == 4096 || t->ve_arena == 8192) ||
         !(t->sub_group == 8 || t->sub_group == 16 || t->sub_group == 32 || t->sub_group == 64)) {
         c.last_error = "tuning value out of range";
         return PFG_EINVAL;
@@ -1021,6 +1021,8 @@ int pfg_set_tuning(pfg_ctx *ctx, const pfg_tuning *t)
     c.tune.clause_grid = (t->clause_grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     c.tune.sub_group = t->sub_group;
     c.tune.lcve_graph = t->lcve_graph;
+    c.tune.ve_arena = t->ve_arena;
+    c.tune.init_grid = (t->init_grid + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     c.stage_words = t->stage_words;
     return PFG_OK;
 }
@@ -1034,6 +1036,8 @@ int pfg_get_tuning(pfg_ctx *ctx, pfg_tuning *t)
     t->sub_group = c.tune.sub_group;
     t->stage_words = c.stage_words;
     t->lcve_graph = c.tune.lcve_graph;
+    t->ve_arena = c.tune.ve_arena;
+    t->init_grid = c.tune.init_grid;
     return PFG_OK;
 }
 

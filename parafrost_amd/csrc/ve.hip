@@ -43,7 +43,7 @@ __device__ __forceinline__ uint32_t *cl_at(const VeArgs &A, uint32_t lit, uint32
     return A.mem + A.occ[A.ot[lit].off + i];
 }
 
-constexpr uint32_t VE_ARENA_WORDS = 8192;   // LDS words per 64-lane block for staged clauses (32 KB)
+constexpr uint32_t VE_ARENA_WORDS = 8192;   // LDS words per 64-lane block for staged clauses (32 KB) at most
 
 // Header word writes keep the staged copy and the pool copy equal: a staged clause
 // writes through to the pool; a pool clause that is also staged (reached through
@@ -608,10 +608,13 @@ __device__ __forceinline__ void ve_detect_var(const VeArgs &A, const VeDetectOut
 // fit in the block's arena together; a variable larger than the whole arena is
 // detected from the pool instead. Detection of one variable touches only its own
 // clauses, so the rounds do not change results.
+template <uint32_t ARENA>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
 k_ve_detect(VeArgs G, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t npvs, uint32_t arena_words, int fun_en)
 {
-    __shared__ uint32_t arena[VE_ARENA_WORDS];
+    static_assert(ARENA <= VE_ARENA_WORDS, "arena larger than the staged-pointer check");
+    __shared__ uint32_t arena[ARENA];
+    if (arena_words > ARENA) arena_words = ARENA;
     const StageSrc S{G.mem, G.ot, G.occ};
     // elected variables come in increasing score order: the heaviest blocks go first
     const uint64_t nblk = (npvs + 63) / 64;
@@ -967,8 +970,12 @@ void ve(Ctx &c)
     {
         const VeDetectOut DO{type, aux, nres, nunits, nwords, nmodel};
         KTimer kt(c, K_VE_DETECT, pv_bytes(c, 0, np, true));
-        k_ve_detect<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np,
-                                                                  std::min(VE_ARENA_WORDS, c.stage_words), c.opts.ve_fun_en);
+        if (c.tune.ve_arena == 4096)
+            k_ve_detect<4096><<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np, c.stage_words,
+                                                                            c.opts.ve_fun_en);
+        else
+            k_ve_detect<8192><<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np, c.stage_words,
+                                                                            c.opts.ve_fun_en);
     }
     if (c.opts.ve_fun_en) {
         const VeDetectOut DO{type, aux, nres, nunits, nwords, nmodel};

@@ -1,6 +1,7 @@
 """Every tuning gives the same results: the occurrence-table build (radix sort or
-counting), the XCD-window grid of the clause passes and the subsumption lane group
-are compared with the oracle pass by pass, single instances and batches."""
+counting), the XCD-window grid of the clause passes, the subsumption lane group, the
+BVE detection arena and the LCVE record-pass grid are compared with the oracle pass by
+pass, single instances and batches."""
 import numpy as np
 import pytest
 
@@ -11,7 +12,8 @@ pytestmark = pytest.mark.gpu
 
 TUNINGS = [dict(ot_build=1), dict(ot_build=1, clause_grid=8), dict(sub_group=8), dict(sub_group=32),
            dict(sub_group=64), dict(ot_build=1, sub_group=8, clause_grid=8192), dict(lcve_graph=1),
-           dict(lcve_graph=1, clause_grid=8)]
+           dict(lcve_graph=1, clause_grid=8), dict(ve_arena=4096), dict(ve_arena=4096, init_grid=8),
+           dict(init_grid=512, ve_arena=4096, stage_words=1500)]
 
 
 @pytest.mark.parametrize("k", range(len(TUNINGS)))
@@ -68,6 +70,8 @@ def test_tuning_rejects_bad_values():
             g.set_tuning(sub_group=12)
         with pytest.raises(pf.SimplifierError):
             g.set_tuning(ot_build=2)
+        with pytest.raises(pf.SimplifierError):
+            g.set_tuning(ve_arena=2048)
         assert g.tuning()["sub_group"] in (8, 16, 32, 64)
     finally:
         g.close()
