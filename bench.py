@@ -293,6 +293,7 @@ def run_c4(args):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     kt = s.kernel_times()
+    pass_ms = {k: round(v / args.steps, 2) for k, v in s.times().items()}
     s.reset_times()
     step(account=True)
     ka = s.kernel_times()
@@ -353,7 +354,7 @@ def run_c4(args):
                    "[1000, 200000], ratios U[4.0,4.3]/U[9.5,10.0], seed 4",
                    "instances": B.ninst, "variables": B.nvars, "clauses": B.ncls, "input_literals": B.nlits,
                    "batches": len(groups), "batch_literal_budget": args.batch_lits, "refuted_instances": unsat,
-                   "generation_s": round(gen_s, 1)},
+                   "generation_s": round(gen_s, 1), "pass_ms_per_step": pass_ms},
         "plain": {"value": plain, "unit": "literals/s",
                   "sample": f"{len(pick)} instances spread over the batch, {plain_l} literals, one context each, "
                             "second of two runs timed"},

@@ -175,59 +175,81 @@ __global__ void k_ere_pairs(EreArgs A, const uint32_t *__restrict__ pvs, uint32_
     }
 }
 
+// last q in [lo, hi) with poff[q] <= g
+__device__ __forceinline__ uint32_t pair_owner(const uint64_t *poff, uint32_t lo, uint32_t hi, uint64_t g)
+{
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (poff[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// pair g = poff[q] + i * |other| + j of elected variable q
+__device__ __forceinline__ void ere_probe_pair(const EreArgs &A, const uint32_t *pvs, const uint64_t *poff, uint32_t q,
+                                               uint64_t g, uint32_t *flag, bool account, unsigned long long &bytes)
+{
+    const uint32_t x = pvs[q];
+    uint32_t dx = 2 * x, fx = dx + 1;
+    if (A.ot[dx].sz > A.ot[fx].sz) { dx = fx; fx = 2 * x; }
+    const OL me = A.ot[dx], other = A.ot[fx];
+    const uint64_t r = g - poff[q];
+    const uint32_t i = (uint32_t)(r / other.sz), j = (uint32_t)(r % other.sz);
+    const uint32_t *ci = A.mem + A.occ[me.off + i];
+    const uint32_t *cj = A.mem + A.occ[other.off + j];
+    if (c_del(ci) || c_del(cj)) return;
+    EreRes R;
+    if (!ere_resolvent<false>(A, x, ci, cj, R)) return;
+    const uint64_t off = A.ot[R.best].off;
+    // the probe asks whether any clause matches, so four candidates' headers are
+    // loaded together and checked before the merge
+    bool hit = false;
+    uint32_t k = 0;
+    for (; k + 4 <= R.minsize && !hit; k += 4) {
+        const uint32_t *c0 = A.mem + A.occ[off + k], *c1 = A.mem + A.occ[off + k + 1],
+                       *c2 = A.mem + A.occ[off + k + 2], *c3 = A.mem + A.occ[off + k + 3];
+        const uint32_t *cc[4] = {c0, c1, c2, c3};
+        const uint32_t h[4] = {c0[0], c1[0], c2[0], c3[0]}, z[4] = {c0[2], c1[2], c2[2], c3[2]},
+                       gs[4] = {c0[1], c1[1], c2[1], c3[1]};
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool head = (int)z[u] == R.len && ((h[u] & ST_LEARNT) || (h[u] & 3u) == R.type) &&
+                              sig_sub(R.msig, gs[u]);
+            if (account) bytes += 20ull + (!hit && head ? 4ull * (uint64_t)R.len : 0ull);
+            if (!hit && head && merge_ere(x, ci, cj, cc[u])) hit = true;
+        }
+    }
+    for (; k < R.minsize && !hit; k++) {
+        const uint32_t *t = A.mem + A.occ[off + k];
+        hit = ere_match(A, x, ci, cj, R, t);
+        if (account) bytes += 20ull + (hit ? 4ull * (uint64_t)R.len : 0ull);
+    }
+    if (hit) {
+        flag[q] = 1u;
+        atomicAdd(&A.cnt[CNT_C], 1ull);
+    }
+}
+
+// A block takes 256 consecutive pairs: two threads find the variables of its first
+// and last pair, and every thread then searches only between them (a few variables,
+// their offsets cached), not over all of poff.
 // acct (accounting only): adds the bytes of the resolvent checks -- per candidate
 // examined its list entry (8) and header (12), per candidate whose size, status and
 // signature pass the literals compared (4 per literal of the resolvent)
-__global__ void k_ere_probe(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs, const uint64_t *__restrict__ poff,
-                            uint64_t total, uint32_t *__restrict__ flag, unsigned long long *acct)
+__global__ void __launch_bounds__(256) k_ere_probe(EreArgs A, const uint32_t *__restrict__ pvs, uint32_t npvs,
+                                                  const uint64_t *__restrict__ poff, uint64_t total,
+                                                  uint32_t *__restrict__ flag, unsigned long long *acct)
 {
+    __shared__ uint32_t s_q[2];
     unsigned long long bytes = 0;
-    for (uint64_t g = gtid(); g < total; g += gstride()) {
-        uint32_t lo = 0, hi = npvs;   // last q with poff[q] <= g
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (poff[mid] <= g) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t q = lo, x = pvs[q];
-        uint32_t dx = 2 * x, fx = dx + 1;
-        if (A.ot[dx].sz > A.ot[fx].sz) { dx = fx; fx = 2 * x; }
-        const OL me = A.ot[dx], other = A.ot[fx];
-        const uint64_t r = g - poff[q];
-        const uint32_t i = (uint32_t)(r / other.sz), j = (uint32_t)(r % other.sz);
-        const uint32_t *ci = A.mem + A.occ[me.off + i];
-        const uint32_t *cj = A.mem + A.occ[other.off + j];
-        if (c_del(ci) || c_del(cj)) continue;
-        EreRes R;
-        if (!ere_resolvent<false>(A, x, ci, cj, R)) continue;
-        const uint64_t off = A.ot[R.best].off;
-        // the probe asks whether any clause matches, so four candidates' headers are
-        // loaded together and checked before the merge
-        bool hit = false;
-        uint32_t k = 0;
-        for (; k + 4 <= R.minsize && !hit; k += 4) {
-            const uint32_t *c0 = A.mem + A.occ[off + k], *c1 = A.mem + A.occ[off + k + 1],
-                           *c2 = A.mem + A.occ[off + k + 2], *c3 = A.mem + A.occ[off + k + 3];
-            const uint32_t *cc[4] = {c0, c1, c2, c3};
-            const uint32_t h[4] = {c0[0], c1[0], c2[0], c3[0]}, z[4] = {c0[2], c1[2], c2[2], c3[2]},
-                           g[4] = {c0[1], c1[1], c2[1], c3[1]};
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const bool head = (int)z[u] == R.len && ((h[u] & ST_LEARNT) || (h[u] & 3u) == R.type) &&
-                                  sig_sub(R.msig, g[u]);
-                if (acct) bytes += 20ull + (!hit && head ? 4ull * (uint64_t)R.len : 0ull);
-                if (!hit && head && merge_ere(x, ci, cj, cc[u])) hit = true;
-            }
-        }
-        for (; k < R.minsize && !hit; k++) {
-            const uint32_t *t = A.mem + A.occ[off + k];
-            hit = ere_match(A, x, ci, cj, R, t);
-            if (acct) bytes += 20ull + (hit ? 4ull * (uint64_t)R.len : 0ull);
-        }
-        if (hit) {
-            flag[q] = 1u;
-            atomicAdd(&A.cnt[CNT_C], 1ull);
-        }
+    for (uint64_t b = (uint64_t)blockIdx.x * 256; b < total; b += (uint64_t)gridDim.x * 256) {
+        const uint64_t last = b + 255 < total ? b + 255 : total - 1;
+        if (threadIdx.x < 2) s_q[threadIdx.x] = pair_owner(poff, 0, npvs, threadIdx.x ? last : b);
+        __syncthreads();
+        const uint64_t g = b + threadIdx.x;
+        if (g < total) ere_probe_pair(A, pvs, poff, pair_owner(poff, s_q[0], s_q[1] + 1, g), g, flag, acct != nullptr, bytes);
+        __syncthreads();
     }
     if (acct) {
         bytes = block_sum(bytes);
