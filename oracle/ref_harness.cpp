@@ -26,7 +26,8 @@
  *   length in bytes (u64) when the simplifier starts, so that the simplifier's part
  *   of the proof can be told from the parser's and shrinkTop's lines
  * env PFREF_TIME=1: print "PFREF_TIME <seconds>" to stderr, the wall time of the
- *   simplifier (rootify .. shrinkSimp; not the parse, not the dump)
+ *   simplifier (rootify .. shrinkSimp, or .. the refutation; not the parse, not the
+ *   dump of a simplified run; a refuted run's time includes its dump)
  * env PFREF_FREEZE=<file>: u32 variables (little endian) frozen as by ifreeze()
  *   (src/cpu/assume.cpp:45-52) with the solver in incremental mode, so that
  *   iassumed() (src/cpu/solve.h:864-867) holds for them during LCVE
@@ -215,10 +216,11 @@ int main(int argc, char** argv)
 		return 0;
 	};
 	const auto t_start = std::chrono::steady_clock::now();
-	if (sigmifying(out, "")) return 0;
-	if (getenv("PFREF_TIME"))
+	const int outcome = sigmifying(out, "");
+	if (getenv("PFREF_TIME"))   /* a refuted run includes the dump of its final state */
 		fprintf(stderr, "PFREF_TIME %.6f\n",
 			std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count());
+	if (outcome) return 0;
 	if (getenv("PFREF_MAP")) {
 		s->map(true);
 		dump_mapped(s, out);
