@@ -53,7 +53,6 @@ void ere(Ctx &c);
 // appends them; proof_units appends the units trail[from, to) of a propagation
 struct ProofLog;
 ProofLog proof_begin(Ctx &c, uint32_t np, uint64_t cap);
-void proof_restart(Ctx &c, uint32_t np);   // drop what was logged since proof_begin
 void proof_end(Ctx &c, uint32_t np, bool decide);
 void proof_units(Ctx &c, uint32_t from, uint32_t to);
 void proof_empty(Ctx &c);                  // the empty clause; nothing is written after it

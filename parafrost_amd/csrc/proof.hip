@@ -177,13 +177,6 @@ ProofLog proof_begin(Ctx &c, uint32_t np, uint64_t cap)
     return L;
 }
 
-void proof_restart(Ctx &c, uint32_t np)
-{
-    if (!proof_on(c)) return;
-    PFG_HIP(hipMemsetAsync(c.pnev.p, 0, 4ull * ((uint64_t)np + 1), c.stream));
-    PFG_HIP(hipMemsetAsync(c.pcnt.p, 0, 8, c.stream));
-}
-
 void proof_empty(Ctx &c)
 {
     if (!proof_on(c)) return;
