@@ -73,7 +73,7 @@ constexpr uint64_t CONF_CAP = 1ull << 20;
 enum KId {
     K_UPLOAD, K_HIST, K_OT_FILL, K_OT_SORT, K_PROP, K_LCVE_RADIX, K_LCVE_INIT, K_LCVE_PASS_A, K_LCVE_PASS_B,
     K_SORT_PV, K_SUB, K_VE_DETECT, K_VE_APPLY, K_BCE_COUNT, K_BCE_APPLY, K_ERE, K_GC, K_COUNT, K_SUB_REST,
-    K_COUNT_INST, K_LCVE_GRAPH, K_VE_FUN, K_NUM
+    K_COUNT_INST, K_LCVE_GRAPH, K_VE_FUN, K_VE_RES, K_NUM
 };
 extern const char *const kernel_names[K_NUM];
 extern const char *const kernel_symbols[K_NUM];

@@ -58,11 +58,11 @@ PassTimer::~PassTimer()
 const char *const kernel_names[K_NUM] = {
     "upload_prep", "hist", "ot_fill", "ot_sort", "prop", "lcve_radix", "lcve_init", "lcve_pass_a", "lcve_pass_b",
     "sort_pv_lists", "sub", "ve_detect", "ve_apply", "bce_count", "bce_apply", "ere_probe", "gc_move", "count",
-    "sub_rest", "count_inst", "lcve_graph", "ve_fun"};
+    "sub_rest", "count_inst", "lcve_graph", "ve_fun", "ve_res"};
 const char *const kernel_symbols[K_NUM] = {
     "k_upload", nullptr, nullptr, nullptr, nullptr, nullptr, "k_elect_init", "k_elect_pass_a", "k_elect_pass_b",
     "k_sort_pv_lists", "k_sub_grp", "k_ve_detect", "k_ve_apply", nullptr, nullptr, "k_ere_probe", "k_gc_move", "k_count",
-    nullptr, "k_count_inst", nullptr, "k_ve_fun"};
+    nullptr, "k_count_inst", nullptr, "k_ve_fun", "k_ve_res"};
 
 KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
 {

@@ -21,8 +21,12 @@
 
 namespace pfg {
 
-// T_FUNPEND: no gate of the other kinds; the function-table gate (k_ve_fun) decides
-enum { T_NONE = 0, T_PURE = 1, T_BN = 2, T_SUBST = 3, T_RES = 4, T_CORE = 5, T_FUNPEND = 6 };
+// T_FUNPEND: no gate of the other kinds; the function-table gate (k_ve_fun) decides.
+// T_RESPEND: no gate; plain resolution is counted by a lane group (k_ve_res).
+enum { T_NONE = 0, T_PURE = 1, T_BN = 2, T_SUBST = 3, T_RES = 4, T_CORE = 5, T_FUNPEND = 6, T_RESPEND = 7 };
+
+// resolvent pairs from which plain resolution is counted by a lane group instead of the detecting lane
+constexpr uint64_t RES_WAVE_MIN = 128;
 
 struct VeArgs {
     uint32_t *mem;
@@ -585,7 +589,10 @@ __device__ __forceinline__ void ve_detect_var(const VeArgs &A, const VeDetectOut
                     t = T_FUNPEND;
                     a = nAdded != 0;
                 }
-                else if (!nAdded && !count_subst(A, v, nOrgCls, p, n, R_ALL, nAdded, words, units)) t = T_RES;
+                else if (!nAdded) {
+                    if ((uint64_t)A.ot[p].sz * A.ot[n].sz >= RES_WAVE_MIN) t = T_RESPEND;
+                    else if (!count_subst(A, v, nOrgCls, p, n, R_ALL, nAdded, words, units)) t = T_RES;
+                }
                 if (t == T_SUBST || t == T_RES) {
                     nr = (uint32_t)nAdded;
                     nw = words;
@@ -780,6 +787,144 @@ __global__ void __launch_bounds__(64) k_ve_fun(VeArgs A, VeDetectOut O, const ui
                 O.nunits[q] = t != T_NONE ? units : 0u;
                 O.nmodel[q] = t != T_NONE ? model_words(A, p, pOrgs, nOrgs) : 0ull;
             }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ plain resolution by lane groups
+// countResolvents (src/cpu/simplify.h:255-287) for the variables detection left
+// pending (at least RES_WAVE_MIN resolvent pairs): a group of RES_G lanes takes one
+// variable, RES_G pairs of |occ(x)| x |occ(~x)| at a time; a wave runs 64 / RES_G
+// variables at once. The sequential count fails at the first resolvent that makes the
+// count exceed the clauses before or is longer than the clause limit; that happens iff
+// the total count exceeds the bound or some counted resolvent is too long, so the group
+// checks both after every step and the decision and the counts on success are the same.
+constexpr int RES_G = 16;
+
+template <typename T>
+__device__ __forceinline__ T group_total(T v)
+{
+#pragma unroll
+    for (int o = RES_G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, RES_G);
+    return v;
+}
+
+// LDS accesses of the wave's lanes ordered across the point (the wave runs in lockstep)
+__device__ __forceinline__ void res_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS per group: the variable's clause images [ref_lo, ref_hi, word0, sig, size, lits..]
+// (as staged by stage.hpp) and, per side, each entry's image offset; a variable that
+// does not fit is read from the pool
+constexpr uint32_t RES_WORDS = 1024, RES_ENT = 128;
+
+__global__ void __launch_bounds__(64) k_ve_res(VeArgs A, VeDetectOut O, const uint32_t *__restrict__ pvs, uint64_t npvs)
+{
+    constexpr uint32_t NG = 64 / RES_G;
+    __shared__ uint32_t img_all[NG][RES_WORDS];
+    __shared__ uint16_t pos_all[NG][2][RES_ENT];
+    const uint32_t lane = threadIdx.x, gl = lane % RES_G, g = lane / RES_G;
+    const uint64_t gmask = (RES_G == 64 ? ~0ull : ((1ull << RES_G) - 1ull)) << (g * RES_G);
+    uint32_t *img = img_all[g];
+    uint16_t(*pos)[RES_ENT] = pos_all[g];
+    for (uint64_t base = (uint64_t)blockIdx.x * 64; base < npvs; base += (uint64_t)gridDim.x * 64) {
+        const bool pend = base + lane < npvs && (O.type[base + lane] & 0xFFu) == T_RESPEND;
+        uint64_t todo = __ballot(pend);
+        while (todo) {
+            // group g takes the g-th pending variable of this step
+            uint64_t t = todo;
+            for (uint32_t k = 0; k < g && t; k++) t &= t - 1;
+            for (uint32_t k = 0; k < NG && todo; k++) todo &= todo - 1;
+            if (!t) continue;
+            const uint64_t q = base + (uint64_t)__builtin_ctzll(t);
+            const uint32_t v = pvs[q], p = 2 * v, n = p + 1;
+            const OL lp = A.ot[p], ln = A.ot[n];
+            // stage the images (prefix sums of their sizes over the group), counting originals
+            const uint32_t ne = lp.sz + ln.sz;
+            bool fits = lp.sz <= RES_ENT && ln.sz <= RES_ENT;
+            uint32_t po = 0, no = 0, total_w = 0;
+            unsigned long long before = 0;
+            for (uint32_t k0 = 0; k0 < ne; k0 += RES_G) {
+                const uint32_t k = k0 + gl;
+                uint64_t r = 0;
+                uint32_t w = 0;
+                if (k < ne) {
+                    r = k < lp.sz ? A.occ[lp.off + k] : A.occ[ln.off + (k - lp.sz)];
+                    const uint32_t *c = A.mem + r;
+                    const uint32_t sz = c[2];
+                    if (c_orig(c)) {
+                        (k < lp.sz ? po : no)++;
+                        before += sz;
+                    }
+                    w = 5u + sz;
+                }
+                uint32_t incl = w;
+#pragma unroll
+                for (int o = 1; o < RES_G; o <<= 1) {
+                    const uint32_t x = __shfl_up(incl, o, RES_G);
+                    if (gl >= (uint32_t)o) incl += x;
+                }
+                const uint32_t at = total_w + incl - w;
+                if (fits && k < ne && at + w <= RES_WORDS) {
+                    const uint32_t *src = A.mem + r;
+                    img[at] = (uint32_t)r;
+                    img[at + 1] = (uint32_t)(r >> 32);
+                    for (uint32_t x = 0; x + 2 < w; x++) img[at + 2 + x] = src[x];
+                    if (k < lp.sz) pos[0][k] = (uint16_t)(at + 2);
+                    else pos[1][k - lp.sz] = (uint16_t)(at + 2);
+                }
+                total_w += __shfl(incl, RES_G - 1, RES_G);
+                fits = fits && total_w <= RES_WORDS;
+            }
+            res_wave_sync();   // the group's images are in LDS before any lane reads them
+            po = group_total(po);
+            no = group_total(no);
+            before = group_total(before);
+            const uint32_t clsbefore = po + no;
+            const uint64_t npairs = (uint64_t)lp.sz * ln.sz;
+            uint32_t added = 0, unitc = 0, total = 0;
+            unsigned long long lits = 0;
+            bool over = false, fail = false;
+            uint32_t i = gl / ln.sz, j = gl % ln.sz;   // pair b0 + gl = i |occ(~x)| + j, advanced by RES_G
+            const uint32_t di = RES_G / ln.sz, dj = RES_G % ln.sz;
+            for (uint64_t b0 = 0; b0 < npairs && !fail; b0 += RES_G) {
+                if (b0 + gl < npairs) {
+                    const uint32_t *ci = fits ? img + pos[0][i] : A.mem + A.occ[lp.off + i];
+                    const uint32_t *cj = fits ? img + pos[1][j] : A.mem + A.occ[ln.off + j];
+                    if (c_orig(ci) && c_orig(cj)) {
+                        const int rs = merge_len(v, ci, cj);
+                        if (rs > 1) {
+                            added++;
+                            lits += (unsigned)rs;
+                            if (A.rlimit && rs > A.rlimit) over = true;
+                        }
+                        else if (rs == 1) unitc++;
+                    }
+                }
+                i += di;
+                j += dj;
+                if (j >= ln.sz) { j -= ln.sz; i++; }
+                total = group_total(added);
+                fail = total > clsbefore || (__ballot(over) & gmask) != 0;
+            }
+            if (!fail) {
+                lits = group_total(lits);
+                unitc = group_total(unitc);
+                if (A.lbound && lits > before) fail = true;
+            }
+            if (gl == 0) {
+                O.type[q] = fail ? T_NONE : T_RES;
+                O.aux[q] = (uint32_t)(po > no);   // model side
+                O.nres[q] = fail ? 0u : total;
+                O.nwords[q] = fail ? 0ull : 3ull * total + lits;
+                O.nunits[q] = fail || !total ? 0u : unitc;
+                O.nmodel[q] = fail ? 0ull : model_words(A, p, (int)po, (int)no);
+            }
+            res_wave_sync();   // reads of this variable's images end before the next staging
         }
     }
 }
@@ -981,6 +1126,11 @@ void ve(Ctx &c)
         const VeDetectOut DO{type, aux, nres, nunits, nwords, nmodel};
         KTimer kt(c, K_VE_FUN, pv_bytes(c, 0, np, true));
         k_ve_fun<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np);
+    }
+    {
+        const VeDetectOut DO{type, aux, nres, nunits, nwords, nmodel};
+        KTimer kt(c, K_VE_RES, 0.0);
+        k_ve_res<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np);
     }
     // exclusive scans with the totals stored one past the end
     scan_u32_to_u64(c, nres, roff, np, roff + np);
