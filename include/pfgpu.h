@@ -90,6 +90,12 @@ int pfg_sigmify_batch(pfg_ctx *ctx, int32_t *status);
  * in its own numbering; NULL buffers query the sizes */
 int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *nwords, uint64_t *ncls, uint32_t *resolved,
                        uint64_t *nres, uint8_t *vstate, uint32_t *trail, uint32_t *ntrail);
+/* No counterpart in the reference (its write-back copies, transfer.cu:104-179): the clause
+ * images and model records of pfg_batch_download without the copy -- pointers into the
+ * context's pinned host memory, valid until the next relayout, upload, sigmify, reset or
+ * destroy of the context. For serving loops that hand results on or copy only what they keep. */
+int pfg_batch_view(pfg_ctx *ctx, uint32_t inst, const uint32_t **words, uint64_t *nwords, uint64_t *ncls,
+                   const uint32_t **resolved, uint64_t *nres);
 /* Write-back for incremental calls, kept in HBM (cacheCNF/writeBack, src/gpu/transfer.cu:44-179,
  * without the host copy): one instance's clauses after pfg_sigmify_batch as CSR in its own
  * numbering, into caller-owned device memory: d_offs gets ncls + 1 offsets (from 0), d_lits

@@ -65,7 +65,7 @@ API = ["pfg_default_opts", "pfg_create", "pfg_destroy", "pfg_upload_cnf", "pfg_u
        "pfg_set_accounting", "pfg_set_stage_words", "pfg_create_batch", "pfg_batch_ninst", "pfg_batch_relayout", "pfg_upload_batch",
        "pfg_sigmify_batch", "pfg_batch_download", "pfg_ifreeze", "pfg_batch_export", "pfg_download_mapped",
        "pfg_download_proof", "pfg_interrupt", "pfg_set_vorg", "pfg_path_counts", "pfg_set_tuning", "pfg_get_tuning",
-       "pfg_batch_export_noalloc"]
+       "pfg_batch_export_noalloc", "pfg_batch_view"]
 
 
 def lib_path() -> str:
@@ -124,6 +124,8 @@ def lib():
                                          ctypes.POINTER(u64), vp, vp, ctypes.POINTER(u32)]
         L.pfg_batch_export.argtypes = [vp, u32, vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.pfg_batch_export_noalloc.argtypes = [vp, u32, vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        pp32 = ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32))
+        L.pfg_batch_view.argtypes = [vp, u32, pp32, ctypes.POINTER(u64), ctypes.POINTER(u64), pp32, ctypes.POINTER(u64)]
         L.pfg_reset.argtypes = [vp]
         L.pfg_interrupt.argtypes = [vp]
         L.pfg_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
@@ -507,6 +509,24 @@ class BatchSimplifier(Simplifier):
         self._check(self.L.pfg_batch_export(self.h, i, lits.data_ptr(), offs.data_ptr(), ctypes.byref(nl),
                                             ctypes.byref(nc)))
         return lits[:nl.value], offs
+
+    def instance_view(self, i: int):
+        """instance(i) without copying the clause words and model records: arrays that view
+        the context's pinned host memory, valid until its next relayout, upload or sigmify
+        (trail and variable states are copies)"""
+        pw, pr = ctypes.POINTER(ctypes.c_uint32)(), ctypes.POINTER(ctypes.c_uint32)()
+        nw, nc, nr = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.L.pfg_batch_view(self.h, i, ctypes.byref(pw), ctypes.byref(nw), ctypes.byref(nc),
+                                          ctypes.byref(pr), ctypes.byref(nr)))
+        w = np.ctypeslib.as_array(pw, (nw.value,)) if nw.value else np.zeros(0, np.uint32)
+        r = np.ctypeslib.as_array(pr, (nr.value,)) if nr.value else np.zeros(0, np.uint32)
+        nt = ctypes.c_uint32()
+        self._check(self.L.pfg_batch_download(self.h, i, None, None, None, None, None, None, None, ctypes.byref(nt)))
+        t = np.empty(nt.value, np.uint32)
+        vs = np.zeros(int(self.inst_nvars[i]) + 1, np.uint8)
+        self._check(self.L.pfg_batch_download(self.h, i, None, None, None, None, None, vs.ctypes.data, t.ctypes.data,
+                                              ctypes.byref(nt)))
+        return w, t, r, vs
 
     def instance(self, i: int, pool=None):
         """(clause words, trail, model records, variable states) of instance i in its own numbering;

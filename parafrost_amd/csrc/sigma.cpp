@@ -790,6 +790,24 @@ int pfg_batch_download(pfg_ctx *ctx, uint32_t inst, uint32_t *words, uint64_t *n
     API_END
 }
 
+// pfg_batch_download's clause images and model records of one instance, in place in the
+// pinned cache (valid until the batch changes)
+int pfg_batch_view(pfg_ctx *ctx, uint32_t inst, const uint32_t **words, uint64_t *nwords, uint64_t *ncls,
+                   const uint32_t **resolved, uint64_t *nres)
+{
+    API_BEGIN
+    if (inst >= c.ninst) return PFG_EINVAL;
+    batch_cache(c);
+    const uint64_t w0 = c.cache_start[inst], w1 = c.cache_start[inst + 1];
+    const uint64_t r0 = c.cache_rstart[inst], r1 = c.cache_rstart[inst + 1];
+    if (words) *words = c.cache_mem.p + w0;
+    if (nwords) *nwords = w1 - w0;
+    if (ncls) *ncls = c.cache_cstart[inst + 1] - c.cache_cstart[inst];
+    if (resolved) *resolved = c.pin_res.p + r0;
+    if (nres) *nres = r1 - r0;
+    API_END
+}
+
 // one instance's clauses as CSR in caller-owned device memory, in the instance's
 // numbering (clause order of the download); NULL buffers only query the sizes
 int pfg_batch_export(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint64_t *d_offs, uint64_t *nlits, uint64_t *ncls)

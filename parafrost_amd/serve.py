@@ -31,9 +31,10 @@ pass; requests that arrive meanwhile wait for the next free slot, so the work of
 the slots' batches overlaps on the device. The resident store has an HBM budget: the
 least recently used idle instances beyond it are moved to host memory and
 brought back by their next request. Latency of a request = results on the host
-- arrival. Results (clauses, units, model records, variable states) land in host
-memory each slot reuses from batch to batch, valid until the slot's next batch;
-keep_results=True gives every request arrays of its own instead.
+- arrival. Results (clauses, units, model records, variable states) are host arrays:
+the clause words and model records view the slot context's pinned host memory,
+where the batch's results are downloaded in one copy, valid until the slot's next
+batch; keep_results=True gives every request arrays of its own instead.
 """
 from __future__ import annotations
 
@@ -243,8 +244,6 @@ class Server:
     def _serve(self, cap_vars: int):
         torch = self.torch
         ctx = self.ctx_cls([cap_vars], self.opts)
-        # results not kept: downloaded into this slot's reused host memory, valid until its next batch
-        pool = None if self.keep else HostPool()
         if self.dev.type == "cuda":
             torch.cuda.set_device(self.dev)
         try:
@@ -293,11 +292,11 @@ class Server:
                 td = time.perf_counter()
                 new = []
                 t_dl = t_ex = 0.0
-                if pool is not None:
-                    pool.reset()
                 for i, (r, res, _) in enumerate(work):
                     t0 = time.perf_counter()
-                    w, t, rec, vs = ctx.instance(i, pool)
+                    # results not kept: clause words and model records viewed in the slot context's
+                    # pinned memory, valid until its next batch (as the reused host memory was)
+                    w, t, rec, vs = ctx.instance(i) if self.keep else ctx.instance_view(i)
                     t1 = time.perf_counter()
                     t_dl += t1 - t0
                     r.status = int(st[i])

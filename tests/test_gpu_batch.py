@@ -184,6 +184,24 @@ def _csr(words):
     return (np.concatenate(lits) if lits else np.zeros(0, np.uint32)), np.array(offs, np.uint64)
 
 
+def test_batch_view_matches_download():
+    """pfg_batch_view: the clause words and model records in place in the pinned cache equal
+    the downloaded copies, for every instance (refuted ones included)."""
+    import parafrost_amd as pf
+    rng = np.random.default_rng(4750)
+    cnfs = [_instance(rng, k) for k in range(11)] + [_full_3cnf()]
+    g = pf.BatchSimplifier([c.nvars for c in cnfs], pf.default_opts())
+    try:
+        g.upload_instances(cnfs)
+        g.sigmify_batch()
+        for i in range(len(cnfs)):
+            a, b = g.instance(i), g.instance_view(i)
+            for x, y, what in zip(a, b, ("words", "trail", "records", "vstate")):
+                assert x.dtype == y.dtype and np.array_equal(x, y), f"instance {i}: {what}"
+    finally:
+        g.close()
+
+
 def test_batch_export_matches_download():
     """The HBM write-back of every instance is the CSR form of its downloaded clauses."""
     import torch
