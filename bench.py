@@ -17,8 +17,8 @@ Prints one JSON line. Extra objects:
   plain, vs_plain -- the plain per-instance path on a spread sample of the instances
   served          -- the serving workload: literals / s, p50 / p99 latency per offered
                      load, and its plain chain
-  writeback_ms    -- downloading every instance's results (clauses, model records,
-                     variable states, units) to host memory once, outside the timed steps
+  writeback_ms    -- every instance's results (clauses, model records, variable states,
+                     units) brought to host memory once, outside the timed steps
   roofline        -- HBM roofline of the dominant kernel: algorithmic bytes per launch
                      (computed on the device by one extra, untimed step) over the
                      kernel's average HIP-event duration, against 8 TB/s
@@ -303,16 +303,16 @@ def run_c4(args):
         if roof["traffic"] is not None:
             roof["traffic_source"] = "profiles/traffic_c4.txt (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE)"
     unsat = int(sum(int((x != 0).sum()) for x in st))
-    # write-back: every instance's results to host memory (reused arrays), outside the timed steps
-    pool = pf.HostPool()
+    # write-back: every instance's results to host memory, outside the timed steps: the batch's
+    # clause images and model records grouped by instance and copied once into the context's
+    # pinned memory (each instance's arrays viewed there), units and variable states copied
     wb = {"s": 0.0, "words": 0}
 
     def download(n):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for i in range(n):
-            pool.reset()
-            w, t, r, vs = s.instance(i, pool)
+            w, t, r, vs = s.instance_view(i)
             wb["words"] += len(w) + len(t) + len(r) + len(vs) // 4
         wb["s"] += time.perf_counter() - t1
 
@@ -361,8 +361,9 @@ def run_c4(args):
         "vs_plain": value / plain if plain else None,
         "tuning": tuning,
         "writeback_ms": round(wb["s"] * 1e3, 1),
-        "writeback": f"every instance's clauses, model records, variable states and units downloaded into reused "
-                     f"host memory after its batch ({wb['words']} words), once, outside the timed steps",
+        "writeback": f"every instance's clauses, model records, variable states and units in host memory after its "
+                     f"batch ({wb['words']} words): one grouped copy per batch into pinned memory, instances viewed "
+                     f"there; once, outside the timed steps",
         "roofline": roof, "kernels": kernels,
     }
     if not args.no_cpu_baseline:
@@ -377,8 +378,10 @@ def run_c5(args):
     seed 5), replayed as a Poisson stream at each offered load. Each request simplifies
     its instance's resident simplified clauses plus its new frames (units enqueued,
     interface variables frozen). Three slots of continuous batching (--c5-slots), 2.5e8 literals per
-    batch. Latency = results on the host - arrival (both paths download results into
-    host memory they reuse from call to call). The plain path runs the same
+    batch. Latency = results on the host - arrival (the served path's in its slot
+    contexts' pinned memory, the plain chain's in host memory it reuses). The
+    tuning is the record for this workload (parafrost_amd/tuning/c5.json; --autotune
+    measures it: each candidate is a served run at the last offered load). The plain path runs the same
     request chain one request at a time in a single-instance context with the state
     on the host; both count the literals of the formulas they simplify (equal)."""
     import parafrost_amd as pf  # noqa: F401  (loads torch and the library first)
@@ -390,12 +393,21 @@ def run_c5(args):
     # warm-up: load the code objects
     w = serve.make_requests(3, seed=99, gates=(2000, 4000), max_frames=4)
     serve.Server(slots=2).run(w, rate=1000.0)
+
+    def reset():
+        for r in reqs:
+            r.done = r.arrival = 0.0
+            r.status, r.result, r.nlits = -1, None, 0
+
+    tuning = serve_tuning(args, reqs, rates[-1], reset)
+    knobs = {k: v for k, v in tuning.items() if k != "source"}
     sweep = []
     for rate in rates:
         for r in reqs:
             r.done = r.arrival = 0.0
             r.status, r.result, r.nlits = -1, None, 0
-        st = serve.Server(slots=args.c5_slots, batch_lits=args.c5_batch_lits).run(reqs, rate=rate)
+        st = serve.Server(slots=args.c5_slots, batch_lits=args.c5_batch_lits, tuning=knobs).run(reqs, rate=rate)
+        release_device_memory()
         sweep.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()})
         print(f"# c5 rate {rate}: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms "
               f"{st['literals_per_s'] / 1e6:.1f} M literals/s ({st['literals']} literals)", file=sys.stderr, flush=True)
@@ -426,8 +438,53 @@ def run_c5(args):
                   "path": "the same request chain one at a time: host formula uploaded, single-instance "
                           "context, results downloaded, clauses written back on the host"},
         "vs_plain": best["literals_per_s"] / plain,
+        "tuning": tuning,
     }
     return out
+
+
+def release_device_memory():
+    """drop a finished server's resident instances (device tensors) before the next run"""
+    import gc
+    import torch
+    gc.collect()
+    torch.cuda.empty_cache()
+
+
+def serve_tuning(args, reqs, rate, reset):
+    """the slots' tuning for the serving workload: --no-tuning the library defaults,
+    --autotune measured now (coordinate descent, each candidate a served run at `rate`),
+    otherwise the record parafrost_amd/tuning/c5.json when it matches the device"""
+    import parafrost_amd as pf
+    from parafrost_amd import serve, tune
+    probe = pf.BatchSimplifier([16], pf.default_opts())
+    defaults = probe.tuning()
+    probe.close()
+    if args.no_tuning:
+        return {"source": "library defaults", **defaults}
+
+    class Slots:   # the tuning every slot context of the next run gets
+        def __init__(self):
+            self.t = dict(defaults)
+
+        def tuning(self):
+            return dict(self.t)
+
+        def set_tuning(self, **kw):
+            self.t.update(kw)
+
+    slots = Slots()
+    if args.autotune:
+        def step():
+            reset()
+            serve.Server(slots=args.c5_slots, batch_lits=args.c5_batch_lits, tuning=slots.t).run(reqs, rate=rate)
+            release_device_memory()
+        rec = tune.tune(slots, step, lambda: None, "c5", reps=1, workload="served c5 workload",
+                        log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+        return {"source": f"measured now ({len(rec['trials'])} candidates)", **slots.tuning()}
+    rec = tune.apply(slots, "c5")
+    src = "parafrost_amd/tuning/c5.json" if rec else f"library defaults (no record of c5 for {tune._device_arch()})"
+    return {"source": src, **slots.tuning()}
 
 
 def setup_tuning(args, s, step, name, workload):
@@ -522,7 +579,7 @@ def run_headline(args):
     c5 = run_c5(args)
     out["served"] = {"workload": c5["config"]["workload"], "path": c5["config"]["path"],
                      "value": c5["value"], "unit": "literals/s", "p50_ms": c5["p50_ms"], "p99_ms": c5["p99_ms"],
-                     "sweep": c5["served"], "plain": c5["plain"], "vs_plain": c5["vs_plain"]}
+                     "sweep": c5["served"], "plain": c5["plain"], "vs_plain": c5["vs_plain"], "tuning": c5["tuning"]}
     out["config"]["served_workload"] = c5["config"]["workload"]
     return out
 

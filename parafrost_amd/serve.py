@@ -153,9 +153,10 @@ class Server:
     store_bytes: HBM budget of the resident instances."""
 
     def __init__(self, slots: int = 2, batch_lits: float = 2.5e8, store_bytes: float = 64e9, opts=None,
-                 keep_results: bool = False, device: str = "cuda:0", context=None):
+                 keep_results: bool = False, device: str = "cuda:0", context=None, tuning=None):
         """context: the batch context class (BatchSimplifier; tests of the request loop
-        itself pass a host stand-in together with device="cpu")."""
+        itself pass a host stand-in together with device="cpu"); tuning: launch choices
+        (pfg_tuning fields) set on every slot context."""
         import torch
         self.torch = torch
         self.dev = torch.device(device)
@@ -167,6 +168,7 @@ class Server:
         self.batch_lits = batch_lits
         self.store_bytes = store_bytes
         self.opts = opts or default_opts()
+        self.tuning = dict(tuning or {})
         self.keep = keep_results
         self.store: dict[int, Resident] = {}
         self.store_used = 0
@@ -241,9 +243,15 @@ class Server:
         if self.dev.type == "cuda":
             self.torch.cuda.current_stream().synchronize()
 
+    def _context(self, cap_vars: int):
+        ctx = self.ctx_cls([cap_vars], self.opts)
+        if self.tuning:
+            ctx.set_tuning(**self.tuning)
+        return ctx
+
     def _serve(self, cap_vars: int):
         torch = self.torch
-        ctx = self.ctx_cls([cap_vars], self.opts)
+        ctx = self._context(cap_vars)
         if self.dev.type == "cuda":
             torch.cuda.set_device(self.dev)
         try:
@@ -267,7 +275,7 @@ class Server:
                     base = gen.batch_bases(nv)
                     if int(base[-1]) > ctx.nvars_cap:
                         ctx.close()
-                        ctx = self.ctx_cls([int(base[-1])], self.opts)
+                        ctx = self._context(int(base[-1]))
                     parts_l, parts_o, at = [], [torch.zeros(1, dtype=torch.int64, device=self.dev)], 0
                     for (r, _, (l, o)), b in zip(work, base[:-1]):
                         r.nlits = int(l.numel())
