@@ -294,8 +294,15 @@ struct Sigma {
                 fprintf(stderr, "[pfg] batch round %d: %u of %u instances running, literals %lld of %lld\n", round, run, ni,
                         (long long)lits_run, (long long)lits_all);
             }
-            const int times = round + 1;   // compaction keeps clause order, so its schedule is free
-            if (times > 1 && c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0) shrinkSimp();
+            // Compaction keeps clause order, so its schedule is free: between rounds it runs
+            // only when deleted images take a tenth of the pool or more (the passes skip
+            // deleted clauses either way); the call ends with one as always.
+            const int times = round + 1;
+            if (times > 1 && c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0) {
+                uint64_t live = 0;
+                for (uint32_t i = 0; i < ni; i++) live += 3ull * (uint64_t)nc[i] + (uint64_t)nl[i];
+                if (10 * live < 9 * c.msz) shrinkSimp();
+            }
             createOT();
             propagate();
             collect_conflicts(st);
