@@ -427,21 +427,87 @@ void reduce_ot(Ctx &c)
 // batch's units (and of their negations) are claimed once and brought to the state
 // the sequential loop leaves them in: satisfied -> deleted, false literals removed
 // (order kept, signature recomputed), new units assigned and appended to the trail.
-__global__ void k_prop_collect(const uint32_t *__restrict__ trail, uint32_t b0, uint32_t b1, const OL *__restrict__ ot,
-                               const uint64_t *__restrict__ occ, uint32_t *__restrict__ claim,
-                               uint64_t *__restrict__ cand, unsigned long long *cnt)
+// claim the clause at r; fresh claims go out with one atomic per wave
+__device__ __forceinline__ void prop_claim(bool valid, uint64_t r, uint32_t *__restrict__ claim, uint64_t *__restrict__ cand,
+                                           unsigned long long *cnt)
 {
-    const uint32_t nu = b1 - b0;
-    for (uint64_t q = blockIdx.x; q < 2ull * nu; q += gridDim.x) {
-        const uint32_t u = trail[b0 + (uint32_t)(q >> 1)];
-        const uint32_t l = (q & 1) ? u : (u ^ 1u);
-        const OL o = ot[l];
-        for (uint32_t k = threadIdx.x; k < o.sz; k += blockDim.x) {
-            const uint64_t r = occ[o.off + k];
-            const uint64_t key = r >> 1;
-            const uint32_t bit = 1u << (key & 31u);
-            const uint32_t old = atomicOr(&claim[key >> 5], bit);
-            if (!(old & bit)) cand[atomicAdd(&cnt[CNT_A], 1ull)] = r;
+    bool fresh = false;
+    if (valid) {
+        const uint64_t key = r >> 1;
+        const uint32_t bit = 1u << (key & 31u);
+        fresh = !(atomicOr(&claim[key >> 5], bit) & bit);
+    }
+    const uint64_t fm = __ballot(fresh);
+    unsigned long long at = 0;
+    if (__lane_id() == 0 && fm) at = atomicAdd(&cnt[CNT_A], (unsigned long long)__popcll(fm));
+    at = __shfl(at, 0, 64);
+    if (fresh) cand[at + __popcll(fm & ((1ull << __lane_id()) - 1ull))] = r;
+}
+
+// A wave takes 64 lists (a unit's literal and its negation, for 32 units): their
+// entries are walked as one run, a lane per entry and step (a lane finds its entry's
+// list by a binary search over the wave's prefix of list sizes), so short lists do
+// not leave lanes idle. Lists longer than PROP_LONG go to a worklist (count in
+// CNT_B) that k_prop_collect_long walks a block per list.
+constexpr uint32_t PROP_LONG = 1024;
+__global__ void __launch_bounds__(256) k_prop_collect(const uint32_t *__restrict__ trail, uint32_t b0, uint32_t b1,
+                                                     const OL *__restrict__ ot, const uint64_t *__restrict__ occ,
+                                                     uint32_t *__restrict__ claim, uint64_t *__restrict__ cand,
+                                                     unsigned long long *cnt, uint32_t *__restrict__ longq)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    const uint64_t nq = 2ull * (b1 - b0);
+    for (uint64_t base = wave * 64; base < nq; base += nwaves * 64) {
+        const uint64_t q = base + lane;
+        uint64_t off = 0;
+        uint32_t sz = 0;
+        if (q < nq) {
+            const uint32_t u = trail[b0 + (uint32_t)(q >> 1)];
+            const OL o = ot[(q & 1) ? u : (u ^ 1u)];
+            off = o.off;
+            sz = o.sz;
+        }
+        wave_append(sz > PROP_LONG, (uint32_t)q, longq, &cnt[CNT_B]);
+        if (sz > PROP_LONG) sz = 0;
+        uint32_t incl = sz;
+#pragma unroll
+        for (int k = 1; k < 64; k <<= 1) {
+            const uint32_t t = __shfl_up(incl, k, 64);
+            if ((int)lane >= k) incl += t;
+        }
+        const uint32_t pre = incl - sz, total = __shfl(incl, 63, 64);
+        const uint32_t olo = (uint32_t)off, ohi = (uint32_t)(off >> 32);
+        for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+            const uint32_t e = t0 + lane;
+            uint32_t m = 0;   // the last list whose prefix is at most e
+#pragma unroll
+            for (uint32_t step = 32; step > 0; step >>= 1)
+                if ((uint32_t)__shfl((int)pre, (int)(m + step), 64) <= e) m += step;
+            const uint32_t pm = (uint32_t)__shfl((int)pre, (int)m, 64);
+            const uint64_t om = ((uint64_t)(uint32_t)__shfl((int)ohi, (int)m, 64) << 32) |
+                                (uint32_t)__shfl((int)olo, (int)m, 64);
+            const uint64_t r = e < total ? occ[om + (e - pm)] : 0ull;
+            prop_claim(e < total, r, claim, cand, cnt);
+        }
+    }
+}
+
+// the long lists, a block each (the worklist's count is read on the device)
+__global__ void __launch_bounds__(256) k_prop_collect_long(const uint32_t *__restrict__ trail, uint32_t b0,
+                                                          const OL *__restrict__ ot, const uint64_t *__restrict__ occ,
+                                                          uint32_t *__restrict__ claim, uint64_t *__restrict__ cand,
+                                                          unsigned long long *cnt, const uint32_t *__restrict__ longq)
+{
+    const uint64_t nl = cnt[CNT_B];
+    for (uint64_t k = blockIdx.x; k < nl; k += gridDim.x) {
+        const uint32_t q = longq[k];
+        const uint32_t u = trail[b0 + (q >> 1)];
+        const OL o = ot[(q & 1) ? u : (u ^ 1u)];
+        for (uint32_t e0 = 0; e0 < o.sz; e0 += blockDim.x) {
+            const uint32_t e = e0 + threadIdx.x;
+            prop_claim(e < o.sz, e < o.sz ? occ[o.off + e] : 0ull, claim, cand, cnt);
         }
     }
 }
@@ -494,9 +560,12 @@ bool prop(Ctx &c)
     ensure(c, c.tmpu64b, c.occ.cap);
     while (c.propagated < c.ntrail) {
         const uint32_t b0 = c.propagated, b1 = c.ntrail;
-        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 8, c.stream));
-        k_prop_collect<<<grid_for(2ull * (b1 - b0) * 64, 64, 8192), 64, 0, c.stream>>>(
-            c.trail.p, b0, b1, c.ot.p, c.occ.p, c.claim.p, c.tmpu64b.p, (unsigned long long *)c.cnt.p);
+        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));   // CNT_A candidates, CNT_B long lists
+        ensure(c, c.plong, 2ull * (b1 - b0));
+        k_prop_collect<<<grid_for(2ull * (b1 - b0), 256, 8192), 256, 0, c.stream>>>(
+            c.trail.p, b0, b1, c.ot.p, c.occ.p, c.claim.p, c.tmpu64b.p, (unsigned long long *)c.cnt.p, c.plong.p);
+        k_prop_collect_long<<<(unsigned)std::min<uint64_t>(2ull * (b1 - b0), 1024), 256, 0, c.stream>>>(
+            c.trail.p, b0, c.ot.p, c.occ.p, c.claim.p, c.tmpu64b.p, (unsigned long long *)c.cnt.p, c.plong.p);
         read_counters(c, 2);
         const uint64_t nc = c.hcnt[CNT_A];
         if (nc) k_prop_apply<<<grid_for(nc, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.tmpu64b.p, nc, c.value.p, c.vstate.p,

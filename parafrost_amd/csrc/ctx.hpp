@@ -149,6 +149,7 @@ struct Ctx {
     DBuf<uint64_t> tmpu64a, tmpu64b, tmpu64c;
     DBuf<uint32_t> owner;       // per (ref >> 1): ERE target claims (all-ones when idle)
     DBuf<uint32_t> claim;       // bitmap over (ref >> 1): prop claims
+    DBuf<uint32_t> plong;       // prop: the long lists of a batch of units
     DBuf<uint32_t> erec;        // LCVE: per-clause candidate-rank records
     DBuf<uint32_t> ufirst;      // per variable: first unit occurrence (all-ones when idle)
     DBuf<uint64_t> cnt;         // small device counters

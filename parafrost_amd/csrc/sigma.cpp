@@ -485,7 +485,7 @@ int pfg_destroy(pfg_ctx *ctx)
     release(c, c.trail); release(c, c.resolved); release(c, c.mchunk); release(c, c.img_off); release(c, c.pvs); release(c, c.units);
     release(c, c.scores); release(c, c.eligible); release(c, c.rank); release(c, c.tmpu32a); release(c, c.tmpu32b); release(c, c.tmpu32c); release(c, c.tmpu32d);
     release(c, c.tmpu8); release(c, c.tmpu64a); release(c, c.tmpu64b); release(c, c.tmpu64c);
-    release(c, c.owner); release(c, c.claim); release(c, c.erec); release(c, c.ufirst); release(c, c.cnt);
+    release(c, c.owner); release(c, c.claim); release(c, c.plong); release(c, c.erec); release(c, c.ufirst); release(c, c.cnt);
     release(c, c.scan_tmp); release(c, c.radix_tmp);
     release(c, c.proof); release(c, c.pev); release(c, c.pord); release(c, c.pnev); release(c, c.pbytes);
     release(c, c.pcnt); release(c, c.poff); release(c, c.pboff); release(c, c.pfirst);
