@@ -575,6 +575,8 @@ __device__ __forceinline__ void sub_wave_body(const SubArgs &A, const uint32_t *
     uint16_t *msz = L.msz, *osz = L.osz;
     uint32_t *msig = L.msig, *osig = L.osig, *mbin = L.mbin, *obin = L.obin;
     const uint32_t lane = threadIdx.x;
+    // units this variable assigns are those past this point of the trail (k_sub_oblivion)
+    if (lane == 0 && A.defer_oblivion) A.cnt[CNT_SUBFROM] = A.cnt[CNT_NTRAIL];
     const uint32_t v = pvs[q], p = 2 * v;
     if (A.ot[p].sz > (uint32_t)A.sub_limit || A.ot[p + 1].sz > (uint32_t)A.sub_limit) return;
     if (lane == 0) s_seq = 0;   // proof events: lane 0 strengthens, one lane deletes, barriers between
@@ -842,7 +844,6 @@ void sub(Ctx &c)
         run_range(u);
         A.allow_units = 1;
         if (c.opts.sub_limit <= (int)SW_CAP) {
-            PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_SUBFROM, c.cnt.p + CNT_NTRAIL, 8, hipMemcpyDeviceToDevice, c.stream));
             A.defer_oblivion = 1;
             k_sub_wave<<<1, 64, 0, c.stream>>>(A, c.pvs.p, u);
             k_sub_oblivion<<<256, 256, 0, c.stream>>>(A);
@@ -852,7 +853,6 @@ void sub(Ctx &c)
         else {   // lists longer than the LDS arrays: the same wave with its arrays in HBM
             const uint32_t cap = (uint32_t)c.opts.sub_limit + 1;
             ensure(c, c.tmpu32d, 5ull * cap);
-            PFG_HIP(hipMemcpyAsync(c.cnt.p + CNT_SUBFROM, c.cnt.p + CNT_NTRAIL, 8, hipMemcpyDeviceToDevice, c.stream));
             A.defer_oblivion = 1;
             k_sub_wave_big<<<1, 64, 0, c.stream>>>(A, c.pvs.p, u, c.tmpu32d.p, cap);
             k_sub_oblivion<<<256, 256, 0, c.stream>>>(A);
