@@ -240,10 +240,11 @@ struct Sigma {
     // contradictions reported since the last call refute their instances
     void collect_conflicts(std::vector<uint8_t> &st)
     {
-        uint64_t h[2] = {0, 0};
-        PFG_HIP(hipMemcpyAsync(&h[0], c.cnt.p + CNT_FLAGS, 8, hipMemcpyDeviceToHost, c.stream));
-        PFG_HIP(hipMemcpyAsync(&h[1], c.cnt.p + CNT_CONF, 8, hipMemcpyDeviceToHost, c.stream));
+        // the flags and the contradiction count in one copy (slots CNT_FLAGS .. CNT_CONF)
+        uint64_t hc[CNT_CONF - CNT_FLAGS + 1];
+        PFG_HIP(hipMemcpyAsync(hc, c.cnt.p + CNT_FLAGS, sizeof(hc), hipMemcpyDeviceToHost, c.stream));
         PFG_HIP(hipStreamSynchronize(c.stream));
+        const uint64_t h[2] = {hc[0], hc[CNT_CONF - CNT_FLAGS]};
         if (!(h[0] & FLAG_UNSAT)) return;
         if (h[1] > CONF_CAP) throw Error(PFG_EINTERNAL, "more contradictions in one pass than the conflict list holds");
         std::vector<uint64_t> vars(h[1]);
