@@ -219,10 +219,7 @@ void ensure(Ctx &c, DBuf<T> &b, uint64_t n, bool keep = false)
     dalloc_raw(c, &np, ncap * sizeof(T));
     if (keep && b.p && b.cap)
         PFG_HIP(hipMemcpyAsync(np, b.p, b.cap * sizeof(T), hipMemcpyDeviceToDevice, c.stream));
-    if (b.p) {
-        if (keep) PFG_HIP(hipStreamSynchronize(c.stream));
-        dfree_raw(c, b.p, b.cap * sizeof(T));
-    }
+    if (b.p) dfree_raw(c, b.p, b.cap * sizeof(T));   // queued after the copy on the same stream
     b.p = (T *)np;
     b.cap = ncap;
 }

@@ -10,26 +10,46 @@
 #include <algorithm>
 #include <new>
 #include <cstdlib>
+#include <mutex>
+#include <set>
 
 namespace pfg {
+
+// Device buffers come from the device's stream-ordered pool on the context's stream:
+// a free waits for the work queued before it on that stream only, where hipFree would
+// wait for the whole device (other contexts' streams included: serving slots, a
+// caller's own work). The pool keeps freed blocks for reuse instead of returning them
+// to the driver at every synchronization.
+static void pool_setup(int device)
+{
+    static std::mutex mu;
+    static std::set<int> done;
+    std::lock_guard<std::mutex> lock(mu);
+    if (done.count(device)) return;
+    hipMemPool_t pool = nullptr;
+    PFG_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t keep = UINT64_MAX;
+    PFG_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    done.insert(device);
+}
 
 void dalloc_raw(Ctx &c, void **p, uint64_t bytes)
 {
     if (c.budget && c.allocated + bytes > c.budget)
         throw Error(PFG_ENOMEM, "HBM budget exceeded (" + std::to_string(c.allocated + bytes) + " > " +
                                     std::to_string(c.budget) + " bytes)");
-    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    hipError_t e = hipMallocAsync(p, bytes ? bytes : 1, c.stream);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         throw Error(e == hipErrorOutOfMemory ? PFG_ENOMEM : PFG_EDEVICE,
-                    std::string("hipMalloc: ") + hipGetErrorString(e));
+                    std::string("hipMallocAsync: ") + hipGetErrorString(e));
     }
     c.allocated += bytes;
 }
 
 void dfree_raw(Ctx &c, void *p, uint64_t bytes)
 {
-    (void)hipFree(p);
+    (void)hipFreeAsync(p, c.stream);
     c.allocated -= std::min(bytes, c.allocated);
 }
 
@@ -446,6 +466,7 @@ int pfg_create(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t nvars, c
     normalize_opts(c.opts);
     try {
         PFG_HIP(hipSetDevice(device));
+        pool_setup(device);
         PFG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         for (auto &e : c.ev) { PFG_HIP(hipEventCreate(&e[0])); PFG_HIP(hipEventCreate(&e[1])); }
         PFG_HIP(hipHostMalloc((void **)&c.hcnt, (CNT_CONF + 1) * sizeof(uint64_t), hipHostMallocDefault));
@@ -490,7 +511,8 @@ int pfg_destroy(pfg_ctx *ctx)
     release(c, c.scan_tmp); release(c, c.radix_tmp);
     release(c, c.proof); release(c, c.pev); release(c, c.pord); release(c, c.pnev); release(c, c.pbytes);
     release(c, c.pcnt); release(c, c.poff); release(c, c.pboff); release(c, c.pfirst);
-    release(c, c.ibase); release(c, c.imu); release(c, c.ib0); release(c, c.inel); release(c, c.icnt);
+    release(c, c.ibase); release(c, c.imu); release(c, c.ib0); release(c, c.inel); release(c, c.icnt); release(c, c.vorg);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);   // the frees are queued on the stream
     c.cache_mem.release(); c.pin_res.release(); c.pin_trail.release(); c.pin_refs.release(); c.pin_vstate.release();
     if (c.hcnt) (void)hipHostFree(c.hcnt);
     for (auto &e : c.kev) { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); }
