@@ -7,6 +7,7 @@
 #include <vector>
 #include <array>
 #include <atomic>
+#include <algorithm>
 #include "../../include/pfgpu.h"
 
 namespace pfg {
@@ -30,23 +31,29 @@ struct Error : std::runtime_error {
             throw ::pfg::Error(PFG_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Growable pinned host buffer of T (download staging).
+// Growable pinned host buffer of T (download staging). A grown buffer's old block is
+// freed only at release(): hipHostFree waits for the whole device, which would stall
+// the other contexts' streams (serving slots) at every growth. Growth is by half, so
+// the retired blocks add up to at most twice the current one.
 template <typename T>
 struct PinBuf {
     T *p = nullptr;
     uint64_t cap = 0;
+    std::vector<T *> retired;
     void ensure(uint64_t n)
     {
         if (n <= cap) return;
-        if (p) (void)hipHostFree(p);
+        if (p) retired.push_back(p);
         p = nullptr;
+        const uint64_t nc = std::max<uint64_t>(n + n / 4 + 1024, cap + cap / 2);
         cap = 0;
-        uint64_t nc = n + n / 4 + 1024;
         PFG_HIP(hipHostMalloc((void **)&p, nc * sizeof(T), hipHostMallocDefault));
         cap = nc;
     }
     void release()
     {
+        for (T *q : retired) (void)hipHostFree(q);
+        retired.clear();
         if (p) (void)hipHostFree(p);
         p = nullptr;
         cap = 0;
