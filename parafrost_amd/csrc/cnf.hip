@@ -512,8 +512,12 @@ __global__ void __launch_bounds__(256) k_prop_collect_long(const uint32_t *__res
     }
 }
 
+// hist (when live) stays the live-occurrence histogram for every variable LCVE reads:
+// a deleted clause's unassigned literals are taken off it. Assigned variables (the
+// false literals removed, the true ones) are passed over by LCVE, so theirs are left.
 __global__ void k_prop_apply(uint32_t *__restrict__ mem, const uint64_t *__restrict__ cand, uint64_t n, int8_t *value,
-                             uint8_t *vstate, uint32_t *trail, unsigned long long *cnt, uint32_t *__restrict__ claim)
+                             uint8_t *vstate, uint32_t *trail, unsigned long long *cnt, uint32_t *__restrict__ claim,
+                             uint32_t *__restrict__ hist)
 {
     for (uint64_t i = gtid(); i < n; i += gstride()) {
         const uint64_t r = cand[i];
@@ -526,7 +530,13 @@ __global__ void k_prop_apply(uint32_t *__restrict__ mem, const uint64_t *__restr
         bool sat = false;
         for (int k = 0; k < sz; k++)
             if (__atomic_load_n(&value[lits[k]], __ATOMIC_RELAXED) > 0) { sat = true; break; }
-        if (sat) { c_markdel(c); continue; }
+        if (sat) {
+            c_markdel(c);
+            if (hist)
+                for (int k = 0; k < sz; k++)
+                    if (__atomic_load_n(&value[lits[k]], __ATOMIC_RELAXED) < 0) atomicSub(&hist[lits[k]], 1u);
+            continue;
+        }
         int j = 0;
         uint32_t sig = 0;
         for (int k = 0; k < sz; k++) {
@@ -569,7 +579,8 @@ bool prop(Ctx &c)
         read_counters(c, 2);
         const uint64_t nc = c.hcnt[CNT_A];
         if (nc) k_prop_apply<<<grid_for(nc, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.tmpu64b.p, nc, c.value.p, c.vstate.p,
-                                                                             c.trail.p, (unsigned long long *)c.cnt.p, c.claim.p);
+                                                                             c.trail.p, (unsigned long long *)c.cnt.p, c.claim.p,
+                                                                             c.hist_live ? c.hist.p : nullptr);
         k_prop_clear<<<grid_for(b1 - b0, 256, 1024), 256, 0, c.stream>>>(c.trail.p, b0, b1, c.ot.p);
         read_counters(c, 10);
         c.propagated = b1;
@@ -579,7 +590,6 @@ bool prop(Ctx &c)
             if (!c.batch) return false;   // a batch propagates the other instances to the end
         }
     }
-    if (c.propagated != before) c.hist_live = false;
     if (c.propagated != before && !c.opts.sub_en) reduce_ot(c);
     return !(!c.batch && c.unsat);
 }
