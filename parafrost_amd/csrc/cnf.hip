@@ -552,8 +552,14 @@ __global__ void k_prop_apply(uint32_t *__restrict__ mem, const uint64_t *__restr
     }
 }
 
-__global__ void k_prop_clear(const uint32_t *__restrict__ trail, uint32_t b0, uint32_t b1, OL *__restrict__ ot)
+// empties the units' lists; resets the candidate and long-list counts for the next batch
+__global__ void k_prop_clear(const uint32_t *__restrict__ trail, uint32_t b0, uint32_t b1, OL *__restrict__ ot,
+                             unsigned long long *cnt)
 {
+    if (gtid() == 0) {
+        cnt[CNT_A] = 0;
+        cnt[CNT_B] = 0;
+    }
     for (uint64_t q = gtid() + b0; q < b1; q += gstride()) {
         const uint32_t u = trail[q];
         ot[u].sz = 0;
@@ -568,9 +574,9 @@ bool prop(Ctx &c)
     ensure(c, c.claim, (c.mem.cap >> 6) + 2);
     PFG_HIP(hipMemsetAsync(c.claim.p, 0, c.claim.cap * 4, c.stream));
     ensure(c, c.tmpu64b, c.occ.cap);
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));   // CNT_A candidates, CNT_B long lists (then k_prop_clear)
     while (c.propagated < c.ntrail) {
         const uint32_t b0 = c.propagated, b1 = c.ntrail;
-        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 16, c.stream));   // CNT_A candidates, CNT_B long lists
         ensure(c, c.plong, 2ull * (b1 - b0));
         k_prop_collect<<<grid_for(2ull * (b1 - b0), 256, 8192), 256, 0, c.stream>>>(
             c.trail.p, b0, b1, c.ot.p, c.occ.p, c.claim.p, c.tmpu64b.p, (unsigned long long *)c.cnt.p, c.plong.p);
@@ -581,7 +587,8 @@ bool prop(Ctx &c)
         if (nc) k_prop_apply<<<grid_for(nc, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.tmpu64b.p, nc, c.value.p, c.vstate.p,
                                                                              c.trail.p, (unsigned long long *)c.cnt.p, c.claim.p,
                                                                              c.hist_live ? c.hist.p : nullptr);
-        k_prop_clear<<<grid_for(b1 - b0, 256, 1024), 256, 0, c.stream>>>(c.trail.p, b0, b1, c.ot.p);
+        k_prop_clear<<<grid_for(b1 - b0, 256, 1024), 256, 0, c.stream>>>(c.trail.p, b0, b1, c.ot.p,
+                                                                          (unsigned long long *)c.cnt.p);
         read_counters(c, 10);
         c.propagated = b1;
         c.ntrail = (uint32_t)c.hcnt[CNT_NTRAIL];
