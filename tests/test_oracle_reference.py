@@ -34,6 +34,13 @@ def test_c1_full():
     _run(gen.config("c1"), po.default_opts())
 
 
+def test_c2_full():
+    """the 2M-variable mixed-width workload: the oracle's final state equals the reference's word for word
+    (about 25 s for the oracle, 50 s for the reference)"""
+    _run(gen.config("c2"), po.default_opts())
+
+
+
 @pytest.mark.parametrize("seed", range(12))
 def test_mid_size(seed):
     rng = np.random.default_rng(500 + seed)
