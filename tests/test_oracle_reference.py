@@ -40,7 +40,6 @@ def test_c2_full():
     _run(gen.config("c2"), po.default_opts())
 
 
-
 @pytest.mark.parametrize("seed", range(12))
 def test_mid_size(seed):
     rng = np.random.default_rng(500 + seed)
