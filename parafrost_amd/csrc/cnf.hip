@@ -1496,15 +1496,21 @@ struct EntLess {
     }
 };
 
+// work (optional): the list indices to sort, *nwork of them (read on the device);
+// without it every elected variable's two lists
 __global__ void __launch_bounds__(64) k_sort_pv_lists(const uint32_t *__restrict__ pvs, uint32_t npvs,
                                                      const OL *__restrict__ ot, uint64_t *occ,
                                                      const uint32_t *__restrict__ mem, int stable, uint64_t *tmp,
-                                                     SortEnt *ents, unsigned long long *nents)
+                                                     SortEnt *ents, unsigned long long *nents,
+                                                     const uint32_t *__restrict__ work,
+                                                     const unsigned long long *__restrict__ nwork)
 {
     __shared__ uint64_t s_hi[SORT_SHORT][64], s_lo[SORT_SHORT][64], s_ref[SORT_SHORT][64];
     const int lane = threadIdx.x;
     const KeyLess lt{mem};
-    for (uint64_t q = gtid(); q < 2ull * npvs; q += gstride()) {
+    const uint64_t nq = work ? *nwork : 2ull * npvs;
+    for (uint64_t k = gtid(); k < nq; k += gstride()) {
+        const uint64_t q = work ? work[k] : k;
         const uint32_t lit = 2 * pvs[q >> 1] + (uint32_t)(q & 1);
         const OL o = ot[lit];
         uint64_t *d = occ + o.off;
@@ -1575,6 +1581,120 @@ __global__ void __launch_bounds__(64) k_sort_pv_lists(const uint32_t *__restrict
     }
 }
 
+// Wave-cooperative sortOT. A wave takes 64 elected lists (one per lane), stages the
+// keys (the two words of short_key) and references of as many of them as fit in
+// SPW_CAP entries into its LDS (cooperative loads: a lane per entry, the entry's list
+// found by a binary search over the wave's prefix of list sizes), and places every
+// entry at its stable rank in its list: the entries with a smaller key plus the equal
+// ones before it. That is the reference's insertion sort for lists of up to
+// SORT_SHORT entries (stable) and, when no two keys of a list are equal, the one
+// order any sort gives; a longer list with equal keys (where std::sort's own tie
+// order counts), or one longer than SPW_MAX, goes to the work list for
+// k_sort_pv_lists. With stable_sort on, stable ranks are the order for every list.
+constexpr uint32_t SPW_CAP = 512, SPW_MAX = 256;
+
+__global__ void __launch_bounds__(64) k_sort_pv_wave(const uint32_t *__restrict__ pvs, uint32_t npvs,
+                                                    const OL *__restrict__ ot, uint64_t *occ,
+                                                    const uint32_t *__restrict__ mem, int stable,
+                                                    uint32_t *__restrict__ work, unsigned long long *nwork)
+{
+    __shared__ uint64_t s_hi[SPW_CAP], s_lo[SPW_CAP], s_ref[SPW_CAP];
+    __shared__ uint32_t s_tie[64];
+    const uint32_t lane = threadIdx.x;
+    const uint64_t nq = 2ull * npvs;
+    for (uint64_t base = (uint64_t)blockIdx.x * 64; base < nq; base += (uint64_t)gridDim.x * 64) {
+        const uint64_t q = base + lane;
+        OL o{0, 0, 0};
+        if (q < nq) o = ot[2 * pvs[q >> 1] + (uint32_t)(q & 1)];
+        const bool toolong = o.sz > SPW_MAX;
+        wave_append(toolong, (uint32_t)q, work, nwork);
+        bool pending = o.sz > 1 && !toolong;
+        while (__ballot(pending)) {
+            // this round: the pending lists, in lane order, whose entries fit together
+            const uint32_t want = pending ? o.sz : 0u;
+            uint32_t incl = want;
+#pragma unroll
+            for (int k = 1; k < 64; k <<= 1) {
+                const uint32_t t = __shfl_up(incl, k, 64);
+                if ((int)lane >= k) incl += t;
+            }
+            const bool take = pending && incl <= SPW_CAP;
+            const uint32_t n = take ? o.sz : 0u;
+            uint32_t pre = n;
+#pragma unroll
+            for (int k = 1; k < 64; k <<= 1) {
+                const uint32_t t = __shfl_up(pre, k, 64);
+                if ((int)lane >= k) pre += t;
+            }
+            const uint32_t total = __shfl(pre, 63, 64);
+            pre -= n;
+            const uint32_t olo = (uint32_t)o.off, ohi = (uint32_t)(o.off >> 32);
+            s_tie[lane] = 0;
+            // stage keys and references
+            for (uint32_t e0 = 0; e0 < total; e0 += 64) {
+                const uint32_t e = e0 + lane;
+                uint32_t m = 0;
+#pragma unroll
+                for (uint32_t step = 32; step > 0; step >>= 1)
+                    if ((uint32_t)__shfl((int)pre, (int)(m + step), 64) <= e) m += step;
+                const uint32_t pm = (uint32_t)__shfl((int)pre, (int)m, 64);
+                const uint64_t om = ((uint64_t)(uint32_t)__shfl((int)ohi, (int)m, 64) << 32) |
+                                    (uint32_t)__shfl((int)olo, (int)m, 64);
+                if (e < total) {
+                    const uint64_t r = occ[om + (e - pm)];
+                    short_key(mem, r, s_hi[e], s_lo[e]);
+                    s_ref[e] = r;
+                }
+            }
+            __syncthreads();
+            // stable ranks (two passes over the staged entries: ties first, then the writes)
+            uint32_t rk[SPW_CAP / 64];
+#pragma unroll
+            for (uint32_t k = 0; k < SPW_CAP / 64; k++) {
+                const uint32_t e = k * 64 + lane;
+                uint32_t m = 0;
+#pragma unroll
+                for (uint32_t step = 32; step > 0; step >>= 1)
+                    if ((uint32_t)__shfl((int)pre, (int)(m + step), 64) <= e) m += step;
+                const uint32_t pm = (uint32_t)__shfl((int)pre, (int)m, 64);
+                const uint32_t nm = (uint32_t)__shfl((int)n, (int)m, 64);
+                rk[k] = 0;
+                if (e < total) {
+                    const uint64_t h = s_hi[e], l = s_lo[e];
+                    uint32_t r = 0;
+                    bool tie = false;
+                    for (uint32_t j = pm; j < pm + nm; j++) {
+                        const uint64_t hj = s_hi[j], lj = s_lo[j];
+                        if (hj < h || (hj == h && lj < l)) r++;
+                        else if (hj == h && lj == l && j != e) {
+                            tie = true;
+                            if (j < e) r++;
+                        }
+                    }
+                    rk[k] = r;
+                    if (tie && !stable && nm > (uint32_t)SORT_SHORT) s_tie[m] = 1;
+                }
+            }
+            __syncthreads();
+            // lists with equal keys past SORT_SHORT entries go to the exact introsort
+            wave_append(take && s_tie[lane] != 0, (uint32_t)q, work, nwork);
+#pragma unroll
+            for (uint32_t k = 0; k < SPW_CAP / 64; k++) {
+                const uint32_t e = k * 64 + lane;
+                uint32_t m = 0;
+#pragma unroll
+                for (uint32_t step = 32; step > 0; step >>= 1)
+                    if ((uint32_t)__shfl((int)pre, (int)(m + step), 64) <= e) m += step;
+                const uint64_t om = ((uint64_t)(uint32_t)__shfl((int)ohi, (int)m, 64) << 32) |
+                                    (uint32_t)__shfl((int)olo, (int)m, 64);
+                if (e < total && s_tie[m] == 0) occ[om + rk[k]] = s_ref[e];
+            }
+            __syncthreads();
+            if (take) pending = false;
+        }
+    }
+}
+
 // entries of the lists longer than SORT_SHORT (the introsort's scratch entries)
 __global__ void k_long_entries(const uint32_t *__restrict__ pvs, uint32_t npvs, const OL *__restrict__ ot,
                                unsigned long long *out)
@@ -1634,10 +1754,18 @@ void sort_ot(Ctx &c)
             ents = (SortEnt *)c.tmpu64b.p;
         }
     }
-    KTimer kt(c, K_SORT_PV, 2.0 * pv_bytes(c, 0, c.npvs, false));   // lists read and written once
-    k_sort_pv_lists<<<grid_for(2ull * c.npvs, 64, 16384), 64, 0, c.stream>>>(c.pvs.p, c.npvs, c.ot.p, c.occ.p, c.mem.p,
-                                                                              c.opts.stable_sort, tmp, ents,
-                                                                              (unsigned long long *)c.cnt.p + CNT_B);
+    ensure(c, c.tmpu32a, 2ull * c.npvs + 1);   // lists left to k_sort_pv_lists, count in CNT_C
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_C, 0, 8, c.stream));
+    {
+        KTimer kt(c, K_SORT_PV, 2.0 * pv_bytes(c, 0, c.npvs, false));   // lists read and written once
+        k_sort_pv_wave<<<grid_for(2ull * c.npvs, 64, 16384), 64, 0, c.stream>>>(c.pvs.p, c.npvs, c.ot.p, c.occ.p, c.mem.p,
+                                                                                c.opts.stable_sort, c.tmpu32a.p,
+                                                                                (unsigned long long *)c.cnt.p + CNT_C);
+    }
+    k_sort_pv_lists<<<grid_for(2ull * c.npvs, 64, 2048), 64, 0, c.stream>>>(c.pvs.p, c.npvs, c.ot.p, c.occ.p, c.mem.p,
+                                                                             c.opts.stable_sort, tmp, ents,
+                                                                             (unsigned long long *)c.cnt.p + CNT_B,
+                                                                             c.tmpu32a.p, (unsigned long long *)c.cnt.p + CNT_C);
 }
 
 }  // namespace pfg

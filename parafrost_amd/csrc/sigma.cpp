@@ -81,7 +81,7 @@ const char *const kernel_names[K_NUM] = {
     "sub_rest", "count_inst", "lcve_graph", "ve_fun", "ve_res"};
 const char *const kernel_symbols[K_NUM] = {
     "k_upload", nullptr, nullptr, nullptr, nullptr, nullptr, "k_elect_init", "k_elect_pass_a", "k_elect_pass_b",
-    "k_sort_pv_lists", "k_sub_grp", "k_ve_detect", "k_ve_apply", nullptr, nullptr, "k_ere_probe", "k_gc_move", "k_count",
+    "k_sort_pv_wave", "k_sub_grp", "k_ve_detect", "k_ve_apply", nullptr, nullptr, "k_ere_probe", "k_gc_move", "k_count",
     nullptr, "k_count_inst", nullptr, "k_ve_fun", "k_ve_res"};
 
 KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
