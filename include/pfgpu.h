@@ -236,7 +236,9 @@ int pfg_set_vorg(pfg_ctx *ctx, const uint32_t *vorg, uint32_t n);
  * counterpart in the reference): [0] extra ERE claiming rounds (run when a target was
  * claimed by two elected variables), [1] unit-capable SUB variables run alone on one
  * wave, [2] of those, the ones whose lists exceed the wave's LDS arrays (sub_limit
- * above 3000), [3] LCVE calls whose election rounds ran from a captured graph.
+ * above 3000), [3] LCVE calls whose election rounds ran from a captured graph,
+ * [4] elected occurrence lists sorted by the exact introsort (past 20 entries with two
+ * equal keys, or past 256 entries) rather than by stable ranks.
  * *n = the number of counters; at most max are written. */
 int pfg_path_counts(pfg_ctx *ctx, uint64_t *counts, int max, int *n);
 

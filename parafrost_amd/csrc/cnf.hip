@@ -1503,12 +1503,14 @@ __global__ void __launch_bounds__(64) k_sort_pv_lists(const uint32_t *__restrict
                                                      const uint32_t *__restrict__ mem, int stable, uint64_t *tmp,
                                                      SortEnt *ents, unsigned long long *nents,
                                                      const uint32_t *__restrict__ work,
-                                                     const unsigned long long *__restrict__ nwork)
+                                                     const unsigned long long *__restrict__ nwork,
+                                                     unsigned long long *nexact)
 {
     __shared__ uint64_t s_hi[SORT_SHORT][64], s_lo[SORT_SHORT][64], s_ref[SORT_SHORT][64];
     const int lane = threadIdx.x;
     const KeyLess lt{mem};
     const uint64_t nq = work ? *nwork : 2ull * npvs;
+    if (work && gtid() == 0 && nq) atomicAdd(nexact, nq);   // lists sorted here, summed over calls
     for (uint64_t k = gtid(); k < nq; k += gstride()) {
         const uint64_t q = work ? work[k] : k;
         const uint32_t lit = 2 * pvs[q >> 1] + (uint32_t)(q & 1);
@@ -1765,7 +1767,8 @@ void sort_ot(Ctx &c)
     k_sort_pv_lists<<<grid_for(2ull * c.npvs, 64, 2048), 64, 0, c.stream>>>(c.pvs.p, c.npvs, c.ot.p, c.occ.p, c.mem.p,
                                                                              c.opts.stable_sort, tmp, ents,
                                                                              (unsigned long long *)c.cnt.p + CNT_B,
-                                                                             c.tmpu32a.p, (unsigned long long *)c.cnt.p + CNT_C);
+                                                                             c.tmpu32a.p, (unsigned long long *)c.cnt.p + CNT_C,
+                                                                             (unsigned long long *)c.cnt.p + CNT_SORTX);
 }
 
 }  // namespace pfg

@@ -70,7 +70,7 @@ struct DBuf {
 // device counter slots (Ctx::cnt) and flags. Slot CNT_CONF counts contradictions;
 // the variables involved follow it (at most CONF_CAP), so that a batch can tell
 // which instances were refuted.
-enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_SUBFROM = 5, CNT_NTRAIL = 8, CNT_FLAGS = 9, CNT_ACCT = 10,
+enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_SUBFROM = 5, CNT_SORTX = 7, CNT_NTRAIL = 8, CNT_FLAGS = 9, CNT_ACCT = 10,
        CNT_ROUND = 11, CNT_CONF = 16 };
 enum { FLAG_UNSAT = 1, FLAG_INTERNAL = 2 };
 constexpr uint64_t CONF_CAP = 1ull << 20;
@@ -92,6 +92,7 @@ enum PathId {
     PATH_SUB_WAVE,       // unit-capable variables run alone on one wave
     PATH_SUB_WAVE_BIG,   // ... of them with lists longer than the LDS arrays
     PATH_LCVE_GRAPH,     // LCVE calls whose election rounds ran as a captured graph
+    PATH_SORT_EXACT,     // elected lists sorted by the exact introsort (equal keys past 20 entries, or long)
     PATH_NUM
 };
 

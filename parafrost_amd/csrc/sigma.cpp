@@ -690,7 +690,9 @@ int pfg_batch_relayout(pfg_ctx *ctx, uint32_t ninst, const uint32_t *nvars)
     PFG_HIP(hipMemsetAsync(c.value.p, 0xFF, c.value.cap, c.stream));
     PFG_HIP(hipMemsetAsync(c.vstate.p, 0, c.vstate.cap, c.stream));
     PFG_HIP(hipMemsetAsync(c.ifrozen.p, 0, c.ifrozen.cap, c.stream));
-    PFG_HIP(hipMemsetAsync(c.cnt.p, 0, (CNT_CONF + 1) * 8, c.stream));
+    // every counter but CNT_SORTX (a path count, kept until pfg_reset_stats)
+    PFG_HIP(hipMemsetAsync(c.cnt.p, 0, CNT_SORTX * 8, c.stream));
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_SORTX + 1, 0, (CNT_CONF - CNT_SORTX) * 8, c.stream));
     PFG_HIP(hipStreamSynchronize(c.stream));
     c.ntrail = c.propagated = 0;
     c.nres = c.nchunk = 0;
@@ -1018,6 +1020,7 @@ int pfg_reset_stats(pfg_ctx *ctx)
     for (auto &k : c.kstat) k = KStat{};
     c.times = pfg_times{};
     for (auto &p : c.path) p = 0;
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_SORTX, 0, 8, c.stream));
     API_END_OK
 }
 
@@ -1027,7 +1030,9 @@ int pfg_reset(pfg_ctx *ctx)
     PFG_HIP(hipMemsetAsync(c.value.p, 0xFF, c.value.cap, c.stream));
     PFG_HIP(hipMemsetAsync(c.vstate.p, 0, c.vstate.cap, c.stream));
     PFG_HIP(hipMemsetAsync(c.ifrozen.p, 0, c.ifrozen.cap, c.stream));
-    PFG_HIP(hipMemsetAsync(c.cnt.p, 0, (CNT_CONF + 1) * 8, c.stream));
+    // every counter but CNT_SORTX (a path count, kept until pfg_reset_stats)
+    PFG_HIP(hipMemsetAsync(c.cnt.p, 0, CNT_SORTX * 8, c.stream));
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_SORTX + 1, 0, (CNT_CONF - CNT_SORTX) * 8, c.stream));
     PFG_HIP(hipStreamSynchronize(c.stream));
     c.ntrail = c.propagated = 0;
     c.nres = c.nchunk = 0;
@@ -1115,11 +1120,16 @@ int pfg_set_vorg(pfg_ctx *ctx, const uint32_t *vorg, uint32_t n)
 int pfg_path_counts(pfg_ctx *ctx, uint64_t *counts, int max, int *n)
 {
     if (!ctx || (max > 0 && !counts)) return PFG_EINVAL;
-    const Ctx &c = ctx->c;
+    API_BEGIN
+    // counted on the device (no host read in the pass itself)
+    uint64_t sx = 0;
+    PFG_HIP(hipMemcpyAsync(&sx, c.cnt.p + CNT_SORTX, 8, hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    c.path[PATH_SORT_EXACT] = sx;
     const int k = std::min<int>(max, PATH_NUM);
     for (int i = 0; i < k; i++) counts[i] = c.path[i];
     if (n) *n = PATH_NUM;
-    return PFG_OK;
+    API_END_OK
 }
 
 const char *pfg_last_error(pfg_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }

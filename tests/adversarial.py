@@ -112,6 +112,44 @@ def sub_long_unit_capable(nlong: int = 3500, seed: int = 0):
     return gen.Cnf(nv, lits, offs), np.arange(2, nv + 1, dtype=np.uint32)
 
 
+def sort_ties(seed: int = 0):
+    """Elected variables whose occurrence lists are long and hold clauses with equal
+    CNF_CMP_KEY (same size, first and last literal, signature): duplicates. Each
+    variable x (every other variable frozen by ifreeze, so that LCVE elects the x's)
+    has on its positive side n clauses (x a b) over fresh a, b, some of them twice, and
+    one negative clause. The lists of 21 to 256 entries with equal keys are sorted by
+    the exact introsort, those without by stable ranks; one list of 300 entries is past
+    the wave sort's limit. Returns the formula and the variables to freeze."""
+    rng = np.random.default_rng(9950 + seed)
+    cls = []
+    nv = 0
+
+    def fresh(k):
+        nonlocal nv
+        out = list(range(nv + 1, nv + 1 + k))
+        nv += k
+        return out
+
+    xs = []
+    for n, dups in ((24, 3), (40, 0), (64, 10), (120, 1), (300, 5), (16, 4)):
+        x, c = fresh(2)
+        xs.append(x)
+        side = []
+        for _ in range(n - dups):
+            a, b = fresh(2)
+            side.append([2 * x, 2 * a + int(rng.integers(0, 2)), 2 * b + int(rng.integers(0, 2))])
+        for i in rng.choice(len(side), dups, replace=False):
+            side.append(list(side[i]))
+        order = rng.permutation(len(side))
+        cls += [side[i] for i in order]
+        cls.append([2 * x + 1, 2 * c])
+    clean = [sorted(c) for c in cls]
+    lits = np.array([l for c in clean for l in c], np.uint32)
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in clean])]).astype(np.uint64)
+    frozen = np.array([v for v in range(1, nv + 1) if v not in set(xs)], np.uint32)
+    return gen.Cnf(nv, lits, offs), frozen
+
+
 def function_gates(seed: int, nv_free: int = 8):
     """Small formulas for function-table gates (ve_fun_en), at most 20 variables for brute
     force, with the variables to freeze so that only the gadget variables x are elected.

@@ -6,6 +6,8 @@ the oracle through the C ABI:
   equal clause (tests/adversarial.py: ere_shared_targets);
 * the unit-capable subsumption wave with lists longer than its LDS arrays
   (sub_limit above 3000);
+* sortOT's exact introsort for elected lists past 20 entries holding equal keys, and
+  past the wave sort's 256 entries, beside the stable ranks of the other lists;
 * the caller's variable mapping (vorg): a second simplifier call on the formula the
   reference's map(true) left, with model records and proof lines in original numbering;
 * pfg_interrupt: a call stopped before its first round, and one stopped while it runs;
@@ -23,7 +25,7 @@ import pytest
 import lockstep as ls
 import pyoracle as po
 import parafrost_amd as pf
-from adversarial import adversarial, ere_shared_targets, function_gates, function_gates_tiled, sub_long_unit_capable
+from adversarial import adversarial, ere_shared_targets, function_gates, function_gates_tiled, sort_ties, sub_long_unit_capable
 from parafrost_amd import gen
 from test_oracle_golden import SECOND, second_call_case
 
@@ -65,6 +67,18 @@ def test_sub_wave_with_long_lists():
     ls.run_lockstep(cnf, sub_limit=5000, ere_limit=5000, mu_pos=100000, mu_neg=100000, phases=1, freeze=frozen,
                     proof_en=1, inspect=lambda g: seen.update(g.path_counts()))
     assert seen["sub_wave_big_vars"] >= 1
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_sort_ties_take_the_exact_introsort(seed):
+    """Elected lists of 16 to 300 entries, duplicates among them: the lists past 20
+    entries with equal keys (and the one past 256) are sorted by the exact introsort,
+    the others by stable ranks; the lists after sortOT equal the oracle's."""
+    print(f"sort ties seed={seed}", flush=True)
+    cnf, frozen = sort_ties(seed)
+    seen = {}
+    ls.run_lockstep(cnf, phases=1, freeze=frozen, inspect=lambda g: seen.update(g.path_counts()))
+    assert seen["sort_exact_lists"] >= 4
 
 
 @pytest.mark.parametrize("name", SECOND)
