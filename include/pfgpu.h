@@ -118,8 +118,14 @@ int pfg_batch_export_noalloc(pfg_ctx *ctx, uint32_t inst, uint32_t *d_lits, uint
  * Results never depend on them; parafrost_amd/tune.py measures candidates on a
  * workload and records the best. */
 typedef struct pfg_tuning {
-    int32_t ot_build;      /* occurrence table: 0 LSD radix sort of (literal, reference) keys,
-                              1 counting (histogram, atomic fill, every list sorted by reference) */
+    int32_t ot_build;      /* occurrence table: 0 LSD radix sort of (literal, reference) keys, and
+                              from the second round of a call on an update of the last table (the
+                              keys of clauses deleted or rewritten since are dropped, those of
+                              rewritten and appended clauses sorted and merged in); 1 counting
+                              (histogram, atomic fill, every list sorted by reference); 2 the radix
+                              sort every round; 3 as 0, updating however much changed, every update
+                              checked against a full sort (PFG_EINTERNAL on a difference; a test
+                              setting) */
     uint32_t clause_grid;  /* 256-thread blocks of the clause-streaming passes (LCVE, histogram, fill),
                               rounded up to a multiple of 8 (one window per XCD) */
     uint32_t sub_group;    /* lanes per elected variable in subsumption: 8, 16, 32 or 64 */
@@ -238,7 +244,8 @@ int pfg_set_vorg(pfg_ctx *ctx, const uint32_t *vorg, uint32_t n);
  * wave, [2] of those, the ones whose lists exceed the wave's LDS arrays (sub_limit
  * above 3000), [3] LCVE calls whose election rounds ran from a captured graph,
  * [4] elected occurrence lists sorted by the exact introsort (past 20 entries with two
- * equal keys, or past 256 entries) rather than by stable ranks.
+ * equal keys, or past 256 entries) rather than by stable ranks, [5] occurrence tables
+ * made by updating the previous round's table instead of sorting every key again.
  * *n = the number of counters; at most max are written. */
 int pfg_path_counts(pfg_ctx *ctx, uint64_t *counts, int max, int *n);
 

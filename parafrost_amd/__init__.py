@@ -19,7 +19,8 @@ _LIB = None
 
 PFG_OK, PFG_EINVAL, PFG_ENOMEM, PFG_EDEVICE, PFG_EUNSAT, PFG_EINTERNAL, PFG_EINTR = range(7)
 _ERRNAMES = {1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EUNSAT", 5: "EINTERNAL", 6: "EINTR"}
-PATH_NAMES = ("ere_extra_rounds", "sub_wave_vars", "sub_wave_big_vars", "lcve_graph_calls", "sort_exact_lists")
+PATH_NAMES = ("ere_extra_rounds", "sub_wave_vars", "sub_wave_big_vars", "lcve_graph_calls", "sort_exact_lists",
+              "ot_updates")
 
 
 class Opts(ctypes.Structure):

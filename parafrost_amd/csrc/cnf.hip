@@ -59,6 +59,7 @@ void upload(Ctx &c, const uint32_t *d_lits, const uint64_t *d_offs, uint64_t ncl
     c.nclauses = (int64_t)ncls;
     c.nliterals = (int64_t)nlits;
     c.ot_valid = false;
+    c.otk_valid = false;
     c.hist_live = false;
 }
 
@@ -270,8 +271,11 @@ __global__ void k_live_sizes(const uint32_t *__restrict__ mem, const uint64_t *_
 // so stores and literal loads are contiguous across the wave; a lane finds its clause
 // by a binary search over the wave's prefix of sizes (the last clause whose prefix is
 // not past the key, which skips deleted and empty clauses).
+// With szs given, clause i contributes its first szs[i] literals (0: none) instead of
+// its live size.
 __global__ void __launch_bounds__(256) k_ot_expand(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs,
-                                                  uint64_t n, const uint64_t *__restrict__ pos, uint64_t *__restrict__ keys)
+                                                  uint64_t n, const uint32_t *__restrict__ szs,
+                                                  const uint64_t *__restrict__ pos, uint64_t *__restrict__ keys)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -282,8 +286,11 @@ __global__ void __launch_bounds__(256) k_ot_expand(const uint32_t *__restrict__ 
         uint32_t sz = 0;
         if (i < n) {
             r = refs[i];
-            const uint32_t *c = mem + r;
-            sz = c_del(c) ? 0u : (uint32_t)c_size(c);
+            if (szs) sz = szs[i];
+            else {
+                const uint32_t *c = mem + r;
+                sz = c_del(c) ? 0u : (uint32_t)c_size(c);
+            }
         }
         uint32_t incl = sz;
 #pragma unroll
@@ -332,6 +339,180 @@ __global__ void k_ot_finish(OL *__restrict__ ot, uint32_t *__restrict__ hist, ui
     }
 }
 
+// ---- update of the last radix-built table (tuning ot_build 0 and 3)
+// A table is the sorted array of (literal << 32 | reference) keys of the live
+// clauses. Between two builds clauses are deleted, shrunk (strengthening, false
+// literals removed by propagation), rewritten in place (BVE's equivalence
+// substitution) and appended (resolvents); references do not move (compaction and
+// new uploads end the update chain). A clause of the last build whose live size
+// changed or that was rewritten (its bit in the rewrite map) is "changed": its old
+// keys are among the keys of its current literals (a deleted clause keeps its
+// literals) and of the literals removed from it in place (the removal log). So the
+// next array is the old array without those keys, merged with the keys of the
+// changed clauses still live and of the appended ones, sorted by literal: the same
+// array a full sort gives, so every list is the same. Few clauses change per round,
+// so the dropped keys are found by binary search and marked by position, and the
+// old array is read once.
+
+// Per clause: its live size now (nsz), the literals to add (asz: every live literal
+// of a changed or appended clause) and to drop (dsz: every literal of a changed
+// clause that had keys at the last build).
+__global__ void k_otu_sizes(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n, uint64_t n0,
+                            const uint32_t *__restrict__ osz, const uint32_t *__restrict__ rewritten,
+                            uint32_t *__restrict__ nsz, uint32_t *__restrict__ asz, uint32_t *__restrict__ dsz)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint64_t r = refs[i];
+        const uint32_t *c = mem + r;
+        const uint32_t sz = c_del(c) ? 0u : (uint32_t)c_size(c);
+        nsz[i] = sz;
+        bool changed = true;
+        uint32_t d = 0;
+        if (i < n0) {
+            const uint64_t b = r >> 2;
+            const uint32_t o = osz[i];
+            changed = sz != o || ((rewritten[b >> 5] >> (b & 31u)) & 1u);
+            if (changed && o) d = (uint32_t)c_size(c);
+        }
+        asz[i] = changed ? sz : 0u;
+        dsz[i] = d;
+    }
+}
+
+constexpr uint32_t OTU_BLOCK = 256, OTU_TILE = 4096, OTU_ITEMS = OTU_TILE / OTU_BLOCK, OTU_WORDS = OTU_TILE / 32;
+
+// first position in a[lo, hi) whose key is not below key
+__device__ __forceinline__ uint64_t lower_bound_u64(const uint64_t *__restrict__ a, uint64_t lo, uint64_t hi, uint64_t key)
+{
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// a key to drop: its position in the old array gets its bit (once; keys that are not
+// there, like a substituted literal, are passed over) and its tile one dropped key
+__global__ void k_otu_locate(const uint64_t *__restrict__ okeys, uint64_t L0, const uint64_t *__restrict__ dkeys, uint64_t nd,
+                             uint32_t *__restrict__ pbits, unsigned long long *__restrict__ dcnt)
+{
+    for (uint64_t j = gtid(); j < nd; j += gstride()) {
+        const uint64_t key = dkeys[j];
+        const uint64_t p = lower_bound_u64(okeys, 0, L0, key);
+        if (p >= L0 || okeys[p] != key) continue;
+        const uint32_t bit = 1u << (p & 31u);
+        if (!(atomicOr(&pbits[p >> 5], bit) & bit)) atomicAdd(&dcnt[p / OTU_TILE], 1ull);
+    }
+}
+
+// nb[t] = the first new key not below tile t's first old key (nb[0] = 0): new keys
+// [nb[t], nb[t + 1]) are merged into tile t, the ones past the last tile's first key
+// into the last tile
+__global__ void k_otu_splits(const uint64_t *__restrict__ okeys, uint64_t ntiles, const uint64_t *__restrict__ knew,
+                             uint64_t M, uint64_t *__restrict__ nb)
+{
+    for (uint64_t t = gtid(); t < ntiles; t += gstride()) nb[t] = t ? lower_bound_u64(knew, 0, M, okeys[t * OTU_TILE]) : 0;
+}
+
+// keys out of tile t: its old keys less the dropped ones, plus its new keys
+__global__ void k_otu_tot(uint64_t L0, const unsigned long long *__restrict__ dcnt, const uint64_t *__restrict__ nb,
+                          uint64_t ntiles, uint64_t M, uint64_t *__restrict__ tot)
+{
+    for (uint64_t t = gtid(); t < ntiles; t += gstride()) {
+        const uint64_t tn = L0 - t * OTU_TILE < OTU_TILE ? L0 - t * OTU_TILE : OTU_TILE;
+        tot[t] = tn - dcnt[t] + (t + 1 < ntiles ? nb[t + 1] : M) - nb[t];
+    }
+}
+
+// One block per tile: the tile's old keys and drop bits in LDS, the kept keys'
+// exclusive ranks (and the tile's new keys, up to OTU_NEW of them); every kept key
+// goes to outbase[t] + its rank + the tile's new keys below it, and every new key of
+// the tile to outbase[t] + its index among them + the kept keys below it. Kept and
+// new keys never coincide (a changed clause's old keys are all dropped), so the
+// positions are distinct.
+constexpr uint32_t OTU_NEW = 1024;
+__global__ void __launch_bounds__(OTU_BLOCK) k_otu_merge(const uint64_t *__restrict__ okeys, uint64_t L0,
+                                                         const uint32_t *__restrict__ pbits, const uint64_t *__restrict__ knew,
+                                                         uint64_t M, const uint64_t *__restrict__ nb, uint64_t ntiles,
+                                                         const uint64_t *__restrict__ outbase, uint64_t *__restrict__ out)
+{
+    __shared__ uint64_t s_key[OTU_TILE];
+    __shared__ uint64_t s_new[OTU_NEW];
+    __shared__ uint16_t s_rank[OTU_TILE + 1];
+    __shared__ uint32_t s_bits[OTU_WORDS];
+    __shared__ uint32_t s_w[OTU_BLOCK / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t t = blockIdx.x, base = t * OTU_TILE;
+    const uint32_t tn = (uint32_t)(L0 - base < (uint64_t)OTU_TILE ? L0 - base : (uint64_t)OTU_TILE);
+    const uint64_t n0 = nb[t], n1 = t + 1 < ntiles ? nb[t + 1] : M, ob = outbase[t];
+    const bool staged = n1 - n0 <= OTU_NEW;
+    for (uint32_t p = tid; p < OTU_TILE; p += OTU_BLOCK) s_key[p] = p < tn ? okeys[base + p] : ~0ull;
+    if (tid < OTU_WORDS) s_bits[tid] = tid * 32 < tn ? pbits[(base >> 5) + tid] : 0u;
+    if (staged)
+        for (uint32_t j = tid; j < (uint32_t)(n1 - n0); j += OTU_BLOCK) s_new[j] = knew[n0 + j];
+    __syncthreads();
+    // thread tid ranks positions [tid * OTU_ITEMS, (tid + 1) * OTU_ITEMS): 16 bits of one drop word
+    const uint32_t p0 = tid * OTU_ITEMS;
+    const uint32_t dropw = (s_bits[p0 >> 5] >> (p0 & 31u)) & 0xFFFFu;
+    const uint32_t valid = p0 >= tn ? 0u : tn - p0 >= OTU_ITEMS ? 0xFFFFu : (1u << (tn - p0)) - 1u;
+    const uint32_t flags = valid & ~dropw;
+    const uint32_t cnt = (uint32_t)__popc(flags);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t x = (uint32_t)__shfl_up(incl, o, 64);
+        if ((int)__lane_id() >= o) incl += x;
+    }
+    if (__lane_id() == 63) s_w[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t r = incl - cnt;
+    for (uint32_t w = 0; w < (tid >> 6); w++) r += s_w[w];
+#pragma unroll
+    for (uint32_t u = 0; u < OTU_ITEMS; u++) {
+        s_rank[p0 + u] = (uint16_t)r;
+        r += (flags >> u) & 1u;
+    }
+    if (tid == OTU_BLOCK - 1) s_rank[OTU_TILE] = (uint16_t)r;
+    __syncthreads();
+    const uint32_t nnew = (uint32_t)(n1 - n0);
+    for (uint32_t p = tid; p < tn; p += OTU_BLOCK) {
+        if (s_rank[p + 1] == s_rank[p]) continue;   // dropped
+        const uint64_t key = s_key[p];
+        uint64_t below = 0;
+        if (nnew) {
+            if (staged) {
+                uint32_t lo = 0, hi = nnew;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_new[mid] < key) lo = mid + 1;
+                    else hi = mid;
+                }
+                below = lo;
+            }
+            else below = lower_bound_u64(knew, n0, n1, key) - n0;
+        }
+        out[ob + s_rank[p] + below] = key;
+    }
+    for (uint64_t j = n0 + tid; j < n1; j += OTU_BLOCK) {
+        const uint64_t key = staged ? s_new[j - n0] : knew[j];
+        uint32_t lo = 0, hi = tn;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_key[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        out[ob + (j - n0) + s_rank[lo]] = key;
+    }
+}
+
+__global__ void k_otu_diff(const uint64_t *__restrict__ a, const uint64_t *__restrict__ b, uint64_t n, unsigned long long *cnt)
+{
+    uint32_t d = 0;
+    for (uint64_t i = gtid(); i < n; i += gstride()) d += a[i] != b[i];
+    if (d) atomicAdd(cnt, (unsigned long long)d);
+}
+
 static int bits_of(uint64_t v)
 {
     int b = 0;
@@ -339,33 +520,133 @@ static int bits_of(uint64_t v)
     return b;
 }
 
+// live size of every clause into sizes, their exclusive scan into pos; returns the total
+static uint64_t live_sizes_scan(Ctx &c, uint32_t *sizes, uint64_t *pos)
+{
+    const uint64_t n = c.nrefs;
+    k_live_sizes<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, sizes);
+    scan_u32_to_u64(c, sizes, pos, n, c.cnt.p + CNT_TOTAL);
+    read_counters(c, 1);
+    return c.hcnt[CNT_TOTAL];
+}
+
+// the keys of every clause (szs: of its first szs[i] literals) at pos, in clause order
+static void expand_keys(Ctx &c, const uint32_t *szs, const uint64_t *pos, uint64_t L, uint64_t *kbuf)
+{
+    const uint64_t n = c.nrefs;
+    KTimer kt(c, K_OT_FILL, 20.0 * n + 12.0 * L);
+    k_ot_expand<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, szs, pos, kbuf);
+}
+
+// ... sorted by literal: returns kbuf or alt (the radix temporary when alt is null),
+// whichever holds the L sorted keys
+static uint64_t *expand_sort(Ctx &c, const uint32_t *szs, const uint64_t *pos, uint64_t L, uint64_t *kbuf, uint64_t *alt)
+{
+    expand_keys(c, szs, pos, L, kbuf);
+    return radix_sort_u64_keys(c, kbuf, L, bits_of(c.nlits2 - 1), 32, alt);
+}
+
+// The update: the new sorted keys in c.otk (their count in L); false when the removal
+// log overflowed or so much changed that sorting every key is the cheaper way (a
+// quarter of the old keys; nothing is changed then).
+static bool update_keys(Ctx &c, uint64_t &L)
+{
+    const uint64_t n = c.nrefs, n0 = c.otk_nrefs, L0 = c.otk_n;
+    ensure(c, c.tmpu32a, n + 1);
+    ensure(c, c.tmpu32b, n + 1);
+    ensure(c, c.tmpu32c, n + 1);
+    ensure(c, c.tmpu64a, n + 1);
+    ensure(c, c.tmpu64c, n + 1);
+    k_otu_sizes<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, n0, c.otsz.p, c.otdirty.p,
+                                                                c.tmpu32a.p, c.tmpu32b.p, c.tmpu32c.p);
+    scan_u32_to_u64(c, c.tmpu32b.p, c.tmpu64a.p, n, c.cnt.p + CNT_A);   // keys to add
+    scan_u32_to_u64(c, c.tmpu32c.p, c.tmpu64c.p, n, c.cnt.p + CNT_B);   // keys of changed clauses to drop
+    PFG_HIP(hipMemcpyAsync(c.hcnt, c.cnt.p, 8 * (CNT_OTRM + 1), hipMemcpyDeviceToHost, c.stream));
+    PFG_HIP(hipStreamSynchronize(c.stream));
+    const uint64_t M = c.hcnt[CNT_A], MD = c.hcnt[CNT_B], NR = c.hcnt[CNT_OTRM];
+    // the checked setting updates whatever changed, so that the update itself is tested
+    if (NR > c.otrm_cap || (c.tune.ot_build != 3 && 4 * (M + MD + NR) > L0)) return false;
+    ensure(c, c.tmpu64b, std::max(M, MD) + 1024);
+    const uint64_t ntiles = (L0 + OTU_TILE - 1) / OTU_TILE;
+    ensure(c, c.otpos, L0 / 32 + 1);
+    ensure(c, c.otcnt, ntiles + 1);   // per tile: dropped keys
+    unsigned long long *dcnt = (unsigned long long *)c.otcnt.p;
+    PFG_HIP(hipMemsetAsync(c.otpos.p, 0, (L0 / 32 + 1) * 4, c.stream));
+    PFG_HIP(hipMemsetAsync(dcnt, 0, 8 * (ntiles + 1), c.stream));
+    if (MD) {   // the changed clauses' current literals
+        expand_keys(c, c.tmpu32c.p, c.tmpu64c.p, MD, c.tmpu64b.p);
+        k_otu_locate<<<grid_for(MD, 256, 8192), 256, 0, c.stream>>>(c.otk.p, L0, c.tmpu64b.p, MD, c.otpos.p, dcnt);
+    }
+    if (NR)     // the literals removed from them in place
+        k_otu_locate<<<grid_for(NR, 256, 8192), 256, 0, c.stream>>>(c.otk.p, L0, c.otrm.p, NR, c.otpos.p, dcnt);
+    const uint64_t *knew = M ? expand_sort(c, c.tmpu32b.p, c.tmpu64a.p, M, c.tmpu64b.p, nullptr) : c.tmpu64b.p;
+    ensure(c, c.tmpu64c, 3 * (ntiles + 1));
+    uint64_t *nb = c.tmpu64c.p, *tot = nb + ntiles + 1, *ob = tot + ntiles + 1;
+    k_otu_splits<<<grid_for(ntiles, 256, 4096), 256, 0, c.stream>>>(c.otk.p, ntiles, knew, M, nb);
+    k_otu_tot<<<grid_for(ntiles, 256, 4096), 256, 0, c.stream>>>(L0, dcnt, nb, ntiles, M, tot);
+    scan_u64(c, tot, ob, ntiles, c.cnt.p + CNT_TOTAL);
+    read_counters(c, 1);
+    L = c.hcnt[CNT_TOTAL];
+    ensure(c, c.otk2, L + 1024);
+    {
+        KTimer kt(c, K_OT_UPDATE, 8.0 * ((double)L0 + (double)L + (double)M));
+        k_otu_merge<<<(unsigned)ntiles, OTU_BLOCK, 0, c.stream>>>(c.otk.p, L0, c.otpos.p, knew, M, nb, ntiles, ob,
+                                                                 c.otk2.p);
+    }
+    std::swap(c.otk, c.otk2);
+    std::swap(c.otsz, c.tmpu32a);   // live sizes at this build
+    c.path[PATH_OT_UPDATE]++;
+    return true;
+}
+
 static void create_ot_radix(Ctx &c)
 {
     const uint64_t n = c.nrefs;
+    const bool keep_keys = c.tune.ot_build == 0 || c.tune.ot_build == 3;
     ensure(c, c.ot, c.nlits2);
     ensure(c, c.hist, c.nlits2);
-    ensure(c, c.tmpu32a, n + 1);
-    ensure(c, c.tmpu64a, n + 1);
-    k_live_sizes<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu32a.p);
-    scan_u32_to_u64(c, c.tmpu32a.p, c.tmpu64a.p, n, c.cnt.p + CNT_TOTAL);
-    read_counters(c, 1);
-    const uint64_t L = c.hcnt[CNT_TOTAL];
-    ensure(c, c.occ, L + 1024);
-    ensure(c, c.tmpu64b, L + 1024);
-    PFG_HIP(hipMemsetAsync(c.ot.p, 0, c.nlits2 * sizeof(OL), c.stream));
-    if (L) {
-        {
-            KTimer kt(c, K_OT_FILL, 20.0 * n + 12.0 * L);
-            k_ot_expand<<<grid_for(n, 256, 16384), 256, 0, c.stream>>>(c.mem.p, c.refs.p, n, c.tmpu64a.p, c.tmpu64b.p);
-        }
-        {
-            const int lbits = bits_of(c.nlits2 - 1);
-            KTimer kt(c, K_OT_SORT, 32.0 * L * ((lbits + 7) / 8));
-            const uint64_t *sorted = radix_sort_u64_keys(c, c.tmpu64b.p, L, lbits, 32);
-            k_ot_bounds<<<grid_for(L, 256, 16384), 256, 0, c.stream>>>(sorted, L, c.ot.p, c.occ.p);
+    uint64_t L = 0;
+    bool updated = false;
+    if (keep_keys && c.otk_valid && c.otk_n && n >= c.otk_nrefs) updated = update_keys(c, L);
+    if (updated && c.tune.ot_build == 3) {   // checked against a full sort
+        ensure(c, c.tmpu32c, n + 1);
+        ensure(c, c.tmpu64c, n + 1);
+        const uint64_t Lf = live_sizes_scan(c, c.tmpu32c.p, c.tmpu64c.p);
+        if (Lf != L) throw Error(PFG_EINTERNAL, "occurrence-table update: key count differs from a full build");
+        ensure(c, c.tmpu64b, L + 1024);
+        ensure(c, c.otk2, L + 1024);
+        const uint64_t *full = L ? expand_sort(c, nullptr, c.tmpu64c.p, L, c.tmpu64b.p, c.otk2.p) : c.tmpu64b.p;
+        PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_A, 0, 8, c.stream));
+        k_otu_diff<<<grid_for(L, 256, 4096), 256, 0, c.stream>>>(full, c.otk.p, L, (unsigned long long *)c.cnt.p + CNT_A);
+        read_counters(c, 2);
+        if (c.hcnt[CNT_A]) throw Error(PFG_EINTERNAL, "occurrence-table update differs from a full build");
+    }
+    if (!updated) {
+        ensure(c, c.otsz, n + 1);
+        ensure(c, c.tmpu64a, n + 1);
+        L = live_sizes_scan(c, c.otsz.p, c.tmpu64a.p);
+        ensure(c, c.otk, L + 1024);
+        ensure(c, c.otk2, L + 1024);
+        if (L) {
+            KTimer kt(c, K_OT_SORT, 32.0 * L * ((bits_of(c.nlits2 - 1) + 7) / 8));
+            const uint64_t *sorted = expand_sort(c, nullptr, c.tmpu64a.p, L, c.otk2.p, c.otk.p);
+            if (sorted == c.otk2.p) std::swap(c.otk, c.otk2);
         }
     }
+    ensure(c, c.occ, L + 1024);
+    PFG_HIP(hipMemsetAsync(c.ot.p, 0, c.nlits2 * sizeof(OL), c.stream));
+    if (L) k_ot_bounds<<<grid_for(L, 256, 16384), 256, 0, c.stream>>>(c.otk.p, L, c.ot.p, c.occ.p);
     k_ot_finish<<<grid_for(c.nlits2, 256, 8192), 256, 0, c.stream>>>(c.ot.p, c.hist.p, c.nlits2);
+    // what the next build updates from; the rewrite map and the removal log start empty
+    c.otk_n = L;
+    c.otk_nrefs = n;
+    c.otk_valid = keep_keys;
+    c.otdirty_words = (c.msz >> 7) + 1;
+    ensure(c, c.otdirty, c.otdirty_words);
+    PFG_HIP(hipMemsetAsync(c.otdirty.p, 0, c.otdirty_words * 4, c.stream));
+    c.otrm_cap = std::max<uint64_t>(L / 4, 1u << 16);
+    ensure(c, c.otrm, c.otrm_cap);
+    PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_OTRM, 0, 8, c.stream));
     c.ot_valid = true;
     c.hist_live = true;
 }
@@ -377,10 +658,11 @@ static void create_ot_radix(Ctx &c)
 // always take the counting way.
 void create_ot(Ctx &c)
 {
-    if (c.tune.ot_build == 0 && c.msz < (1ull << 32) && c.msz + 3 * c.nrefs < (1ull << 32)) {
+    if (c.tune.ot_build != 1 && c.msz < (1ull << 32) && c.msz + 3 * c.nrefs < (1ull << 32)) {
         create_ot_radix(c);
         return;
     }
+    c.otk_valid = false;
     if (!c.hist_live) hist(c);
     ensure(c, c.ot, c.nlits2);
     ensure(c, c.tmpu64a, c.nlits2);
@@ -517,7 +799,7 @@ __global__ void __launch_bounds__(256) k_prop_collect_long(const uint32_t *__res
 // false literals removed, the true ones) are passed over by LCVE, so theirs are left.
 __global__ void k_prop_apply(uint32_t *__restrict__ mem, const uint64_t *__restrict__ cand, uint64_t n, int8_t *value,
                              uint8_t *vstate, uint32_t *trail, unsigned long long *cnt, uint32_t *__restrict__ claim,
-                             uint32_t *__restrict__ hist)
+                             uint32_t *__restrict__ hist, uint64_t *__restrict__ rmlog, uint64_t rmcap)
 {
     for (uint64_t i = gtid(); i < n; i += gstride()) {
         const uint64_t r = cand[i];
@@ -541,7 +823,10 @@ __global__ void k_prop_apply(uint32_t *__restrict__ mem, const uint64_t *__restr
         uint32_t sig = 0;
         for (int k = 0; k < sz; k++) {
             const uint32_t l = lits[k];
-            if (__atomic_load_n(&value[l], __ATOMIC_RELAXED) == 0) continue;
+            if (__atomic_load_n(&value[l], __ATOMIC_RELAXED) == 0) {
+                log_removed(rmlog, &cnt[CNT_OTRM], rmcap, l, r);
+                continue;
+            }
             lits[j++] = l;
             sig |= maphash(l);
         }
@@ -586,7 +871,8 @@ bool prop(Ctx &c)
         const uint64_t nc = c.hcnt[CNT_A];
         if (nc) k_prop_apply<<<grid_for(nc, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.tmpu64b.p, nc, c.value.p, c.vstate.p,
                                                                              c.trail.p, (unsigned long long *)c.cnt.p, c.claim.p,
-                                                                             c.hist_live ? c.hist.p : nullptr);
+                                                                             c.hist_live ? c.hist.p : nullptr, c.otrm.p,
+                                                                             c.otrm_cap);
         k_prop_clear<<<grid_for(b1 - b0, 256, 1024), 256, 0, c.stream>>>(c.trail.p, b0, b1, c.ot.p,
                                                                           (unsigned long long *)c.cnt.p);
         read_counters(c, 10);
@@ -1176,6 +1462,7 @@ void shrink(Ctx &c)
     c.msz = words;
     c.nrefs = live;
     c.ot_valid = false;
+    c.otk_valid = false;   // references moved: the next table is sorted in full
 }
 
 // ------------------------------------------------------------------ ordered compaction

@@ -71,7 +71,7 @@ struct DBuf {
 // the variables involved follow it (at most CONF_CAP), so that a batch can tell
 // which instances were refuted.
 enum { CNT_TOTAL = 0, CNT_A = 1, CNT_B = 2, CNT_C = 3, CNT_D = 4, CNT_SUBFROM = 5, CNT_SORTX = 7, CNT_NTRAIL = 8, CNT_FLAGS = 9, CNT_ACCT = 10,
-       CNT_ROUND = 11, CNT_CONF = 16 };
+       CNT_ROUND = 11, CNT_OTRM = 12, CNT_CONF = 16 };
 enum { FLAG_UNSAT = 1, FLAG_INTERNAL = 2 };
 constexpr uint64_t CONF_CAP = 1ull << 20;
 
@@ -80,7 +80,7 @@ constexpr uint64_t CONF_CAP = 1ull << 20;
 enum KId {
     K_UPLOAD, K_HIST, K_OT_FILL, K_OT_SORT, K_PROP, K_LCVE_RADIX, K_LCVE_INIT, K_LCVE_PASS_A, K_LCVE_PASS_B,
     K_SORT_PV, K_SUB, K_VE_DETECT, K_VE_APPLY, K_BCE_COUNT, K_BCE_APPLY, K_ERE, K_GC, K_COUNT, K_SUB_REST,
-    K_COUNT_INST, K_LCVE_GRAPH, K_VE_FUN, K_VE_RES, K_NUM
+    K_COUNT_INST, K_LCVE_GRAPH, K_VE_FUN, K_VE_RES, K_OT_UPDATE, K_NUM
 };
 extern const char *const kernel_names[K_NUM];
 extern const char *const kernel_symbols[K_NUM];
@@ -93,12 +93,14 @@ enum PathId {
     PATH_SUB_WAVE_BIG,   // ... of them with lists longer than the LDS arrays
     PATH_LCVE_GRAPH,     // LCVE calls whose election rounds ran as a captured graph
     PATH_SORT_EXACT,     // elected lists sorted by the exact introsort (equal keys past 20 entries, or long)
+    PATH_OT_UPDATE,      // occurrence tables made by updating the last one
     PATH_NUM
 };
 
 // launch and algorithm choices (pfg_tuning); results never depend on them
 struct Tuning {
-    int32_t ot_build = 0;          // 0 radix, 1 counting
+    int32_t ot_build = 0;          // 0 radix (updated between rounds), 1 counting, 2 radix every round,
+                                   // 3 as 0 with every update checked against a full build
     uint32_t clause_grid = 2048;   // blocks of 256 threads for the XCD-windowed clause passes
     uint32_t sub_group = 16;       // lanes per variable in the first subsumption configuration
     int32_t lcve_graph = 0;        // LCVE election rounds replayed from a captured graph
@@ -131,6 +133,21 @@ struct Ctx {
     DBuf<uint32_t> hist;        // per literal
     bool ot_valid = false;
     bool hist_live = false;     // hist holds the live-literal counts of the current database
+    // what the next radix build updates from (tuning ot_build 0 and 3): the sorted
+    // (literal << 32 | reference) keys of the last build, every clause's live size at
+    // that build, a bitmap over reference >> 2 of the clauses rewritten in place since
+    // then (equivalence substitution in BVE keeps the size), and a log of the keys of
+    // literals removed from clauses in place since then (count in cnt[CNT_OTRM])
+    DBuf<uint64_t> otk, otk2;
+    uint64_t otk_n = 0, otk_nrefs = 0;
+    DBuf<uint32_t> otsz;
+    DBuf<uint32_t> otdirty;
+    uint64_t otdirty_words = 0;
+    DBuf<uint64_t> otrm;
+    uint64_t otrm_cap = 0;
+    DBuf<uint32_t> otpos;       // the update: bit per old key position, set for the dropped keys
+    DBuf<uint64_t> otcnt;       // ... and per tile of old keys, the dropped ones
+    bool otk_valid = false;
     bool trace = false;         // PFG_TRACE set: per-round progress on stderr
     bool account = false;       // compute algorithmic bytes of the per-variable kernels (extra reductions)
     uint32_t stage_words = 0xFFFFFFFFu;   // cap on the LDS words per block the staged kernels use

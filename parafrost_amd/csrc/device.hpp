@@ -34,6 +34,24 @@ __device__ __forceinline__ void c_markdel(uint32_t *c) { c[0] = (c[0] & ~3u) | S
 __device__ __forceinline__ void c_setst(uint32_t *c, uint32_t st) { c[0] = (c[0] & ~3u) | st; }
 __device__ __forceinline__ int c_size(const uint32_t *c) { return (int)c[2]; }
 
+// a clause rewritten in place with its size kept: its bit (reference >> 2) in the
+// occurrence-table update's rewrite map, which covers the pool of the last build
+__device__ __forceinline__ void mark_rewritten(uint32_t *map, uint64_t words, uint64_t ref)
+{
+    const uint64_t b = ref >> 2;
+    if (map && (b >> 5) < words) atomicOr(&map[b >> 5], 1u << (b & 31u));
+}
+
+// literal lit removed from the clause at ref in place: its key goes to the
+// occurrence-table update's removal log (cnt: its count; past cap the count alone
+// grows, and the next table is sorted in full)
+__device__ __forceinline__ void log_removed(uint64_t *log, unsigned long long *cnt, uint64_t cap, uint32_t lit, uint64_t ref)
+{
+    if (!log) return;
+    const unsigned long long k = atomicAdd(cnt, 1ull);
+    if (k < cap) log[k] = ((uint64_t)lit << 32) | ref;
+}
+
 // signature filters (src/cpu/simplify.h:81-87)
 __device__ __forceinline__ bool sig_sub(uint32_t A, uint32_t B) { return !(A & ~B); }
 __device__ __forceinline__ bool sig_selfsub(uint32_t A, uint32_t B)

@@ -359,18 +359,18 @@ static void radix_pass(Ctx &c, const K *a, const uint32_t *va, K *b, uint32_t *v
 // Returns the buffer holding the sorted keys (keys or the context's temporary); with
 // copy_back the result is always moved into keys/vals.
 template <typename K, bool V>
-static K *radix_impl(Ctx &c, K *keys, uint32_t *vals, uint64_t n, int nbits, int lo, bool copy_back)
+static K *radix_impl(Ctx &c, K *keys, uint32_t *vals, uint64_t n, int nbits, int lo, bool copy_back, K *alt = nullptr)
 {
     if (n <= 1 || nbits <= 0) return keys;
     const uint64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
     const uint64_t cnt_n = 256 * ntiles;   // digits of at most 8 bits
     auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
     const uint32_t ch = (uint32_t)((ntiles + 31) / 32), nchunks = (uint32_t)((ntiles + ch - 1) / ch);
-    const uint64_t bk = al(n * sizeof(K)), bv = V ? al(n * 4) : 0, bc = al(cnt_n * 4), bo = al(cnt_n * 8),
+    const uint64_t bk = alt ? 0 : al(n * sizeof(K)), bv = V ? al(n * 4) : 0, bc = al(cnt_n * 4), bo = al(cnt_n * 8),
                    bp = al(256ull * nchunks * 8);
     ensure(c, c.radix_tmp, bk + bv + bc + bo + bp);
     uint8_t *t = c.radix_tmp.p;
-    K *kt = (K *)t;
+    K *kt = alt ? alt : (K *)t;
     uint32_t *vt = V ? (uint32_t *)(t + bk) : nullptr;
     uint32_t *counts = (uint32_t *)(t + bk + bv);
     uint64_t *offs = (uint64_t *)(t + bk + bv + bc);
@@ -404,9 +404,9 @@ void radix_sort_u32_pairs(Ctx &c, uint32_t *keys, uint32_t *vals, uint64_t n, in
     radix_impl<uint32_t, true>(c, keys, vals, n, nbits, 0, true);
 }
 
-uint64_t *radix_sort_u64_keys(Ctx &c, uint64_t *keys, uint64_t n, int nbits, int lo)
+uint64_t *radix_sort_u64_keys(Ctx &c, uint64_t *keys, uint64_t n, int nbits, int lo, uint64_t *alt)
 {
-    return radix_impl<uint64_t, false>(c, keys, nullptr, n, nbits, lo, false);
+    return radix_impl<uint64_t, false>(c, keys, nullptr, n, nbits, lo, false, alt);
 }
 
 __global__ void k_max_u32(const uint32_t *__restrict__ in, uint64_t n, unsigned long long *out)

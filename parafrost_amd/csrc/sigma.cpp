@@ -78,11 +78,11 @@ PassTimer::~PassTimer()
 const char *const kernel_names[K_NUM] = {
     "upload_prep", "hist", "ot_fill", "ot_sort", "prop", "lcve_radix", "lcve_init", "lcve_pass_a", "lcve_pass_b",
     "sort_pv_lists", "sub", "ve_detect", "ve_apply", "bce_count", "bce_apply", "ere_probe", "gc_move", "count",
-    "sub_rest", "count_inst", "lcve_graph", "ve_fun", "ve_res"};
+    "sub_rest", "count_inst", "lcve_graph", "ve_fun", "ve_res", "ot_update"};
 const char *const kernel_symbols[K_NUM] = {
     "k_upload", nullptr, nullptr, nullptr, nullptr, nullptr, "k_elect_init", "k_elect_pass_a", "k_elect_pass_b",
     "k_sort_pv_wave", "k_sub_grp", "k_ve_detect", "k_ve_apply", nullptr, nullptr, "k_ere_probe", "k_gc_move", "k_count",
-    nullptr, "k_count_inst", nullptr, "k_ve_fun", "k_ve_res"};
+    nullptr, "k_count_inst", nullptr, "k_ve_fun", "k_ve_res", "k_otu_merge"};
 
 KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
 {
@@ -509,6 +509,8 @@ int pfg_destroy(pfg_ctx *ctx)
     release(c, c.tmpu8); release(c, c.tmpu64a); release(c, c.tmpu64b); release(c, c.tmpu64c);
     release(c, c.owner); release(c, c.claim); release(c, c.plong); release(c, c.erec); release(c, c.ufirst); release(c, c.cnt);
     release(c, c.scan_tmp); release(c, c.radix_tmp);
+    release(c, c.otk); release(c, c.otk2); release(c, c.otsz); release(c, c.otdirty);
+    release(c, c.otrm); release(c, c.otpos); release(c, c.otcnt);
     release(c, c.proof); release(c, c.pev); release(c, c.pord); release(c, c.pnev); release(c, c.pbytes);
     release(c, c.pcnt); release(c, c.poff); release(c, c.pboff); release(c, c.pfirst);
     release(c, c.ibase); release(c, c.imu); release(c, c.ib0); release(c, c.inel); release(c, c.icnt); release(c, c.vorg);
@@ -703,6 +705,7 @@ int pfg_batch_relayout(pfg_ctx *ctx, uint32_t ninst, const uint32_t *nvars)
     c.nclauses = c.nliterals = 0;
     c.unsat = false;
     c.ot_valid = c.hist_live = false;
+    c.otk_valid = false;
     c.batch_gen++;
     API_END
 }
@@ -1045,6 +1048,7 @@ int pfg_reset(pfg_ctx *ctx)
     c.unsat = false;
     c.ot_valid = false;
     c.hist_live = false;
+    c.otk_valid = false;
     c.nproof = 0;
     c.proof_closed = false;
     c.has_vorg = false;
@@ -1064,7 +1068,7 @@ int pfg_set_tuning(pfg_ctx *ctx, const pfg_tuning *t)
 {
     if (!ctx || !t) return PFG_EINVAL;
     Ctx &c = ctx->c;
-    if (t->ot_build < 0 || t->ot_build > 1 || t->lcve_graph < 0 || t->lcve_graph > 1 || !t->clause_grid ||
+    if (t->ot_build < 0 || t->ot_build > 3 || t->lcve_graph < 0 || t->lcve_graph > 1 || !t->clause_grid ||
         t->clause_grid > (1u << 20) || t->init_grid > (1u << 20) || !(t->ve_arena == 4096 || t->ve_arena == 8192) ||
         !(t->sub_group == 8 || t->sub_group == 16 || t->sub_group == 32 || t->sub_group == 64)) {
         c.last_error = "tuning value out of range";

@@ -31,6 +31,8 @@ struct SubArgs {
     int allow_units;
     int defer_oblivion;   // a unit's clauses are deleted by the caller (k_sub_oblivion)
     ProofLog P;           // proof events: strengthened clauses, units, subsumed clauses
+    uint64_t *rmlog;      // the occurrence-table update's removal log (strengthened literals)
+    uint64_t rmcap;
 };
 
 __device__ __forceinline__ bool sub_cl(const uint32_t *sm, const uint32_t *lr)
@@ -91,6 +93,7 @@ __device__ __forceinline__ uint32_t strengthen(const SubArgs &A, uint32_t *c, ui
     }
     c[1] = sig;
     c[2] = (uint32_t)(sz - 1);
+    log_removed(A.rmlog, &A.cnt[CNT_OTRM], A.rmcap, me, ref);
     if (n == 1) {
         if (!A.allow_units) { atomicOr(&A.cnt[CNT_FLAGS], (unsigned long long)FLAG_INTERNAL); return 0; }
         const uint32_t unit = lits[0];
@@ -800,7 +803,7 @@ void sub(Ctx &c)
     if (!c.npvs || !(c.opts.sub_en || c.opts.ve_en)) return;
     const uint32_t np = c.npvs;
     SubArgs A{c.mem.p, c.ot.p, c.occ.p, c.value.p, c.vstate.p, c.trail.p, (unsigned long long *)c.cnt.p,
-              c.opts.sub_limit, c.opts.lbd_tier1, 0, 0, proof_begin(c, np, 2 * c.nrefs + 2ull * np)};
+              c.opts.sub_limit, c.opts.lbd_tier1, 0, 0, proof_begin(c, np, 2 * c.nrefs + 2ull * np), c.otrm.p, c.otrm_cap};
     ensure(c, c.tmpu32a, np);
     ensure(c, c.tmpu32b, np);
     ensure(c, c.scores, np);

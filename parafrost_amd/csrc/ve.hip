@@ -938,6 +938,11 @@ struct VeOut {
     uint64_t ref_base, data_base, model_base;
     ProofLog P;   // proof events: resolvents, substituted clauses and units, decided after the pass
     const uint32_t *vorg;   // the caller's variable mapping for the model records (null: identity)
+    uint32_t *rw;           // rewrite map of the occurrence-table update (substituted clauses)
+    uint64_t rw_words;
+    uint64_t *rmlog;        // and its removal log (the substituted literal)
+    uint64_t rmcap;
+    unsigned long long *cnt;
 };
 
 __device__ __forceinline__ void save_side(const VeArgs &A, const uint32_t *vorg, uint32_t *&rs, uint32_t p, bool pbig)
@@ -1012,6 +1017,8 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
                     if (c_learnt(c) || c_molten(c) || has_lit(c, kill)) c_markdel(c);
                     else if (c_orig(c)) {
                         const uint32_t *u0 = us;
+                        mark_rewritten(O.rw, O.rw_words, A.occ[o.off + i]);
+                        log_removed(O.rmlog, &O.cnt[CNT_OTRM], O.rmcap, lit, A.occ[o.off + i]);
                         substitute_one(lit, c, into, us);
                         if (us != u0) plog(O.P, (uint32_t)q, seq, PE_CAND_UNIT, us[-1]);
                         else plog(O.P, (uint32_t)q, seq, PE_CAND_ADD, A.occ[o.off + i]);
@@ -1151,7 +1158,8 @@ void ve(Ctx &c)
     A.mem = c.mem.p;
     uint32_t *units = c.units.p;
     VeOut O{c.resolved.p, units, c.refs.p, c.vstate.p, c.pvs.p, c.nrefs, c.msz, c.nres,
-            proof_begin(c, np, c.nrefs + add_refs + add_units + np), c.has_vorg ? c.vorg.p : nullptr};
+            proof_begin(c, np, c.nrefs + add_refs + add_units + np), c.has_vorg ? c.vorg.p : nullptr,
+            c.otdirty.p, c.otdirty.p ? c.otdirty_words : 0, c.otrm.p, c.otrm_cap, (unsigned long long *)c.cnt.p};
     {
         KTimer kt(c, K_VE_APPLY, 4.0 * add_words + 8.0 * add_refs + 4.0 * add_model);
         k_ve_apply<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, O, type, aux, np, roff, woff, uoff, moff);

@@ -25,7 +25,7 @@ TUNING_DIR = os.path.join(HERE, "tuning")
 
 # candidate values per knob, in the order coordinate descent visits them
 SPACE = {
-    "ot_build": [0, 1],
+    "ot_build": [0, 1, 2],
     "clause_grid": [1024, 2048, 4096, 8192],
     "sub_group": [8, 16, 32, 64],
     "lcve_graph": [0, 1],

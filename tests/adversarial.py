@@ -223,3 +223,24 @@ def function_gates_tiled(copies: int, seed: int):
         nv += cnf.nvars
     return (gen.Cnf(nv, np.concatenate(lits).astype(np.uint32), np.concatenate(offs)),
             np.concatenate(freeze).astype(np.uint32))
+
+
+def equivalences(seed: int, frac: float = 0.15):
+    """Random 3-SAT with planted equivalences x = a (the binaries (-x|a) and (x|-a)):
+    BVE substitutes a for x in x's other clauses, rewriting them in place, mostly with
+    their size kept."""
+    rng = np.random.default_rng(9900 + seed)
+    nv = int(rng.integers(3000, 6000))
+    base = gen.uniform_ksat(nv, int(nv * 3.0), 3, 9900 + seed)
+    cls = [sorted(map(int, c)) for c in base.clauses()]
+    xs = rng.choice(np.arange(1, nv + 1), size=int(nv * frac), replace=False)
+    for x in xs:
+        a = int(rng.integers(1, nv + 1))
+        if a == x:
+            continue
+        s = int(rng.integers(0, 2))
+        cls.append(sorted([2 * int(x) + 1, 2 * a + s]))
+        cls.append(sorted([2 * int(x), 2 * a + (1 - s)]))
+    lits = np.array([l for c in cls for l in c], np.uint32)
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in cls])]).astype(np.uint64)
+    return gen.Cnf(nv, lits, offs)
