@@ -467,13 +467,55 @@ __device__ __forceinline__ uint64_t xor_size_counts(const VeArgs &A, uint32_t dx
     return packed;
 }
 
-__device__ __forceinline__ bool find_xor_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words, uint32_t &units)
+// The size and sequence counts cover both lists, so the two XOR searches of a
+// variable (positive, then negative side) share them; pre.have says they are made.
+struct XorPre {
+    uint64_t counts = 0;
+    SeqCounts seq;
+    bool have = false;
+};
+
+__device__ __forceinline__ bool any_molten(const VeArgs &A, uint32_t dx)
+{
+    for (int s = 0; s < 2; s++) {
+        const uint32_t l = dx ^ (uint32_t)s, sz = A.ot[l].sz;
+        for (uint32_t i = 0; i < sz; i++) {
+            const uint32_t *c = cl_at(A, l, i);
+            if (c_size(c) > 2 && c_molten(c)) return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool find_xor_gate(const VeArgs &A, uint32_t dx, int nOrgCls, int &nAdded, uint64_t &words,
+                                              uint32_t &units, XorPre &pre)
 {
     const uint32_t fx = dx ^ 1u, v = dx >> 1;
     const uint32_t is = A.ot[dx].sz;
     bool molten = false;
-    SeqCounts seq;
-    const uint64_t counts = xor_size_counts(A, dx, molten, seq);
+    if (!pre.have) {
+        pre.counts = xor_size_counts(A, dx, molten, pre.seq);
+        pre.have = true;
+    }
+    else molten = any_molten(A, dx);
+    const uint64_t counts = pre.counts;
+    const SeqCounts &seq = pre.seq;
+    // A clause passes the bucket test below only if its bucket holds at least 4 (size
+    // 3, which also needs 4 clauses of size 3) or 7 (larger sizes): with no such bucket
+    // every attempt fails, and the loop's only effect is the first failing attempt's
+    // cleanup.
+    if (!seq.b2 || (!(seq.b0 & seq.b1 & seq.b2) && (counts & 0xFFu) < 4u)) {
+        if (!molten) return false;
+        for (uint32_t i = 0; i < is; i++) {
+            const uint32_t *ci = cl_at(A, dx, i);
+            const int size = c_size(ci);
+            if (c_orig(ci) && size >= 3 && size - 1 <= A.xor_max_arity) {
+                freeze_arities(A, dx, fx);
+                break;
+            }
+        }
+        return false;
+    }
     uint32_t lits[21];
     for (uint32_t i = 0; i < is; i++) {
         uint32_t *ci = cl_at(A, dx, i);
@@ -579,10 +621,11 @@ __device__ __forceinline__ void ve_detect_var(const VeArgs &A, const VeDetectOut
                     else if (!nAdded && find_ao_gate(A, p, nOrgCls, nAdded, words, units)) gate = true;
                 }
                 if (!gate && nOrgCls > 3) {
+                    XorPre xpre;
                     if (find_ite_gate(A, p, nOrgCls, nAdded, words, units)) gate = true;
                     else if (!nAdded && find_ite_gate(A, n, nOrgCls, nAdded, words, units)) gate = true;
-                    else if (find_xor_gate(A, p, nOrgCls, nAdded, words, units)) gate = true;
-                    else if (!nAdded && find_xor_gate(A, n, nOrgCls, nAdded, words, units)) gate = true;
+                    else if (find_xor_gate(A, p, nOrgCls, nAdded, words, units, xpre)) gate = true;
+                    else if (!nAdded && find_xor_gate(A, n, nOrgCls, nAdded, words, units, xpre)) gate = true;
                 }
                 if (gate) t = T_SUBST;
                 else if (fun_en && nOrgCls > 2) {

@@ -244,3 +244,24 @@ def equivalences(seed: int, frac: float = 0.15):
     lits = np.array([l for c in cls for l in c], np.uint32)
     offs = np.concatenate([[0], np.cumsum([len(c) for c in cls])]).astype(np.uint64)
     return gen.Cnf(nv, lits, offs)
+
+
+def xor_gates(seed: int):
+    """Random 3-SAT with planted XOR constraints x + y1 + .. + yk = b (k = 2..4, every
+    clause of the parity encoding: 2^k clauses of size k + 1), so that BVE's XOR gate
+    search succeeds and substitutes; some gate variables appear nowhere else."""
+    rng = np.random.default_rng(9950 + seed)
+    nv = int(rng.integers(1500, 3000))
+    base = gen.uniform_ksat(nv, int(nv * rng.uniform(1.5, 3.0)), 3, 9950 + seed)
+    cls = [sorted(map(int, c)) for c in base.clauses()]
+    for _ in range(int(nv * 0.08)):
+        k = int(rng.integers(2, 5))
+        vs = rng.choice(np.arange(1, nv + 1), size=k + 1, replace=False)
+        b = int(rng.integers(0, 2))
+        for a in range(1 << (k + 1)):
+            if bin(a).count("1") % 2 == b:
+                continue   # the clause forbidding assignment a (bit i: value of vs[i])
+            cls.append(sorted(2 * int(v) + ((a >> i) & 1) for i, v in enumerate(vs)))
+    lits = np.array([l for c in cls for l in c], np.uint32)
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in cls])]).astype(np.uint64)
+    return gen.Cnf(nv, lits, offs)

@@ -348,6 +348,15 @@ def test_lockstep_adversarial(seed):
                     mu_neg=int(rng.integers(10, 80)), bce_en=seed % 2, proof_en=(seed // 2) % 2)
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_lockstep_xor_gates(seed):
+    """Planted XOR constraints: gate search successes, failures and their cleanup."""
+    print(f"lockstep xor gates seed={seed}", flush=True)
+    from adversarial import xor_gates
+    rng = np.random.default_rng(seed)
+    ls.run_lockstep(xor_gates(seed), phases=int(rng.integers(2, 6)), xor_max_arity=int(rng.integers(2, 6)))
+
+
 def test_empty_and_tiny():
     import parafrost_amd as pf
     g = pf.Simplifier(5)

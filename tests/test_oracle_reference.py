@@ -176,6 +176,27 @@ def test_adversarial(seed):
 
 
 @pytest.mark.parametrize("seed", range(6))
+def test_xor_gates(seed):
+    """Planted XOR constraints: the XOR gate search succeeds (substitution) and fails
+    (cleanup of molten clauses) as in the reference."""
+    from adversarial import xor_gates
+    rng = np.random.default_rng(seed)
+    opts = po.default_opts(phases=int(rng.integers(2, 6)), xor_max_arity=int(rng.integers(2, 6)))
+    cnf = xor_gates(seed)
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "f.cnf")
+        cnf.write_dimacs(p)
+        ref = po.run_reference(p, os.path.join(td, "f.bin"), opts)
+    o = po.Oracle(cnf, opts)
+    rc = o.sigmify()
+    st = o.state()
+    assert (rc != 0) == bool(ref.unsat)
+    if not ref.unsat:
+        for f in ("cnf", "trail", "resolved", "vstate"):
+            assert np.array_equal(getattr(st, f), getattr(ref, f)), f
+
+
+@pytest.mark.parametrize("seed", range(6))
 def test_mid_size_second_call_vorg(seed):
     """A second simplifier call after map(true): the mapped formula with its vorg and
     kept unit; model records (the stack's suffix) and proof lines in original numbering."""
