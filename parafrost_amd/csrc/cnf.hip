@@ -379,7 +379,7 @@ __global__ void k_otu_sizes(const uint32_t *__restrict__ mem, const uint64_t *__
     }
 }
 
-constexpr uint32_t OTU_BLOCK = 256, OTU_TILE = 4096, OTU_ITEMS = OTU_TILE / OTU_BLOCK, OTU_WORDS = OTU_TILE / 32;
+constexpr uint32_t OTU_BLOCK = 256, OTU_TILE = 2048, OTU_ITEMS = OTU_TILE / OTU_BLOCK, OTU_WORDS = OTU_TILE / 32;
 
 // first position in a[lo, hi) whose key is not below key
 __device__ __forceinline__ uint64_t lower_bound_u64(const uint64_t *__restrict__ a, uint64_t lo, uint64_t hi, uint64_t key)
@@ -430,14 +430,19 @@ __global__ void k_otu_tot(uint64_t L0, const unsigned long long *__restrict__ dc
 // goes to outbase[t] + its rank + the tile's new keys below it, and every new key of
 // the tile to outbase[t] + its index among them + the kept keys below it. Kept and
 // new keys never coincide (a changed clause's old keys are all dropped), so the
-// positions are distinct.
-constexpr uint32_t OTU_NEW = 1024;
+// positions are distinct. The keys go out (the next update's array) with their
+// references (the occurrence lists); a tile whose new keys were staged also merges
+// in LDS, so that it writes the list bounds of the literals that start (OL.off) or
+// end (OL.cap, one past) inside its output; k_otu_edges does the tile's two ends and
+// the whole output of a tile that was not staged.
+constexpr uint32_t OTU_NEW = 1024, OTU_NITEMS = OTU_NEW / OTU_BLOCK, OTU_OUT = OTU_TILE + OTU_NEW;
 __global__ void __launch_bounds__(OTU_BLOCK) k_otu_merge(const uint64_t *__restrict__ okeys, uint64_t L0,
                                                          const uint32_t *__restrict__ pbits, const uint64_t *__restrict__ knew,
                                                          uint64_t M, const uint64_t *__restrict__ nb, uint64_t ntiles,
-                                                         const uint64_t *__restrict__ outbase, uint64_t *__restrict__ out)
+                                                         const uint64_t *__restrict__ outbase, uint64_t *__restrict__ out,
+                                                         uint64_t *__restrict__ occ, OL *__restrict__ ot)
 {
-    __shared__ uint64_t s_key[OTU_TILE];
+    __shared__ uint64_t s_key[OTU_OUT];   // the tile's old keys, then its merged output
     __shared__ uint64_t s_new[OTU_NEW];
     __shared__ uint16_t s_rank[OTU_TILE + 1];
     __shared__ uint32_t s_bits[OTU_WORDS];
@@ -452,10 +457,10 @@ __global__ void __launch_bounds__(OTU_BLOCK) k_otu_merge(const uint64_t *__restr
     if (staged)
         for (uint32_t j = tid; j < (uint32_t)(n1 - n0); j += OTU_BLOCK) s_new[j] = knew[n0 + j];
     __syncthreads();
-    // thread tid ranks positions [tid * OTU_ITEMS, (tid + 1) * OTU_ITEMS): 16 bits of one drop word
+    // thread tid ranks positions [tid * OTU_ITEMS, (tid + 1) * OTU_ITEMS): bits of one drop word
     const uint32_t p0 = tid * OTU_ITEMS;
-    const uint32_t dropw = (s_bits[p0 >> 5] >> (p0 & 31u)) & 0xFFFFu;
-    const uint32_t valid = p0 >= tn ? 0u : tn - p0 >= OTU_ITEMS ? 0xFFFFu : (1u << (tn - p0)) - 1u;
+    const uint32_t dropw = (s_bits[p0 >> 5] >> (p0 & 31u)) & ((1u << OTU_ITEMS) - 1u);
+    const uint32_t valid = p0 >= tn ? 0u : tn - p0 >= OTU_ITEMS ? (1u << OTU_ITEMS) - 1u : (1u << (tn - p0)) - 1u;
     const uint32_t flags = valid & ~dropw;
     const uint32_t cnt = (uint32_t)__popc(flags);
     uint32_t incl = cnt;
@@ -476,33 +481,105 @@ __global__ void __launch_bounds__(OTU_BLOCK) k_otu_merge(const uint64_t *__restr
     if (tid == OTU_BLOCK - 1) s_rank[OTU_TILE] = (uint16_t)r;
     __syncthreads();
     const uint32_t nnew = (uint32_t)(n1 - n0);
-    for (uint32_t p = tid; p < tn; p += OTU_BLOCK) {
-        if (s_rank[p + 1] == s_rank[p]) continue;   // dropped
-        const uint64_t key = s_key[p];
-        uint64_t below = 0;
-        if (nnew) {
-            if (staged) {
-                uint32_t lo = 0, hi = nnew;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_new[mid] < key) lo = mid + 1;
-                    else hi = mid;
-                }
-                below = lo;
+    if (!staged) {   // keys and references straight out; k_otu_edges finds the bounds
+        for (uint32_t p = tid; p < tn; p += OTU_BLOCK) {
+            if (s_rank[p + 1] == s_rank[p]) continue;   // dropped
+            const uint64_t key = s_key[p];
+            const uint64_t q = ob + s_rank[p] + lower_bound_u64(knew, n0, n1, key) - n0;
+            out[q] = key;
+            occ[q] = key & 0xFFFFFFFFull;
+        }
+        for (uint64_t j = n0 + tid; j < n1; j += OTU_BLOCK) {
+            const uint64_t key = knew[j];
+            uint32_t lo = 0, hi = tn;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_key[mid] < key) lo = mid + 1;
+                else hi = mid;
             }
-            else below = lower_bound_u64(knew, n0, n1, key) - n0;
+            const uint64_t q = ob + (j - n0) + s_rank[lo];
+            out[q] = key;
+            occ[q] = key & 0xFFFFFFFFull;
         }
-        out[ob + s_rank[p] + below] = key;
+        return;
     }
-    for (uint64_t j = n0 + tid; j < n1; j += OTU_BLOCK) {
-        const uint64_t key = staged ? s_new[j - n0] : knew[j];
-        uint32_t lo = 0, hi = tn;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_key[mid] < key) lo = mid + 1;
-            else hi = mid;
+    // positions in the tile's output, kept in registers until the old keys are read
+    uint64_t kk[OTU_ITEMS], nk[OTU_NITEMS];
+    uint32_t kq[OTU_ITEMS], nq[OTU_NITEMS];
+#pragma unroll
+    for (uint32_t u = 0; u < OTU_ITEMS; u++) {
+        const uint32_t p = tid + u * OTU_BLOCK;
+        kq[u] = 0xFFFFFFFFu;
+        kk[u] = 0;
+        if (p < tn && s_rank[p + 1] != s_rank[p]) {
+            const uint64_t key = s_key[p];
+            uint32_t lo = 0, hi = nnew;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_new[mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            kk[u] = key;
+            kq[u] = s_rank[p] + lo;
         }
-        out[ob + (j - n0) + s_rank[lo]] = key;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < OTU_NITEMS; u++) {
+        const uint32_t j = tid + u * OTU_BLOCK;
+        nq[u] = 0xFFFFFFFFu;
+        nk[u] = 0;
+        if (j < nnew) {
+            const uint64_t key = s_new[j];
+            uint32_t lo = 0, hi = tn;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_key[mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            nk[u] = key;
+            nq[u] = j + s_rank[lo];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t u = 0; u < OTU_ITEMS; u++)
+        if (kq[u] != 0xFFFFFFFFu) s_key[kq[u]] = kk[u];
+#pragma unroll
+    for (uint32_t u = 0; u < OTU_NITEMS; u++)
+        if (nq[u] != 0xFFFFFFFFu) s_key[nq[u]] = nk[u];
+    __syncthreads();
+    const uint32_t T = s_rank[OTU_TILE] + nnew;
+    for (uint32_t q = tid; q < T; q += OTU_BLOCK) {
+        const uint64_t key = s_key[q];
+        const uint32_t lit = (uint32_t)(key >> 32);
+        out[ob + q] = key;
+        occ[ob + q] = key & 0xFFFFFFFFull;
+        if (q > 0 && (uint32_t)(s_key[q - 1] >> 32) != lit) ot[lit].off = ob + q;
+        if (q + 1 < T && (uint32_t)(s_key[q + 1] >> 32) != lit) ot[lit].cap = (uint32_t)(ob + q + 1);
+    }
+}
+
+// the bounds k_otu_merge left: a staged tile's first and last output key against the
+// neighbouring tiles' keys; every key of a tile that was not staged (as k_ot_bounds)
+__global__ void __launch_bounds__(256) k_otu_edges(const uint64_t *__restrict__ out, uint64_t L, const uint64_t *__restrict__ nb,
+                                                   uint64_t ntiles, uint64_t M, const uint64_t *__restrict__ outbase,
+                                                   const uint64_t *__restrict__ tot, OL *__restrict__ ot)
+{
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t T = tot[t], ob = outbase[t];
+        if (!T) continue;
+        const uint64_t n0 = nb[t], n1 = t + 1 < ntiles ? nb[t + 1] : M;
+        const bool staged = n1 - n0 <= OTU_NEW;
+        // staged: thread 0 takes the first key, thread 1 the last (thread 0 both if one)
+        const uint64_t qend = staged ? (threadIdx.x < 2 ? ob + threadIdx.x + 1 : ob) : ob + T;
+        for (uint64_t q0 = ob + threadIdx.x; q0 < qend; q0 += blockDim.x) {
+            const uint64_t q = staged && threadIdx.x == 1 ? ob + T - 1 : q0;
+            if (staged && threadIdx.x == 1 && T == 1) break;
+            const uint32_t lit = (uint32_t)(out[q] >> 32);
+            const bool first = !staged || q == ob, last = !staged || q == ob + T - 1;
+            if (first && (q == 0 || (uint32_t)(out[q - 1] >> 32) != lit)) ot[lit].off = q;
+            if (last && (q + 1 == L || (uint32_t)(out[q + 1] >> 32) != lit)) ot[lit].cap = (uint32_t)(q + 1);
+        }
     }
 }
 
@@ -588,11 +665,15 @@ static bool update_keys(Ctx &c, uint64_t &L)
     read_counters(c, 1);
     L = c.hcnt[CNT_TOTAL];
     ensure(c, c.otk2, L + 1024);
+    ensure(c, c.occ, L + 1024);
+    PFG_HIP(hipMemsetAsync(c.ot.p, 0, c.nlits2 * sizeof(OL), c.stream));
     {
-        KTimer kt(c, K_OT_UPDATE, 8.0 * ((double)L0 + (double)L + (double)M));
+        KTimer kt(c, K_OT_UPDATE, 8.0 * ((double)L0 + 2.0 * (double)L + (double)M));
         k_otu_merge<<<(unsigned)ntiles, OTU_BLOCK, 0, c.stream>>>(c.otk.p, L0, c.otpos.p, knew, M, nb, ntiles, ob,
-                                                                 c.otk2.p);
+                                                                 c.otk2.p, c.occ.p, c.ot.p);
     }
+    k_otu_edges<<<(unsigned)std::min<uint64_t>(ntiles, 65536), 256, 0, c.stream>>>(c.otk2.p, L, nb, ntiles, M, ob, tot,
+                                                                                   c.ot.p);
     std::swap(c.otk, c.otk2);
     std::swap(c.otsz, c.tmpu32a);   // live sizes at this build
     c.path[PATH_OT_UPDATE]++;
@@ -620,6 +701,17 @@ static void create_ot_radix(Ctx &c)
         k_otu_diff<<<grid_for(L, 256, 4096), 256, 0, c.stream>>>(full, c.otk.p, L, (unsigned long long *)c.cnt.p + CNT_A);
         read_counters(c, 2);
         if (c.hcnt[CNT_A]) throw Error(PFG_EINTERNAL, "occurrence-table update differs from a full build");
+        // the lists and list bounds the merge wrote against those the full build derives
+        const uint64_t lo = (L + 2) & ~1ull;
+        ensure(c, c.otchk, lo + 2 * c.nlits2);
+        OL *ot2 = (OL *)(c.otchk.p + lo);
+        PFG_HIP(hipMemsetAsync(ot2, 0, c.nlits2 * sizeof(OL), c.stream));
+        if (L) k_ot_bounds<<<grid_for(L, 256, 16384), 256, 0, c.stream>>>(c.otk.p, L, ot2, c.otchk.p);
+        k_otu_diff<<<grid_for(L, 256, 4096), 256, 0, c.stream>>>(c.otchk.p, c.occ.p, L, (unsigned long long *)c.cnt.p + CNT_A);
+        k_otu_diff<<<grid_for(2 * c.nlits2, 256, 4096), 256, 0, c.stream>>>((const uint64_t *)ot2, (const uint64_t *)c.ot.p,
+                                                                           2 * c.nlits2, (unsigned long long *)c.cnt.p + CNT_A);
+        read_counters(c, 2);
+        if (c.hcnt[CNT_A]) throw Error(PFG_EINTERNAL, "occurrence-table update: lists differ from a full build");
     }
     if (!updated) {
         ensure(c, c.otsz, n + 1);
@@ -633,9 +725,11 @@ static void create_ot_radix(Ctx &c)
             if (sorted == c.otk2.p) std::swap(c.otk, c.otk2);
         }
     }
-    ensure(c, c.occ, L + 1024);
-    PFG_HIP(hipMemsetAsync(c.ot.p, 0, c.nlits2 * sizeof(OL), c.stream));
-    if (L) k_ot_bounds<<<grid_for(L, 256, 16384), 256, 0, c.stream>>>(c.otk.p, L, c.ot.p, c.occ.p);
+    if (!updated) {   // the update's merge wrote the lists and their bounds itself
+        ensure(c, c.occ, L + 1024);
+        PFG_HIP(hipMemsetAsync(c.ot.p, 0, c.nlits2 * sizeof(OL), c.stream));
+        if (L) k_ot_bounds<<<grid_for(L, 256, 16384), 256, 0, c.stream>>>(c.otk.p, L, c.ot.p, c.occ.p);
+    }
     k_ot_finish<<<grid_for(c.nlits2, 256, 8192), 256, 0, c.stream>>>(c.ot.p, c.hist.p, c.nlits2);
     // what the next build updates from; the rewrite map and the removal log start empty
     c.otk_n = L;

@@ -147,6 +147,7 @@ struct Ctx {
     uint64_t otrm_cap = 0;
     DBuf<uint32_t> otpos;       // the update: bit per old key position, set for the dropped keys
     DBuf<uint64_t> otcnt;       // ... and per tile of old keys, the dropped ones
+    DBuf<uint64_t> otchk;       // ot_build 3: the lists and bounds of a full build, for the check
     bool otk_valid = false;
     bool trace = false;         // PFG_TRACE set: per-round progress on stderr
     bool account = false;       // compute algorithmic bytes of the per-variable kernels (extra reductions)

@@ -510,7 +510,7 @@ int pfg_destroy(pfg_ctx *ctx)
     release(c, c.owner); release(c, c.claim); release(c, c.plong); release(c, c.erec); release(c, c.ufirst); release(c, c.cnt);
     release(c, c.scan_tmp); release(c, c.radix_tmp);
     release(c, c.otk); release(c, c.otk2); release(c, c.otsz); release(c, c.otdirty);
-    release(c, c.otrm); release(c, c.otpos); release(c, c.otcnt);
+    release(c, c.otrm); release(c, c.otpos); release(c, c.otcnt); release(c, c.otchk);
     release(c, c.proof); release(c, c.pev); release(c, c.pord); release(c, c.pnev); release(c, c.pbytes);
     release(c, c.pcnt); release(c, c.poff); release(c, c.pboff); release(c, c.pfirst);
     release(c, c.ibase); release(c, c.imu); release(c, c.ib0); release(c, c.inel); release(c, c.icnt); release(c, c.vorg);
