@@ -620,6 +620,7 @@ static void expand_keys(Ctx &c, const uint32_t *szs, const uint64_t *pos, uint64
 static uint64_t *expand_sort(Ctx &c, const uint32_t *szs, const uint64_t *pos, uint64_t L, uint64_t *kbuf, uint64_t *alt)
 {
     expand_keys(c, szs, pos, L, kbuf);
+    KTimer kt(c, K_OT_SORT, 32.0 * L * ((bits_of(c.nlits2 - 1) + 7) / 8));
     return radix_sort_u64_keys(c, kbuf, L, bits_of(c.nlits2 - 1), 32, alt);
 }
 
@@ -720,7 +721,6 @@ static void create_ot_radix(Ctx &c)
         ensure(c, c.otk, L + 1024);
         ensure(c, c.otk2, L + 1024);
         if (L) {
-            KTimer kt(c, K_OT_SORT, 32.0 * L * ((bits_of(c.nlits2 - 1) + 7) / 8));
             const uint64_t *sorted = expand_sort(c, nullptr, c.tmpu64a.p, L, c.otk2.p, c.otk.p);
             if (sorted == c.otk2.p) std::swap(c.otk, c.otk2);
         }

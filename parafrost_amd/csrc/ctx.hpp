@@ -225,6 +225,7 @@ struct Ctx {
     KStat kstat[K_NUM];
     std::vector<std::array<hipEvent_t, 2>> kev;     // event pairs of pending kernel timers
     std::vector<std::pair<int, double>> kpend;      // (kernel, algorithmic bytes) per pending pair
+    int kopen = 0;                                  // timers whose end event is not recorded yet
     hipEvent_t ev[8][2] = {};   // per nesting depth of pass timers
     int ev_depth = 0;
     std::string last_error;

@@ -84,9 +84,12 @@ const char *const kernel_symbols[K_NUM] = {
     "k_sort_pv_wave", "k_sub_grp", "k_ve_detect", "k_ve_apply", nullptr, nullptr, "k_ere_probe", "k_gc_move", "k_count",
     nullptr, "k_count_inst", nullptr, "k_ve_fun", "k_ve_res", "k_otu_merge"};
 
+// Pending event pairs are resolved in batches, never while a timer is open (its end
+// event is not recorded yet).
 KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
 {
-    if (c.kpend.size() >= 2048) resolve_ktimers(c);
+    if (c.kpend.size() >= 2048 && !c.kopen) resolve_ktimers(c);
+    c.kopen++;
     slot = c.kpend.size();
     if (c.kev.size() <= slot) {
         std::array<hipEvent_t, 2> e{};
@@ -98,11 +101,15 @@ KTimer::KTimer(Ctx &cc, int kid, double bytes) : c(cc)
     (void)hipEventRecord(c.kev[slot][0], c.stream);
 }
 
-KTimer::~KTimer() { (void)hipEventRecord(c.kev[slot][1], c.stream); }
+KTimer::~KTimer()
+{
+    (void)hipEventRecord(c.kev[slot][1], c.stream);
+    c.kopen--;
+}
 
 void resolve_ktimers(Ctx &c)
 {
-    if (c.kpend.empty()) return;
+    if (c.kpend.empty() || c.kopen) return;
     (void)hipStreamSynchronize(c.stream);
     for (size_t i = 0; i < c.kpend.size(); i++) {
         float ms = 0.f;
@@ -113,6 +120,8 @@ void resolve_ktimers(Ctx &c)
             k.launches++;
         }
     }
+    // a failed query must not stay behind as the calling thread's last HIP error
+    (void)hipGetLastError();
     c.kpend.clear();
 }
 
@@ -306,18 +315,22 @@ struct Sigma {
                 if (!running) break;
                 // Compaction keeps clause order, so its schedule is free: between rounds it
                 // runs only when deleted images take a tenth of the pool or more (the passes
-                // skip deleted clauses either way; it looks at the last counts made, the
-                // clause counts of the call's start); the call ends with one as always.
+                // skip deleted clauses either way; a round that may compact counts the
+                // instances' clauses first); the call ends with one as always.
                 const int times = round + 1;
-                if (c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0) {
+                const bool gc_due = c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0;
+                if (gc_due) {
+                    {
+                        PassTimer t(c, &c.times.count);
+                        count_inst(c, nc, nl);
+                    }
                     uint64_t live = 0;
                     for (uint32_t i = 0; i < ni; i++) live += 3ull * (uint64_t)nc[i] + (uint64_t)nl[i];
                     if (10 * live < 9 * c.msz) shrinkSimp();
                 }
-                // the literals each instance has after the last round, summed from this
-                // round's histogram (the same counts a pass over the clauses makes)
                 createOT();
-                {
+                if (!gc_due) {   // the literals each instance has after the last round, summed
+                                 // from this round's histogram (the counts a clause pass makes)
                     PassTimer t(c, &c.times.count);
                     inst_lits(c, nl);
                 }
@@ -414,7 +427,15 @@ struct pfg_ctx {
     Ctx &c = ctx->c;               \
     try {                          \
         PFG_HIP(hipSetDevice(c.device));
+// a kernel launch that failed in the call is reported here, not left behind as the
+// calling thread's last HIP error
+#define API_LAUNCH_CHECK                                                                            \
+    {                                                                                               \
+        const hipError_t le_ = hipGetLastError();                                                   \
+        if (le_ != hipSuccess) throw ::pfg::Error(PFG_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(le_)); \
+    }
 #define API_END                                  \
+    API_LAUNCH_CHECK                             \
     }                                            \
     catch (const Error &e) {                     \
         c.last_error = e.what();                 \
@@ -427,6 +448,7 @@ struct pfg_ctx {
     return c.unsat ? PFG_EUNSAT : PFG_OK;
 // bookkeeping calls (statistics, settings) succeed whatever the formula's state
 #define API_END_OK                               \
+    API_LAUNCH_CHECK                             \
     }                                            \
     catch (const Error &e) {                     \
         c.last_error = e.what();                 \

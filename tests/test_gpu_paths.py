@@ -220,3 +220,26 @@ def test_function_gates_on_general_formulas(seed):
     print(f"function gates general seed={seed}", flush=True)
     cnf = adversarial(seed) if seed % 2 else gen.scalefree(3000, 9000, 2.5, 0.3, 70 + seed)
     ls.run_lockstep(cnf, ve_fun_en=1, phases=3, mu_pos=40, mu_neg=40, proof_en=1)
+
+
+def test_many_timed_calls_leave_no_hip_error():
+    """Thousands of timed launches in one context, with the sorts of full table builds
+    every round (ot_build 2): the pending event pairs are resolved between timers,
+    never while one is open, and no HIP error is left behind for the calling thread
+    (torch launches in between and afterwards succeed)."""
+    import torch
+    cnf = gen.uniform_ksat(3000, 12600, 3, 77)
+    g = pf.Simplifier(cnf.nvars, pf.default_opts())
+    try:
+        g.set_tuning(ot_build=2)
+        x = None
+        for k in range(60):
+            g.upload(cnf)
+            g.sigmify()
+            x = torch.ones(4, device="cuda") * float(k)   # a torch launch on this thread
+        torch.cuda.synchronize()
+        assert float(x.sum()) == 4.0 * 59
+        st = g.kernel_times()
+        assert st["ot_sort"]["launches"] >= 60 and st["ot_fill"]["launches"] >= 60
+    finally:
+        g.close()
