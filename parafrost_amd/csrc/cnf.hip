@@ -1046,34 +1046,6 @@ __global__ void __launch_bounds__(CI_BLOCK) k_count_inst(const uint32_t *__restr
     }
 }
 
-// Live literals per instance from a fresh table's histogram (hist[l] = live clauses
-// holding l, so an instance's sum over its literals is its live literal count): a
-// block per instance sums its literal range (padding variables have no occurrences;
-// ibase[ninst] ends the last instance's range, a context may hold more variables).
-__global__ void __launch_bounds__(256) k_inst_lits(const uint32_t *__restrict__ hist, const uint32_t *__restrict__ ibase,
-                                                  uint32_t ninst, unsigned long long *__restrict__ out)
-{
-    for (uint32_t i = blockIdx.x; i < ninst; i += gridDim.x) {
-        const uint64_t l0 = 2ull * ((uint64_t)ibase[i] + 1), l1 = 2ull * ((uint64_t)ibase[i + 1] + 1);
-        unsigned long long sum = 0;
-        for (uint64_t l = l0 + threadIdx.x; l < l1; l += blockDim.x) sum += hist[l];
-        sum = block_sum(sum);
-        if (threadIdx.x == 0) out[i] = sum;
-    }
-}
-
-void inst_lits(Ctx &c, std::vector<int64_t> &lits)
-{
-    if (!c.hist_live) throw Error(PFG_EINTERNAL, "instance literal counts need a live histogram");
-    k_inst_lits<<<std::min<uint32_t>(c.ninst, 65536u), 256, 0, c.stream>>>(c.hist.p, c.ibase.p, c.ninst,
-                                                                            (unsigned long long *)c.icnt.p);
-    std::vector<uint64_t> h(c.ninst);
-    PFG_HIP(hipMemcpyAsync(h.data(), c.icnt.p, 8ull * c.ninst, hipMemcpyDeviceToHost, c.stream));
-    PFG_HIP(hipStreamSynchronize(c.stream));
-    lits.resize(c.ninst);
-    for (uint32_t i = 0; i < c.ninst; i++) lits[i] = (int64_t)h[i];
-}
-
 void count_inst(Ctx &c, std::vector<int64_t> &cls, std::vector<int64_t> &lits)
 {
     PFG_HIP(hipMemsetAsync(c.icnt.p, 0, 16ull * c.ninst, c.stream));

@@ -20,7 +20,6 @@ uint64_t compact_u32(Ctx &c, const uint32_t *in, const uint32_t *flags, uint64_t
 void enqueue_units_host(Ctx &c, const uint32_t *lits, uint32_t n);
 // batches
 void count_inst(Ctx &c, std::vector<int64_t> &cls, std::vector<int64_t> &lits);
-void inst_lits(Ctx &c, std::vector<int64_t> &lits);   // live literals per instance from a fresh table
 void partition_pvs(Ctx &c, const std::vector<uint8_t> &cls, uint32_t &na, uint32_t &nb);
 void batch_images(Ctx &c, uint32_t *out, std::vector<uint64_t> &start, std::vector<uint64_t> &cstart);
 void batch_export(Ctx &c, const uint32_t *img, uint64_t c0, uint64_t ncls, uint64_t nlits, uint32_t *d_lits,

@@ -308,41 +308,6 @@ struct Sigma {
         bool intr = false;
         for (;;) {
             if ((intr = interrupted())) break;
-            if (round > 0) {
-                // no instance left running: no table to build
-                bool running = false;
-                for (uint32_t i = 0; i < ni && !running; i++) running = st[i] == B_RUN;
-                if (!running) break;
-                // Compaction keeps clause order, so its schedule is free: between rounds it
-                // runs only when deleted images take a tenth of the pool or more (the passes
-                // skip deleted clauses either way; a round that may compact counts the
-                // instances' clauses first); the call ends with one as always.
-                const int times = round + 1;
-                const bool gc_due = c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0;
-                if (gc_due) {
-                    {
-                        PassTimer t(c, &c.times.count);
-                        count_inst(c, nc, nl);
-                    }
-                    uint64_t live = 0;
-                    for (uint32_t i = 0; i < ni; i++) live += 3ull * (uint64_t)nc[i] + (uint64_t)nl[i];
-                    if (10 * live < 9 * c.msz) shrinkSimp();
-                }
-                createOT();
-                if (!gc_due) {   // the literals each instance has after the last round, summed
-                                 // from this round's histogram (the counts a clause pass makes)
-                    PassTimer t(c, &c.times.count);
-                    inst_lits(c, nl);
-                }
-                for (uint32_t i = 0; i < ni; i++) {
-                    if (cls[i] != 1 || st[i] != B_RUN) continue;
-                    diff[i] = before[i] - nl[i];
-                    before[i] = nl[i];
-                    phase[i]++;
-                    mu[i]++;
-                    mu[i] += (phase[i] == c.opts.phases);
-                }
-            }
             bool any = false;
             for (uint32_t i = 0; i < ni; i++) {
                 if (st[i] == B_RUN && before[i] == 0) st[i] = B_DONE;
@@ -359,7 +324,16 @@ struct Sigma {
                 fprintf(stderr, "[pfg] batch round %d: %u of %u instances running, literals %lld of %lld\n", round, run, ni,
                         (long long)lits_run, (long long)lits_all);
             }
-            if (round == 0) createOT();
+            // Compaction keeps clause order, so its schedule is free: between rounds it runs
+            // only when deleted images take a tenth of the pool or more (the passes skip
+            // deleted clauses either way); the call ends with one as always.
+            const int times = round + 1;
+            if (times > 1 && c.opts.shrink_rate && (times % c.opts.shrink_rate) == 0) {
+                uint64_t live = 0;
+                for (uint32_t i = 0; i < ni; i++) live += 3ull * (uint64_t)nc[i] + (uint64_t)nl[i];
+                if (10 * live < 9 * c.msz) shrinkSimp();
+            }
+            createOT();
             propagate();
             collect_conflicts(st);
             for (uint32_t i = 0; i < ni; i++) c.imu_host[i] = st[i] == B_RUN ? (uint8_t)std::min<uint32_t>(mu[i], 254u) : 0xFFu;
@@ -395,7 +369,18 @@ struct Sigma {
                 collect_conflicts(st);
                 BCE();
             }
-            // the round's results are counted at the next round's table build (above)
+            {
+                PassTimer t(c, &c.times.count);
+                count_inst(c, nc, nl);
+            }
+            for (uint32_t i = 0; i < ni; i++) {
+                if (cls[i] != 1 || st[i] != B_RUN) continue;
+                diff[i] = before[i] - nl[i];
+                before[i] = nl[i];
+                phase[i]++;
+                mu[i]++;
+                mu[i] += (phase[i] == c.opts.phases);
+            }
             if (++round > (1 << 20)) throw Error(PFG_EINTERNAL, "batch did not finish");
         }
         if (c.ntrail > c.propagated) {
