@@ -63,6 +63,11 @@ typedef struct pfg_opts {
                                   src/gpu/options.cpp:31, function.cuh:263-311; src/cpu has none: the core
                                   variables are numbered by first appearance, which no result
                                   depends on); off by default, as in the reference */
+    int32_t ve_atomic_en;      /* 0    BVE places each eliminated variable's resolvents, units and model
+                                  records by atomic reservation instead of in elected order (the GPU
+                                  reference's veatomic, src/gpu/options.cu:24, ve_k kernels.cu:298-373,
+                                  off by default): the same clauses and records, their order in the pool
+                                  and on the stack set by the schedule; single-instance contexts only */
 } pfg_opts;
 
 /* ---- Batches (no counterpart call in the reference, which simplifies one formula per

@@ -454,6 +454,7 @@ void pfg_default_opts(pfg_opts *o)
     o->sub_limit = 3000; o->ere_limit = 3000; o->ere_max_resolvent = 1000;
     o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
     o->bce_en = 0; o->bce_limit = 3000; o->proof_en = 0; o->ve_fun_en = 0;
+    o->ve_atomic_en = 0;
 }
 
 
@@ -679,6 +680,8 @@ int pfg_create_batch(pfg_ctx **out, int device, uint64_t hbm_budget, uint32_t ni
 {
     if (!out || !ninst || !nvars) return PFG_EINVAL;
     if (opts && opts->proof_en) return PFG_EINVAL;   // the proof is one instance's log
+    // a batch groups each instance's model records from the elected order
+    if (opts && opts->ve_atomic_en) return PFG_EINVAL;
     for (uint32_t i = 0; i < ninst; i++)
         if (!nvars[i]) return PFG_EINVAL;
     std::vector<uint32_t> base;

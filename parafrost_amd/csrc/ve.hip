@@ -1038,8 +1038,9 @@ __device__ __forceinline__ void substitute_one(uint32_t dx, uint32_t *org, uint3
 }
 
 __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type, const uint32_t *__restrict__ aux,
-                           uint32_t npvs, const uint64_t *__restrict__ roff, const uint64_t *__restrict__ woff,
-                           const uint64_t *__restrict__ uoff, const uint64_t *__restrict__ moff)
+                           uint32_t npvs, const uint32_t *__restrict__ nres, const uint64_t *__restrict__ roff,
+                           const uint64_t *__restrict__ woff, const uint64_t *__restrict__ uoff,
+                           const uint64_t *__restrict__ moff)
 {
     for (uint64_t q = gtid(); q < npvs; q += gstride()) {
         const uint32_t tw = type[q], t = tw & 0xFFu;
@@ -1071,7 +1072,7 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
         }
         else {
             const bool pbig = aux[q] != 0;
-            if ((t == T_SUBST || t == T_RES || t == T_CORE) && roff[q + 1] > roff[q]) {   // xsubstitute/xresolve only if nAddedCls
+            if ((t == T_SUBST || t == T_RES || t == T_CORE) && nres[q]) {   // xsubstitute/xresolve only if nAddedCls
                 uint32_t dx = p, fx = n;
                 if (A.ot[dx].sz > A.ot[fx].sz) { dx = n; fx = p; }
                 const OL om = A.ot[dx], oo = A.ot[fx];
@@ -1120,6 +1121,38 @@ __global__ void k_ve_apply(VeArgs A, VeOut O, const uint32_t *__restrict__ type,
     }
 }
 
+// The reference's atomic BVE (ve_k, src/gpu/kernels.cu:298-373, option veatomic,
+// src/gpu/options.cu:24, off by default) reserves each eliminated variable's space in
+// the pool, the unit list and the model stack with atomics as it goes, so their order
+// depends on the schedule. With ve_atomic_en the detection is the same and each block
+// reserves the regions of its 256 elected variables with one atomic per counter
+// (elected order inside a block, block order as the blocks arrive) instead of the four
+// scans; cur[0..3] count references, units, words and model words.
+__global__ void __launch_bounds__(256) k_ve_reserve(const uint32_t *__restrict__ nres,
+                                                    const uint32_t *__restrict__ nunits,
+                                                    const uint64_t *__restrict__ nwords,
+                                                    const uint64_t *__restrict__ nmodel, uint32_t npvs,
+                                                    uint64_t *__restrict__ roff, uint64_t *__restrict__ uoff,
+                                                    uint64_t *__restrict__ woff, uint64_t *__restrict__ moff,
+                                                    unsigned long long *cur)
+{
+    // the loop bound is the same for every thread of a block: block_reserve needs all of them
+    for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x; q0 < npvs; q0 += gstride()) {
+        const uint64_t q = q0 + threadIdx.x;
+        const bool in = q < npvs;
+        const uint64_t r = block_reserve(in ? nres[q] : 0u, cur + 0);
+        const uint64_t u = block_reserve(in ? nunits[q] : 0u, cur + 1);
+        const uint64_t w = block_reserve(in ? nwords[q] : 0u, cur + 2);
+        const uint64_t m = block_reserve(in ? nmodel[q] : 0u, cur + 3);
+        if (in) {
+            roff[q] = r;
+            uoff[q] = u;
+            woff[q] = w;
+            moff[q] = m;
+        }
+    }
+}
+
 // batch: the starts of the non-empty record chunks of one pass, appended in elected
 // order (= stack order) to the chunk list
 __global__ void k_chunk_flags(const uint64_t *__restrict__ moff, uint32_t np, uint32_t *__restrict__ flags)
@@ -1157,7 +1190,7 @@ void ve(Ctx &c)
     ensure(c, c.tmpu32a, 2ull * np + 2);
     ensure(c, c.tmpu32b, 2ull * np + 2);
     ensure(c, c.tmpu64a, 2ull * np + 2);
-    ensure(c, c.tmpu64b, 4ull * np + 8);
+    ensure(c, c.tmpu64b, 4ull * np + 12);   // + the reservation counters of ve_atomic_en
     uint32_t *type = c.tmpu32a.p, *aux = c.tmpu32a.p + np + 1;
     uint32_t *nres = c.tmpu32b.p, *nunits = c.tmpu32b.p + np + 1;
     uint64_t *nwords = c.tmpu64a.p, *nmodel = c.tmpu64a.p + np + 1;
@@ -1182,16 +1215,24 @@ void ve(Ctx &c)
         KTimer kt(c, K_VE_RES, 0.0);
         k_ve_res<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, DO, c.pvs.p, np);
     }
-    // exclusive scans with the totals stored one past the end
-    scan_u32_to_u64(c, nres, roff, np, roff + np);
-    scan_u32_to_u64(c, nunits, uoff, np, uoff + np);
-    scan_u64(c, nwords, woff, np, woff + np);
-    scan_u64(c, nmodel, moff, np, moff + np);
     uint64_t tot[4 * 2];
-    PFG_HIP(hipMemcpyAsync(&tot[0], roff + np, 8, hipMemcpyDeviceToHost, c.stream));
-    PFG_HIP(hipMemcpyAsync(&tot[1], uoff + np, 8, hipMemcpyDeviceToHost, c.stream));
-    PFG_HIP(hipMemcpyAsync(&tot[2], woff + np, 8, hipMemcpyDeviceToHost, c.stream));
-    PFG_HIP(hipMemcpyAsync(&tot[3], moff + np, 8, hipMemcpyDeviceToHost, c.stream));
+    if (c.opts.ve_atomic_en && !c.batch) {
+        uint64_t *cur = c.tmpu64b.p + 4ull * np + 8;
+        PFG_HIP(hipMemsetAsync(cur, 0, 32, c.stream));
+        k_ve_reserve<<<grid_for(np, 256, 8192), 256, 0, c.stream>>>(nres, nunits, nwords, nmodel, np, roff, uoff,
+                                                                    woff, moff, (unsigned long long *)cur);
+        PFG_HIP(hipMemcpyAsync(tot, cur, 32, hipMemcpyDeviceToHost, c.stream));
+    }
+    else {   // exclusive scans in elected order, the totals stored one past the end
+        scan_u32_to_u64(c, nres, roff, np, roff + np);
+        scan_u32_to_u64(c, nunits, uoff, np, uoff + np);
+        scan_u64(c, nwords, woff, np, woff + np);
+        scan_u64(c, nmodel, moff, np, moff + np);
+        PFG_HIP(hipMemcpyAsync(&tot[0], roff + np, 8, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipMemcpyAsync(&tot[1], uoff + np, 8, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipMemcpyAsync(&tot[2], woff + np, 8, hipMemcpyDeviceToHost, c.stream));
+        PFG_HIP(hipMemcpyAsync(&tot[3], moff + np, 8, hipMemcpyDeviceToHost, c.stream));
+    }
     PFG_HIP(hipStreamSynchronize(c.stream));
     const uint64_t add_refs = tot[0], add_units = tot[1], add_words = tot[2], add_model = tot[3];
     ensure(c, c.mem, c.msz + add_words + 1, true);
@@ -1205,7 +1246,7 @@ void ve(Ctx &c)
             c.otdirty.p, c.otdirty.p ? c.otdirty_words : 0, c.otrm.p, c.otrm_cap, (unsigned long long *)c.cnt.p};
     {
         KTimer kt(c, K_VE_APPLY, 4.0 * add_words + 8.0 * add_refs + 4.0 * add_model);
-        k_ve_apply<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, O, type, aux, np, roff, woff, uoff, moff);
+        k_ve_apply<<<grid_for(np, 64, 16384), 64, 0, c.stream>>>(A, O, type, aux, np, nres, roff, woff, uoff, moff);
     }
     if (c.batch && add_model) append_chunks(c, moff, np);
     c.nrefs += add_refs;
