@@ -68,6 +68,13 @@ typedef struct pfg_opts {
                                   reference's veatomic, src/gpu/options.cu:24, ve_k kernels.cu:298-373,
                                   off by default): the same clauses and records, their order in the pool
                                   and on the stack set by the schedule; single-instance contexts only */
+    uint32_t lcve_max;         /* 0    the GPU reference's lcvemax (src/gpu/options.cpp:36, lcve.cu:68; 3000
+                                  there, no such limit in src/cpu): the election stops at a candidate with
+                                  more live occurrences of either literal; 0: no limit */
+    uint32_t lcve_clause_max;  /* 0    the GPU reference's lcveclausemax (options.cpp:37, lcve.cu:105-127;
+                                  30000 there): a candidate whose positive list holds a live clause longer
+                                  than this is passed over; one whose negative list does freezes its positive
+                                  list's neighbours and is not elected; 0: no limit */
 } pfg_opts;
 
 /* ---- Batches (no counterpart call in the reference, which simplifies one formula per

@@ -338,6 +338,7 @@ void pfo_default_opts(pfo_opts *o)
     o->sub_limit = 3000; o->ere_limit = 3000; o->ere_max_resolvent = 1000;
     o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
     o->bce_en = 0; o->bce_limit = 3000; o->proof_en = 0; o->ve_fun_en = 0;
+    o->lcve_max = 0; o->lcve_clause_max = 0;
 }
 
 pfo *pfo_create(uint32_t nvars, const pfo_opts *o, const uint32_t *lits, const uint64_t *offs,
@@ -664,8 +665,18 @@ static void dep_freeze(pfo *s, ol_t *ol, uint32_t cand, uint32_t pt, uint32_t nt
     }
 }
 
+static int has_long_clause(const pfo *s, const ol_t *ol, uint32_t limit)
+{   /* the size test of the GPU reference's depFreeze (src/gpu/lcve.cu:105-120) */
+    for (int32_t i = 0; i < ol->sz; i++) {
+        const uint32_t *c = CL(s, ol->d[i]);
+        if (!c_del(c) && (uint32_t)c_size(c) > limit) return 1;
+    }
+    return 0;
+}
+
 int pfo_lcve(pfo *s)
-{   /* LCVE (lcve.cpp:46-81) */
+{   /* LCVE (lcve.cpp:46-81), with the GPU reference's two optional limits
+       (src/gpu/lcve.cu:68, 105-127; both off by default) */
     var_reorder(s);
     s->npvs = 0;
     s->nfstack = 0;
@@ -678,7 +689,17 @@ int pfo_lcve(pfo *s)
         const uint32_t p = 2 * cand, n = p + 1;
         const uint32_t psz = (uint32_t)s->ot[p].sz, nsz = (uint32_t)s->ot[n].sz;
         if (s->ps[cand] == 0 && s->ns[cand] == 0) continue;
+        if (s->opts.lcve_max && (s->ps[cand] > s->opts.lcve_max || s->ns[cand] > s->opts.lcve_max)) break;
         if (psz >= pt && nsz >= nt) break;
+        if (s->opts.lcve_clause_max) {
+            /* depFreeze(cand, ot[p]) && depFreeze(cand, ot[n]): a failing list's own
+               freezes are undone, the positive list's stay when only the negative fails */
+            if (has_long_clause(s, &s->ot[p], s->opts.lcve_clause_max)) continue;
+            if (has_long_clause(s, &s->ot[n], s->opts.lcve_clause_max)) {
+                dep_freeze(s, &s->ot[p], cand, pt, nt);
+                continue;
+            }
+        }
         s->pvs[s->npvs++] = cand;
         dep_freeze(s, &s->ot[p], cand, pt, nt);
         dep_freeze(s, &s->ot[n], cand, pt, nt);

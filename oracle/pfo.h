@@ -42,6 +42,12 @@ typedef struct pfo_opts {
     int32_t ve_fun_en;         /* GPU reference's vefun (src/gpu/options.cpp:31): function-table gates
                                   (no src/cpu counterpart: the variable numbering is chosen
                                   here; no result depends on it) = 0 */
+    uint32_t lcve_max;         /* GPU reference's lcvemax (src/gpu/options.cpp:36, lcve.cu:68): the walk
+                                  stops at a candidate with more live occurrences of either literal
+                                  (0: no limit; src/cpu has none) = 0 */
+    uint32_t lcve_clause_max;  /* GPU reference's lcveclausemax (options.cpp:37, lcve.cu:105-120): a
+                                  candidate whose list holds a live clause longer than this is not
+                                  elected (0: no limit) = 0 */
 } pfo_opts;
 
 typedef struct pfo pfo;

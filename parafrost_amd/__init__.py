@@ -31,7 +31,8 @@ class Opts(ctypes.Structure):
                 ("ere_max_resolvent", ctypes.c_int32), ("sub_en", ctypes.c_int32), ("ve_en", ctypes.c_int32),
                 ("ere_en", ctypes.c_int32), ("lbd_tier1", ctypes.c_int32), ("stable_sort", ctypes.c_int32),
                 ("bce_en", ctypes.c_int32), ("bce_limit", ctypes.c_int32), ("proof_en", ctypes.c_int32),
-                ("ve_fun_en", ctypes.c_int32), ("ve_atomic_en", ctypes.c_int32)]
+                ("ve_fun_en", ctypes.c_int32), ("ve_atomic_en", ctypes.c_int32),
+                ("lcve_max", ctypes.c_uint32), ("lcve_clause_max", ctypes.c_uint32)]
 
 
 class Times(ctypes.Structure):

@@ -29,7 +29,10 @@
 
 namespace pfg {
 
-enum { K_SKIP = 0, K_CAND = 1, K_DYNBREAK = 2, K_BREAK = 3 };
+// K_HALF (option lcve_clause_max only): a candidate whose negative list holds a clause
+// over the limit; reached unfrozen it freezes the cold neighbours of its positive list
+// and is not elected (src/gpu/lcve.cu:105-127)
+enum { K_SKIP = 0, K_CAND = 1, K_DYNBREAK = 2, K_BREAK = 3, K_HALF = 4 };
 enum { S_UNDEC = 0, S_IN = 1, S_OUT = 2 };
 constexpr uint32_t NO_CAND = 0xFFFFFFFFu;
 
@@ -55,6 +58,13 @@ struct InstArgs {
     uint32_t mu_pos, mu_neg;
     unsigned long long *ib0, *ibrk, *inel;   // inel is counted for batches only
     int batch;
+    // the GPU reference's optional limits (src/gpu/lcve.cu:68, 105-127; off when 0 / null):
+    // a candidate with more than lmax live occurrences of either literal is a break;
+    // lng[lit] marks literals with a live clause over lcve_clause_max, half[v] the
+    // K_HALF candidates
+    uint32_t lmax;
+    const uint8_t *lng;
+    uint8_t *half;
 };
 
 __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, const uint32_t *__restrict__ hist,
@@ -74,10 +84,16 @@ __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, co
             // iassumed (lcve.cpp:57) and eliminated / assigned (lcve.cpp:58) variables are passed over
             if (mi != IMU_OFF && !ifrozen[v] && !vstate[v] && (ps || ns)) {
                 const uint32_t pt = I.mu_pos << mi, nt = I.mu_neg << mi;
-                const bool hot = ot[2 * v].sz >= pt && ot[2 * v + 1].sz >= nt;
+                const bool hot = (ot[2 * v].sz >= pt && ot[2 * v + 1].sz >= nt) ||
+                                 (I.lmax && (ps > I.lmax || ns > I.lmax));
                 const bool freezable = ps < pt || ns < nt;
                 k = (hot && !freezable) ? K_BREAK : hot ? K_DYNBREAK : K_CAND;
+                // depFreeze on the positive list, then the negative one, each failing on a
+                // long clause: a positive failure leaves nothing, a negative one the
+                // positive list's freezes
+                if (k == K_CAND && I.lng) k = I.lng[2 * v] ? K_SKIP : I.lng[2 * v + 1] ? K_HALF : K_CAND;
             }
+            if (I.half) I.half[v] = k == K_HALF;
             kind[i] = k;
             status[i] = S_UNDEC;
             blocked[i] = 0;
@@ -93,6 +109,17 @@ __global__ void k_classify(const uint32_t *__restrict__ elig, uint32_t nvars, co
             if (__lane_id() == leader) atomicMin(&I.ib0[li], (unsigned long long)i);
             m &= ~grp;
         }
+    }
+}
+
+// option lcve_clause_max: the literals of every live clause longer than the limit
+__global__ void k_long_lits(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs, uint64_t n,
+                            uint32_t limit, uint8_t *__restrict__ lng)
+{
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t *c = mem + refs[i];
+        if (c_del(c) || (uint32_t)c_size(c) <= limit) continue;
+        for (int t = 0; t < c_size(c); t++) lng[c[3 + t]] = 1;
     }
 }
 
@@ -116,8 +143,8 @@ __global__ void __launch_bounds__(256) k_crank(const uint32_t *__restrict__ elig
         if (i < nvars) {
             const uint32_t v = elig[i];
             const uint8_t k = kind[i];
-            if (k == K_CAND || k == K_DYNBREAK) cand = i < I.ib0[find_inst(I.ibase, I.ninst, v)];
-            crank[v] = cand ? (((uint32_t)i << 1) | (k == K_CAND ? 1u : 0u)) : NO_CAND;
+            if (k == K_CAND || k == K_DYNBREAK || k == K_HALF) cand = i < I.ib0[find_inst(I.ibase, I.ninst, v)];
+            crank[v] = cand ? (((uint32_t)i << 1) | (k != K_DYNBREAK ? 1u : 0u)) : NO_CAND;
         }
         block_append(cand, (uint32_t)i, work, nwork);
     }
@@ -129,12 +156,20 @@ __global__ void __launch_bounds__(256) k_crank(const uint32_t *__restrict__ elig
 // rounds stream records instead of clauses and rank lookups. Round 0's pass A is
 // fused in: nothing is elected yet, so every candidate ranked after the clause's
 // earliest freezing candidate is blocked. The first ELECT_CACHE candidate codes of
-// a clause stay in LDS between the counting and the writing loop.
+// a clause stay in LDS between the counting and the writing loop. With HALF (option
+// lcve_clause_max) the code of a K_HALF candidate's negative literal has no freezer bit:
+// its freezing reaches the clauses of its positive list only.
 constexpr int ELECT_CACHE = 16;
+template <bool HALF>
 __global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t *__restrict__ refs, uint64_t n,
                                                    uint32_t *__restrict__ rec, uint64_t *__restrict__ active,
-                                                   unsigned long long *nactive, unsigned long long *cnt)
+                                                   unsigned long long *nactive, unsigned long long *cnt,
+                                                   const uint8_t *__restrict__ half)
 {
+    auto code = [&](uint32_t a, uint32_t x) -> uint32_t {
+        if (HALF && (a & 1u) && x != NO_CAND && half[a >> 1]) return x & ~1u;
+        return x;
+    };
     __shared__ uint32_t s_cr[ELECT_CACHE][256];
     const unsigned tid = threadIdx.x;
     for (uint64_t b = xcd_block() * blockDim.x; b < n; b += gstride()) {
@@ -155,14 +190,14 @@ __global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t 
                 int t = 0;
                 for (; t + 4 <= sz; t += 4) {   // four rank gathers in flight, taken in order
                     const uint32_t a0 = c[3 + t], a1 = c[4 + t], a2 = c[5 + t], a3 = c[6 + t];
-                    const uint32_t x0 = A.crank[a0 >> 1], x1 = A.crank[a1 >> 1], x2 = A.crank[a2 >> 1],
-                                   x3 = A.crank[a3 >> 1];
+                    const uint32_t x0 = code(a0, A.crank[a0 >> 1]), x1 = code(a1, A.crank[a1 >> 1]),
+                                   x2 = code(a2, A.crank[a2 >> 1]), x3 = code(a3, A.crank[a3 >> 1]);
                     take(x0);
                     take(x1);
                     take(x2);
                     take(x3);
                 }
-                for (; t < sz; t++) take(A.crank[c[3 + t] >> 1]);
+                for (; t < sz; t++) take(code(c[3 + t], A.crank[c[3 + t] >> 1]));
             }
         }
         const bool keep = k >= 2u;
@@ -180,7 +215,7 @@ __global__ void __launch_bounds__(256) k_elect_init(ElectArgs A, const uint64_t 
             if (k > (uint32_t)ELECT_CACHE) {
                 uint32_t w = 0;
                 for (int t = 0; t < sz; t++) {
-                    const uint32_t x = A.crank[c[3 + t] >> 1];
+                    const uint32_t x = code(c[3 + t], A.crank[c[3 + t] >> 1]);
                     if (x == NO_CAND) continue;
                     if (++w <= (uint32_t)ELECT_CACHE) continue;
                     const uint32_t j = x >> 1;
@@ -210,7 +245,13 @@ __device__ __forceinline__ void elect_pass_a(const ElectArgs &A, const uint32_t 
             r = __builtin_nontemporal_load(active + i);
             const uint32_t *rr = rec + r;
             const uint32_t k = __builtin_nontemporal_load(rr);
-            bool in_freezer = false;
+            // fz: the earliest elected candidate whose code freezes (none: ~0). Only the
+            // undecided candidates ranked after it are frozen by it: one ranked before
+            // it can only be a candidate that does not freeze through this clause (a
+            // dynamic break, the negative literal of a K_HALF candidate), which an
+            // earlier freezing candidate of the clause would have blocked otherwise,
+            // and which the sequential walk reaches before the freezer.
+            uint32_t fz = 0xFFFFFFFFu;
             uint32_t m = 0xFFFFFFFFu;
             uint32_t undec = 0, alive = 0;
             constexpr uint32_t KR = 8;   // undecided candidates of the first KR entries are kept in registers
@@ -219,7 +260,7 @@ __device__ __forceinline__ void elect_pass_a(const ElectArgs &A, const uint32_t 
             auto take = [&](uint32_t t, uint32_t x, uint8_t st) {
                 const uint32_t j = x >> 1;
                 const bool freezer = x & 1u;
-                if (st == S_IN && freezer) in_freezer = true;
+                if (st == S_IN && freezer && j < fz) fz = j;
                 if (st == S_UNDEC) {
                     undec++;
                     if (freezer && j < m) m = j;
@@ -247,17 +288,17 @@ __device__ __forceinline__ void elect_pass_a(const ElectArgs &A, const uint32_t 
                 // gets the same mark again, or a blocked mark that pass B ignores
                 for (uint32_t q = 0; q < nund; q++) {
                     const uint32_t j = und[q];
-                    if (in_freezer) A.status[j] = S_OUT;
+                    if (j > fz) A.status[j] = S_OUT;
                     else if (j > m) A.blocked[j] = 1;
                 }
                 for (uint32_t t = KR + 1; t <= k; t++) {
                     const uint32_t j = rr[t] >> 1;
                     if (A.status[j] != S_UNDEC) continue;
-                    if (in_freezer) A.status[j] = S_OUT;
+                    if (j > fz) A.status[j] = S_OUT;
                     else if (j > m) A.blocked[j] = 1;
                 }
             }
-            keep = !in_freezer && undec >= 1u && alive >= 2u;
+            keep = fz == 0xFFFFFFFFu && undec >= 1u && alive >= 2u;
         }
         const uint64_t o = block_reserve(keep ? 1ull : 0ull, nnext);
         if (keep) next[o] = r;
@@ -415,7 +456,8 @@ static int bits_of32(uint32_t x)
 static InstArgs inst_args(Ctx &c)
 {
     return InstArgs{c.ibase.p, c.ninst, c.imu.p, c.opts.mu_pos, c.opts.mu_neg, (unsigned long long *)c.ib0.p,
-                    (unsigned long long *)c.ib0.p + c.ninst, (unsigned long long *)c.inel.p, c.batch ? 1 : 0};
+                    (unsigned long long *)c.ib0.p + c.ninst, (unsigned long long *)c.inel.p, c.batch ? 1 : 0,
+                    c.opts.lcve_max, nullptr, nullptr};
 }
 
 uint32_t lcve(Ctx &c)
@@ -425,7 +467,8 @@ uint32_t lcve(Ctx &c)
     ensure(c, c.scores, nv + 1);
     ensure(c, c.eligible, std::max<uint64_t>(c.nlits2, nv + 1));
     ensure(c, c.rank, nv + 1);
-    ensure(c, c.tmpu8, 3ull * nv + 3);
+    const bool lim = c.opts.lcve_clause_max != 0;
+    ensure(c, c.tmpu8, (lim ? 6ull : 3ull) * (nv + 1));
     ensure(c, c.tmpu32a, nv + 1);
     ensure(c, c.tmpu32b, nv + 1);
     ensure(c, c.pvs, nv + 1);
@@ -440,7 +483,18 @@ uint32_t lcve(Ctx &c)
         KTimer kt(c, K_LCVE_RADIX, 16.0 * nv * ((bits + 7) / 8));
         radix_sort_u32_pairs(c, c.scores.p, c.eligible.p, nv, ((bits + 7) / 8) * 8);
     }
-    const InstArgs I = inst_args(c);
+    InstArgs I = inst_args(c);
+    uint8_t *half = nullptr;
+    if (lim) {   // lng (2 nv + 2 bytes), then half (nv + 1), after kind / status / blocked
+        uint8_t *lng = c.tmpu8.p + 3ull * (nv + 1);
+        half = lng + 2ull * (nv + 1);
+        PFG_HIP(hipMemsetAsync(lng, 0, 3ull * (nv + 1), c.stream));
+        if (c.nrefs)
+            k_long_lits<<<grid_for(c.nrefs, 256, 8192), 256, 0, c.stream>>>(c.mem.p, c.refs.p, c.nrefs,
+                                                                             c.opts.lcve_clause_max, lng);
+        I.lng = lng;
+        I.half = half;
+    }
     if (c.ninst > 1) {   // (instance, score, variable) order
         k_inst_keys<<<grid_for(nv, 256, 8192), 256, 0, c.stream>>>(c.eligible.p, nv, I, c.scores.p);
         radix_sort_u32_pairs(c, c.scores.p, c.eligible.p, nv, bits_of32(c.ninst - 1));
@@ -465,9 +519,13 @@ uint32_t lcve(Ctx &c)
     if (c.nrefs) {
         KTimer kt(c, K_LCVE_INIT, 20.0 * c.nrefs + 8.0 * c.nliterals);
         PFG_HIP(hipMemsetAsync(c.cnt.p + CNT_D, 0, 8, c.stream));
-        k_elect_init<<<xcd_grid(c.nrefs, 256, c.tune.init_grid ? c.tune.init_grid : c.tune.clause_grid), 256, 0,
-                       c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act,
-                                                                          ra + 1, (unsigned long long *)c.cnt.p);
+        const unsigned g = xcd_grid(c.nrefs, 256, c.tune.init_grid ? c.tune.init_grid : c.tune.clause_grid);
+        if (half)
+            k_elect_init<true><<<g, 256, 0, c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act, ra + 1,
+                                                        (unsigned long long *)c.cnt.p, half);
+        else
+            k_elect_init<false><<<g, 256, 0, c.stream>>>(A, c.refs.p, c.nrefs, c.erec.p, act, ra + 1,
+                                                         (unsigned long long *)c.cnt.p, nullptr);
     }
     uint64_t hw[2] = {0, 0};
     PFG_HIP(hipMemcpyAsync(hw, rw, 8, hipMemcpyDeviceToHost, c.stream));

@@ -454,7 +454,7 @@ void pfg_default_opts(pfg_opts *o)
     o->sub_limit = 3000; o->ere_limit = 3000; o->ere_max_resolvent = 1000;
     o->sub_en = 1; o->ve_en = 1; o->ere_en = 1; o->lbd_tier1 = 2; o->stable_sort = 0;
     o->bce_en = 0; o->bce_limit = 3000; o->proof_en = 0; o->ve_fun_en = 0;
-    o->ve_atomic_en = 0;
+    o->ve_atomic_en = 0; o->lcve_max = 0; o->lcve_clause_max = 0;
 }
 
 
