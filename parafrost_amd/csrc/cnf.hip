@@ -1018,17 +1018,37 @@ __device__ __forceinline__ void ci_flush_wave(unsigned long long *icnt, uint32_t
 
 __global__ void __launch_bounds__(CI_BLOCK) k_count_inst(const uint32_t *__restrict__ mem, const uint64_t *__restrict__ refs,
                                                         uint64_t n, const uint32_t *__restrict__ ibase, uint32_t ninst,
-                                                        unsigned long long *__restrict__ icnt)
+                                                        unsigned long long *__restrict__ icnt, uint64_t msz)
 {
     uint32_t cur = 0, lo = 1, hi = 0;   // instance counted for; its variables are (lo, hi]
     unsigned long long nc = 0, nl = 0;
     for (uint64_t t0 = (uint64_t)blockIdx.x * CI_BLOCK * CI_ITEMS; t0 < n; t0 += (uint64_t)gridDim.x * CI_BLOCK * CI_ITEMS) {
+        // every item's reference, then every item's header words, in flight together
+        uint64_t r[CI_ITEMS];
+#pragma unroll
         for (int k = 0; k < CI_ITEMS; k++) {
             const uint64_t i = t0 + (uint64_t)k * CI_BLOCK + threadIdx.x;
-            if (i >= n) break;
-            const uint32_t *cl = mem + refs[i];
-            if (c_del(cl) || !c_size(cl)) continue;   // an empty clause (refuted instance) names no instance
-            const uint32_t v = cl[3] >> 1;
+            r[k] = i < n ? refs[i] : ~0ull;
+        }
+        uint32_t h0[CI_ITEMS], h2[CI_ITEMS], h3[CI_ITEMS];
+#pragma unroll
+        for (int k = 0; k < CI_ITEMS; k++) {
+            h0[k] = ST_DELETED;
+            h2[k] = h3[k] = 0;
+            if (r[k] != ~0ull) {
+                const uint32_t *cl = mem + r[k];
+                h0[k] = cl[0];
+                h2[k] = cl[2];
+                // the first literal; only an empty image at the end of the pool has no
+                // word after its header
+                if (r[k] + 3 < msz) h3[k] = cl[3];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CI_ITEMS; k++) {
+            // deleted, past the end, or empty (a refuted instance's empty clause names no instance)
+            if ((h0[k] & ST_DELETED) || !h2[k]) continue;
+            const uint32_t v = h3[k] >> 1;
             if (v <= lo || v > hi) {
                 if (nc) {
                     atomicAdd(&icnt[2 * cur], nc);
@@ -1040,7 +1060,7 @@ __global__ void __launch_bounds__(CI_BLOCK) k_count_inst(const uint32_t *__restr
                 hi = cur + 1 < ninst ? ibase[cur + 1] : 0xFFFFFFFFu;
             }
             nc++;
-            nl += (unsigned)c_size(cl);
+            nl += h2[k];
         }
         ci_flush_wave(icnt, cur, nc, nl);
     }
@@ -1052,7 +1072,7 @@ void count_inst(Ctx &c, std::vector<int64_t> &cls, std::vector<int64_t> &lits)
     if (c.nrefs) {
         KTimer kt(c, K_COUNT_INST, 20.0 * c.nrefs + 4.0 * c.nliterals);
         k_count_inst<<<grid_for((c.nrefs + CI_ITEMS - 1) / CI_ITEMS, CI_BLOCK, 8192), CI_BLOCK, 0, c.stream>>>(
-            c.mem.p, c.refs.p, c.nrefs, c.ibase.p, c.ninst, (unsigned long long *)c.icnt.p);
+            c.mem.p, c.refs.p, c.nrefs, c.ibase.p, c.ninst, (unsigned long long *)c.icnt.p, c.msz);
     }
     std::vector<uint64_t> h(2ull * c.ninst);
     PFG_HIP(hipMemcpyAsync(h.data(), c.icnt.p, 16ull * c.ninst, hipMemcpyDeviceToHost, c.stream));
